@@ -1,0 +1,368 @@
+"""MI355X-native DMRG hot path (dmrgcc drop-in), Python host binding.
+
+Thin ctypes layer over the C ABI in ``include/dmrgcc_hip.h`` implemented by
+``libdmrgcc_amd.so`` (HIP kernels for gfx950 + C++ host engine).  Device
+buffers are passed as raw HIP pointers (torch CUDA tensors' ``data_ptr()``);
+PyTorch is used only to own device memory.
+
+The library must be built (``python dmrgcc_amd/build.py`` or
+``__graft_entry__.build()``); importing the compute entry points without it
+raises -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdmrgcc_amd.so")
+
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISION_NAMES = {FP64: "fp64", FP32: "fp32", BF16X3: "bf16x3"}
+
+_lib = None
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_i64_p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "dmrgcc_last_error": (ctypes.c_char_p, []),
+    "dmrgcc_version": (ctypes.c_int, []),
+    "dmrgcc_rand_draws": (ctypes.c_int, [ctypes.c_uint, ctypes.c_int64, ctypes.c_int, _c_int_p]),
+    "dmrgcc_quicksort": (ctypes.c_int, [ctypes.c_int, _c_dbl_p, _c_int_p, ctypes.c_int]),
+    "dmrgcc_truncate_dims": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_dbl_p, ctypes.c_int,
+                                            ctypes.c_double, _c_int_p]),
+    "dmrgcc_model_spinless_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_model_destroy": (ctypes.c_int, [_vp]),
+    "dmrgcc_model_num_site": (ctypes.c_int, [_vp]),
+    "dmrgcc_model_space_table": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p]),
+    "dmrgcc_model_mpo_site": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p]),
+    "dmrgcc_model_mpo_table": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p]),
+    "dmrgcc_block_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_vp), _c_int_p, _c_int_p, _c_int_p,
+                                         ctypes.c_int, _vp]),
+    "dmrgcc_heff_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p,
+                                          ctypes.c_int, ctypes.c_int, _c_int_p,
+                                          ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                          ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                          _c_int_p, _c_i64_p, _vp,
+                                          ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                          _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
+    "dmrgcc_heff_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "dmrgcc_heff_info": (ctypes.c_int, [_vp, _c_dbl_p]),
+    "dmrgcc_heff_destroy": (ctypes.c_int, [_vp]),
+    "dmrgcc_lanczos": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p,
+                                      _c_int_p]),
+    "dmrgcc_blocks_count": (ctypes.c_int, [_vp]),
+    "dmrgcc_blocks_get": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                         _c_int_p, _c_i64_p]),
+    "dmrgcc_blocks_data": (_vp, [_vp]),
+    "dmrgcc_blocks_download": (ctypes.c_int, [_vp, _c_dbl_p]),
+    "dmrgcc_blocks_destroy": (ctypes.c_int, [_vp]),
+    "dmrgcc_env_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _c_dbl_p, ctypes.c_int, ctypes.c_int, _c_int_p,
+                                         ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _vp,
+                                         ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                         _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
+    "dmrgcc_sector_svd": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, _c_int_p,
+                                         ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(_vp), _vp]),
+    "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                  _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+}
+
+
+class DmrgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdmrgcc_amd.so is not built; run python dmrgcc_amd/build.py")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().dmrgcc_last_error().decode()
+        raise DmrgError("dmrgcc status %d: %s" % (rc, msg))
+
+
+def _ip(a):
+    return a.ctypes.data_as(_c_int_p)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def _lp(a):
+    return a.ctypes.data_as(_c_i64_p)
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _i64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int64))
+
+
+# --------------------------------------------------------------------------
+# host-side restatements
+# --------------------------------------------------------------------------
+def rand_draws(n, seed=1, skip=0):
+    out = np.zeros(n, dtype=np.int32)
+    _check(lib().dmrgcc_rand_draws(seed, skip, n, _ip(out)))
+    return out
+
+
+def quicksort(vals, index=None, flag=0):
+    v = _f64(vals).copy()
+    idx = None if index is None else _i32(index).copy()
+    _check(lib().dmrgcc_quicksort(len(v), _dp(v), None if idx is None else _ip(idx), flag))
+    return v, idx
+
+
+def truncate_dims(svals, max_dim, canpre):
+    """ComputeTruncateDim: svals is a list of descending arrays (one per
+    sector).  Returns (kept dims, svals after the sigma := 0 forcing)."""
+    counts = _i32([len(s) for s in svals])
+    flat = _f64(np.concatenate([np.asarray(s, dtype=np.float64) for s in svals]))
+    trunc = np.zeros(len(svals), dtype=np.int32)
+    _check(lib().dmrgcc_truncate_dims(len(svals), _ip(counts), _dp(flat), max_dim, canpre, _ip(trunc)))
+    out, acc = [], 0
+    for c in counts:
+        out.append(flat[acc:acc + c].copy())
+        acc += c
+    return trunc, out
+
+
+class SpinlessModel:
+    """RealSpinlessBoseSquareSpace + RealSpinlessBoseSquareHamiltonian."""
+
+    def __init__(self, nx, ny, num_boson, physics_dim, hop_x=-1.0, hop_y=-1.0, U=0.0, mu=0.0,
+                 period_x=False, period_y=False):
+        self.nx, self.ny, self.N, self.d = nx, ny, num_boson, physics_dim
+        self.L = nx * ny
+        h = _vp()
+        _check(lib().dmrgcc_model_spinless_create(nx, ny, num_boson, physics_dim, hop_x, hop_y, U,
+                                                  mu, int(period_x), int(period_y),
+                                                  ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.dmrgcc_model_destroy(self._h)
+            self._h = None
+
+    def space_table(self, bond):
+        c = ctypes.c_int()
+        _check(lib().dmrgcc_model_space_table(self._h, bond, ctypes.byref(c), None))
+        q = np.zeros(c.value, dtype=np.int32)
+        _check(lib().dmrgcc_model_space_table(self._h, bond, ctypes.byref(c), _ip(q)))
+        return q
+
+    def mpo_site(self, site):
+        lb, rb = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dmrgcc_model_mpo_site(self._h, site, ctypes.byref(lb), ctypes.byref(rb), None))
+        W = np.zeros((lb.value, rb.value, self.d, self.d))
+        _check(lib().dmrgcc_model_mpo_site(self._h, site, ctypes.byref(lb), ctypes.byref(rb), _dp(W)))
+        return W
+
+    def mpo_table(self, bond):
+        c = ctypes.c_int()
+        _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), None))
+        q = np.zeros(c.value, dtype=np.int32)
+        _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), _ip(q)))
+        return q
+
+    def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
+        """RealTensorRundmrg::Run.  table: rows (maxm, maxb, niter, canpre,
+        noise_exponent) per half sweep.  Returns a dict of per-half-sweep
+        energies, discarded weights, seconds and run statistics."""
+        rows = [tuple(r) for r in table]
+        n = len(rows)
+        maxm = _i32([r[0] for r in rows])
+        maxb = _i32([r[1] for r in rows])
+        niter = _i32([r[2] for r in rows])
+        canpre = _f64([r[3] for r in rows])
+        noise = _f64([r[4] for r in rows])
+        e = np.zeros(n)
+        t = np.zeros(n)
+        s = np.zeros(n)
+        st = np.zeros(8)
+        _check(lib().dmrgcc_run(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
+                                _dp(canpre), _dp(noise), precision, int(refine), int(verbose), _dp(e),
+                                _dp(t), _dp(s), _dp(st)))
+        return {"energies": e, "trunc_err": t, "seconds": s, "total_seconds": st[0],
+                "matvecs": int(st[1]), "dense_solves": int(st[2]), "matvec_seconds": st[3],
+                "matvec_flops": st[4], "rand_draws": int(st[5]), "solves": int(st[6]),
+                "max_bond": int(st[7])}
+
+
+# --------------------------------------------------------------------------
+# device entry points
+# --------------------------------------------------------------------------
+def _ptr(t):
+    return _vp(t.data_ptr()) if t is not None else _vp()
+
+
+def _sync_torch():
+    """Order the caller's pending torch work before a library call."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.synchronize()
+
+
+def block_gemm(As, Bs, Cs, precision=FP64, stream=None):
+    """C_i = A_i @ B_i for lists of contiguous float64 device tensors."""
+    n = len(As)
+    arr = lambda ts: (_vp * max(n, 1))(*[_vp(t.data_ptr()) for t in ts])
+    M = _i32([a.shape[0] for a in As])
+    K = _i32([a.shape[1] for a in As])
+    N = _i32([b.shape[1] for b in Bs])
+    _sync_torch()
+    _check(lib().dmrgcc_block_gemm(n, arr(As), arr(Bs), arr(Cs), _ip(M), _ip(N), _ip(K), precision,
+                                   _vp(stream) if stream else _vp()))
+
+
+class BlockList:
+    """Host description of a block-sparse matrix: per block (channel, l, r,
+    rows, cols, offset)."""
+
+    def __init__(self, o, l, r, rows, cols, off):
+        self.o, self.l, self.r = _i32(o), _i32(l), _i32(r)
+        self.rows, self.cols, self.off = _i32(rows), _i32(cols), _i64(off)
+
+    def __len__(self):
+        return len(self.l)
+
+
+class HeffPlan:
+    """H_eff matvec plan of one site (MultiplyEffectHamilton)."""
+
+    def __init__(self, W, phys, ket_l, ket_r, ket_rows, ket_cols, Lb, L_data, Rb, R_data, stream=None):
+        W = _f64(W)
+        lb, rb, d = W.shape[0], W.shape[1], W.shape[2]
+        phys = _i32(phys)
+        nl, nr = phys.shape
+        self._keep = (W, phys, _i32(ket_l), _i32(ket_r), _i32(ket_rows), _i32(ket_cols), Lb, Rb,
+                      L_data, R_data)
+        _, _, kl, kr, krows, kcols, _, _, _, _ = self._keep
+        h = _vp()
+        _sync_torch()
+        _check(lib().dmrgcc_heff_create(
+            d, lb, rb, _dp(W), nl, nr, _ip(phys), len(kl), _ip(kl), _ip(kr), _ip(krows), _ip(kcols),
+            len(Lb), _ip(Lb.o), _ip(Lb.l), _ip(Lb.r), _ip(Lb.rows), _ip(Lb.cols), _lp(Lb.off),
+            _ptr(L_data), len(Rb), _ip(Rb.o), _ip(Rb.l), _ip(Rb.r), _ip(Rb.rows), _ip(Rb.cols),
+            _lp(Rb.off), _ptr(R_data), _vp(stream) if stream else _vp(), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.dmrgcc_heff_destroy(self._h)
+            self._h = None
+
+    def apply(self, x, y, precision=FP64):
+        """y += H_eff x (device float64 tensors)."""
+        _sync_torch()
+        _check(lib().dmrgcc_heff_apply(self._h, _ptr(x), _ptr(y), precision))
+
+    def info(self):
+        v = np.zeros(6)
+        _check(lib().dmrgcc_heff_info(self._h, _dp(v)))
+        return {"psi_dim": int(v[0]), "flops_ref": v[1], "flops_exec": v[2], "bytes_min": v[3],
+                "tasks": int(v[4]), "tiles": int(v[5])}
+
+    def lanczos(self, ket, num_iter, precision=FP64, refine=True):
+        e = ctypes.c_double()
+        it = ctypes.c_int()
+        _sync_torch()
+        _check(lib().dmrgcc_lanczos(self._h, _ptr(ket), num_iter, precision, int(refine),
+                                    ctypes.byref(e), ctypes.byref(it)))
+        return e.value, it.value
+
+
+def env_update(leigh, W, phys, ket_l, ket_r, ket_rows, ket_cols, ket_data, Eb, E_data, stream=None):
+    """Left (leigh 0) / right (leigh 1) environment update.  Returns
+    (list of (o, l, r, rows, cols, offset), device data pointer holder)."""
+    W = _f64(W)
+    lb, rb, d = W.shape[0], W.shape[1], W.shape[2]
+    phys = _i32(phys)
+    nl, nr = phys.shape
+    kl, kr, krows, kcols = _i32(ket_l), _i32(ket_r), _i32(ket_rows), _i32(ket_cols)
+    h = _vp()
+    _sync_torch()
+    _check(lib().dmrgcc_env_update(
+        leigh, d, lb, rb, _dp(W), nl, nr, _ip(phys), len(kl), _ip(kl), _ip(kr), _ip(krows),
+        _ip(kcols), _ptr(ket_data), len(Eb), _ip(Eb.o), _ip(Eb.l), _ip(Eb.r), _ip(Eb.rows),
+        _ip(Eb.cols), _lp(Eb.off), _ptr(E_data), _vp(stream) if stream else _vp(), ctypes.byref(h)))
+    return EnvResult(h)
+
+
+class EnvResult:
+    def __init__(self, h):
+        self._h = h
+        n = lib().dmrgcc_blocks_count(h)
+        self.blocks = []
+        o, l, r, rows, cols = (ctypes.c_int() for _ in range(5))
+        off = ctypes.c_int64()
+        for i in range(n):
+            _check(lib().dmrgcc_blocks_get(h, i, ctypes.byref(o), ctypes.byref(l), ctypes.byref(r),
+                                           ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(off)))
+            self.blocks.append((o.value, l.value, r.value, rows.value, cols.value, off.value))
+        self.data_ptr = lib().dmrgcc_blocks_data(h)
+        self.size = sum(b[3] * b[4] for b in self.blocks)
+
+    def to_host(self):
+        out = np.zeros(max(self.size, 1))
+        _check(lib().dmrgcc_blocks_download(self._h, _dp(out)))
+        return out[:self.size]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.dmrgcc_blocks_destroy(self._h)
+            self._h = None
+
+
+def sector_svd(mats, stream=None):
+    """Batched thin SVD of device float64 matrices.  Returns lists (U, S, Vt)
+    of device tensors with S descending."""
+    import torch
+    n = len(mats)
+    U = [torch.empty((m.shape[0], min(m.shape)), dtype=torch.float64, device=m.device) for m in mats]
+    S = [torch.empty((min(m.shape),), dtype=torch.float64, device=m.device) for m in mats]
+    Vt = [torch.empty((min(m.shape), m.shape[1]), dtype=torch.float64, device=m.device) for m in mats]
+    arr = lambda ts: (_vp * max(n, 1))(*[_vp(t.data_ptr()) for t in ts])
+    M = _i32([m.shape[0] for m in mats])
+    N = _i32([m.shape[1] for m in mats])
+    _sync_torch()
+    _check(lib().dmrgcc_sector_svd(n, arr(mats), _ip(M), _ip(N), arr(U), arr(S), arr(Vt),
+                                   _vp(stream) if stream else _vp()))
+    return U, S, Vt

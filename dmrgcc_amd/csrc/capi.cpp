@@ -1,0 +1,477 @@
+// extern "C" entry points (declared in include/dmrgcc_hip.h).  Each wraps one
+// piece of the engine, converting exceptions into status codes.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+
+#include "../../include/dmrgcc_hip.h"
+#include "dmrg.h"
+#include "quicksort.h"
+#include "rng.h"
+
+using namespace dmrg;
+
+struct dmrgcc_model {
+  std::unique_ptr<RealSpinlessBoseSquareSpace> space;
+  std::unique_ptr<RealSpinlessBoseSquareHamiltonian> ham;
+  std::vector<QNTable> tables;
+};
+
+struct dmrgcc_heff {
+  std::unique_ptr<Context> cx;
+  Lattice lat;
+  SiteOperator W;
+  Env L, R;
+  HeffPlan plan;
+  LanczosStats st;
+};
+
+struct dmrgcc_blocks {
+  std::vector<int> o, l, r, rows, cols;
+  std::vector<int64_t> off;
+  DevBuf<double> data;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return kErrInternal;
+  }
+}
+
+SiteOperator make_op(int d, int lb, int rb, const double* W) {
+  require(W != nullptr && d > 0 && lb > 0 && rb > 0, "invalid MPO site tensor", kErrArgument);
+  SiteOperator op;
+  op.d = d; op.lb = lb; op.rb = rb;
+  op.w.assign(W, W + (size_t)lb * rb * d * d);
+  return op;
+}
+
+void make_lattice(Lattice& lat, int d, int nl, int nr, const int* phys, int nket, const int* kl,
+                  const int* kr, const int* krows, const int* kcols, const double* data) {
+  require(nl >= 0 && nr >= 0 && nket >= 0, "negative sizes", kErrArgument);
+  lat.d = d;
+  lat.lb.resize(nl);
+  lat.rb.resize(nr);
+  std::iota(lat.lb.begin(), lat.lb.end(), 0);
+  std::iota(lat.rb.begin(), lat.rb.end(), 0);
+  lat.ld.assign(nl, 0);
+  lat.rd.assign(nr, 0);
+  lat.phys.assign(nl, std::vector<int>(nr, -1));
+  for (int i = 0; i < nl; ++i)
+    for (int j = 0; j < nr; ++j) lat.phys[i][j] = phys[(size_t)i * nr + j];
+  int64_t acc = 0;
+  for (int b = 0; b < nket; ++b) {
+    require(kl[b] >= 0 && kl[b] < nl && kr[b] >= 0 && kr[b] < nr, "ket block sector out of range");
+    require(lat.phys[kl[b]][kr[b]] != -1, "ket block without a physical index");
+    lat.ket.blk.push_back(Block{kl[b], kr[b], krows[b], kcols[b], acc});
+    acc += (int64_t)krows[b] * kcols[b];
+    lat.ld[kl[b]] = krows[b];
+    lat.rd[kr[b]] = kcols[b];
+  }
+  lat.ket.size = acc;
+  lat.ket.ext = data;
+}
+
+Env make_env(int nbond, int n, const int* o, const int* l, const int* r, const int* rows,
+             const int* cols, const int64_t* off, const double* data) {
+  Env e(nbond);
+  for (int i = 0; i < n; ++i) {
+    require(o[i] >= 0 && o[i] < nbond, "environment channel out of range");
+    e[o[i]].blk.push_back(Block{l[i], r[i], rows[i], cols[i], off[i]});
+  }
+  for (auto& b : e) b.ext = data;
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dmrgcc_last_error(void) { return last_error(); }
+int dmrgcc_version(void) { return 1; }
+
+int dmrgcc_rand_draws(unsigned seed, int64_t skip, int n, int* out) {
+  return guarded([&] {
+    require(out != nullptr && n >= 0 && skip >= 0, "bad arguments", kErrArgument);
+    GlibcRand g(seed);
+    g.skip(skip);
+    for (int i = 0; i < n; ++i) out[i] = g.next();
+  });
+}
+
+int dmrgcc_quicksort(int n, double* vals, int* index, int flag) {
+  return guarded([&] {
+    require(vals != nullptr && n >= 0 && (flag == 0 || flag == 1), "bad arguments", kErrArgument);
+    std::vector<double> v(vals, vals + n);
+    std::vector<int> idx;
+    if (index) idx.assign(index, index + n);
+    quicksort(v, index ? &idx : nullptr, flag);
+    std::memcpy(vals, v.data(), n * sizeof(double));
+    if (index) std::memcpy(index, idx.data(), n * sizeof(int));
+  });
+}
+
+int dmrgcc_truncate_dims(int nsec, const int* counts, double* svals, int max_dim, double canpre,
+                         int* trunc) {
+  return guarded([&] {
+    require(nsec >= 0 && counts && svals && trunc, "bad arguments", kErrArgument);
+    std::vector<std::vector<double>> sv(nsec);
+    int64_t acc = 0;
+    for (int s = 0; s < nsec; ++s) {
+      require(counts[s] >= 1, "every sector needs at least one singular value", kErrArgument);
+      sv[s].assign(svals + acc, svals + acc + counts[s]);
+      acc += counts[s];
+    }
+    std::vector<int> t = truncate_dims(max_dim, canpre, sv);
+    acc = 0;
+    for (int s = 0; s < nsec; ++s) {
+      trunc[s] = t[s];
+      svals[acc] = sv[s][0];
+      acc += counts[s];
+    }
+  });
+}
+
+int dmrgcc_model_spinless_create(int nx, int ny, int num_boson, int physics_dim, double hop_x,
+                                 double hop_y, double U, double mu, int period_x, int period_y,
+                                 dmrgcc_model** out) {
+  return guarded([&] {
+    require(out && nx > 0 && ny > 0 && physics_dim > 1 && num_boson >= 0, "bad model arguments",
+            kErrArgument);
+    auto m = std::make_unique<dmrgcc_model>();
+    m->space = std::make_unique<RealSpinlessBoseSquareSpace>(nx, ny, num_boson, physics_dim);
+    m->ham = std::make_unique<RealSpinlessBoseSquareHamiltonian>(nx, ny, physics_dim, 0.0, hop_x, hop_y,
+                                                                 U, mu, period_x != 0, period_y != 0);
+    m->ham->DefineTensorHamiltonian();
+    m->tables = m->space->DefineQuantumTable();
+    *out = m.release();
+  });
+}
+
+int dmrgcc_model_destroy(dmrgcc_model* m) {
+  delete m;
+  return 0;
+}
+
+int dmrgcc_model_num_site(const dmrgcc_model* m) { return m ? m->space->num_site() : -1; }
+
+int dmrgcc_model_space_table(const dmrgcc_model* m, int bond, int* count, int* qn) {
+  return guarded([&] {
+    require(m && count && bond >= 0 && bond < (int)m->tables.size(), "bad arguments", kErrArgument);
+    *count = (int)m->tables[bond].size();
+    if (qn)
+      for (int i = 0; i < *count; ++i) qn[i] = m->tables[bond][i][0];
+  });
+}
+
+int dmrgcc_model_mpo_site(const dmrgcc_model* m, int site, int* lb, int* rb, double* W) {
+  return guarded([&] {
+    require(m && lb && rb && site >= 0 && site < m->ham->num_site(), "bad arguments", kErrArgument);
+    const SiteOperator& op = m->ham->sites()[site];
+    *lb = op.lb;
+    *rb = op.rb;
+    if (W) std::memcpy(W, op.w.data(), op.w.size() * sizeof(double));
+  });
+}
+
+int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn) {
+  return guarded([&] {
+    require(m && count, "bad arguments", kErrArgument);
+    const auto& t = m->ham->bond_tables();
+    require(bond >= 0 && bond < (int)t.size(), "bond out of range", kErrArgument);
+    *count = (int)t[bond].size();
+    if (qn)
+      for (int i = 0; i < *count; ++i) qn[i] = t[bond][i][0];
+  });
+}
+
+int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, double* const* C,
+                      const int* M, const int* N, const int* K, int precision, void* stream) {
+  return guarded([&] {
+    require(n >= 0 && (n == 0 || (A && B && C && M && N && K)), "bad arguments", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GemmBatch gb;
+    for (int i = 0; i < n; ++i) {
+      require(M[i] >= 0 && N[i] >= 0 && K[i] >= 0, "negative GEMM size");
+      gb.begin_item(C[i], N[i], M[i], N[i], 0);
+      gb.add(A[i], K[i], B[i], N[i], K[i], 0, 1.0);
+    }
+    gb.finalize(s);
+    gb.launch(s, precision);
+    DMRG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
+                       int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                       const int* ket_cols, int nL, const int* L_o, const int* L_l,
+                       const int* L_r, const int* L_rows, const int* L_cols,
+                       const int64_t* L_off, const double* L_data, int nR, const int* R_o,
+                       const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
+                       const int64_t* R_off, const double* R_data, void* stream,
+                       dmrgcc_heff** out) {
+  return guarded([&] {
+    require(out != nullptr, "null output", kErrArgument);
+    auto h = std::make_unique<dmrgcc_heff>();
+    h->cx = std::make_unique<Context>(static_cast<hipStream_t>(stream));
+    h->W = make_op(d, lb, rb, W);
+    make_lattice(h->lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, nullptr);
+    h->L = make_env(lb, nL, L_o, L_l, L_r, L_rows, L_cols, L_off, L_data);
+    h->R = make_env(rb, nR, R_o, R_l, R_r, R_rows, R_cols, R_off, R_data);
+    h->plan.build(h->lat, h->W, h->L, h->R, h->cx->stream);
+    *out = h.release();
+  });
+}
+
+int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision) {
+  return guarded([&] {
+    require(h && x && y, "null argument", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    h->plan.apply(x, y, precision, h->cx->stream);
+    h->cx->sync();
+  });
+}
+
+int dmrgcc_heff_info(const dmrgcc_heff* h, double* info) {
+  return guarded([&] {
+    require(h && info, "null argument", kErrArgument);
+    info[0] = (double)h->plan.psi_dim();
+    info[1] = h->plan.fref();
+    info[2] = h->plan.flops();
+    info[3] = (double)h->plan.bytes_min();
+    info[4] = (double)h->plan.tasks();
+    info[5] = (double)(h->plan.stage1().tiles.size() + h->plan.stage2().tiles.size());
+  });
+}
+
+int dmrgcc_heff_destroy(dmrgcc_heff* h) {
+  delete h;
+  return 0;
+}
+
+int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int refine,
+                   double* energy, int* iters_out) {
+  return guarded([&] {
+    require(h && ket && energy, "null argument", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    int it = 0;
+    *energy = lanczos_solve(h->plan, ket, num_iter, precision, refine, *h->cx, h->st, &it);
+    if (iters_out) *iters_out = it;
+  });
+}
+
+int dmrgcc_blocks_count(const dmrgcc_blocks* b) { return b ? (int)b->o.size() : -1; }
+
+int dmrgcc_blocks_get(const dmrgcc_blocks* b, int i, int* o, int* l, int* r, int* rows, int* cols,
+                      int64_t* off) {
+  return guarded([&] {
+    require(b && i >= 0 && i < (int)b->o.size(), "block index out of range", kErrArgument);
+    if (o) *o = b->o[i];
+    if (l) *l = b->l[i];
+    if (r) *r = b->r[i];
+    if (rows) *rows = b->rows[i];
+    if (cols) *cols = b->cols[i];
+    if (off) *off = b->off[i];
+  });
+}
+
+const double* dmrgcc_blocks_data(const dmrgcc_blocks* b) { return b ? b->data.data() : nullptr; }
+
+int dmrgcc_blocks_download(const dmrgcc_blocks* b, double* host) {
+  return guarded([&] {
+    require(b && host, "null argument", kErrArgument);
+    int64_t total = 0;
+    for (size_t i = 0; i < b->rows.size(); ++i) total += (int64_t)b->rows[i] * b->cols[i];
+    if (total > 0)
+      DMRG_HIP(hipMemcpy(host, b->data.data(), total * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int dmrgcc_blocks_destroy(dmrgcc_blocks* b) {
+  delete b;
+  return 0;
+}
+
+int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
+                      const int* phys, int nket, const int* ket_l, const int* ket_r,
+                      const int* ket_rows, const int* ket_cols, const double* ket_data, int nE,
+                      const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
+                      const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
+                      dmrgcc_blocks** out) {
+  return guarded([&] {
+    require(out && (leigh == 0 || leigh == 1), "bad arguments", kErrArgument);
+    Context cx(static_cast<hipStream_t>(stream));
+    SiteOperator op = make_op(d, lb, rb, W);
+    Lattice lat;
+    make_lattice(lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, ket_data);
+    Env e = make_env(leigh == 0 ? lb : rb, nE, E_o, E_l, E_r, E_rows, E_cols, E_off, E_data);
+    Env res = leigh == 0 ? left_env_update(lat, op, e, cx) : right_env_update(lat, op, e, cx);
+    auto b = std::make_unique<dmrgcc_blocks>();
+    int64_t total = 0;
+    for (const auto& t : res) total += t.size;
+    b->data.resize((size_t)std::max<int64_t>(total, 1));
+    int64_t acc = 0;
+    for (size_t o = 0; o < res.size(); ++o) {
+      for (const Block& k : res[o].blk) {
+        b->o.push_back((int)o);
+        b->l.push_back(k.l);
+        b->r.push_back(k.r);
+        b->rows.push_back(k.rows);
+        b->cols.push_back(k.cols);
+        b->off.push_back(acc + k.off);
+      }
+      if (res[o].size > 0)
+        DMRG_HIP(hipMemcpyAsync(b->data.data() + acc, res[o].data.data(), res[o].size * sizeof(double),
+                                hipMemcpyDeviceToDevice, cx.stream));
+      acc += res[o].size;
+    }
+    cx.sync();
+    *out = b.release();
+  });
+}
+
+int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N, double* const* U,
+                      double* const* S, double* const* Vt, void* stream) {
+  return guarded([&] {
+    require(n >= 0 && (n == 0 || (A && M && N && U && S && Vt)), "bad arguments", kErrArgument);
+    Context cx(static_cast<hipStream_t>(stream));
+    struct P { int M, N, m, k; bool trans; int64_t g, v, s; };
+    std::vector<P> ps(n);
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+      require(M[i] > 0 && N[i] > 0, "empty matrix in sector SVD");
+      P& p = ps[i];
+      p.trans = M[i] < N[i];
+      p.m = std::max(M[i], N[i]);
+      p.k = std::min(M[i], N[i]);
+      p.M = M[i]; p.N = N[i];
+      p.g = w; w += (int64_t)p.m * p.k;
+      p.v = w; w += (int64_t)p.k * p.k;
+      p.s = w; w += p.k;
+    }
+    DevBuf<double> work((size_t)std::max<int64_t>(w, 1));
+    std::vector<CopyTask> cps;
+    std::vector<SvdProblem> probs;
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      CopyTask c{A[i], work.data() + p.g, p.M, p.N, p.N, 1, 0, 0, 1.0};
+      if (!p.trans) { c.dsr = 1; c.dsc = p.m; }  // G = A, column-major
+      else { c.dsr = p.m; c.dsc = 1; }           // G = A^T
+      cps.push_back(c);
+      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s, p.m, p.k});
+    }
+    launch_copies(cps, cx.copy_scratch, cx.stream);
+    jacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);
+    std::vector<int> hperm;
+    std::vector<double> hsc, hone;
+    std::vector<int64_t> poff(n);
+    std::vector<std::vector<double>> sorted(n);
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      std::vector<double> s(p.k);
+      DMRG_HIP(hipMemcpyAsync(s.data(), work.data() + p.s, p.k * sizeof(double), hipMemcpyDeviceToHost,
+                              cx.stream));
+      cx.sync();
+      std::vector<int> ord(p.k);
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return s[a] > s[b]; });
+      poff[i] = hperm.size();
+      for (int j = 0; j < p.k; ++j) {
+        hperm.push_back(ord[j]);
+        sorted[i].push_back(s[ord[j]]);
+        hsc.push_back(s[ord[j]] > 0 ? 1.0 / s[ord[j]] : std::numeric_limits<double>::quiet_NaN());
+      }
+    }
+    const int64_t ones = hsc.size();
+    int kmax = 1;
+    for (const P& p : ps) kmax = std::max(kmax, p.k);
+    for (int j = 0; j < kmax; ++j) hsc.push_back(1.0);
+    upload(cx.ivec, hperm.empty() ? std::vector<int>{0} : hperm, cx.stream);
+    upload(cx.dvec, hsc, cx.stream);
+    std::vector<ColGatherTask> gts;
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      const int* pp = cx.ivec.data() + poff[i];
+      const double* inv = cx.dvec.data() + poff[i];
+      const double* one = cx.dvec.data() + ones;
+      const double* G = work.data() + p.g;
+      const double* V = work.data() + p.v;
+      // U (M x k) row-major, Vt (k x N) row-major
+      ColGatherTask u{}, v{};
+      u.perm = v.perm = pp;
+      u.ncol = v.ncol = p.k;
+      u.dst = U[i]; u.dsr = p.k; u.dsc = 1; u.rows = p.M;
+      v.dst = Vt[i]; v.dsr = 1; v.dsc = p.N; v.rows = p.N;
+      if (!p.trans) {  // A = (G V^-1) -> U = G/sigma, V = V
+        u.src = G; u.ld = p.m; u.scale = inv;
+        v.src = V; v.ld = p.k; v.scale = one;
+      } else {         // A^T = G -> U = V, V = G/sigma
+        u.src = V; u.ld = p.k; u.scale = one;
+        v.src = G; v.ld = p.m; v.scale = inv;
+      }
+      gts.push_back(u);
+      gts.push_back(v);
+      DMRG_HIP(hipMemcpyAsync(S[i], sorted[i].data(), p.k * sizeof(double), hipMemcpyHostToDevice,
+                              cx.stream));
+    }
+    launch_col_gather(gts, cx.gather_scratch, cx.stream);
+    cx.sync();
+  });
+}
+
+int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+               const int* maxb, const int* niter, const double* canpre, const double* noise,
+               int precision, int refine, int verbose, double* energies, double* trunc,
+               double* seconds, double* stats) {
+  return guarded([&] {
+    require(m && nhalf > 0 && maxm && maxb && niter && canpre && noise, "bad arguments", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    RunParams p;
+    p.first_block = first_block;
+    p.first_dim = first_dim;
+    for (int h = 0; h < nhalf; ++h) {
+      p.maxm.push_back(maxm[h]);
+      p.maxb.push_back(maxb[h]);
+      p.niter.push_back(niter[h]);
+      p.canpre.push_back(canpre[h]);
+      p.noise.push_back(std::pow(0.1, noise[h]));
+    }
+    p.precision = precision;
+    p.refine = refine;
+    p.verbose = verbose;
+    RealTensorRundmrg run(*m->space, *m->ham, p);
+    RunResult r = run.Run();
+    for (int h = 0; h < nhalf && h < (int)r.energies.size(); ++h) {
+      if (energies) energies[h] = r.energies[h];
+      if (trunc) trunc[h] = r.trunc_err[h];
+      if (seconds) seconds[h] = r.half_seconds[h];
+    }
+    if (stats) {
+      stats[0] = r.total_seconds;
+      stats[1] = r.stats.matvecs;
+      stats[2] = r.stats.dense;
+      stats[3] = r.stats.matvec_seconds;
+      stats[4] = r.stats.flops;
+      stats[5] = (double)r.rng_draws;
+      stats[6] = r.stats.solves;
+      int mx = 0;
+      for (int v : r.final_bond_dims) mx = std::max(mx, v);
+      stats[7] = mx;
+    }
+  });
+}
+
+}  // extern "C"

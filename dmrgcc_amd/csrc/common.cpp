@@ -1,0 +1,82 @@
+// Caching device allocator and error plumbing.
+#include "common.h"
+
+#include <algorithm>
+
+namespace dmrg {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& s) { g_last_error = s; }
+const char* last_error() { return g_last_error.c_str(); }
+
+DevicePool& DevicePool::get() {
+  // Intentionally leaked: buffers may be released by static destructors
+  // after the HIP runtime itself has begun tearing down.
+  static DevicePool* pool = new DevicePool();
+  return *pool;
+}
+
+size_t DevicePool::round(size_t b) {
+  if (b <= 4096) return (b + 255) & ~size_t(255);
+  size_t r = 4096;
+  while (r < b) r <<= 1;
+  return r;
+}
+
+void* DevicePool::alloc(size_t bytes) {
+  const size_t r = round(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = free_.find(r);
+  if (it != free_.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    cached_ -= r;
+    in_use_ += r;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, r);
+  if (e != hipSuccess) {
+    // release the cache and retry once
+    for (auto& kv : free_)
+      for (void* q : kv.second) {
+        (void)hipFree(q);
+        size_of_.erase(q);
+      }
+    free_.clear();
+    cached_ = 0;
+    (void)hipGetLastError();
+    e = hipMalloc(&p, r);
+    if (e != hipSuccess)
+      throw Error(kErrHip, std::string("hipMalloc(") + std::to_string(r) + "): " + hipGetErrorString(e));
+  }
+  size_of_[p] = r;
+  in_use_ += r;
+  return p;
+}
+
+void DevicePool::release(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = size_of_.find(p);
+  if (it == size_of_.end()) return;
+  free_[it->second].push_back(p);
+  in_use_ -= it->second;
+  cached_ += it->second;
+}
+
+void DevicePool::trim() {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& kv : free_)
+    for (void* q : kv.second) {
+      (void)hipFree(q);
+      size_of_.erase(q);
+    }
+  free_.clear();
+  cached_ = 0;
+}
+
+}  // namespace dmrg

@@ -1,0 +1,204 @@
+// Host-side DMRG engine: block-sparse site tensors and environments resident
+// in HBM, host bookkeeping of quantum-number sectors, device kernels for all
+// numerics.  Mirrors the reference classes RealTensorLattice,
+// RealTensorContraction, RealTensorLanczos, RealTensorSpace,
+// RealTensorNetwork and RealTensorRundmrg (src/dmrg/realdmrg/).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "model.h"
+#include "rng.h"
+
+namespace dmrg {
+
+// One block of a block-sparse matrix: sector positions (l, r), shape, element
+// offset into the tensor's device buffer (row-major).
+struct Block {
+  int l, r, rows, cols;
+  int64_t off;
+};
+
+// RealMatrixBlock: blocks concatenated in block order in one HBM buffer; the
+// flat buffer is exactly the reference's VectorizeMatrixBlock layout.
+struct BlockTensor {
+  std::vector<Block> blk;
+  int64_t size = 0;
+  DevBuf<double> data;
+  const double* ext = nullptr;  // non-owning view of caller memory (C ABI)
+
+  int n() const { return (int)blk.size(); }
+  void define(const std::vector<int>& l, const std::vector<int>& r);
+  void allocate();  // offsets from shapes, fresh (uninitialised) buffer
+  int find(int l, int r) const;
+  std::vector<int> same(int leigh, int target) const;
+  double* base() const { return ext ? const_cast<double*>(ext) : data.data(); }
+  double* ptr(int i) const { return base() + blk[i].off; }
+  int64_t elems(int i) const { return (int64_t)blk[i].rows * blk[i].cols; }
+};
+
+using Env = std::vector<BlockTensor>;  // one block-sparse matrix per MPO channel
+
+// RealTensorLattice: one MPS site.
+struct Lattice {
+  int d = 0;
+  std::vector<int> lb, ld, rb, rd;   // sector QN-table indices and dims
+  std::vector<std::vector<int>> phys; // [nl][nr], -1 where no block
+  BlockTensor ket;
+  std::vector<int> match_dim;
+  bool has_can = false;
+  BlockTensor can;  // canonical_tensor_: one block per sector
+  int nl() const { return (int)lb.size(); }
+  int nr() const { return (int)rb.size(); }
+  void define_sectors(const std::vector<int>& l, const std::vector<int>& r,
+                      const std::vector<int>& ldim, const std::vector<int>& rdim);
+};
+
+// Per-instance device resources.
+struct Context {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  GlibcRand rng;
+  DevBuf<double> partial, scal;
+  DevBuf<CopyTask> copy_scratch;
+  DevBuf<ColGatherTask> gather_scratch;
+  DevBuf<SvdProblem> svd_scratch;
+  DevBuf<int> svd_sweeps;
+  DevBuf<int> ivec;
+  DevBuf<double> dvec;
+  DevBuf<double> work;
+  explicit Context(hipStream_t s = nullptr);
+  ~Context();
+  void sync() { DMRG_HIP(hipStreamSynchronize(stream)); }
+};
+
+// H_eff matvec plan of one site (MultiplyEffectHamilton,
+// real_tensor_contraction.cc:1223-1291): stage 1 T = K(k,l) R[o2](l,j) per
+// (o2, R block, ket block); stage 2 out(i,j) += sum w L[o1](i,k) T per output
+// block.  Built once per Lanczos solve, launched once per iteration.
+class HeffPlan {
+ public:
+  void build(const Lattice& lat, const SiteOperator& W, const Env& L, const Env& R,
+             hipStream_t stream);
+  // y (+)= H x ; x, y in ket-vector layout (psi_dim elements).
+  void apply(const double* x, double* y, int precision, hipStream_t stream);
+  double flops() const { return s1_.flops + s2_.flops; }  // executed
+  double fref() const { return fref_; }                      // reference order
+  int64_t psi_dim() const { return psi_dim_; }
+  int64_t tasks() const { return (int64_t)s1_.items.size(); }
+  int64_t bytes_min() const { return bytes_min_; }
+  const GemmBatch& stage1() const { return s1_; }
+  const GemmBatch& stage2() const { return s2_; }
+
+ private:
+  GemmBatch s1_, s2_;
+  DevBuf<double> T_;
+  int64_t psi_dim_ = 0;
+  int64_t bytes_min_ = 0;
+  double fref_ = 0;
+};
+
+// Environment updates (LeftComputeTensorContraction :782-940,
+// RightComputeTensorContraction :959-1119).
+Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx);
+Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx);
+Env boundary_env(int nbond, Context& cx);
+
+// Dense symmetric eigen-decomposition (cyclic Jacobi), ascending eigenvalues,
+// eigenvectors in columns of the row-major n x n `vec`.  Used for the Lanczos
+// tridiagonal and the dense edge-site H_eff (RealSymMatrixDiag,
+// src/util/general.cc:47-64), both FP64 on the host.
+void sym_eig(int n, std::vector<double> a, std::vector<double>& eval, std::vector<double>& vec);
+
+// ComputeTruncateDim (real_tensor_lattice.cc:501-555): kept states per
+// sector from per-sector descending singular values (modified in place when a
+// sector is forced to keep one state with sigma 0).
+std::vector<int> truncate_dims(int max_dim, double canpre, std::vector<std::vector<double>>& sv);
+
+struct LanczosStats {
+  int matvecs = 0, dense = 0, solves = 0;
+  double matvec_seconds = 0, flops = 0;
+};
+
+// Lanczos on one site's H_eff (LanczosMethod, real_tensor_lanczos.cc:14-169):
+// psi_dim < 100 takes the dense path (dense H_eff + host eigensolver), else
+// device Lanczos with modified Gram-Schmidt.  `ket` (device, n doubles) is
+// the start vector and receives the ground vector.  Returns the energy.
+double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, int refine,
+                     Context& cx, LanczosStats& st, int* iters = nullptr);
+
+struct RunParams {
+  int first_block = 20, first_dim = 10;
+  std::vector<int> maxm, maxb, niter;          // per half sweep
+  std::vector<double> canpre, noise;           // noise already 0.1^x
+  int precision = kFP64;  // H_eff arithmetic inside Lanczos
+  int refine = 1;         // mixed precision: final FP64 Rayleigh quotient
+  int verbose = 0;
+};
+
+struct RunResult {
+  std::vector<double> energies;   // per half sweep (right, left, right, ...)
+  std::vector<double> trunc_err;  // max discarded weight per half sweep
+  std::vector<double> half_seconds;
+  double total_seconds = 0;
+  LanczosStats stats;
+  uint64_t rng_draws = 0;
+  std::vector<int> final_bond_dims;
+};
+
+// RealTensorRundmrg: owns the MPS, the environments and drives the sweeps.
+class RealTensorRundmrg {
+ public:
+  RealTensorRundmrg(const RealTensorSpace& space, const RealTensorHamiltonian& ham,
+                    const RunParams& p, hipStream_t stream = nullptr);
+  RunResult Run();
+
+  // pieces exposed for tests
+  void DefineTensorSpace(int first_block, int first_dim);
+  void Initialize();
+  double LanczosMethod(int num_iter, int site);
+  Lattice& lattice(int i) { return lat_[i]; }
+  Env& left_env(int i) { return L_[i]; }
+  Env& right_env(int i) { return R_[i]; }
+  const SiteOperator& op(int i) const { return ham_.sites()[i]; }
+  Context& context() { return cx_; }
+
+ private:
+  void set_params(int maxb, int maxm, double canpre, double noise) {
+    max_block_ = maxb; max_dim_ = maxm; canpre_ = canpre; noise_ = noise;
+  }
+  bool expansion_active() const { return !(max_block_ < 0 || max_dim_ < 0 || noise_ > 1); }
+  int check(int site, int lq, int rq) const;
+  void lattice_block(int site, std::vector<int>& left, std::vector<int>& right) const;
+  void tensor_index(int site, const Lattice& lat, std::vector<int>& li, std::vector<int>& ri,
+                    std::vector<int>& pi) const;
+  void random_blocks(Lattice& lat, const std::vector<int>& li, const std::vector<int>& ri,
+                     const std::vector<int>& pi, int rows, int cols);
+  double canonical(int leigh, int site);  // returns discarded weight
+  void merge(int leigh, int site);
+  std::vector<std::vector<int>> expan_lattice(int leigh, int site, const QNTable& op_table,
+                                              Lattice& E);
+  void expan_contraction(int leigh, int site, Lattice& E, const std::vector<std::vector<int>>& mapping);
+  void expand(int leigh, int site, Lattice& E);
+  void match(int leigh, int site);
+
+  const RealTensorSpace& space_;
+  const RealTensorHamiltonian& ham_;
+  RunParams p_;
+  Context cx_;
+  int L_sites_ = 0, d_ = 0;
+  std::vector<QNTable> tables_;
+  std::vector<Lattice> lat_;
+  std::vector<Env> L_, R_;
+  int max_block_ = 0, max_dim_ = -1;
+  double canpre_ = 1e-30, noise_ = 10;
+  LanczosStats stats_;
+};
+
+}  // namespace dmrg

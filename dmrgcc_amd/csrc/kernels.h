@@ -1,0 +1,142 @@
+// Device primitives of the DMRG hot path: descriptor structs shared by host
+// and device code, and the host-side launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace dmrg {
+
+// Arithmetic form of a contraction.
+enum Precision : int {
+  kFP64 = 0,    // FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulate
+  kFP32 = 1,    // FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulate
+  kBF16x3 = 2,  // split bf16 hi/lo, three v_mfma_f32_16x16x32_bf16 products,
+                // FP32 accumulate
+};
+
+// One product term  scale * op(A) * op(B)  accumulated into an output item.
+// A is M x K (row-major with row stride lda) or, with kTransA, stored K x M;
+// B is K x N (row stride ldb) or, with kTransB, stored N x K.
+// kRelA/kRelB: the operand pointer is an element offset from the launch's
+// `in` base; kRelC on an item: C is an offset from the launch's `out` base.
+// One uploaded plan then serves any input/output vector pair.
+enum : int { kTransA = 1, kTransB = 2, kRelA = 4, kRelB = 8 };
+enum : int { kRelC = 1 };
+struct GemmContrib {
+  const double* A;
+  const double* B;
+  int64_t lda, ldb;
+  int K;
+  int flags;
+  double scale;
+};
+
+// One row-major output matrix C (M x N, row stride ldc) and its product terms
+// [c0, c1).  accumulate=1: C += sum; accumulate=0: C = sum.
+struct GemmItem {
+  double* C;
+  int64_t ldc;
+  int M, N;
+  int c0, c1;
+  int accumulate;
+  int flags;
+};
+
+// One workgroup: output tile (m0, n0) of item `item`.
+struct GemmTile {
+  int item, m0, n0, pad;
+};
+
+// A grouped GEMM launch: items with their contribution lists.  Built on the
+// host, uploaded once, launched many times (the H_eff plan reuses it for every
+// Lanczos iteration).
+struct GemmBatch {
+  std::vector<GemmItem> items;
+  std::vector<GemmContrib> contribs;
+  std::vector<GemmTile> tiles;
+  DevBuf<GemmItem> d_items;
+  DevBuf<GemmContrib> d_contribs;
+  DevBuf<GemmTile> d_tiles;
+  double flops = 0;  // executed real flops (2*M*N*K summed)
+
+  void clear() { items.clear(); contribs.clear(); tiles.clear(); flops = 0; }
+  // Start an output item; subsequent add() calls attach terms to it.
+  void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0);
+  void add(const double* A, int64_t lda, const double* B, int64_t ldb, int K,
+           int flags, double scale);
+  // Finish construction: tile every item and upload the descriptors.
+  void finalize(hipStream_t stream, int tile_m = 64, int tile_n = 64);
+  void launch(hipStream_t stream, int precision, const double* in = nullptr,
+              double* out = nullptr) const;
+  bool empty() const { return tiles.empty(); }
+};
+
+// Strided 2-D copy task: dst[i*dsr + j*dsc] = alpha * src[i*ssr + j*ssc]
+// for i < rows, j < cols.  src == nullptr writes zeros.
+struct CopyTask {
+  const double* src;
+  double* dst;
+  int rows, cols;
+  int64_t ssr, ssc, dsr, dsc;
+  double alpha;
+};
+void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
+                   hipStream_t stream);
+
+// ---- vector primitives (Lanczos) ----
+// Deterministic two-pass dot product: result written to *out (device).
+void dot(const double* x, const double* y, int64_t n, double* out,
+         DevBuf<double>& partial, hipStream_t stream);
+// y += sign * (*alpha) * x   (alpha on device)
+void axpy_dev(double* y, const double* x, int64_t n, const double* alpha,
+              double sign, hipStream_t stream);
+// y = a*y + b*x (host scalars)
+void axpby(double* y, const double* x, int64_t n, double a, double b,
+           hipStream_t stream);
+// mode 0: x /= sqrt(*nrm2)   (GramSchmidtMethod, real_tensor_lanczos.cc:198-200)
+// mode 1: x *= sqrt(1/(*nrm2)) (NormalizeMatrixBlock, real_matrix_block.cc:164-171)
+void scale_inv_sqrt(double* x, int64_t n, const double* nrm2, hipStream_t stream, int mode = 0);
+// psi_next = h / beta; h = -beta * psi   (real_tensor_lanczos.cc:140-151)
+void lanczos_step(double* psi_next, double* h, const double* psi, int64_t n, double beta,
+                  hipStream_t stream);
+// y = sum_j coef[j] * V[j*stride + :]   (coef on device; Ritz vector,
+// real_tensor_lanczos.cc:116-123, summed in increasing j)
+void combine(double* y, const double* V, int64_t stride, const double* coef, int k,
+             int64_t n, hipStream_t stream);
+void fill(double* x, int64_t n, double v, hipStream_t stream);
+
+// Column gather for the SVD scatter: for i < rows, jj < ncol:
+//   dst[i*dsr + jj*dsc] = scale[jj] * src[(r0 + i) + perm[jj]*ld]
+// (src column-major).  A NaN scale writes the unit vector e_jj instead
+// (completion of a zero singular vector).  perm/scale are device arrays.
+struct ColGatherTask {
+  const double* src;
+  int64_t ld;
+  int r0, rows, ncol, pad;
+  const int* perm;
+  const double* scale;
+  double* dst;
+  int64_t dsr, dsc;
+};
+void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGatherTask>& scratch,
+                       hipStream_t stream);
+
+// ---- batched one-sided Jacobi SVD ----
+// Each problem: column-major G (M x N, M >= N, leading dim M) is rotated in
+// place into G*V with mutually orthogonal columns; V (N x N column-major) is
+// initialised to identity and accumulates the rotations; sigma[j] = |G e_j|.
+struct SvdProblem {
+  double* G;
+  double* V;
+  double* sigma;
+  int M, N;
+};
+void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
+                DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 60);
+
+}  // namespace dmrg

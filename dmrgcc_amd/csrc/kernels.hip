@@ -1,0 +1,627 @@
+// HIP kernels of the DMRG hot path for gfx950 (MI355X, CDNA4).
+//
+// * grouped block GEMM: every sector product of the H_eff matvec, the
+//   environment updates, the merges and the expansion runs through one
+//   persistent-descriptor launch.  A workgroup owns one 64x64 output tile of
+//   one output block and accumulates all of that block's product terms
+//   (K-concatenation), so there are no atomics and results are deterministic.
+//   Four waves per workgroup, each a 32x32 sub-tile = 2x2 MFMA 16x16 blocks.
+//   FP64 uses v_mfma_f64_16x16x4f64, FP32 v_mfma_f32_16x16x4f32, BF16x3
+//   splits both operands into bf16 hi+lo and issues three
+//   v_mfma_f32_16x16x32_bf16 per block (hi*hi + hi*lo + lo*hi).
+// * strided block copies (stacking for the SVD, padding, scatter)
+// * Lanczos vector primitives with deterministic two-pass reductions
+// * batched one-sided (Hestenes) Jacobi SVD, one workgroup per sector.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.h"
+
+namespace dmrg {
+
+namespace {
+
+constexpr int kTile = 64;     // output tile edge
+constexpr int kThreads = 256; // 4 waves
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 MFMA tile kernel.  LDS holds A and B k-major (As[k][m],
+// Bs[k][n]) so that an MFMA operand fetch (lane l reads row k = l/16, column
+// l%16) is a contiguous 16-element run per quarter wave.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef d4v V;
+  __device__ static V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef f4v V;
+  __device__ static V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
+    const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
+    const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
+    double* __restrict__ out_base) {
+  constexpr int KT = 16;            // k-tile
+  constexpr int LD = kTile + 2;     // padded LDS row
+  __shared__ T As[KT][LD];
+  __shared__ T Bs[KT][LD];
+  typedef typename MfmaAcc<T>::V V;
+
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmItem it = items[tl.item];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int M = it.M, N = it.N;
+  const int m0 = tl.m0, n0 = tl.n0;
+
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
+
+  for (int c = it.c0; c < it.c1; ++c) {
+    const GemmContrib cb = cons[c];
+    const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+    const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+    const bool ta = cb.flags & kTransA;
+    const bool tb = cb.flags & kTransB;
+    const double sc = cb.scale;
+    const int K = cb.K;
+    for (int k0 = 0; k0 < K; k0 += KT) {
+      __syncthreads();
+      // A tile (kTile rows x KT) -> As[k][m]
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int m, k;
+        if (!ta) { m = tid >> 2; k = (tid & 3) * 4 + q; }
+        else     { k = tid >> 4; m = (tid & 15) * 4 + q; }
+        const int gm = m0 + m, gk = k0 + k;
+        double v = 0.0;
+        if (gm < M && gk < K) v = ta ? A[(int64_t)gk * cb.lda + gm] : A[(int64_t)gm * cb.lda + gk];
+        As[k][m] = (T)(v * sc);
+      }
+      // B tile (KT x kTile) -> Bs[k][n]
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int n, k;
+        if (!tb) { k = tid >> 4; n = (tid & 15) * 4 + q; }
+        else     { n = tid >> 2; k = (tid & 3) * 4 + q; }
+        const int gn = n0 + n, gk = k0 + k;
+        double v = 0.0;
+        if (gn < N && gk < K) v = tb ? B[(int64_t)gn * cb.ldb + gk] : B[(int64_t)gk * cb.ldb + gn];
+        Bs[k][n] = (T)v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < KT; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T af[2], bf[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) af[a] = As[kr][wm * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bf[b] = Bs[kr][wn * 32 + b * 16 + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
+      }
+    }
+  }
+  // epilogue: lane holds D[4*(lane/16)+v][lane%16] of each 16x16 block
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
+        const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
+        if (gm < M && gn < N) {
+          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
+          const double r = (double)acc[a][b][v];
+          *p = it.accumulate ? (*p + r) : r;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 kernel.  K-tile 32; LDS keeps hi/lo bf16 planes m-major with k
+// contiguous (Ah[m][k]) so lane l fetches 8 consecutive k of row l%16 in one
+// 16-byte read: exactly the v_mfma_f32_16x16x32_bf16 operand layout (lane l
+// supplies A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16]).
+// ---------------------------------------------------------------------------
+__device__ inline unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+__global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
+    const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
+    const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
+    double* __restrict__ out_base) {
+  constexpr int KT = 32;
+  constexpr int LD = KT + 8;  // 80-byte rows keep 16-byte alignment
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[kTile][LD];
+  __shared__ __attribute__((aligned(16))) unsigned short Al[kTile][LD];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[kTile][LD];
+  __shared__ __attribute__((aligned(16))) unsigned short Bl[kTile][LD];
+
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmItem it = items[tl.item];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int M = it.M, N = it.N;
+  const int m0 = tl.m0, n0 = tl.n0;
+
+  f4v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+
+  for (int c = it.c0; c < it.c1; ++c) {
+    const GemmContrib cb = cons[c];
+    const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+    const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+    const bool ta = cb.flags & kTransA;
+    const bool tb = cb.flags & kTransB;
+    const double sc = cb.scale;
+    const int K = cb.K;
+    for (int k0 = 0; k0 < K; k0 += KT) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        int m, k;
+        if (!ta) { m = tid >> 2; k = (tid & 3) * 8 + q; }
+        else     { k = tid >> 3; m = (tid & 7) * 8 + q; }
+        const int gm = m0 + m, gk = k0 + k;
+        double v = 0.0;
+        if (gm < M && gk < K) v = ta ? A[(int64_t)gk * cb.lda + gm] : A[(int64_t)gm * cb.lda + gk];
+        const float f = (float)(v * sc);
+        const unsigned short h = f2bf(f);
+        Ah[m][k] = h;
+        Al[m][k] = f2bf(f - bf2f(h));
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        int n, k;
+        if (!tb) { k = tid >> 3; n = (tid & 7) * 8 + q; }
+        else     { n = tid >> 2; k = (tid & 3) * 8 + q; }
+        const int gn = n0 + n, gk = k0 + k;
+        double v = 0.0;
+        if (gn < N && gk < K) v = tb ? B[(int64_t)gn * cb.ldb + gk] : B[(int64_t)gk * cb.ldb + gn];
+        const float f = (float)v;
+        const unsigned short h = f2bf(f);
+        Bh[n][k] = h;
+        Bl[n][k] = f2bf(f - bf2f(h));
+      }
+      __syncthreads();
+      const int kc = 8 * (lane >> 4);
+      bf8v ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int r = wm * 32 + a * 16 + (lane & 15);
+        ah[a] = *reinterpret_cast<const bf8v*>(&Ah[r][kc]);
+        al[a] = *reinterpret_cast<const bf8v*>(&Al[r][kc]);
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int r = wn * 32 + b * 16 + (lane & 15);
+        bh[b] = *reinterpret_cast<const bf8v*>(&Bh[r][kc]);
+        bl[b] = *reinterpret_cast<const bf8v*>(&Bl[r][kc]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
+        const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
+        if (gm < M && gn < N) {
+          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
+          const double r = (double)acc[a][b][v];
+          *p = it.accumulate ? (*p + r) : r;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// strided copies
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void copy_kernel(const CopyTask* __restrict__ tasks) {
+  const CopyTask t = tasks[blockIdx.x];
+  const int64_t total = (int64_t)t.rows * t.cols;
+  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i = (int)(e / t.cols), j = (int)(e % t.cols);
+    const double v = t.src ? t.alpha * t.src[i * t.ssr + j * t.ssc] : 0.0;
+    t.dst[i * t.dsr + j * t.dsc] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector primitives
+// ---------------------------------------------------------------------------
+constexpr int kDotBlocks = 512;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(const double* __restrict__ x,
+                                                          const double* __restrict__ y,
+                                                          int64_t n, double* __restrict__ part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * y[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void dot_final_kernel(const double* __restrict__ part, int np,
+                                                        double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                                const double* __restrict__ alpha, double sign) {
+  const double a = sign * (*alpha);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                             double a, double b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * y[i] + b * x[i];
+}
+
+__global__ void scale_inv_sqrt_kernel(double* __restrict__ x, int64_t n,
+                                      const double* __restrict__ nrm2, int mode) {
+  if (mode == 0) {
+    const double r = sqrt(*nrm2);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      x[i] /= r;
+  } else {
+    const double r = sqrt(1.0 / (*nrm2));
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      x[i] *= r;
+  }
+}
+
+__global__ void lanczos_step_kernel(double* __restrict__ nxt, double* __restrict__ h,
+                                    const double* __restrict__ psi, int64_t n, double beta) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    nxt[i] = h[i] / beta;
+    h[i] = -beta * psi[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void col_gather_kernel(const ColGatherTask* __restrict__ tasks) {
+  const ColGatherTask t = tasks[blockIdx.x];
+  const int64_t total = (int64_t)t.rows * t.ncol;
+  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int jj = (int)(e / t.rows), i = (int)(e % t.rows);
+    const double s = t.scale[jj];
+    double v;
+    if (s != s) v = (t.r0 + i == jj) ? 1.0 : 0.0;
+    else v = s * t.src[(int64_t)(t.r0 + i) + (int64_t)t.perm[jj] * t.ld];
+    t.dst[(int64_t)i * t.dsr + (int64_t)jj * t.dsc] = v;
+  }
+}
+
+__global__ void combine_kernel(double* __restrict__ y, const double* __restrict__ V, int64_t stride,
+                               const double* __restrict__ coef, int k, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += V[j * stride + i] * coef[j];
+    y[i] = s;
+  }
+}
+
+__global__ void fill_kernel(double* __restrict__ x, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+inline int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+// ---------------------------------------------------------------------------
+// One-sided Jacobi SVD.  One workgroup (8 waves) per problem; columns are
+// paired by the round-robin (circle) ordering so every round rotates N/2
+// disjoint pairs in parallel, one pair per wave at a time.
+// ---------------------------------------------------------------------------
+constexpr int kSvdThreads = 512;
+
+__global__ __launch_bounds__(kSvdThreads) void jacobi_svd_kernel(const SvdProblem* __restrict__ probs,
+                                                                 int* __restrict__ sweeps_out,
+                                                                 int max_sweeps) {
+  const SvdProblem p = probs[blockIdx.x];
+  const int M = p.M, N = p.N;
+  double* __restrict__ G = p.G;
+  double* __restrict__ V = p.V;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwaves = blockDim.x >> 6;
+  __shared__ int rotated;
+
+  for (int64_t e = tid; e < (int64_t)N * N; e += blockDim.x) V[e] = (e % (N + 1) == 0) ? 1.0 : 0.0;
+  __syncthreads();
+
+  const double tol = 2.220446049250313e-16 * sqrt((double)M);
+  const int n2 = N + (N & 1);
+  int sweep = 0;
+  if (N > 1) {
+    for (sweep = 0; sweep < max_sweeps; ++sweep) {
+      if (tid == 0) rotated = 0;
+      __syncthreads();
+      for (int r = 0; r < n2 - 1; ++r) {
+        for (int pr = wave; pr < n2 / 2; pr += nwaves) {
+          int a, b;
+          if (pr == 0) { a = r; b = n2 - 1; }
+          else { a = (r + pr) % (n2 - 1); b = (r + n2 - 1 - pr) % (n2 - 1); }
+          if (a >= N || b >= N) continue;
+          if (a > b) { int t = a; a = b; b = t; }
+          double* ga = G + (int64_t)a * M;
+          double* gb = G + (int64_t)b * M;
+          double al = 0.0, be = 0.0, ga_b = 0.0;
+          for (int i = lane; i < M; i += 64) {
+            const double x = ga[i], y = gb[i];
+            al += x * x; be += y * y; ga_b += x * y;
+          }
+          al = wave_sum(al); be = wave_sum(be); ga_b = wave_sum(ga_b);
+          if (al > 0.0 && be > 0.0 && fabs(ga_b) > tol * sqrt(al) * sqrt(be)) {
+            const double zeta = (be - al) / (2.0 * ga_b);
+            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+            const double cs = 1.0 / sqrt(1.0 + t * t);
+            const double sn = cs * t;
+            for (int i = lane; i < M; i += 64) {
+              const double x = ga[i], y = gb[i];
+              ga[i] = cs * x - sn * y;
+              gb[i] = sn * x + cs * y;
+            }
+            double* va = V + (int64_t)a * N;
+            double* vb = V + (int64_t)b * N;
+            for (int i = lane; i < N; i += 64) {
+              const double x = va[i], y = vb[i];
+              va[i] = cs * x - sn * y;
+              vb[i] = sn * x + cs * y;
+            }
+            if (lane == 0) rotated = 1;
+          }
+        }
+        __syncthreads();
+      }
+      const int any = rotated;
+      __syncthreads();
+      if (!any) break;
+    }
+  }
+  for (int j = wave; j < N; j += nwaves) {
+    const double* g = G + (int64_t)j * M;
+    double s = 0.0;
+    for (int i = lane; i < M; i += 64) s += g[i] * g[i];
+    s = wave_sum(s);
+    if (lane == 0) p.sigma[j] = sqrt(s);
+  }
+  if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host wrappers
+// ===========================================================================
+void GemmBatch::begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags) {
+  GemmItem it{};
+  it.flags = flags;
+  it.C = C;
+  it.ldc = ldc;
+  it.M = M;
+  it.N = N;
+  it.c0 = it.c1 = (int)contribs.size();
+  it.accumulate = accumulate;
+  items.push_back(it);
+}
+
+void GemmBatch::add(const double* A, int64_t lda, const double* B, int64_t ldb, int K, int flags,
+                    double scale) {
+  require(!items.empty(), "GemmBatch::add without begin_item", kErrInternal);
+  GemmContrib c{A, B, lda, ldb, K, flags, scale};
+  contribs.push_back(c);
+  items.back().c1 = (int)contribs.size();
+  flops += 2.0 * items.back().M * (double)items.back().N * K;
+}
+
+void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
+  tiles.clear();
+  // Largest work first so the long items start early.
+  std::vector<int> order(items.size());
+  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
+  std::vector<double> cost(items.size(), 0.0);
+  for (size_t i = 0; i < items.size(); ++i) {
+    double k = 0;
+    for (int c = items[i].c0; c < items[i].c1; ++c) k += contribs[c].K;
+    cost[i] = k;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  for (int i : order) {
+    const GemmItem& it = items[i];
+    if (it.M <= 0 || it.N <= 0) continue;
+    if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
+    for (int m0 = 0; m0 < it.M; m0 += tile_m)
+      for (int n0 = 0; n0 < it.N; n0 += tile_n) tiles.push_back(GemmTile{i, m0, n0, 0});
+  }
+  upload(d_items, items, stream);
+  upload(d_contribs, contribs, stream);
+  upload(d_tiles, tiles, stream);
+}
+
+void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out) const {
+  if (tiles.empty()) return;
+  const dim3 grid((unsigned)tiles.size()), block(kThreads);
+  if (precision == kFP64)
+    hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, d_tiles.data(),
+                       d_items.data(), d_contribs.data(), in, out);
+  else if (precision == kFP32)
+    hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, d_tiles.data(),
+                       d_items.data(), d_contribs.data(), in, out);
+  else if (precision == kBF16x3)
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
+                       d_contribs.data(), in, out);
+  else
+    throw Error(kErrArgument, "unknown precision");
+  DMRG_HIP(hipGetLastError());
+}
+
+void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
+                   hipStream_t stream) {
+  if (tasks.empty()) return;
+  // split large tasks into row chunks of <= 64K elements
+  std::vector<CopyTask> split;
+  split.reserve(tasks.size());
+  for (const CopyTask& t : tasks) {
+    if (t.rows <= 0 || t.cols <= 0) continue;
+    int rows_per = std::max(1, (int)(65536 / std::max(1, t.cols)));
+    for (int r0 = 0; r0 < t.rows; r0 += rows_per) {
+      CopyTask c = t;
+      c.rows = std::min(rows_per, t.rows - r0);
+      if (c.src) c.src = t.src + r0 * t.ssr;
+      c.dst = t.dst + r0 * t.dsr;
+      split.push_back(c);
+    }
+  }
+  if (split.empty()) return;
+  upload(scratch, split, stream);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)split.size()), dim3(256), 0, stream,
+                     scratch.data());
+  DMRG_HIP(hipGetLastError());
+  // the descriptor buffer may be reused by the next call: wait for the kernel
+  DMRG_HIP(hipStreamSynchronize(stream));
+}
+
+void dot(const double* x, const double* y, int64_t n, double* out, DevBuf<double>& partial,
+         hipStream_t stream) {
+  partial.ensure(kDotBlocks);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + 255) / 256));
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(g), dim3(256), 0, stream, x, y, n, partial.data());
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, stream, partial.data(), g, out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void axpy_dev(double* y, const double* x, int64_t n, const double* alpha, double sign,
+              hipStream_t stream) {
+  hipLaunchKernelGGL(axpy_dev_kernel, dim3(grid_for(n)), dim3(256), 0, stream, y, x, n, alpha, sign);
+  DMRG_HIP(hipGetLastError());
+}
+
+void axpby(double* y, const double* x, int64_t n, double a, double b, hipStream_t stream) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, stream, y, x, n, a, b);
+  DMRG_HIP(hipGetLastError());
+}
+
+void scale_inv_sqrt(double* x, int64_t n, const double* nrm2, hipStream_t stream, int mode) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scale_inv_sqrt_kernel, dim3(grid_for(n)), dim3(256), 0, stream, x, n, nrm2, mode);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lanczos_step(double* psi_next, double* h, const double* psi, int64_t n, double beta,
+                  hipStream_t stream) {
+  hipLaunchKernelGGL(lanczos_step_kernel, dim3(grid_for(n)), dim3(256), 0, stream, psi_next, h, psi,
+                     n, beta);
+  DMRG_HIP(hipGetLastError());
+}
+
+void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGatherTask>& scratch,
+                       hipStream_t stream) {
+  std::vector<ColGatherTask> live;
+  for (const ColGatherTask& t : tasks)
+    if (t.rows > 0 && t.ncol > 0) live.push_back(t);
+  if (live.empty()) return;
+  upload(scratch, live, stream);
+  hipLaunchKernelGGL(col_gather_kernel, dim3((unsigned)live.size()), dim3(256), 0, stream,
+                     scratch.data());
+  DMRG_HIP(hipGetLastError());
+  DMRG_HIP(hipStreamSynchronize(stream));
+}
+
+void combine(double* y, const double* V, int64_t stride, const double* coef, int k, int64_t n,
+             hipStream_t stream) {
+  hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, stream, y, V, stride, coef, k, n);
+  DMRG_HIP(hipGetLastError());
+}
+
+void fill(double* x, int64_t n, double v, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, stream, x, n, v);
+  DMRG_HIP(hipGetLastError());
+}
+
+void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
+                DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
+  if (probs.empty()) return;
+  for (const SvdProblem& p : probs)
+    require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
+  upload(scratch, probs, stream);
+  sweeps_out.ensure(probs.size());
+  hipLaunchKernelGGL(jacobi_svd_kernel, dim3((unsigned)probs.size()), dim3(kSvdThreads), 0, stream,
+                     scratch.data(), sweeps_out.data(), max_sweeps);
+  DMRG_HIP(hipGetLastError());
+}
+
+}  // namespace dmrg

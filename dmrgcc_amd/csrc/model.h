@@ -1,0 +1,112 @@
+// Model layer of the host API: quantum-number tables, the QN rule and the
+// incremental MPO builder, mirroring the reference's abstract
+// RealTensorSpace / RealTensorHamiltonian interface
+// (src/dmrg/realdmrg/real_tensor_space.h:137-150,
+//  src/dmrg/realdmrg/real_tensor_hamiltonian.h:6-78) and its spinless
+// Bose-Hubbard subclasses (model/space/real_spinless_bose_square_space.*,
+// model/hamiltonian/real_spinless_bose_square_hamiltonian.*).
+#pragma once
+
+#include <vector>
+
+namespace dmrg {
+
+using QN = std::vector<int>;
+using QNTable = std::vector<QN>;
+
+// Dense operator of one MPO site: W[o1][o2](p1, p2), p1 = bra, p2 = ket,
+// stored as ((o1*rb + o2)*d + p1)*d + p2.
+struct SiteOperator {
+  int lb = 0, rb = 0, d = 0;
+  std::vector<double> w;
+  double at(int o1, int o2, int p1, int p2) const {
+    return w[(((size_t)o1 * rb + o2) * d + p1) * d + p2];
+  }
+  double& at(int o1, int o2, int p1, int p2) {
+    return w[(((size_t)o1 * rb + o2) * d + p1) * d + p2];
+  }
+  // RealTensorOperator::{Left,Right,Mixed}CheckZero
+  // (real_tensor_operator.cc:363-409): any |entry| > 1e-15 along the
+  // remaining indices.
+  bool left_nonzero(int o1, int p1) const;
+  bool right_nonzero(int o2, int p2) const;
+  bool mixed_nonzero(int o1, int p2) const;
+};
+
+// RealTensorSpace: the abstract model-space interface.
+class RealTensorSpace {
+ public:
+  virtual ~RealTensorSpace() = default;
+  virtual int num_site() const = 0;
+  virtual int physics_dim() const = 0;
+  virtual int num_quantum() const = 0;
+  // DefineQuantumTable: one QN table per MPS bond (num_site + 1 of them).
+  virtual std::vector<QNTable> DefineQuantumTable() const = 0;
+  // MergeQuantumTable: QN of an MPO channel applied to an MPS sector.
+  virtual QN MergeQuantumTable(const QN& op, const QN& space) const = 0;
+  // CheckQuantumTable: physical index connecting two sectors, or -1.
+  virtual int CheckQuantumTable(int site, const QN& left, const QN& right) const = 0;
+};
+
+// RealTensorHamiltonian: owns the MPO and its bond QN tables and provides
+// the incremental construction used by model subclasses.
+class RealTensorHamiltonian {
+ public:
+  virtual ~RealTensorHamiltonian() = default;
+  virtual void DefineTensorHamiltonian() = 0;
+
+  int num_site() const { return num_site_; }
+  const std::vector<SiteOperator>& sites() const { return W_; }
+  const std::vector<QNTable>& bond_tables() const { return qt_; }
+
+ protected:
+  void Init(int num_site, int physics_dim, int num_quantum, std::vector<std::vector<double>> basic);
+  // ExpanTensorHamiltonian (real_tensor_hamiltonian.cc:151-187)
+  void ExpanTensorHamiltonian(int site, int op_index, double coefficient, const QN& table);
+  // ParallelTensorHamiltonian (real_tensor_hamiltonian.cc:189-214)
+  void ParallelTensorHamiltonian();
+
+  int num_site_ = 0, d_ = 0, nq_ = 0;
+  std::vector<std::vector<double>> basic_;  // d*d row-major operators
+  std::vector<SiteOperator> W_;
+  std::vector<QNTable> qt_;
+};
+
+// RealMatrix::ParallelMatrix on a row-major rows x cols matrix
+// (real_matrix.cc:438-531).  leigh 0 = columns, 1 = rows.  Returns kept
+// positions; transfer is n x n row-major (n = cols or rows).
+std::vector<int> parallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
+                                 std::vector<double>& transfer);
+
+class RealSpinlessBoseSquareSpace : public RealTensorSpace {
+ public:
+  RealSpinlessBoseSquareSpace(int nx, int ny, int num_boson, int physics_dim)
+      : nx_(nx), ny_(ny), n_(num_boson), d_(physics_dim) {}
+  int num_site() const override { return nx_ * ny_; }
+  int physics_dim() const override { return d_; }
+  int num_quantum() const override { return 1; }
+  std::vector<QNTable> DefineQuantumTable() const override;
+  QN MergeQuantumTable(const QN& op, const QN& space) const override { return QN{op[0] + space[0]}; }
+  int CheckQuantumTable(int site, const QN& left, const QN& right) const override;
+
+ private:
+  int nx_, ny_, n_, d_;
+};
+
+class RealSpinlessBoseSquareHamiltonian : public RealTensorHamiltonian {
+ public:
+  RealSpinlessBoseSquareHamiltonian(int nx, int ny, int physics_dim, double flux, double hop_x,
+                                    double hop_y, double U, double mu, bool period_x,
+                                    bool period_y);
+  void DefineTensorHamiltonian() override;
+
+ private:
+  void Hop1(int p1, int p2, double c);
+  void Hop2(int p1, int p2, double c);
+  void OnSite(int p, double c);
+  int nx_, ny_;
+  double hop_x_, hop_y_, U_, mu_;
+  bool px_, py_;
+};
+
+}  // namespace dmrg

@@ -1,0 +1,160 @@
+/*
+ * dmrgcc_hip.h -- C ABI of the MI355X DMRG hot path.
+ *
+ * The reference (jiandong94/dmrgcc) has no FFI: its hot path sits behind a
+ * C++ class API.  Each entry point below replaces one member function of that
+ * API (cited as file:line); plain pointers and sizes only.  Device pointers
+ * are HIP device addresses (HBM); host pointers are ordinary memory.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a nonzero status:
+ *      1 HIP runtime error, 2 shape / index mismatch (the reference prints and
+ *      calls exit(-1) there), 3 invalid argument, 4 no convergence,
+ *      5 internal error.  dmrgcc_last_error() returns the message.
+ *  - Matrices are row-major FP64.  A block-sparse matrix is a list of blocks
+ *    (sector indices l, r; rows; cols) stored back to back in block order in
+ *    one buffer: exactly the reference's VectorizeMatrixBlock layout
+ *    (src/tensor/real_matrix_block.cc:173-194).
+ *  - MPO site tensors are dense host arrays W[o1][o2][p1][p2] (bra p1, ket
+ *    p2), lb*rb*d*d doubles.
+ *  - precision: 0 = FP64 (MFMA f64), 1 = FP32 (MFMA f32),
+ *               2 = BF16x3 (split bf16 MFMA, FP32 accumulate).
+ *  - stream: a hipStream_t, or NULL for the library's own stream.
+ *  - Not thread-safe per handle.
+ */
+#ifndef DMRGCC_HIP_H_
+#define DMRGCC_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ---------------------------------------------------------- */
+const char* dmrgcc_last_error(void);
+int dmrgcc_version(void);
+
+/* ---- host-side restatements (no GPU needed) ---------------------------- */
+
+/* glibc rand() stream from `seed` (the reference never seeds: seed 1),
+ * src/tensor/real_matrix.cc:201-208.  Writes n successive values. */
+int dmrgcc_rand_draws(unsigned seed, int64_t skip, int n, int* out);
+
+/* QuickSort (src/util/general.ih:6-65), flag 0 descending / 1 ascending,
+ * sorting `vals` in place together with `index`. */
+int dmrgcc_quicksort(int n, double* vals, int* index, int flag);
+
+/* ComputeTruncateDim (src/dmrg/realdmrg/real_tensor_lattice.cc:501-555).
+ * Sector s has counts[s] descending singular values in `svals` (concatenated;
+ * modified in place when a sector is forced to keep one state with sigma 0).
+ * Writes the kept dimension per sector to `trunc`. */
+int dmrgcc_truncate_dims(int nsec, const int* counts, double* svals, int max_dim,
+                         double canpre, int* trunc);
+
+/* Real spinless Bose-Hubbard on an nx x ny square lattice (site j + i*ny):
+ * RealSpinlessBoseSquareSpace / RealSpinlessBoseSquareHamiltonian
+ * (model/space/real_spinless_bose_square_space.cc:26-83,
+ *  model/hamiltonian/real_spinless_bose_square_hamiltonian.cc:41-228).
+ * A model handle owns the MPS QN tables and the deparallelised MPO. */
+typedef struct dmrgcc_model dmrgcc_model;
+int dmrgcc_model_spinless_create(int nx, int ny, int num_boson, int physics_dim, double hop_x,
+                                 double hop_y, double U, double mu, int period_x, int period_y,
+                                 dmrgcc_model** out);
+int dmrgcc_model_destroy(dmrgcc_model* m);
+int dmrgcc_model_num_site(const dmrgcc_model* m);
+/* MPS bond QN table `bond` (0..L): count, then the first QN of each entry. */
+int dmrgcc_model_space_table(const dmrgcc_model* m, int bond, int* count, int* qn);
+/* MPO site tensor: bond dims, then W (lb*rb*d*d doubles) if W != NULL. */
+int dmrgcc_model_mpo_site(const dmrgcc_model* m, int site, int* lb, int* rb, double* W);
+/* MPO bond QN table (0..L): count, then the channel QNs. */
+int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn);
+
+/* ---- block GEMM (RealMatrix::MultiplyToMatrix, real_matrix.cc:291-315) -- */
+/* Batched C_i = A_i * B_i, row-major, device pointers, one launch. */
+int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, double* const* C,
+                      const int* M, const int* N, const int* K, int precision, void* stream);
+
+/* ---- effective Hamiltonian (MultiplyEffectHamilton,
+ *      src/dmrg/realdmrg/real_tensor_contraction.cc:1223-1291) ---------- */
+typedef struct dmrgcc_heff dmrgcc_heff;
+/* Site description:
+ *   d, lb, rb, W (host)          MPO site tensor
+ *   nl, nr, phys (host nl*nr)    physical index per sector pair, -1 if none
+ *   nket, ket_l/r/rows/cols      ket block list (defines the vector layout)
+ *   nL, L_o/l/r/rows/cols/off    left environment blocks: channel o1, bra
+ *                                sector l, ket sector r, offset into L_data
+ *   nR, R_o/l/r/rows/cols/off    right environment blocks: channel o2, ket
+ *                                sector l, bra sector r, offset into R_data
+ * L_data / R_data are device buffers that must outlive the plan. */
+int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
+                       int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                       const int* ket_cols, int nL, const int* L_o, const int* L_l,
+                       const int* L_r, const int* L_rows, const int* L_cols,
+                       const int64_t* L_off, const double* L_data, int nR, const int* R_o,
+                       const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
+                       const int64_t* R_off, const double* R_data, void* stream,
+                       dmrgcc_heff** out);
+/* y += H_eff x  (the reference's accumulate-into-state semantics).
+ * x, y: device vectors of psi_dim doubles. */
+int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision);
+/* info[0] = psi_dim, [1] = F_ref (reference-order flops), [2] = F_exec,
+ * [3] = minimum HBM bytes per matvec, [4] = stage-1 tasks, [5] = tiles. */
+int dmrgcc_heff_info(const dmrgcc_heff* h, double* info);
+int dmrgcc_heff_destroy(dmrgcc_heff* h);
+
+/* Lanczos ground state (RealTensorLanczos::LanczosMethod,
+ * src/dmrg/realdmrg/real_tensor_lanczos.cc:14-169) on the plan's site:
+ * `ket` (device, psi_dim) is the start vector and receives the Ritz vector.
+ * precision != 0 runs the iterations mixed and, if refine, returns the FP64
+ * Rayleigh quotient of the Ritz vector.  iters_out may be NULL. */
+int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int refine,
+                   double* energy, int* iters_out);
+
+/* ---- block-sparse result handle (environment updates) ------------------ */
+typedef struct dmrgcc_blocks dmrgcc_blocks;
+int dmrgcc_blocks_count(const dmrgcc_blocks* b);
+/* Block i: channel, l, r, rows, cols, element offset into the data buffer. */
+int dmrgcc_blocks_get(const dmrgcc_blocks* b, int i, int* o, int* l, int* r, int* rows, int* cols,
+                      int64_t* off);
+const double* dmrgcc_blocks_data(const dmrgcc_blocks* b);
+/* Copy all block data (sum rows*cols doubles) to host memory. */
+int dmrgcc_blocks_download(const dmrgcc_blocks* b, double* host);
+int dmrgcc_blocks_destroy(dmrgcc_blocks* b);
+
+/* Environment update (LeftComputeTensorContraction :782-940 for leigh 0,
+ * RightComputeTensorContraction :959-1119 for leigh 1): the site
+ * description as for dmrgcc_heff_create plus the ket data (device) and one
+ * environment (L for leigh 0, R for leigh 1).  Output: the next environment,
+ * one block-sparse matrix per channel (o2 for leigh 0, o1 for leigh 1). */
+int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
+                      const int* phys, int nket, const int* ket_l, const int* ket_r,
+                      const int* ket_rows, const int* ket_cols, const double* ket_data, int nE,
+                      const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
+                      const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
+                      dmrgcc_blocks** out);
+
+/* ---- sector SVD (RealMatrix::SVDMatrix, real_matrix.cc:588-620) -------- */
+/* Batched thin SVD of n row-major device matrices A_i (M_i x N_i), one-sided
+ * Jacobi on the device.  Outputs (device): U_i (M_i x k_i), S_i (k_i,
+ * descending), Vt_i (k_i x N_i), k_i = min(M_i, N_i). */
+int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
+                      double* const* U, double* const* S, double* const* Vt, void* stream);
+
+/* ---- full sweep (RealTensorRundmrg::Run, real_tensor_rundmrg.cc:115-245) */
+/* Runs the spinless model from DefineTensorSpace through nhalf half sweeps.
+ * Per half sweep h: maxm[h], maxb[h], niter[h], canpre[h], noise[h] (the
+ * table exponent; 0.1^noise is applied as in the reference).  Outputs per
+ * half sweep (host, nhalf entries): energies, discarded weight, seconds.
+ * stats (host, 8): total seconds, matvecs, dense solves, matvec seconds,
+ * matvec flops (F_ref), rand() draws, Lanczos solves, max bond dim. */
+int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+               const int* maxb, const int* niter, const double* canpre, const double* noise,
+               int precision, int refine, int verbose, double* energies, double* trunc,
+               double* seconds, double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRGCC_HIP_H_ */

@@ -1,0 +1,120 @@
+"""Block-sparse matrix and MPS site tensor containers, oracle copy.
+
+Restates src/tensor/real_matrix_block.cc (FindMatrixBlock :314-380,
+ComputePartMatrixBlockDim :154-162, VectorizeMatrixBlock :173-194,
+NormalizeMatrixBlock :164-171) and the RealTensorLattice fields
+(src/dmrg/realdmrg/real_tensor_lattice.h:10-42).  A block is a 2-D float64
+numpy array; an empty (reset) block has shape (0, 0).
+"""
+import numpy as np
+
+
+def empty():
+    return np.zeros((0, 0))
+
+
+class BlockMat:
+    def __init__(self, left=(), right=(), mats=None):
+        self.left = list(left)
+        self.right = list(right)
+        if mats is None:
+            mats = [empty() for _ in self.left]
+        self.mats = list(mats)
+
+    @property
+    def n(self):
+        return len(self.left)
+
+    def copy(self):
+        return BlockMat(self.left, self.right, [m.copy() for m in self.mats])
+
+    def find(self, l, r):
+        """FindMatrixBlock(left, right): first position or -1."""
+        for i in range(self.n):
+            if self.left[i] == l and self.right[i] == r:
+                return i
+        return -1
+
+    def same(self, leigh, target):
+        """FindMatrixBlock(num, positions, leigh, target)."""
+        key = self.left if leigh == 0 else self.right
+        return [i for i in range(self.n) if key[i] == target]
+
+    def dim(self):
+        return sum(m.size for m in self.mats)
+
+    def offsets(self):
+        out, acc = [], 0
+        for m in self.mats:
+            out.append(acc)
+            acc += m.size
+        return out
+
+    def to_vector(self):
+        if not self.mats:
+            return np.zeros(0)
+        return np.concatenate([m.ravel() for m in self.mats])
+
+    def from_vector(self, v):
+        acc = 0
+        for i, m in enumerate(self.mats):
+            self.mats[i] = np.asarray(v[acc:acc + m.size], dtype=m.dtype).reshape(m.shape).copy()
+            acc += m.size
+
+    def normalize(self):
+        s = sum(float(np.sum(m * m)) for m in self.mats)
+        f = np.sqrt(1.0 / s)
+        self.mats = [m * f for m in self.mats]
+
+    def reset(self):
+        self.mats = [empty() for _ in self.mats]
+
+
+class Lattice:
+    """RealTensorLattice: one MPS site."""
+
+    def __init__(self, d):
+        self.d = d
+        self.left_block = []
+        self.left_dim = []
+        self.right_block = []
+        self.right_dim = []
+        self.phys = []  # [nl][nr], -1 where no block
+        self.ket = BlockMat()
+        self.match_dim = None
+        self.canonical = None
+
+    def copy(self):
+        o = Lattice(self.d)
+        o.left_block = list(self.left_block)
+        o.left_dim = list(self.left_dim)
+        o.right_block = list(self.right_block)
+        o.right_dim = list(self.right_dim)
+        o.phys = [list(r) for r in self.phys]
+        o.ket = self.ket.copy()
+        return o
+
+    @property
+    def nl(self):
+        return len(self.left_block)
+
+    @property
+    def nr(self):
+        return len(self.right_block)
+
+    def define_sectors(self, left_block, right_block, left_dim, right_dim):
+        """DefineTensorLattice (real_tensor_lattice.cc:300-340)."""
+        self.left_block = list(left_block)
+        self.left_dim = list(left_dim)
+        self.right_block = list(right_block)
+        self.right_dim = list(right_dim)
+        self.phys = [[-1] * len(right_block) for _ in left_block]
+
+    def define_blocks(self, li, ri, pi, rows, cols, rng):
+        """DefineTensorLattice (real_tensor_lattice.cc:343-359): every block
+        gets a fresh RandomMatrix(rows, cols) (consumes rand())."""
+        mats = []
+        for k in range(len(li)):
+            self.phys[li[k]][ri[k]] = pi[k]
+            mats.append(rng.random_matrix(rows, cols))
+        self.ket = BlockMat(li, ri, mats)

@@ -1,0 +1,134 @@
+"""H_eff matvec, environment updates and Lanczos on the device against the
+oracle, on states taken from real oracle sweeps."""
+import numpy as np
+import pytest
+
+import dmrgcc_amd as dm
+from oracle.dmrg import heff_apply, heff_dense, heff_flops, lanczos, left_env, right_env
+from tests._support import env_blocks, oracle_state, site_description
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+STATES = {
+    # (nx, ny, N, d, maxm, half, site)
+    "3x2_d3": (3, 2, 6, 3, 30, 2, 2),
+    "4x3_d4_left": (4, 3, 12, 4, 120, 2, 5),
+    "4x3_d4_right": (4, 3, 12, 4, 120, 3, 7),
+}
+_cache = {}
+
+
+def state(name):
+    if name not in _cache:
+        nx, ny, N, d, maxm, half, site = STATES[name]
+        _cache[name] = (oracle_state(nx, ny, N, d, maxm=maxm, half=half, site=site, niter=6), site)
+    return _cache[name]
+
+
+def make_plan(dmo, site):
+    lat = dmo.space.lat[site]
+    phys, kl, kr, krows, kcols = site_description(lat)
+    (Lo, Ll, Lr, Lrows, Lcols, Loff), Ld = env_blocks(dmo.Lenv[site])
+    (Ro, Rl, Rr, Rrows, Rcols, Roff), Rd = env_blocks(dmo.Renv[site])
+    Ldev = torch.tensor(Ld if Ld.size else np.zeros(1), device="cuda")
+    Rdev = torch.tensor(Rd if Rd.size else np.zeros(1), device="cuda")
+    plan = dm.HeffPlan(dmo.W[site], phys, kl, kr, krows, kcols,
+                       dm.BlockList(Lo, Ll, Lr, Lrows, Lcols, Loff), Ldev,
+                       dm.BlockList(Ro, Rl, Rr, Rrows, Rcols, Roff), Rdev)
+    plan._dev = (Ldev, Rdev)
+    return plan, lat
+
+
+@pytest.mark.parametrize("name", list(STATES))
+def test_heff_fp64_matches_oracle(name):
+    dmo, site = state(name)
+    plan, lat = make_plan(dmo, site)
+    n = lat.ket.dim()
+    assert plan.info()["psi_dim"] == n
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, y0.copy())
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.tensor(y0, device="cuda")
+    plan.apply(xd, yd, dm.FP64)          # accumulate: y0 + H x
+    got = yd.cpu().numpy()
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    info = plan.info()
+    assert info["flops_ref"] == heff_flops(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site])
+
+
+@pytest.mark.parametrize("prec,tol", [(dm.FP32, 1e-5), (dm.BF16X3, 2e-4)])
+def test_heff_mixed_within_tolerance(prec, tol):
+    dmo, site = state("4x3_d4_left")
+    plan, lat = make_plan(dmo, site)
+    n = lat.ket.dim()
+    x = np.random.default_rng(4).standard_normal(n)
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, np.zeros(n))
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    plan.apply(torch.tensor(x, device="cuda"), yd, prec)
+    err = np.abs(yd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < tol, err
+
+
+def test_heff_is_symmetric_and_equals_dense():
+    dmo, site = state("3x2_d3")
+    plan, lat = make_plan(dmo, site)
+    H = heff_dense(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site])
+    n = H.shape[0]
+    X = torch.eye(n, dtype=torch.float64, device="cuda")
+    cols = []
+    for c in range(n):
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        plan.apply(X[c].contiguous(), y, dm.FP64)
+        cols.append(y.cpu().numpy())
+    Hg = np.stack(cols, axis=1)
+    assert np.abs(Hg - H).max() < 1e-12 * np.abs(H).max()
+    assert np.abs(Hg - Hg.T).max() < 1e-12 * np.abs(H).max()
+
+
+@pytest.mark.parametrize("name", list(STATES))
+@pytest.mark.parametrize("leigh", [0, 1])
+def test_env_update_matches_oracle(name, leigh):
+    dmo, site = state(name)
+    lat = dmo.space.lat[site]
+    phys, kl, kr, krows, kcols = site_description(lat)
+    env = dmo.Lenv[site] if leigh == 0 else dmo.Renv[site]
+    (o, l, r, rows, cols, off), data = env_blocks(env)
+    ket = torch.tensor(lat.ket.to_vector(), device="cuda")
+    E = torch.tensor(data if data.size else np.zeros(1), device="cuda")
+    res = dm.env_update(leigh, dmo.W[site], phys, kl, kr, krows, kcols, ket,
+                        dm.BlockList(o, l, r, rows, cols, off), E)
+    ref = (left_env if leigh == 0 else right_env)(lat, dmo.W[site], env)
+    host = res.to_host()
+    got = {}
+    for (ch, bl, br, nr_, nc, of) in res.blocks:
+        got[(ch, bl, br)] = host[of:of + nr_ * nc].reshape(nr_, nc)
+    nref = sum(b.n for b in ref)
+    assert len(res.blocks) == nref
+    for ch, bm in enumerate(ref):
+        for k in range(bm.n):
+            m = bm.mats[k]
+            g = got[(ch, bm.left[k], bm.right[k])]
+            assert g.shape == m.shape
+            if m.size:
+                assert np.abs(g - m).max() <= 1e-12 * max(1.0, np.abs(m).max())
+
+
+@pytest.mark.parametrize("name", ["4x3_d4_left", "4x3_d4_right"])
+def test_lanczos_matches_oracle(name):
+    dmo, site = state(name)
+    plan, lat = make_plan(dmo, site)
+    ket = torch.tensor(lat.ket.to_vector(), device="cuda")
+    lat2 = lat.copy()
+    e_ref = lanczos(lat2, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], 10)
+    e, iters = plan.lanczos(ket, 10, dm.FP64)
+    assert abs(e - e_ref) < 1e-10 * abs(e_ref)
+    v = ket.cpu().numpy()
+    vr = lat2.ket.to_vector()
+    assert abs(abs(np.dot(v, vr)) - 1.0) < 1e-8
+    # mixed precision with FP64 Rayleigh-quotient refinement
+    ket2 = torch.tensor(lat.ket.to_vector(), device="cuda")
+    e2, _ = plan.lanczos(ket2, 10, dm.BF16X3, refine=True)
+    assert abs(e2 - e_ref) < 1e-8 * abs(e_ref)

@@ -1,0 +1,68 @@
+"""Grouped block GEMM (all three arithmetic forms) and the batched sector SVD
+against numpy FP64, on ragged shapes that exercise every tile edge."""
+import numpy as np
+import pytest
+
+import dmrgcc_amd as dm
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1, 1), (3, 5, 2), (16, 16, 4), (17, 33, 65), (64, 64, 64), (65, 1, 130),
+          (100, 200, 37), (200, 150, 300), (7, 129, 9), (257, 3, 5)]
+
+# relative (to |A||B| scale) tolerances per arithmetic form
+TOL = {dm.FP64: 1e-13, dm.FP32: 2e-6, dm.BF16X3: 5e-5}
+
+
+@pytest.mark.parametrize("prec", [dm.FP64, dm.FP32, dm.BF16X3])
+def test_block_gemm_matches_numpy(prec):
+    rng = np.random.default_rng(5)
+    As, Bs, ref = [], [], []
+    for (M, N, K) in SHAPES:
+        a = rng.standard_normal((M, K))
+        b = rng.standard_normal((K, N))
+        As.append(torch.tensor(a, device="cuda"))
+        Bs.append(torch.tensor(b, device="cuda"))
+        ref.append(a @ b)
+    Cs = [torch.full((M, N), np.nan, dtype=torch.float64, device="cuda") for (M, N, K) in SHAPES]
+    dm.block_gemm(As, Bs, Cs, prec)
+    torch.cuda.synchronize()
+    for (M, N, K), c, r in zip(SHAPES, Cs, ref):
+        c = c.cpu().numpy()
+        scale = np.sqrt(K)  # entries of A@B are O(sqrt(K))
+        err = np.abs(c - r).max() / scale
+        assert np.isfinite(c).all()
+        assert err < TOL[prec], (M, N, K, err)
+
+
+def test_block_gemm_fp64_is_exact_on_integers():
+    # small integers: every product and partial sum is exact in FP64, so the
+    # MFMA result must equal numpy bit for bit (catches any lane/layout slip)
+    rng = np.random.default_rng(1)
+    for (M, N, K) in SHAPES:
+        a = rng.integers(-3, 4, (M, K)).astype(float)
+        b = rng.integers(-3, 4, (K, N)).astype(float)
+        c = torch.zeros((M, N), dtype=torch.float64, device="cuda")
+        dm.block_gemm([torch.tensor(a, device="cuda")], [torch.tensor(b, device="cuda")], [c], dm.FP64)
+        assert np.array_equal(c.cpu().numpy(), a @ b), (M, N, K)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (7, 1), (5, 5), (40, 12), (12, 40), (120, 90),
+                                   (300, 64), (33, 257)])
+def test_sector_svd(shape):
+    rng = np.random.default_rng(9)
+    M, N = shape
+    a = rng.standard_normal((M, N))
+    # graded spectrum down to 1e-12 to check small singular values
+    u, s, vt = np.linalg.svd(a, full_matrices=False)
+    s = np.logspace(0, -12, len(s))
+    a = (u * s) @ vt
+    U, S, Vt = dm.sector_svd([torch.tensor(a, device="cuda")])
+    U, S, Vt = U[0].cpu().numpy(), S[0].cpu().numpy(), Vt[0].cpu().numpy()
+    k = min(M, N)
+    assert np.all(np.diff(S) <= 0)
+    assert np.abs(S - s).max() < 1e-14 * max(M, N)
+    assert np.abs((U * S) @ Vt - a).max() < 1e-14 * max(M, N)
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
+    assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12

@@ -1,0 +1,48 @@
+"""Full sweeps on the device: per-half-sweep energies against the oracle's
+sweeps on the same inputs (same seeded rand() stream) and against exact
+diagonalisation."""
+import numpy as np
+import pytest
+
+import dmrgcc_amd as dm
+from oracle import ed
+from oracle.dmrg import DMRG
+from oracle.model import SpinlessBoseSquare
+from tests._support import sweep_rows
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # nx, ny, N, d, U, period_y, maxm, nhalf, first_dim
+    (3, 2, 6, 3, 2.0, False, 60, 6, 4),
+    (2, 3, 6, 4, 3.0, True, 80, 6, 6),
+    (4, 3, 12, 3, 2.0, False, 40, 6, 6),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_run_matches_oracle_and_ed(case):
+    nx, ny, N, d, U, py, maxm, nhalf, fd = case
+    rows = sweep_rows(maxm, nhalf, niter=10, canpre=1e-12)
+    om = SpinlessBoseSquare(nx, ny, N, d, jx=-1.0, jy=-1.0, U=U, period_y=py)
+    o = DMRG(om, rows, nhalf // 2, first_block=20, first_dim=fd)
+    e_ref = np.array(o.run())
+    pm = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U, period_y=py)
+    r = pm.run(rows, first_block=20, first_dim=fd, precision=dm.FP64)
+    assert r["rand_draws"] == o.rng.count
+    assert np.abs(r["energies"] - e_ref).max() < 1e-9 * np.abs(e_ref).max()
+    if om.L <= 12:
+        e0 = ed.ground_energy(om.L, d, N, om.bonds(), U)
+        if maxm >= 60:
+            assert abs(r["energies"][-1] - e0) < 1e-9 * abs(e0)
+
+
+@pytest.mark.parametrize("prec", [dm.FP32, dm.BF16X3])
+def test_run_mixed_precision_energy(prec):
+    nx, ny, N, d, U, py, maxm, nhalf, fd = CASES[0]
+    rows = sweep_rows(maxm, nhalf, niter=10, canpre=1e-12)
+    pm = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U, period_y=py)
+    r64 = pm.run(rows, first_dim=fd, precision=dm.FP64)
+    rmx = pm.run(rows, first_dim=fd, precision=prec, refine=True)
+    e0 = r64["energies"][-1]
+    assert abs(rmx["energies"][-1] - e0) < 1e-9 * abs(e0)
