@@ -73,6 +73,20 @@ _SIGS = {
     "dmrgcc_sector_svd": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, _c_int_p,
                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(_vp), _vp]),
+    "dmrgcc_dmrg_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
+                                          _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_dmrg_advance": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p]),
+    "dmrgcc_dmrg_set_precision": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "dmrgcc_dmrg_heff": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_dmrg_ket": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), _c_i64_p]),
+    "dmrgcc_dmrg_site_blocks": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_i64_p,
+                                               _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                               _c_i64_p, _c_dbl_p]),
+    "dmrgcc_dmrg_phys": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
+    "dmrgcc_dmrg_result": (ctypes.c_int, [_vp, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+    "dmrgcc_dmrg_destroy": (ctypes.c_int, [_vp]),
+    "dmrgcc_heff_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                   _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
@@ -201,28 +215,110 @@ class SpinlessModel:
         _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), _ip(q)))
         return q
 
+    def start(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
+        """A resumable run (DmrgRun) of this model over the sweep table."""
+        return DmrgRun(self, table, first_block, first_dim, precision, refine, verbose)
+
     def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
         """RealTensorRundmrg::Run.  table: rows (maxm, maxb, niter, canpre,
         noise_exponent) per half sweep.  Returns a dict of per-half-sweep
         energies, discarded weights, seconds and run statistics."""
-        rows = [tuple(r) for r in table]
-        n = len(rows)
-        maxm = _i32([r[0] for r in rows])
-        maxb = _i32([r[1] for r in rows])
-        niter = _i32([r[2] for r in rows])
-        canpre = _f64([r[3] for r in rows])
-        noise = _f64([r[4] for r in rows])
-        e = np.zeros(n)
-        t = np.zeros(n)
-        s = np.zeros(n)
-        st = np.zeros(8)
+        n, maxm, maxb, niter, canpre, noise = _table_args(table)
+        e, t, s, st = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(24)
         _check(lib().dmrgcc_run(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
                                 _dp(canpre), _dp(noise), precision, int(refine), int(verbose), _dp(e),
                                 _dp(t), _dp(s), _dp(st)))
-        return {"energies": e, "trunc_err": t, "seconds": s, "total_seconds": st[0],
-                "matvecs": int(st[1]), "dense_solves": int(st[2]), "matvec_seconds": st[3],
-                "matvec_flops": st[4], "rand_draws": int(st[5]), "solves": int(st[6]),
-                "max_bond": int(st[7])}
+        return _result_dict(e, t, s, st)
+
+
+def _table_args(table):
+    rows = [tuple(r) for r in table]
+    return (len(rows), _i32([r[0] for r in rows]), _i32([r[1] for r in rows]),
+            _i32([r[2] for r in rows]), _f64([r[3] for r in rows]), _f64([r[4] for r in rows]))
+
+
+def _result_dict(e, t, s, st):
+    return {"energies": e, "trunc_err": t, "seconds": s, "total_seconds": st[0],
+            "matvecs": int(st[1]), "dense_solves": int(st[2]), "matvec_seconds": st[3],
+            "matvec_flops": st[4], "rand_draws": int(st[5]), "solves": int(st[6]),
+            "max_bond": int(st[7]),
+            "phase_seconds": {"lanczos": st[8], "expansion": st[9], "expand": st[10],
+                              "canonical": st[11], "svd": st[12], "env": st[13],
+                              "match_merge": st[14]},
+            "svd_max_m": int(st[15]), "svd_max_n": int(st[16]), "svd_calls": int(st[17]),
+            "svd_work": st[18]}
+
+
+class DmrgRun:
+    """Resumable RealTensorRundmrg over a sweep table (see SpinlessModel.run)."""
+
+    def __init__(self, model, table, first_block=20, first_dim=10, precision=FP64, refine=True,
+                 verbose=False):
+        self.model = model
+        self.nhalf, maxm, maxb, niter, canpre, noise = _table_args(table)
+        h = _vp()
+        _check(lib().dmrgcc_dmrg_create(model._h, first_block, first_dim, self.nhalf, _ip(maxm),
+                                        _ip(maxb), _ip(niter), _dp(canpre), _dp(noise), precision,
+                                        int(refine), int(verbose), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.dmrgcc_dmrg_destroy(self._h)
+            self._h = None
+
+    def advance(self, half=-1, site=-1):
+        """Run until the Lanczos solve of `site` in half sweep `half` is next
+        (True) or to the end of the table (False)."""
+        r = ctypes.c_int()
+        _check(lib().dmrgcc_dmrg_advance(self._h, half, site, ctypes.byref(r)))
+        return bool(r.value)
+
+    def set_precision(self, precision, refine=True):
+        _check(lib().dmrgcc_dmrg_set_precision(self._h, precision, int(refine)))
+
+    def heff(self, site):
+        h = _vp()
+        _check(lib().dmrgcc_dmrg_heff(self._h, site, ctypes.byref(h)))
+        return HeffPlan._from_handle(h, self)
+
+    def ket(self, site):
+        p = _vp()
+        n = ctypes.c_int64()
+        _check(lib().dmrgcc_dmrg_ket(self._h, site, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def site_blocks(self, site, which):
+        """(blocks [(o, l, r, rows, cols, off)], host data) of the ket (0),
+        left (1) or right (2) environment of `site`."""
+        c = ctypes.c_int()
+        tot = ctypes.c_int64()
+        _check(lib().dmrgcc_dmrg_site_blocks(self._h, site, which, ctypes.byref(c), ctypes.byref(tot),
+                                             None, None, None, None, None, None, None))
+        n = c.value
+        o, l, r, rows, cols = (np.zeros(max(n, 1), dtype=np.int32) for _ in range(5))
+        off = np.zeros(max(n, 1), dtype=np.int64)
+        data = np.zeros(max(tot.value, 1))
+        _check(lib().dmrgcc_dmrg_site_blocks(self._h, site, which, ctypes.byref(c), ctypes.byref(tot),
+                                             _ip(o), _ip(l), _ip(r), _ip(rows), _ip(cols), _lp(off),
+                                             _dp(data)))
+        blocks = [(int(o[i]), int(l[i]), int(r[i]), int(rows[i]), int(cols[i]), int(off[i]))
+                  for i in range(n)]
+        return blocks, data[:tot.value]
+
+    def phys(self, site):
+        nl, nr = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dmrgcc_dmrg_phys(self._h, site, ctypes.byref(nl), ctypes.byref(nr), None))
+        p = np.zeros(max(nl.value * nr.value, 1), dtype=np.int32)
+        _check(lib().dmrgcc_dmrg_phys(self._h, site, ctypes.byref(nl), ctypes.byref(nr), _ip(p)))
+        return p[:nl.value * nr.value].reshape(nl.value, nr.value)
+
+    def result(self):
+        nd = ctypes.c_int()
+        e, t, s, st = np.zeros(self.nhalf), np.zeros(self.nhalf), np.zeros(self.nhalf), np.zeros(24)
+        _check(lib().dmrgcc_dmrg_result(self._h, ctypes.byref(nd), _dp(e), _dp(t), _dp(s), _dp(st)))
+        k = nd.value
+        return _result_dict(e[:k], t[:k], s[:k], st)
 
 
 # --------------------------------------------------------------------------
@@ -294,11 +390,26 @@ class HeffPlan:
         _sync_torch()
         _check(lib().dmrgcc_heff_apply(self._h, _ptr(x), _ptr(y), precision))
 
+    @classmethod
+    def _from_handle(cls, h, owner):
+        self = cls.__new__(cls)
+        self._h = h
+        self._keep = owner
+        return self
+
     def info(self):
-        v = np.zeros(6)
+        v = np.zeros(10)
         _check(lib().dmrgcc_heff_info(self._h, _dp(v)))
         return {"psi_dim": int(v[0]), "flops_ref": v[1], "flops_exec": v[2], "bytes_min": v[3],
-                "tasks": int(v[4]), "tiles": int(v[5])}
+                "tasks": int(v[4]), "tiles": int(v[5]), "flops_stage1": v[6],
+                "flops_stage2": v[7], "tiles_stage1": int(v[8]), "tiles_stage2": int(v[9])}
+
+    def time(self, precision=FP64, reps=20):
+        """Mean milliseconds per stage-1 launch, stage-2 launch and whole
+        matvec (HIP events on the plan's stream)."""
+        ms = np.zeros(3)
+        _check(lib().dmrgcc_heff_time(self._h, precision, reps, _dp(ms)))
+        return {"stage1_ms": ms[0], "stage2_ms": ms[1], "matvec_ms": ms[2]}
 
     def lanczos(self, ket, num_iter, precision=FP64, refine=True):
         e = ctypes.c_double()

@@ -1,5 +1,6 @@
 // extern "C" entry points (declared in include/dmrgcc_hip.h).  Each wraps one
 // piece of the engine, converting exceptions into status codes.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -26,6 +27,11 @@ struct dmrgcc_heff {
   Env L, R;
   HeffPlan plan;
   LanczosStats st;
+};
+
+struct dmrgcc_dmrg {
+  dmrgcc_model* model;
+  std::unique_ptr<RealTensorRundmrg> run;
 };
 
 struct dmrgcc_blocks {
@@ -95,6 +101,56 @@ Env make_env(int nbond, int n, const int* o, const int* l, const int* r, const i
   return e;
 }
 
+}  // namespace
+
+namespace {
+RunParams make_params(int first_block, int first_dim, int nhalf, const int* maxm, const int* maxb,
+                      const int* niter, const double* canpre, const double* noise, int precision,
+                      int refine, int verbose) {
+  require(nhalf > 0 && maxm && maxb && niter && canpre && noise, "bad sweep table", kErrArgument);
+  require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+  RunParams p;
+  p.first_block = first_block;
+  p.first_dim = first_dim;
+  for (int h = 0; h < nhalf; ++h) {
+    p.maxm.push_back(maxm[h]);
+    p.maxb.push_back(maxb[h]);
+    p.niter.push_back(niter[h]);
+    p.canpre.push_back(canpre[h]);
+    p.noise.push_back(std::pow(0.1, noise[h]));
+  }
+  p.precision = precision;
+  p.refine = refine;
+  p.verbose = verbose;
+  return p;
+}
+
+void fill_result(const RunResult& r, int* nhalf_done, double* energies, double* trunc, double* seconds,
+                 double* stats) {
+  if (nhalf_done) *nhalf_done = (int)r.energies.size();
+  for (size_t h = 0; h < r.energies.size(); ++h) {
+    if (energies) energies[h] = r.energies[h];
+    if (trunc) trunc[h] = r.trunc_err[h];
+    if (seconds) seconds[h] = r.half_seconds[h];
+  }
+  if (stats) {
+    stats[0] = r.total_seconds;
+    stats[1] = r.stats.matvecs;
+    stats[2] = r.stats.dense;
+    stats[3] = r.stats.matvec_seconds;
+    stats[4] = r.stats.flops;
+    stats[5] = (double)r.rng_draws;
+    stats[6] = r.stats.solves;
+    int mx = 0;
+    for (int v : r.final_bond_dims) mx = std::max(mx, v);
+    stats[7] = mx;
+    const PhaseTimes& ph = r.phase;
+    stats[8] = ph.lanczos; stats[9] = ph.expan; stats[10] = ph.expand; stats[11] = ph.canonical;
+    stats[12] = ph.svd; stats[13] = ph.env; stats[14] = ph.merge;
+    stats[15] = (double)ph.svd_max_m; stats[16] = (double)ph.svd_max_n;
+    stats[17] = (double)ph.svd_calls; stats[18] = ph.svd_work;
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -254,6 +310,10 @@ int dmrgcc_heff_info(const dmrgcc_heff* h, double* info) {
     info[3] = (double)h->plan.bytes_min();
     info[4] = (double)h->plan.tasks();
     info[5] = (double)(h->plan.stage1().tiles.size() + h->plan.stage2().tiles.size());
+    info[6] = h->plan.stage1().flops;
+    info[7] = h->plan.stage2().flops;
+    info[8] = (double)h->plan.stage1().tiles.size();
+    info[9] = (double)h->plan.stage2().tiles.size();
   });
 }
 
@@ -371,7 +431,8 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
       if (!p.trans) { c.dsr = 1; c.dsc = p.m; }  // G = A, column-major
       else { c.dsr = p.m; c.dsc = 1; }           // G = A^T
       cps.push_back(c);
-      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s, p.m, p.k});
+      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s, p.m, p.k,
+                                 p.trans ? 1 : 0, 0});
     }
     launch_copies(cps, cx.copy_scratch, cx.stream);
     jacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);
@@ -432,45 +493,158 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
   });
 }
 
+
+int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                       const int* maxb, const int* niter, const double* canpre, const double* noise,
+                       int precision, int refine, int verbose, dmrgcc_dmrg** out) {
+  return guarded([&] {
+    require(m && out, "null argument", kErrArgument);
+    RunParams p = make_params(first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
+                              refine, verbose);
+    auto h = std::make_unique<dmrgcc_dmrg>();
+    h->model = m;
+    h->run = std::make_unique<RealTensorRundmrg>(*m->space, *m->ham, p);
+    h->run->Start();
+    *out = h.release();
+  });
+}
+
+int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached) {
+  return guarded([&] {
+    require(h != nullptr, "null handle", kErrArgument);
+    const bool r = h->run->Advance(half, site);
+    if (reached) *reached = r ? 1 : 0;
+  });
+}
+
+int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine) {
+  return guarded([&] {
+    require(h != nullptr && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    h->run->set_precision(precision, refine);
+  });
+}
+
+int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
+  return guarded([&] {
+    require(h && out && site >= 0 && site < h->run->num_site(), "bad arguments", kErrArgument);
+    auto e = std::make_unique<dmrgcc_heff>();
+    e->cx = std::make_unique<Context>(nullptr);
+    h->run->BuildHeff(site, e->plan);
+    *out = e.release();
+  });
+}
+
+int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n) {
+  return guarded([&] {
+    require(h && ket && n && site >= 0 && site < h->run->num_site(), "bad arguments", kErrArgument);
+    Lattice& lat = h->run->lattice(site);
+    *ket = lat.ket.data.data();
+    *n = lat.ket.size;
+  });
+}
+
+int dmrgcc_dmrg_site_blocks(dmrgcc_dmrg* h, int site, int which, int* count, int64_t* total, int* o,
+                            int* l, int* r, int* rows, int* cols, int64_t* off, double* data) {
+  return guarded([&] {
+    require(h && count && total && site >= 0 && site < h->run->num_site() && which >= 0 && which <= 2,
+            "bad arguments", kErrArgument);
+    std::vector<const BlockTensor*> ts;
+    if (which == 0) ts.push_back(&h->run->lattice(site).ket);
+    else for (const auto& b : (which == 1 ? h->run->left_env(site) : h->run->right_env(site))) ts.push_back(&b);
+    int n = 0;
+    int64_t tot = 0;
+    for (const BlockTensor* t : ts) { n += t->n(); tot += t->size; }
+    *count = n;
+    *total = tot;
+    if (!o) return;
+    int k = 0;
+    int64_t acc = 0;
+    for (size_t c = 0; c < ts.size(); ++c) {
+      const BlockTensor* t = ts[c];
+      for (const Block& b : t->blk) {
+        o[k] = (int)c; l[k] = b.l; r[k] = b.r; rows[k] = b.rows; cols[k] = b.cols; off[k] = acc + b.off;
+        ++k;
+      }
+      if (data && t->size > 0)
+        DMRG_HIP(hipMemcpy(data + acc, t->base(), t->size * sizeof(double), hipMemcpyDeviceToHost));
+      acc += t->size;
+    }
+  });
+}
+
+int dmrgcc_dmrg_phys(dmrgcc_dmrg* h, int site, int* nl, int* nr, int* phys) {
+  return guarded([&] {
+    require(h && nl && nr && site >= 0 && site < h->run->num_site(), "bad arguments", kErrArgument);
+    const Lattice& lat = h->run->lattice(site);
+    *nl = lat.nl();
+    *nr = lat.nr();
+    if (phys)
+      for (int i = 0; i < lat.nl(); ++i)
+        for (int j = 0; j < lat.nr(); ++j) phys[(size_t)i * lat.nr() + j] = lat.phys[i][j];
+  });
+}
+
+int dmrgcc_dmrg_result(dmrgcc_dmrg* h, int* nhalf_done, double* energies, double* trunc,
+                       double* seconds, double* stats) {
+  return guarded([&] {
+    require(h != nullptr, "null handle", kErrArgument);
+    fill_result(h->run->Result(), nhalf_done, energies, trunc, seconds, stats);
+  });
+}
+
+int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h) {
+  delete h;
+  return 0;
+}
+
+int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms) {
+  return guarded([&] {
+    require(h && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    const int64_t n = std::max<int64_t>(h->plan.psi_dim(), 1);
+    DevBuf<double> x((size_t)n), y((size_t)n);
+    fill(x.data(), n, 1e-3, h->cx->stream);
+    fill(y.data(), n, 0.0, h->cx->stream);
+    hipStream_t s = h->cx->stream;
+    hipEvent_t e[3];
+    for (auto& ev : e) DMRG_HIP(hipEventCreate(&ev));
+    // warm-up
+    h->plan.apply(x.data(), y.data(), precision, s);
+    float t1 = 0, t2 = 0, tm = 0;
+    for (int r = 0; r < reps; ++r) {
+      DMRG_HIP(hipEventRecord(e[0], s));
+      h->plan.apply_stage(1, x.data(), y.data(), precision, s);
+      DMRG_HIP(hipEventRecord(e[1], s));
+      h->plan.apply_stage(2, x.data(), y.data(), precision, s);
+      DMRG_HIP(hipEventRecord(e[2], s));
+      DMRG_HIP(hipEventSynchronize(e[2]));
+      float a, b;
+      DMRG_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+      DMRG_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+      t1 += a; t2 += b;
+    }
+    // back-to-back matvecs, no host synchronisation in between
+    DMRG_HIP(hipEventRecord(e[0], s));
+    for (int r = 0; r < reps; ++r) h->plan.apply(x.data(), y.data(), precision, s);
+    DMRG_HIP(hipEventRecord(e[1], s));
+    DMRG_HIP(hipEventSynchronize(e[1]));
+    DMRG_HIP(hipEventElapsedTime(&tm, e[0], e[1]));
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    ms[0] = t1 / reps;
+    ms[1] = t2 / reps;
+    ms[2] = tm / reps;
+  });
+}
+
 int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
                double* seconds, double* stats) {
   return guarded([&] {
-    require(m && nhalf > 0 && maxm && maxb && niter && canpre && noise, "bad arguments", kErrArgument);
-    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
-    RunParams p;
-    p.first_block = first_block;
-    p.first_dim = first_dim;
-    for (int h = 0; h < nhalf; ++h) {
-      p.maxm.push_back(maxm[h]);
-      p.maxb.push_back(maxb[h]);
-      p.niter.push_back(niter[h]);
-      p.canpre.push_back(canpre[h]);
-      p.noise.push_back(std::pow(0.1, noise[h]));
-    }
-    p.precision = precision;
-    p.refine = refine;
-    p.verbose = verbose;
+    require(m != nullptr, "null model", kErrArgument);
+    RunParams p = make_params(first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
+                              refine, verbose);
     RealTensorRundmrg run(*m->space, *m->ham, p);
-    RunResult r = run.Run();
-    for (int h = 0; h < nhalf && h < (int)r.energies.size(); ++h) {
-      if (energies) energies[h] = r.energies[h];
-      if (trunc) trunc[h] = r.trunc_err[h];
-      if (seconds) seconds[h] = r.half_seconds[h];
-    }
-    if (stats) {
-      stats[0] = r.total_seconds;
-      stats[1] = r.stats.matvecs;
-      stats[2] = r.stats.dense;
-      stats[3] = r.stats.matvec_seconds;
-      stats[4] = r.stats.flops;
-      stats[5] = (double)r.rng_draws;
-      stats[6] = r.stats.solves;
-      int mx = 0;
-      for (int v : r.final_bond_dims) mx = std::max(mx, v);
-      stats[7] = mx;
-    }
+    fill_result(run.Run(), nullptr, energies, trunc, seconds, stats);
   });
 }
 

@@ -208,7 +208,12 @@ void sym_eig(int n, std::vector<double> a, std::vector<double>& eval, std::vecto
   vec.assign((size_t)n * n, 0.0);
   for (int j = 0; j < n; ++j) {
     eval[j] = A(order[j], order[j]);
-    for (int i = 0; i < n; ++i) vec[(size_t)i * n + j] = v[(size_t)i * n + order[j]];
+    // sign convention: largest |component| (first on ties) positive
+    int bi = 0;
+    for (int i = 1; i < n; ++i)
+      if (std::fabs(v[(size_t)i * n + order[j]]) > std::fabs(v[(size_t)bi * n + order[j]])) bi = i;
+    const double sg = v[(size_t)bi * n + order[j]] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) vec[(size_t)i * n + j] = sg * v[(size_t)i * n + order[j]];
   }
 }
 
@@ -319,6 +324,11 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
 void HeffPlan::apply(const double* x, double* y, int precision, hipStream_t stream) {
   s1_.launch(stream, precision, x, nullptr);
   s2_.launch(stream, precision, nullptr, y);
+}
+
+void HeffPlan::apply_stage(int stage, const double* x, double* y, int precision, hipStream_t stream) {
+  if (stage == 1) s1_.launch(stream, precision, x, nullptr);
+  else s2_.launch(stream, precision, nullptr, y);
 }
 
 // ===========================================================================
@@ -646,8 +656,11 @@ void RealTensorRundmrg::DefineTensorSpace(int first_block, int first_dim) {
     std::vector<int> li, ri, pi;
     tensor_index(i, lat_[i], li, ri, pi);
     random_blocks(lat_[i], li, ri, pi, i == 0 ? 1 : first_dim, i == L_sites_ - 1 ? 1 : first_dim);
+    if (p_.verbose >= 2) trace(-2, i, 0.0, "define_random");
     canonical(0, i);
+    if (p_.verbose >= 2) trace(-2, i, 0.0, "define_canon");
     if (i > 0) merge(0, i);
+    if (p_.verbose >= 2) trace(-2, i, 0.0, "define_merge");
   }
 }
 
@@ -701,10 +714,12 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
   cx_.work.ensure((size_t)std::max<int64_t>(wsize, 1));
   double* w = cx_.work.data();
   std::vector<SvdProblem> probs;
+  // zero the stacked matrices first (ragged blocks leave gaps); a separate
+  // launch, since copies and fills of one buffer must not race
+  fill(w, wsize, 0.0, cx_.stream);
   for (int s = 0; s < ns; ++s) {
     Sec& S = sec[s];
     if (!S.svd) continue;
-    copies.push_back(CopyTask{nullptr, w + S.g, 1, S.M * S.N, 0, 0, 0, 1, 0.0});
     int acc = 0;
     for (int p : S.pos) {
       const Block& b = ket.blk[p];
@@ -726,10 +741,19 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
       }
       copies.push_back(c);
     }
-    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, S.M, S.N});
+    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, S.M, S.N, S.trans ? 1 : 0, 0});
   }
   launch_copies(copies, cx_.copy_scratch, cx_.stream);
+  const double tsvd = now();
   jacobi_svd(probs, cx_.svd_scratch, cx_.svd_sweeps, cx_.stream);
+  cx_.sync();
+  ph_.svd += now() - tsvd;
+  for (const SvdProblem& q : probs) {
+    ph_.svd_max_m = std::max<int64_t>(ph_.svd_max_m, q.M);
+    ph_.svd_max_n = std::max<int64_t>(ph_.svd_max_n, q.N);
+    ph_.svd_work += (double)q.M * q.N * q.N;
+  }
+  ph_.svd_calls += (int64_t)probs.size();
   // singular values to the host, sorted descending per sector
   std::vector<double> hw((size_t)std::max<int64_t>(wsize, 1));
   std::vector<std::vector<double>> sv(ns), strue(ns);
@@ -966,8 +990,10 @@ void RealTensorRundmrg::Initialize() {
   R_[L_sites_ - 1] = boundary_env(ham_.sites()[L_sites_ - 1].rb, cx_);
   for (int i = L_sites_ - 1; i > 0; --i) {
     canonical(1, i);
+    if (p_.verbose >= 2) trace(-1, i, 0.0, "init_canon");
     R_[i - 1] = right_env_update(lat_[i], ham_.sites()[i], R_[i], cx_);
     merge(1, i - 1);
+    if (p_.verbose >= 2) trace(-1, i - 1, 0.0, "init_merge");
   }
   canonical(1, 0);
 }
@@ -1520,69 +1546,123 @@ void RealTensorRundmrg::match(int leigh, int site) {
 // --------------------------------------------------------------------------
 // Run (real_tensor_rundmrg.cc:115-245), no disk cache
 // --------------------------------------------------------------------------
-RunResult RealTensorRundmrg::Run() {
-  RunResult res;
-  const double t0 = now();
+void RealTensorRundmrg::trace(int h, int j, double e, const char* what) {
+  const Lattice& lat = lat_[j];
+  long ld = 0, rd = 0;
+  for (int v : lat.ld) ld += v;
+  for (int v : lat.rd) rd += v;
+  double nrm = 0.0;
+  if (lat.ket.size > 0) {
+    dot(lat.ket.data.data(), lat.ket.data.data(), lat.ket.size, cx_.scal.data() + 30, cx_.partial, cx_.stream);
+    DMRG_HIP(hipMemcpyAsync(&nrm, cx_.scal.data() + 30, sizeof(double), hipMemcpyDeviceToHost, cx_.stream));
+    cx_.sync();
+  }
+  std::printf("TRACE %d %d %-17s E=%.12f nl=%d nr=%d ld=%ld rd=%ld nblk=%d size=%ld rng=%llu nrm2=%.12e\n", h, j,
+              what, e, lat.nl(), lat.nr(), ld, rd, lat.ket.n(), (long)lat.ket.size,
+              (unsigned long long)cx_.rng.count(), nrm);
+  std::fflush(stdout);
+}
+
+void RealTensorRundmrg::Start() {
+  t0_ = now();
   stats_ = LanczosStats{};
+  ph_ = PhaseTimes{};
+  res_ = RunResult{};
   DefineTensorSpace(p_.first_block, p_.first_dim);
   Initialize();
-  const int nhalf = (int)p_.maxm.size();
+  cur_h_ = 0;
+  cur_j_ = 0;
+  half_e_ = 0;
+  half_terr_ = 0;
+  half_t0_ = now();
+  started_ = true;
+}
+
+// One site step of half sweep h at site j: Lanczos, expansion, canonical
+// form, environment update, match and merge (real_tensor_rundmrg.cc:145-175
+// and :183-214).
+void RealTensorRundmrg::Step(int h, int j) {
   const auto& W = ham_.sites();
-  for (int h = 0; h < nhalf; h += 2) {
-    // left to right
-    double th = now();
+  const bool l2r = (h % 2) == 0;
+  if ((l2r && j == 0) || (!l2r && j == L_sites_ - 1)) {
     set_params(p_.maxb[h], p_.maxm[h], p_.canpre[h], p_.noise[h]);
-    double energy = 0, terr = 0;
-    for (int j = 0; j < L_sites_ - 1; ++j) {
-      energy = LanczosMethod(p_.niter[h], j);
-      for (auto& b : R_[j]) { b.define(std::vector<int>(), std::vector<int>()); }
-      // ResetTensorNetwork keeps the block list but empties the matrices;
-      // R_j is not read again before it is recomputed, so it is dropped.
-      Lattice E;
-      auto mapping = expan_lattice(1, j, ham_.bond_tables()[j + 1], E);
-      expan_contraction(0, j, E, mapping);
-      expand(0, j, E);
-      terr = std::max(terr, canonical(0, j));
-      L_[j + 1] = left_env_update(lat_[j], W[j], L_[j], cx_);
-      match(0, j + 1);
-      merge(0, j + 1);
-    }
-    res.energies.push_back(energy);
-    res.trunc_err.push_back(terr);
-    res.half_seconds.push_back(now() - th);
-    if (p_.verbose) std::printf("right energy = %.12f\n", energy), std::fflush(stdout);
-    if (h + 1 >= nhalf) break;
-    // right to left
-    th = now();
-    set_params(p_.maxb[h + 1], p_.maxm[h + 1], p_.canpre[h + 1], p_.noise[h + 1]);
-    terr = 0;
-    for (int j = L_sites_ - 1; j > 0; --j) {
-      energy = LanczosMethod(p_.niter[h + 1], j);
-      for (auto& b : L_[j]) { b.define(std::vector<int>(), std::vector<int>()); }
-      Lattice E;
-      auto mapping = expan_lattice(0, j, ham_.bond_tables()[j], E);
-      expan_contraction(1, j, E, mapping);
-      expand(1, j, E);
-      terr = std::max(terr, canonical(1, j));
-      R_[j - 1] = right_env_update(lat_[j], W[j], R_[j], cx_);
-      match(1, j - 1);
-      merge(1, j - 1);
-    }
-    res.energies.push_back(energy);
-    res.trunc_err.push_back(terr);
-    res.half_seconds.push_back(now() - th);
-    if (p_.verbose) std::printf("left energy = %.12f\n", energy), std::fflush(stdout);
+    half_terr_ = 0;
+    half_t0_ = now();
   }
+  double tp = now();
+  const double energy = LanczosMethod(p_.niter[h], j);
+  half_e_ = energy;
+  ph_.lanczos += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, j, energy, "lanczos");
+  // ResetTensorNetwork keeps the block list but empties the matrices; the
+  // reset environment is not read again before it is recomputed.
+  for (auto& b : (l2r ? R_[j] : L_[j])) b.define(std::vector<int>(), std::vector<int>());
+  Lattice E;
+  auto mapping = l2r ? expan_lattice(1, j, ham_.bond_tables()[j + 1], E)
+                     : expan_lattice(0, j, ham_.bond_tables()[j], E);
+  expan_contraction(l2r ? 0 : 1, j, E, mapping);
+  ph_.expan += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, j, energy, "expan_contraction");
+  expand(l2r ? 0 : 1, j, E);
+  ph_.expand += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, j, energy, "expand");
+  half_terr_ = std::max(half_terr_, canonical(l2r ? 0 : 1, j));
+  ph_.canonical += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, j, energy, "canonical");
+  if (l2r) L_[j + 1] = left_env_update(lat_[j], W[j], L_[j], cx_);
+  else R_[j - 1] = right_env_update(lat_[j], W[j], R_[j], cx_);
+  ph_.env += now() - tp; tp = now();
+  const int nb = l2r ? j + 1 : j - 1;
+  match(l2r ? 0 : 1, nb);
+  if (p_.verbose >= 2) trace(h, nb, energy, "match");
+  merge(l2r ? 0 : 1, nb);
+  ph_.merge += now() - tp;
+  if (p_.verbose >= 2) trace(h, nb, energy, "merge");
+  if ((l2r && j == L_sites_ - 2) || (!l2r && j == 1)) {
+    res_.energies.push_back(half_e_);
+    res_.trunc_err.push_back(half_terr_);
+    res_.half_seconds.push_back(now() - half_t0_);
+    if (p_.verbose) std::printf("%s energy = %.12f\n", l2r ? "right" : "left", half_e_), std::fflush(stdout);
+  }
+}
+
+bool RealTensorRundmrg::Advance(int stop_h, int stop_j) {
+  if (!started_) Start();
+  const int nhalf = (int)p_.maxm.size();
+  while (cur_h_ < nhalf) {
+    const bool l2r = (cur_h_ % 2) == 0;
+    if (!l2r && cur_j_ == 0) cur_j_ = L_sites_ - 1;
+    if (cur_h_ == stop_h && cur_j_ == stop_j) return true;
+    Step(cur_h_, cur_j_);
+    if (l2r) {
+      if (++cur_j_ == L_sites_ - 1) { ++cur_h_; cur_j_ = 0; }
+    } else {
+      if (--cur_j_ == 0) { ++cur_h_; cur_j_ = 0; }
+    }
+  }
+  return false;
+}
+
+RunResult RealTensorRundmrg::Result() {
   cx_.sync();
-  res.total_seconds = now() - t0;
+  RunResult res = res_;
+  res.total_seconds = now() - t0_;
   res.stats = stats_;
+  res.phase = ph_;
   res.rng_draws = cx_.rng.count();
+  res.final_bond_dims.clear();
   for (int i = 0; i < L_sites_; ++i) {
     int s = 0;
     for (int v : lat_[i].rd) s += v;
     res.final_bond_dims.push_back(s);
   }
   return res;
+}
+
+RunResult RealTensorRundmrg::Run() {
+  Start();
+  Advance(-1, -1);
+  return Result();
 }
 
 }  // namespace dmrg

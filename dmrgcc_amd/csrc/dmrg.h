@@ -88,6 +88,8 @@ class HeffPlan {
              hipStream_t stream);
   // y (+)= H x ; x, y in ket-vector layout (psi_dim elements).
   void apply(const double* x, double* y, int precision, hipStream_t stream);
+  // launch one stage only (timing)
+  void apply_stage(int stage, const double* x, double* y, int precision, hipStream_t stream);
   double flops() const { return s1_.flops + s2_.flops; }  // executed
   double fref() const { return fref_; }                      // reference order
   int64_t psi_dim() const { return psi_dim_; }
@@ -142,7 +144,16 @@ struct RunParams {
   int verbose = 0;
 };
 
+// Wall time per sweep phase (seconds, host clock around device-synchronised
+// work) and SVD sizes seen.
+struct PhaseTimes {
+  double lanczos = 0, expan = 0, expand = 0, canonical = 0, svd = 0, env = 0, merge = 0;
+  int64_t svd_max_m = 0, svd_max_n = 0, svd_calls = 0;
+  double svd_work = 0;  // sum of M*N*N over sector SVDs
+};
+
 struct RunResult {
+  PhaseTimes phase;
   std::vector<double> energies;   // per half sweep (right, left, right, ...)
   std::vector<double> trunc_err;  // max discarded weight per half sweep
   std::vector<double> half_seconds;
@@ -158,6 +169,19 @@ class RealTensorRundmrg {
   RealTensorRundmrg(const RealTensorSpace& space, const RealTensorHamiltonian& ham,
                     const RunParams& p, hipStream_t stream = nullptr);
   RunResult Run();
+  // Resumable driving: Start() builds and right-canonicalises the initial MPS
+  // (DefineTensorSpace + Initialize); Advance(h, j) runs site steps until the
+  // Lanczos solve of site j in half sweep h is next (returns true) or the
+  // sweep table is exhausted (returns false, pass h = -1 to run to the end).
+  void Start();
+  bool Advance(int stop_h, int stop_j);
+  RunResult Result();
+  int num_site() const { return L_sites_; }
+  void set_precision(int precision, int refine) { p_.precision = precision; p_.refine = refine; }
+  // H_eff plan on the current state of `site` (valid when the site is next).
+  void BuildHeff(int site, HeffPlan& plan) {
+    plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
+  }
 
   // pieces exposed for tests
   void DefineTensorSpace(int first_block, int first_dim);
@@ -187,6 +211,12 @@ class RealTensorRundmrg {
   void expan_contraction(int leigh, int site, Lattice& E, const std::vector<std::vector<int>>& mapping);
   void expand(int leigh, int site, Lattice& E);
   void match(int leigh, int site);
+  void trace(int h, int j, double e, const char* what);
+  void Step(int h, int j);
+  bool started_ = false;
+  int cur_h_ = 0, cur_j_ = 0;
+  double half_e_ = 0, half_terr_ = 0, half_t0_ = 0, t0_ = 0;
+  RunResult res_;
 
   const RealTensorSpace& space_;
   const RealTensorHamiltonian& ham_;
@@ -199,6 +229,7 @@ class RealTensorRundmrg {
   int max_block_ = 0, max_dim_ = -1;
   double canpre_ = 1e-30, noise_ = 10;
   LanczosStats stats_;
+  PhaseTimes ph_;
 };
 
 }  // namespace dmrg

@@ -77,7 +77,8 @@ struct GemmBatch {
 };
 
 // Strided 2-D copy task: dst[i*dsr + j*dsc] = alpha * src[i*ssr + j*ssc]
-// for i < rows, j < cols.  src == nullptr writes zeros.
+// for i < rows, j < cols.  src == nullptr writes zeros.  The tasks of one
+// launch run concurrently: their destinations must not overlap.
 struct CopyTask {
   const double* src;
   double* dst;
@@ -130,11 +131,18 @@ void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGather
 // Each problem: column-major G (M x N, M >= N, leading dim M) is rotated in
 // place into G*V with mutually orthogonal columns; V (N x N column-major) is
 // initialised to identity and accumulates the rotations; sigma[j] = |G e_j|.
+// Sign convention (fixes the gauge the reference leaves to LAPACK): the
+// largest-magnitude entry (first one on ties) of every left singular vector
+// is made positive, flipping the matching right singular vector with it.  The
+// left vector is column j of G (u_in_v = 0) or of V (u_in_v = 1, when G holds
+// the transposed matrix).
 struct SvdProblem {
   double* G;
   double* V;
   double* sigma;
   int M, N;
+  int u_in_v;
+  int pad;
 };
 void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 60);

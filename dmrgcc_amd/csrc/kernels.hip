@@ -37,12 +37,16 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 template <typename T>
 struct MfmaAcc;
+// Output element v of a lane sits in column lane%16 and a row that differs
+// between the two instructions (measured on gfx950): the f64 form interleaves
+// rows (lane/16 + 4v), the f32 form blocks them (4*(lane/16) + v).
 template <>
 struct MfmaAcc<double> {
   typedef d4v V;
   __device__ static V mma(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct MfmaAcc<float> {
@@ -50,6 +54,7 @@ struct MfmaAcc<float> {
   __device__ static V mma(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 template <typename T>
@@ -126,14 +131,14 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
       }
     }
   }
-  // epilogue: lane holds D[4*(lane/16)+v][lane%16] of each 16x16 block
+  // epilogue: lane holds D[row(lane, v)][lane%16] of each 16x16 block
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int gm = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
+        const int gm = m0 + wm * 32 + a * 16 + MfmaAcc<T>::row(lane, v);
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
           double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
@@ -382,83 +387,104 @@ inline int grid_for(int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// One-sided Jacobi SVD.  One workgroup (8 waves) per problem; columns are
-// paired by the round-robin (circle) ordering so every round rotates N/2
-// disjoint pairs in parallel, one pair per wave at a time.
+// One-sided (Hestenes) Jacobi SVD, batched over sectors.  Columns are paired
+// by the round-robin (circle) ordering; one launch performs one round for
+// every active sector, one wave per column pair, so a round costs one short
+// kernel however many sectors there are.  The host runs the rounds of a
+// sweep and retires sectors whose sweep made no rotation.
 // ---------------------------------------------------------------------------
-constexpr int kSvdThreads = 512;
+constexpr int kSvdWaves = 4;
 
-__global__ __launch_bounds__(kSvdThreads) void jacobi_svd_kernel(const SvdProblem* __restrict__ probs,
-                                                                 int* __restrict__ sweeps_out,
-                                                                 int max_sweeps) {
+__global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __restrict__ probs) {
+  const SvdProblem p = probs[blockIdx.x];
+  const int64_t nn = (int64_t)p.N * p.N;
+  for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) p.V[e] = (e % (p.N + 1) == 0) ? 1.0 : 0.0;
+}
+
+// pair q of the round -> problem via the prefix array of pairs per problem
+__global__ __launch_bounds__(kSvdWaves * 64) void jacobi_round_kernel(
+    const SvdProblem* __restrict__ probs, const int* __restrict__ active,
+    const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * kSvdWaves + (threadIdx.x >> 6);
+  if (q >= prefix[nactive]) return;
+  int lo = 0, hi = nactive - 1;
+  while (lo < hi) {  // largest k with prefix[k] <= q
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= q) lo = mid; else hi = mid - 1;
+  }
+  const int pi = active[lo];
+  const SvdProblem p = probs[pi];
+  const int M = p.M, N = p.N, n2 = N + (N & 1);
+  if (round >= n2 - 1) return;
+  const int pr = q - prefix[lo];
+  int a, b;
+  if (pr == 0) { a = round; b = n2 - 1; }
+  else { a = (round + pr) % (n2 - 1); b = (round + n2 - 1 - pr) % (n2 - 1); }
+  if (a >= N || b >= N) return;
+  if (a > b) { const int t = a; a = b; b = t; }
+  double* __restrict__ ga = p.G + (int64_t)a * M;
+  double* __restrict__ gb = p.G + (int64_t)b * M;
+  double al = 0.0, be = 0.0, gm = 0.0;
+  for (int i = lane; i < M; i += 64) {
+    const double x = ga[i], y = gb[i];
+    al += x * x; be += y * y; gm += x * y;
+  }
+  al = wave_sum(al); be = wave_sum(be); gm = wave_sum(gm);
+  const double tol = 2.220446049250313e-16 * sqrt((double)M);
+  if (!(al > 0.0 && be > 0.0 && fabs(gm) > tol * sqrt(al) * sqrt(be))) return;
+  const double zeta = (be - al) / (2.0 * gm);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+  const double cs = 1.0 / sqrt(1.0 + t * t);
+  const double sn = cs * t;
+  for (int i = lane; i < M; i += 64) {
+    const double x = ga[i], y = gb[i];
+    ga[i] = cs * x - sn * y;
+    gb[i] = sn * x + cs * y;
+  }
+  double* __restrict__ va = p.V + (int64_t)a * N;
+  double* __restrict__ vb = p.V + (int64_t)b * N;
+  for (int i = lane; i < N; i += 64) {
+    const double x = va[i], y = vb[i];
+    va[i] = cs * x - sn * y;
+    vb[i] = sn * x + cs * y;
+  }
+  if (lane == 0) rotated[pi] = 1;
+}
+
+// singular values and the sign convention, one workgroup per problem
+__global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __restrict__ probs) {
   const SvdProblem p = probs[blockIdx.x];
   const int M = p.M, N = p.N;
-  double* __restrict__ G = p.G;
-  double* __restrict__ V = p.V;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwaves = blockDim.x >> 6;
-  __shared__ int rotated;
-
-  for (int64_t e = tid; e < (int64_t)N * N; e += blockDim.x) V[e] = (e % (N + 1) == 0) ? 1.0 : 0.0;
-  __syncthreads();
-
-  const double tol = 2.220446049250313e-16 * sqrt((double)M);
-  const int n2 = N + (N & 1);
-  int sweep = 0;
-  if (N > 1) {
-    for (sweep = 0; sweep < max_sweeps; ++sweep) {
-      if (tid == 0) rotated = 0;
-      __syncthreads();
-      for (int r = 0; r < n2 - 1; ++r) {
-        for (int pr = wave; pr < n2 / 2; pr += nwaves) {
-          int a, b;
-          if (pr == 0) { a = r; b = n2 - 1; }
-          else { a = (r + pr) % (n2 - 1); b = (r + n2 - 1 - pr) % (n2 - 1); }
-          if (a >= N || b >= N) continue;
-          if (a > b) { int t = a; a = b; b = t; }
-          double* ga = G + (int64_t)a * M;
-          double* gb = G + (int64_t)b * M;
-          double al = 0.0, be = 0.0, ga_b = 0.0;
-          for (int i = lane; i < M; i += 64) {
-            const double x = ga[i], y = gb[i];
-            al += x * x; be += y * y; ga_b += x * y;
-          }
-          al = wave_sum(al); be = wave_sum(be); ga_b = wave_sum(ga_b);
-          if (al > 0.0 && be > 0.0 && fabs(ga_b) > tol * sqrt(al) * sqrt(be)) {
-            const double zeta = (be - al) / (2.0 * ga_b);
-            const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
-            const double cs = 1.0 / sqrt(1.0 + t * t);
-            const double sn = cs * t;
-            for (int i = lane; i < M; i += 64) {
-              const double x = ga[i], y = gb[i];
-              ga[i] = cs * x - sn * y;
-              gb[i] = sn * x + cs * y;
-            }
-            double* va = V + (int64_t)a * N;
-            double* vb = V + (int64_t)b * N;
-            for (int i = lane; i < N; i += 64) {
-              const double x = va[i], y = vb[i];
-              va[i] = cs * x - sn * y;
-              vb[i] = sn * x + cs * y;
-            }
-            if (lane == 0) rotated = 1;
-          }
-        }
-        __syncthreads();
-      }
-      const int any = rotated;
-      __syncthreads();
-      if (!any) break;
-    }
-  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   for (int j = wave; j < N; j += nwaves) {
-    const double* g = G + (int64_t)j * M;
+    double* g = p.G + (int64_t)j * M;
+    double* v = p.V + (int64_t)j * N;
     double s = 0.0;
     for (int i = lane; i < M; i += 64) s += g[i] * g[i];
     s = wave_sum(s);
     if (lane == 0) p.sigma[j] = sqrt(s);
+    // sign convention: largest |u_i| (first on ties) positive
+    const double* u = p.u_in_v ? v : g;
+    const int n = p.u_in_v ? N : M;
+    double best = -1.0, val = 0.0;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < n; i += 64) {
+      const double x = fabs(u[i]);
+      if (x > best) { best = x; bi = i; val = u[i]; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      const double ov = __shfl_xor(val, o, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; val = ov; }
+    }
+    if (val < 0.0) {
+      for (int i = lane; i < M; i += 64) g[i] = -g[i];
+      for (int i = lane; i < N; i += 64) v[i] = -v[i];
+    }
   }
-  if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
 }
 
 }  // namespace
@@ -615,12 +641,50 @@ void fill(double* x, int64_t n, double v, hipStream_t stream) {
 void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
   if (probs.empty()) return;
+  const int np = (int)probs.size();
   for (const SvdProblem& p : probs)
     require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
   upload(scratch, probs, stream);
-  sweeps_out.ensure(probs.size());
-  hipLaunchKernelGGL(jacobi_svd_kernel, dim3((unsigned)probs.size()), dim3(kSvdThreads), 0, stream,
-                     scratch.data(), sweeps_out.data(), max_sweeps);
+  hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  DMRG_HIP(hipGetLastError());
+  // device ints: rotated[np] | active[np] | prefix[np + 1]
+  sweeps_out.ensure((size_t)3 * np + 1);
+  int* rotated = sweeps_out.data();
+  int* dactive = rotated + np;
+  int* dprefix = dactive + np;
+  std::vector<int> active;
+  for (int i = 0; i < np; ++i)
+    if (probs[i].N > 1) active.push_back(i);
+  std::vector<int> hrot(np), meta;
+  for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
+    meta.assign(active.begin(), active.end());
+    int pairs = 0, rounds = 0;
+    std::vector<int> prefix(1, 0);
+    for (int i : active) {
+      const int n2 = probs[i].N + (probs[i].N & 1);
+      pairs += n2 / 2;
+      prefix.push_back(pairs);
+      rounds = std::max(rounds, n2 - 1);
+    }
+    const int na = (int)active.size();
+    std::vector<int> h(np, 0);
+    h.insert(h.end(), active.begin(), active.end());
+    h.resize(2 * np, 0);
+    h.insert(h.end(), prefix.begin(), prefix.end());
+    DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    const int grid = (pairs + kSvdWaves - 1) / kSvdWaves;
+    for (int r = 0; r < rounds; ++r)
+      hipLaunchKernelGGL(jacobi_round_kernel, dim3(grid), dim3(kSvdWaves * 64), 0, stream, scratch.data(),
+                         dactive, dprefix, na, r, rotated);
+    DMRG_HIP(hipGetLastError());
+    DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
+    DMRG_HIP(hipStreamSynchronize(stream));
+    std::vector<int> still;
+    for (int i : active)
+      if (hrot[i]) still.push_back(i);
+    active.swap(still);
+  }
+  hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
 }
 

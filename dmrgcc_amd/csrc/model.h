@@ -7,6 +7,7 @@
 // model/hamiltonian/real_spinless_bose_square_hamiltonian.*).
 #pragma once
 
+#include <cstddef>
 #include <vector>
 
 namespace dmrg {

@@ -99,7 +99,9 @@ int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, c
  * x, y: device vectors of psi_dim doubles. */
 int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision);
 /* info[0] = psi_dim, [1] = F_ref (reference-order flops), [2] = F_exec,
- * [3] = minimum HBM bytes per matvec, [4] = stage-1 tasks, [5] = tiles. */
+ * [3] = minimum HBM bytes per matvec, [4] = stage-1 tasks, [5] = tiles,
+ * [6] = stage-1 flops, [7] = stage-2 flops, [8] = stage-1 tiles,
+ * [9] = stage-2 tiles. */
 int dmrgcc_heff_info(const dmrgcc_heff* h, double* info);
 int dmrgcc_heff_destroy(dmrgcc_heff* h);
 
@@ -146,12 +148,49 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
  * Per half sweep h: maxm[h], maxb[h], niter[h], canpre[h], noise[h] (the
  * table exponent; 0.1^noise is applied as in the reference).  Outputs per
  * half sweep (host, nhalf entries): energies, discarded weight, seconds.
- * stats (host, 8): total seconds, matvecs, dense solves, matvec seconds,
- * matvec flops (F_ref), rand() draws, Lanczos solves, max bond dim. */
+ * stats (host, 24): [0] total seconds, [1] matvecs, [2] dense solves,
+ * [3] matvec seconds, [4] matvec flops (F_ref), [5] rand() draws, [6] Lanczos
+ * solves, [7] max bond dim, [8..14] seconds in Lanczos, expansion
+ * contraction, expansion fill, canonicalisation, of which SVD, environment
+ * update, match+merge, [15..16] largest SVD M and N, [17] SVD count,
+ * [18] sum of M*N*N over SVDs. */
 int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
                double* seconds, double* stats);
+
+/* ---- resumable sweep (RealTensorRundmrg, real_tensor_rundmrg.cc:3-287) -- */
+/* A handle owns the MPS, environments and rand() stream of one run.
+ * create: DefineTensorSpace + Initialize (same sweep-table arguments as
+ * dmrgcc_run).  advance: run site steps until the Lanczos solve of `site`
+ * in half sweep `half` is next (*reached = 1) or the table ends
+ * (*reached = 0; pass half = -1 to run to the end).  heff: the H_eff plan of
+ * `site` on the current state (valid until the next advance); ket: device
+ * pointer and length of that site's ket vector.  result: as dmrgcc_run.
+ * set_precision: arithmetic of the following Lanczos solves. */
+typedef struct dmrgcc_dmrg dmrgcc_dmrg;
+int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                       const int* maxb, const int* niter, const double* canpre, const double* noise,
+                       int precision, int refine, int verbose, dmrgcc_dmrg** out);
+int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached);
+int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine);
+int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
+int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
+/* Export the current blocks of `site` to host memory (tests, CPU baseline):
+ * which 0 = ket, 1 = left environment, 2 = right environment.  With
+ * o == NULL only *count and *total (elements) are written. */
+int dmrgcc_dmrg_site_blocks(dmrgcc_dmrg* h, int site, int which, int* count, int64_t* total, int* o,
+                            int* l, int* r, int* rows, int* cols, int64_t* off, double* data);
+/* Physical-index table of `site` (nl x nr, -1 where no block). */
+int dmrgcc_dmrg_phys(dmrgcc_dmrg* h, int site, int* nl, int* nr, int* phys);
+int dmrgcc_dmrg_result(dmrgcc_dmrg* h, int* nhalf_done, double* energies, double* trunc,
+                       double* seconds, double* stats);
+int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h);
+
+/* Time `reps` H_eff applications (y += H x on internal buffers) with HIP
+ * events on the plan's stream.  ms[0] = mean stage-1 (T = K R) launch,
+ * ms[1] = mean stage-2 (out += w L T) launch, ms[2] = mean whole matvec. */
+int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -265,10 +265,22 @@ def boundary_env(nbond):
 # --------------------------------------------------------------------------
 # Lanczos
 # --------------------------------------------------------------------------
+def fix_signs(vecs):
+    """Sign convention for eigen/singular vectors (columns of `vecs`): the
+    largest-magnitude component (first on ties) is made positive.  The
+    reference leaves the sign to LAPACK; fixing it makes the trajectory
+    reproducible, because the expansion noise contracts a fixed random matrix
+    with the gauge-carrying bond index."""
+    idx = np.argmax(np.abs(vecs), axis=0)
+    sg = np.where(vecs[idx, np.arange(vecs.shape[1])] < 0, -1.0, 1.0)
+    return sg
+
+
 def sym_eig(A):
     """RealSymMatrixDiag (src/util/general.cc:47-64): ascending eigenvalues,
-    eigenvectors in columns."""
-    return np.linalg.eigh(A)
+    eigenvectors in columns (sign convention of fix_signs)."""
+    w, v = np.linalg.eigh(A)
+    return w, v * fix_signs(v)[None, :]
 
 
 def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
@@ -361,6 +373,9 @@ def svd(A):
     factor is returned padded to n rows (rows >= min(m,n) are zero)."""
     m, n = A.shape
     U, s, Vt = np.linalg.svd(A, full_matrices=False)
+    sg = fix_signs(U)
+    U = U * sg[None, :]
+    Vt = Vt * sg[:, None]
     Vfull = np.zeros((n, n))
     Vfull[:Vt.shape[0]] = Vt
     return U, s.copy(), Vfull
@@ -506,6 +521,7 @@ class Space:
         self.canpre = 1e-30
         self.noise = 10.0
         self.expan = None
+        self.tracer = None
 
     def set_params(self, max_block, max_dim, canpre, noise):
         """DefineTensorSpaceParameter (real_tensor_space.cc:59-66)."""
@@ -559,9 +575,15 @@ class Space:
             rows = 1 if i == 0 else first_dim
             cols = 1 if i == L - 1 else first_dim
             self.lat[i].define_blocks(li, ri, pi, rows, cols, self.rng)
+            if self.tracer:
+                self.tracer(-2, i, 0.0, "define_random")
             self.canonical(0, i)
+            if self.tracer:
+                self.tracer(-2, i, 0.0, "define_canon")
             if i > 0:
                 self.merge(0, i)
+            if self.tracer:
+                self.tracer(-2, i, 0.0, "define_merge")
 
     def canonical(self, leigh, site, info=None):
         if leigh == 0:
@@ -791,6 +813,7 @@ def _expan_width(D, d):
 
 def left_expan_contraction(lat, W, Lenv, E, mapping, noise, rng):
     """LeftExpanTensorContraction (real_tensor_contraction.cc:169-459)."""
+
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     same_l = [ket.same(0, l) for l in range(lat.nl)]
@@ -1008,13 +1031,16 @@ def sweep_table(rows, num_sweep):
 class DMRG:
     """RealTensorRundmrg + RealTensorNetwork + RealTensorLanczos."""
 
-    def __init__(self, model, sweeps, num_sweep, first_block=20, first_dim=10, seed=1):
+    def __init__(self, model, sweeps, num_sweep, first_block=20, first_dim=10, seed=1, trace=False):
         self.model = model
         self.rng = GlibcRand(seed)
         self.space = Space(model, self.rng)
         self.W, self.op_tables = model.mpo()
         self.L = model.L
         self.table = sweep_table(sweeps, num_sweep)
+        self.trace = trace
+        if trace:
+            self.space.tracer = self._trace
         self.space.define(first_block, first_dim)
         # tensor_contraction_[site] holds L (left_bond of W[site]) and R
         # (right_bond of W[site]) (real_tensor_network.cc:3-33)
@@ -1032,9 +1058,20 @@ class DMRG:
         sp = self.space
         for i in range(L - 1, 0, -1):
             sp.canonical(1, i)
+            if self.trace:
+                self._trace(-1, i, 0.0, "init_canon")
             self.Renv[i - 1] = right_env(sp.lat[i], self.W[i], self.Renv[i])
             sp.merge(1, i - 1)
+            if self.trace:
+                self._trace(-1, i - 1, 0.0, "init_merge")
         sp.canonical(1, 0)
+
+    def _trace(self, h, j, e, what):
+        lat = self.space.lat[j]
+        v = lat.ket.to_vector()
+        print("TRACE %d %d %-17s E=%.12f nl=%d nr=%d ld=%d rd=%d nblk=%d size=%d rng=%d nrm2=%.12e" % (
+            h, j, what, e if e is not None else 0.0, lat.nl, lat.nr, sum(lat.left_dim), sum(lat.right_dim),
+            lat.ket.n, lat.ket.dim(), self.rng.count, float(np.dot(v, v))), flush=True)
 
     def _discarded(self, info):
         s2 = 0.0
@@ -1060,16 +1097,23 @@ class DMRG:
                 if stop == (2 * s, j):
                     return self.energies
                 energy = lanczos(sp.lat[j], self.W[j], self.Lenv[j], self.Renv[j], niter, self.stats)
+                tr = self._trace if self.trace else (lambda *a: None)
+                tr(2 * s, j, energy, "lanczos")
                 self.Renv[j] = [BlockMat(b.left, b.right) for b in self.Renv[j]]
                 mapping = sp.expan_lattice(1, j, self.op_tables[j + 1])
                 left_expan_contraction(sp.lat[j], self.W[j], self.Lenv[j], sp.expan, mapping, sp.noise, self.rng)
+                tr(2 * s, j, energy, "expan_contraction")
                 sp.expand(0, j)
+                tr(2 * s, j, energy, "expand")
                 info = {}
                 sp.canonical(0, j, info)
+                tr(2 * s, j, energy, "canonical")
                 terr = max(terr, self._discarded(info))
                 self.Lenv[j + 1] = left_env(sp.lat[j], self.W[j], self.Lenv[j])
                 sp.match(0, j + 1)
+                tr(2 * s, j + 1, energy, "match")
                 sp.merge(0, j + 1)
+                tr(2 * s, j + 1, energy, "merge")
             self.energies.append(energy)
             self.trunc_err.append(terr)
             if verbose:
@@ -1081,16 +1125,23 @@ class DMRG:
                 if stop == (2 * s + 1, j):
                     return self.energies
                 energy = lanczos(sp.lat[j], self.W[j], self.Lenv[j], self.Renv[j], niter, self.stats)
+                tr = self._trace if self.trace else (lambda *a: None)
+                tr(2 * s + 1, j, energy, "lanczos")
                 self.Lenv[j] = [BlockMat(b.left, b.right) for b in self.Lenv[j]]
                 mapping = sp.expan_lattice(0, j, self.op_tables[j])
                 right_expan_contraction(sp.lat[j], self.W[j], self.Renv[j], sp.expan, mapping, sp.noise, self.rng)
+                tr(2 * s + 1, j, energy, "expan_contraction")
                 sp.expand(1, j)
+                tr(2 * s + 1, j, energy, "expand")
                 info = {}
                 sp.canonical(1, j, info)
+                tr(2 * s + 1, j, energy, "canonical")
                 terr = max(terr, self._discarded(info))
                 self.Renv[j - 1] = right_env(sp.lat[j], self.W[j], self.Renv[j])
                 sp.match(1, j - 1)
+                tr(2 * s + 1, j - 1, energy, "match")
                 sp.merge(1, j - 1)
+                tr(2 * s + 1, j - 1, energy, "merge")
             self.energies.append(energy)
             self.trunc_err.append(terr)
             if verbose:
