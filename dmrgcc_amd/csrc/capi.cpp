@@ -420,7 +420,7 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
       p.M = M[i]; p.N = N[i];
       p.g = w; w += (int64_t)p.m * p.k;
       p.v = w; w += (int64_t)p.k * p.k;
-      p.s = w; w += p.k;
+      p.s = w; w += p.k + 1;  // sigma[k] and |A|_F^2
     }
     DevBuf<double> work((size_t)std::max<int64_t>(w, 1));
     std::vector<CopyTask> cps;
@@ -431,8 +431,8 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
       if (!p.trans) { c.dsr = 1; c.dsc = p.m; }  // G = A, column-major
       else { c.dsr = p.m; c.dsc = 1; }           // G = A^T
       cps.push_back(c);
-      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s, p.m, p.k,
-                                 p.trans ? 1 : 0, 0});
+      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s,
+                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0, 0});
     }
     launch_copies(cps, cx.copy_scratch, cx.stream);
     jacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);

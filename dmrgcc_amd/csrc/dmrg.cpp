@@ -709,7 +709,7 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
     S.N = std::min(S.rows, S.cols);
     S.g = wsize; wsize += (int64_t)S.M * S.N;
     S.v = wsize; wsize += (int64_t)S.N * S.N;
-    S.s = wsize; wsize += S.N;
+    S.s = wsize; wsize += S.N + 1;  // sigma[N] and |A|_F^2
   }
   cx_.work.ensure((size_t)std::max<int64_t>(wsize, 1));
   double* w = cx_.work.data();
@@ -741,7 +741,7 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
       }
       copies.push_back(c);
     }
-    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, S.M, S.N, S.trans ? 1 : 0, 0});
+    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, w + S.s + S.N, S.M, S.N, S.trans ? 1 : 0, 0});
   }
   launch_copies(copies, cx_.copy_scratch, cx_.stream);
   const double tsvd = now();

@@ -52,19 +52,40 @@ struct GemmTile {
   int item, m0, n0, pad;
 };
 
+// Summed-K threshold above which finalize() splits an item: 512, or the
+// value of the environment variable DMRGCC_SPLIT_K (0 disables splitting).
+int default_split_k();
+
+// Split-K of one heavy output item: partial sums of chunks 1..n-1 land in
+// `part` (n-1 consecutive M x N buffers) and are added into C afterwards in
+// chunk order, so results stay deterministic.
+struct SplitTask {
+  double* C;
+  int64_t ldc;
+  int M, N;
+  int nparts, flags;
+  const double* part;
+};
+
 // A grouped GEMM launch: items with their contribution lists.  Built on the
 // host, uploaded once, launched many times (the H_eff plan reuses it for every
-// Lanczos iteration).
+// Lanczos iteration).  finalize() cuts items whose summed K exceeds
+// `split_k` into contribution chunks (split-K) so that no single workgroup
+// carries the whole launch.
 struct GemmBatch {
   std::vector<GemmItem> items;
   std::vector<GemmContrib> contribs;
   std::vector<GemmTile> tiles;
+  std::vector<SplitTask> splits;
   DevBuf<GemmItem> d_items;
   DevBuf<GemmContrib> d_contribs;
   DevBuf<GemmTile> d_tiles;
+  DevBuf<SplitTask> d_splits;
+  DevBuf<double> part;
+  int split_k = default_split_k();
   double flops = 0;  // executed real flops (2*M*N*K summed)
 
-  void clear() { items.clear(); contribs.clear(); tiles.clear(); flops = 0; }
+  void clear() { items.clear(); contribs.clear(); tiles.clear(); splits.clear(); flops = 0; }
   // Start an output item; subsequent add() calls attach terms to it.
   void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0);
   void add(const double* A, int64_t lda, const double* B, int64_t ldb, int K,
@@ -136,10 +157,14 @@ void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGather
 // is made positive, flipping the matching right singular vector with it.  The
 // left vector is column j of G (u_in_v = 0) or of V (u_in_v = 1, when G holds
 // the transposed matrix).
+// Columns whose squared norm falls below (eps |A|_F)^2 are numerically zero
+// and are not rotated (their singular values sit at the round-off floor); the
+// squared Frobenius norm is computed on the device into fro2.
 struct SvdProblem {
   double* G;
   double* V;
   double* sigma;
+  double* fro2;
   int M, N;
   int u_in_v;
   int pad;

@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "kernels.h"
 
@@ -264,6 +266,21 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
       }
 }
 
+// split-K reduction: C (+)= sum of the partial buffers, in chunk order
+__global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __restrict__ tasks,
+                                                           double* __restrict__ out_base) {
+  const SplitTask t = tasks[blockIdx.y];
+  double* C = (t.flags & kRelC) ? out_base + (intptr_t)t.C : t.C;
+  const int64_t mn = (int64_t)t.M * t.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < t.nparts; ++c) s += t.part[c * mn + e];
+    const int i = (int)(e / t.N), j = (int)(e % t.N);
+    C[(int64_t)i * t.ldc + j] += s;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // strided copies
 // ---------------------------------------------------------------------------
@@ -393,63 +410,145 @@ inline int grid_for(int64_t n) {
 // kernel however many sectors there are.  The host runs the rounds of a
 // sweep and retires sectors whose sweep made no rotation.
 // ---------------------------------------------------------------------------
-constexpr int kSvdWaves = 4;
-
 __global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __restrict__ probs) {
   const SvdProblem p = probs[blockIdx.x];
   const int64_t nn = (int64_t)p.N * p.N;
   for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) p.V[e] = (e % (p.N + 1) == 0) ? 1.0 : 0.0;
+  __shared__ double red[4];
+  double s = 0.0;
+  const int64_t mn = (int64_t)p.M * p.N;
+  for (int64_t e = threadIdx.x; e < mn; e += blockDim.x) s += p.G[e] * p.G[e];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *p.fro2 = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// pair q of the round -> problem via the prefix array of pairs per problem
-__global__ __launch_bounds__(kSvdWaves * 64) void jacobi_round_kernel(
+// Block one-sided Jacobi: the N columns are cut into blocks of kJb; one
+// launch performs one round of the block round-robin, one workgroup per block
+// pair.  The workgroup copies the (up to 2*kJb) columns into LDS, orthogonalises
+// them against each other with scalar rotations (inner sweeps until none is
+// needed), accumulates the rotations in Q, writes the columns back and applies
+// Q to the matching columns of V.  Visit q -> problem via the prefix array of
+// block pairs per problem.
+constexpr int kJb = 8;
+constexpr int kJc = 2 * kJb;
+
+template <bool kLds>
+__global__ __launch_bounds__(512) void jacobi_block_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ active,
     const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
-  const int lane = threadIdx.x & 63;
-  const int q = blockIdx.x * kSvdWaves + (threadIdx.x >> 6);
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = blockIdx.x;
   if (q >= prefix[nactive]) return;
   int lo = 0, hi = nactive - 1;
-  while (lo < hi) {  // largest k with prefix[k] <= q
+  while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (prefix[mid] <= q) lo = mid; else hi = mid - 1;
   }
   const int pi = active[lo];
   const SvdProblem p = probs[pi];
-  const int M = p.M, N = p.N, n2 = N + (N & 1);
-  if (round >= n2 - 1) return;
+  const int M = p.M, N = p.N;
+  const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
+  if (round >= nb2 - 1) return;
   const int pr = q - prefix[lo];
-  int a, b;
-  if (pr == 0) { a = round; b = n2 - 1; }
-  else { a = (round + pr) % (n2 - 1); b = (round + n2 - 1 - pr) % (n2 - 1); }
-  if (a >= N || b >= N) return;
-  if (a > b) { const int t = a; a = b; b = t; }
-  double* __restrict__ ga = p.G + (int64_t)a * M;
-  double* __restrict__ gb = p.G + (int64_t)b * M;
-  double al = 0.0, be = 0.0, gm = 0.0;
-  for (int i = lane; i < M; i += 64) {
-    const double x = ga[i], y = gb[i];
-    al += x * x; be += y * y; gm += x * y;
+  int bi, bj;
+  if (pr == 0) { bi = round; bj = nb2 - 1; }
+  else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+  // columns of the visit
+  __shared__ int col[kJc];
+  __shared__ double Q[kJc][kJc];
+  __shared__ int any_rot, sweep_rot;
+  int nc = 0;
+  for (int b2 = 0; b2 < 2; ++b2) {
+    const int blk = b2 == 0 ? bi : bj;
+    if (blk >= nb) continue;
+    for (int c = blk * kJb; c < min(N, blk * kJb + kJb); ++c) {
+      if (tid == 0) col[nc] = c;
+      ++nc;
+    }
   }
-  al = wave_sum(al); be = wave_sum(be); gm = wave_sum(gm);
-  const double tol = 2.220446049250313e-16 * sqrt((double)M);
-  if (!(al > 0.0 && be > 0.0 && fabs(gm) > tol * sqrt(al) * sqrt(be))) return;
-  const double zeta = (be - al) / (2.0 * gm);
-  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
-  const double cs = 1.0 / sqrt(1.0 + t * t);
-  const double sn = cs * t;
-  for (int i = lane; i < M; i += 64) {
-    const double x = ga[i], y = gb[i];
-    ga[i] = cs * x - sn * y;
-    gb[i] = sn * x + cs * y;
+  if (tid < kJc * kJc) Q[tid / kJc][tid % kJc] = (tid / kJc == tid % kJc) ? 1.0 : 0.0;
+  if (tid == 0) any_rot = 0;
+  __syncthreads();
+  // column storage: LDS copy or the columns in place
+  double* cbuf[kJc];
+  for (int k = 0; k < nc; ++k) cbuf[k] = kLds ? smem + (int64_t)k * M : p.G + (int64_t)col[k] * M;
+  if (kLds) {
+    for (int k = 0; k < nc; ++k) {
+      const double* src = p.G + (int64_t)col[k] * M;
+      for (int i = tid; i < M; i += blockDim.x) cbuf[k][i] = src[i];
+    }
+    __syncthreads();
   }
-  double* __restrict__ va = p.V + (int64_t)a * N;
-  double* __restrict__ vb = p.V + (int64_t)b * N;
-  for (int i = lane; i < N; i += 64) {
-    const double x = va[i], y = vb[i];
-    va[i] = cs * x - sn * y;
-    vb[i] = sn * x + cs * y;
+  const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
+  const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
+  const int nc2 = nc + (nc & 1);
+  // one inner sweep per visit (the outer sweeps repeat the visit)
+  for (int sweep = 0; sweep < 1 && nc > 1; ++sweep) {
+    if (tid == 0) sweep_rot = 0;
+    __syncthreads();
+    for (int r = 0; r < nc2 - 1; ++r) {
+      for (int pp = wave; pp < nc2 / 2; pp += (int)(blockDim.x >> 6)) {
+        int a, b;
+        if (pp == 0) { a = r; b = nc2 - 1; }
+        else { a = (r + pp) % (nc2 - 1); b = (r + nc2 - 1 - pp) % (nc2 - 1); }
+        if (a >= nc || b >= nc) continue;
+        if (a > b) { const int t = a; a = b; b = t; }
+        double* ga = cbuf[a];
+        double* gb = cbuf[b];
+        double al = 0.0, be = 0.0, gm = 0.0;
+        for (int i = lane; i < M; i += 64) {
+          const double x = ga[i], y = gb[i];
+          al += x * x; be += y * y; gm += x * y;
+        }
+        al = wave_sum(al); be = wave_sum(be); gm = wave_sum(gm);
+        if (al > zero2 && be > zero2 && fabs(gm) > tol * sqrt(al) * sqrt(be)) {
+          const double zeta = (be - al) / (2.0 * gm);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+          const double cs = 1.0 / sqrt(1.0 + t * t);
+          const double sn = cs * t;
+          for (int i = lane; i < M; i += 64) {
+            const double x = ga[i], y = gb[i];
+            ga[i] = cs * x - sn * y;
+            gb[i] = sn * x + cs * y;
+          }
+          if (lane < nc) {
+            const double x = Q[lane][a], y = Q[lane][b];
+            Q[lane][a] = cs * x - sn * y;
+            Q[lane][b] = sn * x + cs * y;
+          }
+          if (lane == 0) sweep_rot = 1;
+        }
+      }
+      __syncthreads();
+    }
+    const int any = sweep_rot;
+    __syncthreads();
+    if (!any) break;
+    if (tid == 0) any_rot = 1;
   }
-  if (lane == 0) rotated[pi] = 1;
+  __syncthreads();
+  if (kLds) {
+    for (int k = 0; k < nc; ++k) {
+      double* dst = p.G + (int64_t)col[k] * M;
+      for (int i = tid; i < M; i += blockDim.x) dst[i] = cbuf[k][i];
+    }
+  }
+  if (any_rot) {
+    // V[:, col] <- V[:, col] * Q, row by row
+    for (int i = tid; i < N; i += blockDim.x) {
+      double v[kJc];
+      for (int k = 0; k < nc; ++k) v[k] = p.V[(int64_t)col[k] * N + i];
+      for (int k = 0; k < nc; ++k) {
+        double s = 0.0;
+        for (int m = 0; m < nc; ++m) s += v[m] * Q[m][k];
+        p.V[(int64_t)col[k] * N + i] = s;
+      }
+    }
+    if (tid == 0) rotated[pi] = 1;
+  }
 }
 
 // singular values and the sign convention, one workgroup per problem
@@ -513,8 +612,65 @@ void GemmBatch::add(const double* A, int64_t lda, const double* B, int64_t ldb, 
   flops += 2.0 * items.back().M * (double)items.back().N * K;
 }
 
+int default_split_k() {
+  const char* e = std::getenv("DMRGCC_SPLIT_K");
+  return e ? std::max(0, std::atoi(e)) : 512;
+}
+
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   tiles.clear();
+  splits.clear();
+  // split-K: chunk the contribution lists of heavy items
+  if (split_k > 0) {
+    std::vector<GemmItem> extra;
+    int64_t psize = 0;
+    struct Pending { size_t item; int first_extra, n; };
+    std::vector<Pending> pend;
+    const size_t nitems = items.size();
+    for (size_t i = 0; i < nitems; ++i) {
+      GemmItem& it = items[i];
+      int64_t ktot = 0;
+      for (int c = it.c0; c < it.c1; ++c) ktot += contribs[c].K;
+      if (ktot <= split_k || it.c1 - it.c0 < 2) continue;
+      // chunk boundaries by accumulated K
+      std::vector<int> cuts{it.c0};
+      int64_t acc = 0;
+      for (int c = it.c0; c < it.c1; ++c) {
+        if (acc >= split_k && c > cuts.back()) { cuts.push_back(c); acc = 0; }
+        acc += contribs[c].K;
+      }
+      cuts.push_back(it.c1);
+      const int nch = (int)cuts.size() - 1;
+      if (nch < 2) continue;
+      Pending pd{i, (int)extra.size(), nch - 1};
+      for (int ch = 1; ch < nch; ++ch) {
+        GemmItem e = it;
+        e.c0 = cuts[ch];
+        e.c1 = cuts[ch + 1];
+        e.accumulate = 0;
+        e.flags = 0;
+        e.C = reinterpret_cast<double*>((intptr_t)psize);  // fixed up after allocation
+        e.ldc = it.N;
+        psize += (int64_t)it.M * it.N;
+        extra.push_back(e);
+      }
+      it.c1 = cuts[1];
+      pend.push_back(pd);
+    }
+    if (!extra.empty()) {
+      part.ensure((size_t)psize);
+      const size_t base = items.size();
+      for (GemmItem& e : extra) {
+        e.C = part.data() + (intptr_t)e.C;
+        items.push_back(e);
+      }
+      for (const Pending& pd : pend) {
+        const GemmItem& it = items[pd.item];
+        splits.push_back(SplitTask{it.C, it.ldc, it.M, it.N, pd.n, it.flags,
+                                   items[base + pd.first_extra].C});
+      }
+    }
+  }
   // Largest work first so the long items start early.
   std::vector<int> order(items.size());
   for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
@@ -535,6 +691,7 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   upload(d_items, items, stream);
   upload(d_contribs, contribs, stream);
   upload(d_tiles, tiles, stream);
+  if (!splits.empty()) upload(d_splits, splits, stream);
 }
 
 void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out) const {
@@ -552,6 +709,14 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   else
     throw Error(kErrArgument, "unknown precision");
   DMRG_HIP(hipGetLastError());
+  if (!splits.empty()) {
+    int64_t mx = 0;
+    for (const SplitTask& t : splits) mx = std::max<int64_t>(mx, (int64_t)t.M * t.N);
+    const unsigned gx = (unsigned)std::min<int64_t>(64, (mx + 255) / 256);
+    hipLaunchKernelGGL(split_reduce_kernel, dim3(gx, (unsigned)splits.size()), dim3(256), 0, stream,
+                       d_splits.data(), out);
+    DMRG_HIP(hipGetLastError());
+  }
 }
 
 void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
@@ -642,11 +807,18 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
   if (probs.empty()) return;
   const int np = (int)probs.size();
-  for (const SvdProblem& p : probs)
+  int maxM = 1;
+  for (const SvdProblem& p : probs) {
     require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
+    maxM = std::max(maxM, p.M);
+  }
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
+  // LDS holds the visit's columns when they fit (64 KiB budget keeps two
+  // workgroups per CU); otherwise the columns are rotated in place.
+  const size_t lds = (size_t)kJc * maxM * sizeof(double);
+  const bool use_lds = lds <= 150 * 1024;
   // device ints: rotated[np] | active[np] | prefix[np + 1]
   sweeps_out.ensure((size_t)3 * np + 1);
   int* rotated = sweeps_out.data();
@@ -655,16 +827,17 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
   std::vector<int> active;
   for (int i = 0; i < np; ++i)
     if (probs[i].N > 1) active.push_back(i);
-  std::vector<int> hrot(np), meta;
+  std::vector<int> hrot(np);
+  int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
-    meta.assign(active.begin(), active.end());
-    int pairs = 0, rounds = 0;
+    ++sweeps_done;
+    int visits = 0, rounds = 0;
     std::vector<int> prefix(1, 0);
     for (int i : active) {
-      const int n2 = probs[i].N + (probs[i].N & 1);
-      pairs += n2 / 2;
-      prefix.push_back(pairs);
-      rounds = std::max(rounds, n2 - 1);
+      const int nb = (probs[i].N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
+      visits += nb2 / 2;
+      prefix.push_back(visits);
+      rounds = std::max(rounds, nb2 - 1);
     }
     const int na = (int)active.size();
     std::vector<int> h(np, 0);
@@ -672,10 +845,14 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
     h.resize(2 * np, 0);
     h.insert(h.end(), prefix.begin(), prefix.end());
     DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    const int grid = (pairs + kSvdWaves - 1) / kSvdWaves;
-    for (int r = 0; r < rounds; ++r)
-      hipLaunchKernelGGL(jacobi_round_kernel, dim3(grid), dim3(kSvdWaves * 64), 0, stream, scratch.data(),
-                         dactive, dprefix, na, r, rotated);
+    for (int r = 0; r < rounds; ++r) {
+      if (use_lds)
+        hipLaunchKernelGGL(jacobi_block_round_kernel<true>, dim3(visits), dim3(512), lds, stream,
+                           scratch.data(), dactive, dprefix, na, r, rotated);
+      else
+        hipLaunchKernelGGL(jacobi_block_round_kernel<false>, dim3(visits), dim3(512), 0, stream,
+                           scratch.data(), dactive, dprefix, na, r, rotated);
+    }
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
@@ -686,6 +863,14 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
   }
   hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
+  if (std::getenv("DMRG_SVD_STATS")) {
+    int maxN = 0; int64_t mn2 = 0;
+    for (const SvdProblem& p : probs) { maxN = std::max(maxN, p.N); mn2 += (int64_t)p.M * p.N * p.N; }
+    std::printf("SVDSTAT np=%d maxM=%d maxN=%d sweeps=%d MN2=%lld |", np, maxM, maxN, sweeps_done,
+                (long long)mn2);
+    for (const SvdProblem& p : probs) if ((int64_t)p.M * p.N > 20000) std::printf(" %dx%d", p.M, p.N);
+    std::printf("\n");
+  }
 }
 
 }  // namespace dmrg

@@ -59,6 +59,24 @@ def test_heff_fp64_matches_oracle(name):
     assert info["flops_ref"] == heff_flops(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site])
 
 
+@pytest.mark.parametrize("split_k", [0, 1, 40])
+@pytest.mark.parametrize("prec", [dm.FP64, dm.BF16X3])
+def test_heff_split_k(monkeypatch, split_k, prec):
+    # split-K cuts heavy output blocks into contribution chunks that are summed
+    # by a second kernel; any threshold must give the unsplit result
+    dmo, site = state("4x3_d4_left")
+    monkeypatch.setenv("DMRGCC_SPLIT_K", str(split_k))
+    plan, lat = make_plan(dmo, site)
+    n = lat.ket.dim()
+    x = np.random.default_rng(6).standard_normal(n)
+    y0 = np.random.default_rng(7).standard_normal(n)
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, y0.copy())
+    yd = torch.tensor(y0, device="cuda")
+    plan.apply(torch.tensor(x, device="cuda"), yd, prec)
+    err = np.abs(yd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < (1e-12 if prec == dm.FP64 else 2e-4), err
+
+
 @pytest.mark.parametrize("prec,tol", [(dm.FP32, 1e-5), (dm.BF16X3, 2e-4)])
 def test_heff_mixed_within_tolerance(prec, tol):
     dmo, site = state("4x3_d4_left")
