@@ -72,7 +72,7 @@ _SIGS = {
                                          _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
     "dmrgcc_sector_svd": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, _c_int_p,
                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp),
-                                         ctypes.POINTER(_vp), _vp]),
+                                         ctypes.POINTER(_vp), ctypes.c_int, _vp]),
     "dmrgcc_dmrg_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
                                           _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -462,9 +462,12 @@ class EnvResult:
             self._h = None
 
 
-def sector_svd(mats, stream=None):
+def sector_svd(mats, stream=None, precondition=0):
     """Batched thin SVD of device float64 matrices.  Returns lists (U, S, Vt)
-    of device tensors with S descending."""
+    of device tensors with S descending.  precondition: 0 plain Jacobi; 1 QR
+    first and Jacobi on R^T (long-side factor exactly orthonormal, short-side
+    one only for singular values well above eps*|A|); 2 two QRs (short-side
+    factor exactly orthonormal)."""
     import torch
     n = len(mats)
     U = [torch.empty((m.shape[0], min(m.shape)), dtype=torch.float64, device=m.device) for m in mats]
@@ -475,5 +478,5 @@ def sector_svd(mats, stream=None):
     N = _i32([m.shape[1] for m in mats])
     _sync_torch()
     _check(lib().dmrgcc_sector_svd(n, arr(mats), _ip(M), _ip(N), arr(U), arr(S), arr(Vt),
-                                   _vp(stream) if stream else _vp()))
+                                   int(precondition), _vp(stream) if stream else _vp()))
     return U, S, Vt

@@ -404,9 +404,10 @@ int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl,
 }
 
 int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N, double* const* U,
-                      double* const* S, double* const* Vt, void* stream) {
+                      double* const* S, double* const* Vt, int precondition, void* stream) {
   return guarded([&] {
     require(n >= 0 && (n == 0 || (A && M && N && U && S && Vt)), "bad arguments", kErrArgument);
+    require(precondition >= 0 && precondition <= 2, "precondition must be 0, 1 or 2", kErrArgument);
     Context cx(static_cast<hipStream_t>(stream));
     struct P { int M, N, m, k; bool trans; int64_t g, v, s; };
     std::vector<P> ps(n);
@@ -432,7 +433,8 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
       else { c.dsr = p.m; c.dsc = 1; }           // G = A^T
       cps.push_back(c);
       probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s,
-                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0, 0});
+                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0,
+                                 precondition});
     }
     launch_copies(cps, cx.copy_scratch, cx.stream);
     jacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);

@@ -741,7 +741,11 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
       }
       copies.push_back(c);
     }
-    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, w + S.s + S.N, S.M, S.N, S.trans ? 1 : 0, 0});
+    // QR preconditioning that keeps exact the factor read as orthonormal:
+    // G (left canonical of a tall sector, right canonical of a wide one) or V
+    const int precond = (left && !S.trans) || (!left && S.trans) ? 1 : 2;
+    probs.push_back(SvdProblem{w + S.g, w + S.v, w + S.s, w + S.s + S.N, S.M, S.N, S.trans ? 1 : 0,
+                               precond});
   }
   launch_copies(copies, cx_.copy_scratch, cx_.stream);
   const double tsvd = now();

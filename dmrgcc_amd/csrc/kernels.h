@@ -84,16 +84,26 @@ struct GemmBatch {
   DevBuf<double> part;
   int split_k = default_split_k();
   double flops = 0;  // executed real flops (2*M*N*K summed)
+  // Groups: end_group() closes the items added since the previous call into
+  // one group that launch_group() runs on its own (a chain of dependent
+  // GEMMs described by one upload).  Batches that use groups are not split.
+  std::vector<int> group_items;  // item index where each group ends
+  std::vector<int> group_tiles;  // tile index where each group ends (after finalize)
 
-  void clear() { items.clear(); contribs.clear(); tiles.clear(); splits.clear(); flops = 0; }
+  void clear() {
+    items.clear(); contribs.clear(); tiles.clear(); splits.clear();
+    group_items.clear(); group_tiles.clear(); flops = 0;
+  }
   // Start an output item; subsequent add() calls attach terms to it.
   void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0);
   void add(const double* A, int64_t lda, const double* B, int64_t ldb, int K,
            int flags, double scale);
+  int end_group() { group_items.push_back((int)items.size()); return (int)group_items.size() - 1; }
   // Finish construction: tile every item and upload the descriptors.
   void finalize(hipStream_t stream, int tile_m = 64, int tile_n = 64);
   void launch(hipStream_t stream, int precision, const double* in = nullptr,
               double* out = nullptr) const;
+  void launch_group(hipStream_t stream, int group, int precision = 0) const;
   bool empty() const { return tiles.empty(); }
 };
 
@@ -160,6 +170,17 @@ void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGather
 // Columns whose squared norm falls below (eps |A|_F)^2 are numerically zero
 // and are not rotated (their singular values sit at the round-off floor); the
 // squared Frobenius norm is computed on the device into fro2.
+//
+// precond asks for QR preconditioning when N is large enough (both forms
+// return G = U*Sigma and V as above, with about a third of the sweeps):
+//   1: G = Q R (blocked Householder), Jacobi on R^T = W J^T, G <- Q J and
+//      V <- W / sigma.  U = Q J is orthonormal to working precision for every
+//      singular value; V only for singular values well above eps |A| (zero
+//      sigma gives a zero column).  For callers that take the orthonormal
+//      factor from G.
+//   2: second QR R^T = Q2 R2, Jacobi on R2^T = W J^T, V <- Q2 J and
+//      G <- Q [W; 0].  V is orthonormal to working precision (as in the plain
+//      Jacobi); for callers that take the orthonormal factor from V.
 struct SvdProblem {
   double* G;
   double* V;
@@ -167,7 +188,7 @@ struct SvdProblem {
   double* fro2;
   int M, N;
   int u_in_v;
-  int pad;
+  int precond;
 };
 void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 60);

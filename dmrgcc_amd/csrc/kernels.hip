@@ -554,6 +554,7 @@ __global__ __launch_bounds__(512) void jacobi_block_round_kernel(
 // singular values and the sign convention, one workgroup per problem
 __global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __restrict__ probs) {
   const SvdProblem p = probs[blockIdx.x];
+  if (p.precond) return;  // finished by qr_finish_kernel
   const int M = p.M, N = p.N;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   for (int j = wave; j < N; j += nwaves) {
@@ -583,6 +584,214 @@ __global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __
       for (int i = lane; i < M; i += 64) g[i] = -g[i];
       for (int i = lane; i < N; i += 64) v[i] = -v[i];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// QR preconditioning of the sector SVD: blocked Householder QR A = Q R
+// (panels of b columns, compact WY form Q_k = I - Y_k T_k Y_k^T), X = R^T,
+// and later C <- Q [S; 0] (back-transformation of a seed S).
+// Per job, panel k covers columns c0 = k*b .. c0+bw-1 and rows c0 .. M-1
+// (m = M - c0 of them); Y_k and P_k = Y_k T_k are m x bw column-major (ld m)
+// at offset k*b*M of the job's Y and P buffers.
+// ---------------------------------------------------------------------------
+struct QrJob {
+  double* A;        // M x N column-major (ld M), factorised in place
+  double* Y;        // Householder vectors, panel k at Y + k*b*M
+  double* P;        // Y_k T_k, same layout
+  double* X;        // N x N column-major: R^T
+  double* C;        // M x N (ld M): target of the back-transformation
+  const double* S;  // N x N seed, C = [S; 0] first (null: C already holds it)
+  double* W;        // finish: N x N rotated R^T (sigma_j = |W e_j|)
+  double* V;        // finish: N x N output W / sigma
+  double* sigma;
+  int M, N;
+  int u_in_v, finish;
+};
+
+// Factorise panel k of every problem (one workgroup each).  kLds: the panel is
+// staged in LDS (dynamic, m*b + b*b + b doubles); otherwise it is worked on in
+// place in global memory and LDS holds only T and tau (b*b + b doubles).
+template <bool kLds>
+__global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__ probs, int k,
+                                                       int b) {
+  extern __shared__ double smem[];
+  __shared__ double red[8];
+  __shared__ double bcast[2];
+  const QrJob p = probs[blockIdx.x];
+  const int M = p.M, N = p.N;
+  const int c0 = k * b;
+  if (c0 >= N) return;
+  const int bw = min(b, N - c0);
+  const int m = M - c0;  // >= bw because M >= N
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
+  double* Ag = p.A + c0 + (int64_t)c0 * M;  // panel origin, ld M
+  double* pan;
+  int64_t ld;
+  double* T;
+  if (kLds) {
+    pan = smem;
+    ld = m;
+    T = smem + (int64_t)m * bw;
+    for (int64_t e = tid; e < (int64_t)m * bw; e += blockDim.x) {
+      const int j = (int)(e / m), i = (int)(e % m);
+      pan[e] = Ag[i + (int64_t)j * M];
+    }
+  } else {
+    pan = Ag;
+    ld = M;
+    T = smem;
+  }
+  double* tau = T + b * b;
+  __syncthreads();
+  for (int j = 0; j < bw; ++j) {
+    double* x = pan + (int64_t)j * ld + j;  // x[0 .. len-1], len = m - j
+    const int len = m - j;
+    double ss = 0.0;
+    for (int i = 1 + tid; i < len; i += blockDim.x) ss += x[i] * x[i];
+    ss = wave_sum(ss);
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += red[w];
+      const double alpha = x[0];
+      double tj = 0.0, beta = alpha, scal = 0.0;
+      if (t > 0.0) {  // reflector H = I - tau v v^T with v = (1, x[1:] / (alpha - beta))
+        beta = -copysign(sqrt(alpha * alpha + t), alpha);
+        tj = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      tau[j] = tj;
+      bcast[0] = scal;
+      bcast[1] = beta;
+    }
+    __syncthreads();
+    const double tj = tau[j];
+    if (tj != 0.0) {
+      const double scal = bcast[0];
+      for (int i = 1 + tid; i < len; i += blockDim.x) x[i] *= scal;
+    }
+    __syncthreads();
+    if (tid == 0) x[0] = bcast[1];
+    if (tj != 0.0) {
+      // remaining panel columns: y -= tau v (v^T y), one wave per column
+      for (int c = j + 1 + wave; c < bw; c += nw) {
+        double* y = pan + (int64_t)c * ld + j;
+        double d = 0.0;
+        for (int i = 1 + lane; i < len; i += 64) d += x[i] * y[i];
+        d = (wave_sum(d) + y[0]) * tj;
+        for (int i = 1 + lane; i < len; i += 64) y[i] -= d * x[i];
+        if (lane == 0) y[0] -= d;
+      }
+    }
+    __syncthreads();
+  }
+  // S = Y^T Y (strictly upper part) into the strictly lower triangle of T:
+  // y_l^T y_j = v_l[j] + sum_{i > j} v_l[i] v_j[i]  (l < j)
+  for (int pr = wave; pr < bw * bw; pr += nw) {
+    const int l = pr / bw, j = pr % bw;
+    if (l >= j) continue;
+    double d = 0.0;
+    for (int i = j + 1 + lane; i < m; i += 64) d += pan[(int64_t)l * ld + i] * pan[(int64_t)j * ld + i];
+    d = wave_sum(d);
+    if (lane == 0) T[j * b + l] = d + pan[(int64_t)l * ld + j];
+  }
+  __syncthreads();
+  // T upper triangular (dlarft): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
+  for (int j = 0; j < bw; ++j) {
+    if (tid < j) {
+      double s = 0.0;
+      for (int l = tid; l < j; ++l) s += T[tid * b + l] * T[j * b + l];
+      T[tid * b + j] = -tau[j] * s;
+    } else if (tid == j) {
+      T[j * b + j] = tau[j];
+    }
+    __syncthreads();
+  }
+  // Y_k (unit lower trapezoidal) and P_k = Y_k T_k
+  double* Yk = p.Y + (int64_t)k * b * M;
+  double* Pk = p.P + (int64_t)k * b * M;
+  for (int64_t e = tid; e < (int64_t)m * bw; e += blockDim.x) {
+    const int j = (int)(e / m), i = (int)(e % m);
+    double s = 0.0;
+    const int lmax = min(j, i);
+    for (int l = 0; l <= lmax; ++l) {
+      const double yil = (i == l) ? 1.0 : pan[(int64_t)l * ld + i];
+      s += yil * T[l * b + j];
+    }
+    Yk[e] = i < j ? 0.0 : (i == j ? 1.0 : pan[(int64_t)j * ld + i]);
+    Pk[e] = s;
+  }
+  if (kLds) {  // R block of the panel back to A (upper triangle of its top bw rows)
+    for (int64_t e = tid; e < (int64_t)bw * bw; e += blockDim.x) {
+      const int j = (int)(e / bw), i = (int)(e % bw);
+      if (i <= j) Ag[i + (int64_t)j * M] = pan[(int64_t)j * ld + i];
+    }
+  }
+}
+
+// X = R^T (N x N column-major) from the upper triangle of the factorised A
+__global__ __launch_bounds__(256) void qr_rt_kernel(const QrJob* __restrict__ probs) {
+  const QrJob p = probs[blockIdx.y];
+  const int N = p.N, M = p.M;
+  const int64_t nn = (int64_t)N * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / N), j = (int)(e % N);  // X[j + i N] = R[i][j]
+    p.X[e] = j >= i ? p.A[i + (int64_t)j * M] : 0.0;
+  }
+}
+
+// C = [S; 0] (M x N) ahead of the back-transformation C <- Q C
+__global__ __launch_bounds__(256) void qr_seed_kernel(const QrJob* __restrict__ probs) {
+  const QrJob p = probs[blockIdx.y];
+  if (!p.S) return;
+  const int N = p.N, M = p.M;
+  const int64_t mn = (int64_t)M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / M), i = (int)(e % M);
+    p.C[e] = i < N ? p.S[i + (int64_t)j * N] : 0.0;
+  }
+}
+
+// Jobs with finish set (C = Q J, the left factor, exactly orthonormal):
+// sigma_j = |W e_j|, sign convention on the left vector (column j of C, or of
+// W when the problem holds the transposed matrix), then C_j <- sigma_j C_j and
+// V_j <- W_j / sigma_j (zero when sigma_j = 0).  One wave per column.
+__global__ __launch_bounds__(256) void qr_finish_kernel(const QrJob* __restrict__ probs) {
+  const QrJob p = probs[blockIdx.x];
+  if (!p.finish) return;
+  const int M = p.M, N = p.N;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int j = wave; j < N; j += nwaves) {
+    double* a = p.C + (int64_t)j * M;
+    double* w = p.W + (int64_t)j * N;
+    double* v = p.V + (int64_t)j * N;
+    double s = 0.0;
+    for (int i = lane; i < N; i += 64) s += w[i] * w[i];
+    s = sqrt(wave_sum(s));
+    if (lane == 0) p.sigma[j] = s;
+    const double* u = p.u_in_v ? w : a;
+    const int n = p.u_in_v ? N : M;
+    double best = -1.0, val = 0.0;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < n; i += 64) {
+      const double x = fabs(u[i]);
+      if (x > best) { best = x; bi = i; val = u[i]; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      const double ov = __shfl_xor(val, o, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; val = ov; }
+    }
+    const double sg = val < 0.0 ? -1.0 : 1.0;
+    const double sa = sg * s, sv = s > 0.0 ? sg / s : 0.0;
+    for (int i = lane; i < M; i += 64) a[i] *= sa;
+    for (int i = lane; i < N; i += 64) v[i] = w[i] * sv;
   }
 }
 
@@ -620,8 +829,10 @@ int default_split_k() {
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   tiles.clear();
   splits.clear();
+  group_tiles.clear();
+  if (!group_items.empty() && group_items.back() != (int)items.size()) end_group();
   // split-K: chunk the contribution lists of heavy items
-  if (split_k > 0) {
+  if (split_k > 0 && group_items.empty()) {
     std::vector<GemmItem> extra;
     int64_t psize = 0;
     struct Pending { size_t item; int first_extra, n; };
@@ -671,22 +882,29 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
       }
     }
   }
-  // Largest work first so the long items start early.
-  std::vector<int> order(items.size());
-  for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
+  // Largest work first (within each group) so the long items start early.
   std::vector<double> cost(items.size(), 0.0);
   for (size_t i = 0; i < items.size(); ++i) {
     double k = 0;
     for (int c = items[i].c0; c < items[i].c1; ++c) k += contribs[c].K;
     cost[i] = k;
   }
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  for (int i : order) {
-    const GemmItem& it = items[i];
-    if (it.M <= 0 || it.N <= 0) continue;
-    if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
-    for (int m0 = 0; m0 < it.M; m0 += tile_m)
-      for (int n0 = 0; n0 < it.N; n0 += tile_n) tiles.push_back(GemmTile{i, m0, n0, 0});
+  std::vector<int> ends = group_items;
+  if (ends.empty()) ends.push_back((int)items.size());
+  int first = 0;
+  for (int end : ends) {
+    std::vector<int> order;
+    for (int i = first; i < end; ++i) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    for (int i : order) {
+      const GemmItem& it = items[i];
+      if (it.M <= 0 || it.N <= 0) continue;
+      if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
+      for (int m0 = 0; m0 < it.M; m0 += tile_m)
+        for (int n0 = 0; n0 < it.N; n0 += tile_n) tiles.push_back(GemmTile{i, m0, n0, 0});
+    }
+    group_tiles.push_back((int)tiles.size());
+    first = end;
   }
   upload(d_items, items, stream);
   upload(d_contribs, contribs, stream);
@@ -717,6 +935,24 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
                        d_splits.data(), out);
     DMRG_HIP(hipGetLastError());
   }
+}
+
+void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const {
+  require(group >= 0 && group < (int)group_tiles.size(), "GemmBatch: no such group", kErrInternal);
+  const int t0 = group == 0 ? 0 : group_tiles[group - 1], t1 = group_tiles[group];
+  if (t1 <= t0) return;
+  const dim3 grid((unsigned)(t1 - t0)), block(kThreads);
+  const GemmTile* tl = d_tiles.data() + t0;
+  if (precision == kFP64)
+    hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, tl, d_items.data(),
+                       d_contribs.data(), nullptr, nullptr);
+  else if (precision == kFP32)
+    hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, tl, d_items.data(),
+                       d_contribs.data(), nullptr, nullptr);
+  else
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, tl, d_items.data(),
+                       d_contribs.data(), nullptr, nullptr);
+  DMRG_HIP(hipGetLastError());
 }
 
 void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
@@ -803,20 +1039,203 @@ void fill(double* x, int64_t n, double v, hipStream_t stream) {
   DMRG_HIP(hipGetLastError());
 }
 
-void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
-                DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
-  if (probs.empty()) return;
-  const int np = (int)probs.size();
-  int maxM = 1;
-  for (const SvdProblem& p : probs) {
-    require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
-    maxM = std::max(maxM, p.M);
+namespace {
+constexpr int kQrMinN = 48;  // smaller problems are not worth the QR launches
+
+// One batched blocked-Householder QR over a set of jobs.  The GEMM batch
+// holds, per panel k, the trailing update (groups 2k, 2k+1) and, in reverse
+// panel order, the back-transformation C <- Q_k C (groups 2*npan + ...).
+struct QrPhase {
+  std::vector<QrJob> jobs;
+  DevBuf<QrJob> d_jobs;
+  DevBuf<double> work;
+  GemmBatch gemm;
+  int b = 32, npan = 0, maxN = 0, maxM = 0;
+  bool lds = true;
+  size_t lds_bytes = 0;
+
+  // jobs: A, C, S, W, V, sigma, M, N, u_in_v, finish set by the caller; the
+  // plan adds the X, Y, P buffers.
+  void plan(std::vector<QrJob> in, hipStream_t stream) {
+    jobs = std::move(in);
+    maxM = maxN = 0;
+    for (const QrJob& j : jobs) { maxM = std::max(maxM, j.M); maxN = std::max(maxN, j.N); }
+    // widest panel whose LDS copy fits (m*b + b*b + b doubles plus the static bytes)
+    const size_t budget = 160 * 1024 - 256;
+    lds = false;
+    for (int bb : {32, 16, 8}) {
+      const size_t need = ((size_t)maxM * bb + (size_t)bb * bb + bb) * sizeof(double);
+      if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
+    }
+    if (!lds) { b = 32; lds_bytes = ((size_t)b * b + b) * sizeof(double); }
+    npan = (maxN + b - 1) / b;
+    int64_t total = 0;
+    std::vector<int64_t> off;
+    for (const QrJob& j : jobs) {
+      const int np = (j.N + b - 1) / b;
+      off.push_back(total);
+      total += (int64_t)j.N * j.N + 2 * (int64_t)np * b * j.M + (int64_t)b * j.N;
+    }
+    work.ensure((size_t)std::max<int64_t>(total, 1));
+    std::vector<double*> Z;
+    for (size_t t = 0; t < jobs.size(); ++t) {
+      QrJob& r = jobs[t];
+      const int np = (r.N + b - 1) / b;
+      r.X = work.data() + off[t];
+      r.Y = r.X + (int64_t)r.N * r.N;
+      r.P = r.Y + (int64_t)np * b * r.M;
+      Z.push_back(r.P + (int64_t)np * b * r.M);
+    }
+    GemmBatch& g = gemm;
+    g.clear();
+    g.split_k = 0;
+    // factorisation: Z^T = A_trail^T P_k ; A_trail^T -= Z^T Y_k^T  (row-major
+    // views of the column-major blocks)
+    for (int k = 0; k < npan; ++k) {
+      for (int half = 0; half < 2; ++half) {
+        for (size_t t = 0; t < jobs.size(); ++t) {
+          const QrJob& r = jobs[t];
+          const int c0 = k * b;
+          if (c0 >= r.N) continue;
+          const int bw = std::min(b, r.N - c0), m = r.M - c0, nt = r.N - c0 - bw;
+          if (nt <= 0) continue;
+          double* At = r.A + c0 + (int64_t)(c0 + bw) * r.M;
+          if (half == 0) {
+            g.begin_item(Z[t], bw, nt, bw, 0);
+            g.add(At, r.M, r.P + (int64_t)k * b * r.M, m, m, kTransB, 1.0);
+          } else {
+            g.begin_item(At, r.M, nt, m, 1);
+            g.add(Z[t], bw, r.Y + (int64_t)k * b * r.M, m, bw, 0, -1.0);
+          }
+        }
+        g.end_group();
+      }
+    }
+    // back-transformation, last panel first: Z^T = C^T Y_k ; C^T -= Z^T P_k^T
+    for (int k = npan - 1; k >= 0; --k) {
+      for (int half = 0; half < 2; ++half) {
+        for (size_t t = 0; t < jobs.size(); ++t) {
+          const QrJob& r = jobs[t];
+          const int c0 = k * b;
+          if (c0 >= r.N) continue;
+          const int bw = std::min(b, r.N - c0), m = r.M - c0;
+          if (half == 0) {
+            g.begin_item(Z[t], bw, r.N, bw, 0);
+            g.add(r.C + c0, r.M, r.Y + (int64_t)k * b * r.M, m, m, kTransB, 1.0);
+          } else {
+            g.begin_item(r.C + c0, r.M, r.N, m, 1);
+            g.add(Z[t], bw, r.P + (int64_t)k * b * r.M, m, bw, 0, -1.0);
+          }
+        }
+        g.end_group();
+      }
+    }
+    g.finalize(stream);
+    upload(d_jobs, jobs, stream);
   }
+
+  // A = Q R in place, X = R^T
+  void factor(hipStream_t stream) const {
+    const unsigned nj = (unsigned)jobs.size();
+    for (int k = 0; k < npan; ++k) {
+      if (lds)
+        hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(nj), dim3(512), lds_bytes, stream,
+                           d_jobs.data(), k, b);
+      else
+        hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(nj), dim3(512), lds_bytes, stream,
+                           d_jobs.data(), k, b);
+      DMRG_HIP(hipGetLastError());
+      gemm.launch_group(stream, 2 * k, kFP64);
+      gemm.launch_group(stream, 2 * k + 1, kFP64);
+    }
+    const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxN * maxN + 255) / 256);
+    hipLaunchKernelGGL(qr_rt_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
+    DMRG_HIP(hipGetLastError());
+  }
+
+  // C = Q [S; 0] (or Q C when S is null), then the finish of flagged jobs
+  void back(hipStream_t stream) const {
+    const unsigned nj = (unsigned)jobs.size();
+    const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxM * maxN + 255) / 256);
+    hipLaunchKernelGGL(qr_seed_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
+    DMRG_HIP(hipGetLastError());
+    for (int k = npan - 1, g = 2 * npan; k >= 0; --k, g += 2) {
+      gemm.launch_group(stream, g, kFP64);
+      gemm.launch_group(stream, g + 1, kFP64);
+    }
+    hipLaunchKernelGGL(qr_finish_kernel, dim3(nj), dim3(256), 0, stream, d_jobs.data());
+    DMRG_HIP(hipGetLastError());
+  }
+};
+}  // namespace
+
+void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scratch,
+                DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
+  if (probs_in.empty()) return;
+  const int np = (int)probs_in.size();
+  // Jacobi view of every problem.  precond 1: Jacobi on R^T of A = Q R, then
+  // G <- Q J (exact) and V <- W / sigma.  precond 2: second QR R^T = Q2 R2,
+  // Jacobi on R2^T = W J^T, then V <- Q2 J (exact) and G <- Q [W; 0].
+  std::vector<SvdProblem> probs = probs_in;
+  std::vector<int> mode(np, 0), p1, p2;
+  for (int i = 0; i < np; ++i) {
+    const SvdProblem& p = probs_in[i];
+    require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
+    if (p.N >= kQrMinN && (p.precond == 1 || p.precond == 2)) mode[i] = p.precond;
+    if (mode[i]) p1.push_back(i);
+    if (mode[i] == 2) p2.push_back(i);
+  }
+  QrPhase ph1, ph2;
+  std::vector<int> slot1(np, -1);
+  if (!p1.empty()) {
+    std::vector<QrJob> jobs;
+    for (int i : p1) {
+      const SvdProblem& p = probs_in[i];
+      QrJob j{};
+      j.A = p.G; j.C = p.G; j.V = p.V; j.sigma = p.sigma;
+      j.M = p.M; j.N = p.N; j.u_in_v = p.u_in_v; j.finish = mode[i] == 1 ? 1 : 0;
+      slot1[i] = (int)jobs.size();
+      jobs.push_back(j);
+    }
+    ph1.plan(jobs, stream);
+    if (!p2.empty()) {
+      std::vector<QrJob> jobs2;
+      for (int i : p2) {
+        const SvdProblem& p = probs_in[i];
+        QrJob j{};
+        j.A = ph1.jobs[slot1[i]].X;  // R^T, N x N
+        j.C = p.V;                   // holds J after the Jacobi
+        j.M = p.N; j.N = p.N;
+        jobs2.push_back(j);
+      }
+      ph2.plan(jobs2, stream);
+    }
+    // seeds of the first back-transformation, the Jacobi matrices
+    for (size_t t = 0, t2 = 0; t < p1.size(); ++t) {
+      const int i = p1[t];
+      QrJob& j = ph1.jobs[t];
+      if (mode[i] == 1) {
+        j.S = probs_in[i].V;  // J
+        j.W = j.X;
+        probs[i].G = j.X;
+      } else {
+        j.S = ph2.jobs[t2].X;  // W = R2^T J
+        probs[i].G = ph2.jobs[t2].X;
+        ++t2;
+      }
+      probs[i].M = probs[i].N;
+    }
+    upload(ph1.d_jobs, ph1.jobs, stream);
+    ph1.factor(stream);
+    if (!p2.empty()) ph2.factor(stream);
+  }
+  int maxM = 1;
+  for (const SvdProblem& p : probs) maxM = std::max(maxM, p.M);
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
-  // LDS holds the visit's columns when they fit (64 KiB budget keeps two
-  // workgroups per CU); otherwise the columns are rotated in place.
+  // LDS holds the visit's columns when they fit; otherwise the columns are
+  // rotated in place.
   const size_t lds = (size_t)kJc * maxM * sizeof(double);
   const bool use_lds = lds <= 150 * 1024;
   // device ints: rotated[np] | active[np] | prefix[np + 1]
@@ -861,14 +1280,25 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
       if (hrot[i]) still.push_back(i);
     active.swap(still);
   }
+  if (!p2.empty()) ph2.back(stream);  // V = Q2 J
+  if (!p1.empty()) ph1.back(stream);  // G = Q J (then finish) or Q [W; 0]
+  // sigma and sign convention from G = U Sigma and V (plain and precond 2)
+  for (int i = 0; i < np; ++i) {
+    probs[i] = probs_in[i];
+    probs[i].precond = mode[i] == 1 ? 1 : 0;
+  }
+  upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
+  // the phases' buffers are released when this function returns
+  if (!p1.empty()) DMRG_HIP(hipStreamSynchronize(stream));
   if (std::getenv("DMRG_SVD_STATS")) {
     int maxN = 0; int64_t mn2 = 0;
-    for (const SvdProblem& p : probs) { maxN = std::max(maxN, p.N); mn2 += (int64_t)p.M * p.N * p.N; }
-    std::printf("SVDSTAT np=%d maxM=%d maxN=%d sweeps=%d MN2=%lld |", np, maxM, maxN, sweeps_done,
-                (long long)mn2);
-    for (const SvdProblem& p : probs) if ((int64_t)p.M * p.N > 20000) std::printf(" %dx%d", p.M, p.N);
+    for (const SvdProblem& p : probs_in) { maxN = std::max(maxN, p.N); mn2 += (int64_t)p.M * p.N * p.N; }
+    std::printf("SVDSTAT np=%d qr=%d qr2=%d maxN=%d sweeps=%d MN2=%lld |", np, (int)p1.size(),
+                (int)p2.size(), maxN,
+                sweeps_done, (long long)mn2);
+    for (const SvdProblem& p : probs_in) if ((int64_t)p.M * p.N > 20000) std::printf(" %dx%d", p.M, p.N);
     std::printf("\n");
   }
 }

@@ -139,9 +139,17 @@ int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl,
 /* ---- sector SVD (RealMatrix::SVDMatrix, real_matrix.cc:588-620) -------- */
 /* Batched thin SVD of n row-major device matrices A_i (M_i x N_i), one-sided
  * Jacobi on the device.  Outputs (device): U_i (M_i x k_i), S_i (k_i,
- * descending), Vt_i (k_i x N_i), k_i = min(M_i, N_i). */
+ * descending), Vt_i (k_i x N_i), k_i = min(M_i, N_i).  precondition selects
+ * the QR preconditioning (matrices with k_i >= 48; about a third of the
+ * Jacobi sweeps): 0 = none, plain one-sided Jacobi, both factors orthonormal;
+ * 1 = A = Q R and Jacobi on R^T: the long-side factor (U when M_i >= N_i,
+ * else Vt) is orthonormal to working precision, the short-side one only for
+ * singular values well above eps * |A| (sigma = 0 columns come back zero);
+ * 2 = a second QR of R^T and Jacobi on its R^T: the short-side factor is
+ * orthonormal to working precision, the long-side one as in 0. */
 int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
-                      double* const* U, double* const* S, double* const* Vt, void* stream);
+                      double* const* U, double* const* S, double* const* Vt, int precondition,
+                      void* stream);
 
 /* ---- full sweep (RealTensorRundmrg::Run, real_tensor_rundmrg.cc:115-245) */
 /* Runs the spinless model from DefineTensorSpace through nhalf half sweeps.

@@ -66,3 +66,43 @@ def test_sector_svd(shape):
     assert np.abs((U * S) @ Vt - a).max() < 1e-14 * max(M, N)
     assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
     assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("shapes", [[(120, 90)], [(200, 150), (150, 200), (64, 64), (90, 20)],
+                                    [(600, 560)], [(700, 650), (650, 700)], [(2100, 100)],
+                                    [(3000, 60)]])
+def test_sector_svd_preconditioned(shapes, mode):
+    # QR-preconditioned path: LDS panels of 32 / 16 / 8 columns and the global
+    # panel (M too large for LDS); mixed with problems below the QR size
+    rng = np.random.default_rng(11)
+    mats, spec = [], []
+    for (M, N) in shapes:
+        a = rng.standard_normal((M, N))
+        u, s, vt = np.linalg.svd(a, full_matrices=False)
+        s = np.logspace(0, -12, len(s))
+        mats.append((u * s) @ vt)
+        spec.append(s)
+    U, S, Vt = dm.sector_svd([torch.tensor(a, device="cuda") for a in mats], precondition=mode)
+    for a, s, u, sg, vt in zip(mats, spec, U, S, Vt):
+        u, sg, vt = u.cpu().numpy(), sg.cpu().numpy(), vt.cpu().numpy()
+        M, N = a.shape
+        k = min(M, N)
+        assert np.all(np.diff(sg) <= 0)
+        assert np.abs(sg - s).max() < 1e-14 * max(M, N)
+        assert np.abs((u * sg) @ vt - a).max() < 1e-14 * max(M, N)
+        big = sg > 1e-4
+        if (M >= N) == (mode == 1) or k < 48:  # U exact, V for the well-resolved singular values
+            assert np.abs(u.T @ u - np.eye(k)).max() < 1e-12
+            vb = vt[big]
+            assert np.abs(vb @ vb.T - np.eye(vb.shape[0])).max() < 1e-9
+        else:
+            assert np.abs(vt @ vt.T - np.eye(k)).max() < 1e-12
+            ub = u[:, big]
+            assert np.abs(ub.T @ ub - np.eye(ub.shape[1])).max() < 1e-9
+        # same singular values and sign convention as the plain Jacobi
+        U0, S0, Vt0 = dm.sector_svd([torch.tensor(a, device="cuda")])
+        assert np.abs(S0[0].cpu().numpy() - sg).max() < 1e-14 * max(M, N)
+        u0 = U0[0].cpu().numpy()
+        ok = np.abs(u0[:, big] - u[:, big]).max()
+        assert ok < 1e-8, ok
