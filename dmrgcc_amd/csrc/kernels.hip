@@ -425,20 +425,63 @@ __global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __re
 }
 
 // Block one-sided Jacobi: the N columns are cut into blocks of kJb; one
-// launch performs one round of the block round-robin, one workgroup per block
-// pair.  The workgroup copies the (up to 2*kJb) columns into LDS, orthogonalises
-// them against each other with scalar rotations (inner sweeps until none is
-// needed), accumulates the rotations in Q, writes the columns back and applies
-// Q to the matching columns of V.  Visit q -> problem via the prefix array of
-// block pairs per problem.
+// launch performs one round of the block round-robin, one workgroup (4 waves)
+// per block pair ("visit", up to 16 columns).  A visit forms the 16 x 16 Gram
+// matrix of its columns with FP64 MFMA (the four waves take interleaved
+// 16-row slabs, partial Grams summed in a fixed order), runs one sweep of
+// two-sided Jacobi on it in LDS (wave 0; the rotation angles are those of the
+// one-sided method, computed from the Gram entries), and applies the
+// accumulated rotation Q to its columns of G and V with MFMA.  Pairs whose
+// columns are already orthogonal to tol (or numerically zero) are not
+// rotated; a visit that rotates nothing leaves its columns untouched.
 constexpr int kJb = 8;
 constexpr int kJc = 2 * kJb;
+static_assert(kJc == 16, "a visit is one 16 x 16 MFMA block");
 
-template <bool kLds>
-__global__ __launch_bounds__(512) void jacobi_block_round_kernel(
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// rows [0, rows) of the visit's columns: X[:, col] <- X[:, col] * Q
+__device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int rows,
+                                      const int* col, const double (*Q)[kJc + 1], int lane,
+                                      int wave) {
+  const int i = lane & 15, kk = lane >> 4;
+  int cs[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) cs[t] = col[4 * t + kk];
+  const int cout = col[i];
+  double qb[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) qb[t] = Q[4 * t + kk][i];
+  for (int k0 = wave * 16; k0 < rows; k0 += 64) {
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    double a[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      a[t] = (cs[t] >= 0 && k0 + i < rows) ? X[(int64_t)cs[t] * ld + k0 + i] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], qb[t], acc, 0, 0, 0);
+    if (cout >= 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = k0 + kk + 4 * v;
+        if (r < rows) X[(int64_t)cout * ld + r] = acc[v];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void jacobi_gram_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ active,
     const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
-  extern __shared__ double smem[];
+  __shared__ double Gp[4][kJc][kJc + 1];
+  __shared__ double G[kJc][kJc + 1];
+  __shared__ double Q[kJc][kJc + 1];
+  __shared__ int col[kJc];
+  __shared__ int flag;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = blockIdx.x;
   if (q >= prefix[nactive]) return;
@@ -456,99 +499,87 @@ __global__ __launch_bounds__(512) void jacobi_block_round_kernel(
   int bi, bj;
   if (pr == 0) { bi = round; bj = nb2 - 1; }
   else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
-  // columns of the visit
-  __shared__ int col[kJc];
-  __shared__ double Q[kJc][kJc];
-  __shared__ int any_rot, sweep_rot;
-  int nc = 0;
-  for (int b2 = 0; b2 < 2; ++b2) {
-    const int blk = b2 == 0 ? bi : bj;
-    if (blk >= nb) continue;
-    for (int c = blk * kJb; c < min(N, blk * kJb + kJb); ++c) {
-      if (tid == 0) col[nc] = c;
-      ++nc;
-    }
-  }
-  if (tid < kJc * kJc) Q[tid / kJc][tid % kJc] = (tid / kJc == tid % kJc) ? 1.0 : 0.0;
-  if (tid == 0) any_rot = 0;
-  __syncthreads();
-  // column storage: LDS copy or the columns in place
-  double* cbuf[kJc];
-  for (int k = 0; k < nc; ++k) cbuf[k] = kLds ? smem + (int64_t)k * M : p.G + (int64_t)col[k] * M;
-  if (kLds) {
-    for (int k = 0; k < nc; ++k) {
-      const double* src = p.G + (int64_t)col[k] * M;
-      for (int i = tid; i < M; i += blockDim.x) cbuf[k][i] = src[i];
-    }
-    __syncthreads();
-  }
-  const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
-  const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
-  const int nc2 = nc + (nc & 1);
-  // one inner sweep per visit (the outer sweeps repeat the visit)
-  for (int sweep = 0; sweep < 1 && nc > 1; ++sweep) {
-    if (tid == 0) sweep_rot = 0;
-    __syncthreads();
-    for (int r = 0; r < nc2 - 1; ++r) {
-      for (int pp = wave; pp < nc2 / 2; pp += (int)(blockDim.x >> 6)) {
-        int a, b;
-        if (pp == 0) { a = r; b = nc2 - 1; }
-        else { a = (r + pp) % (nc2 - 1); b = (r + nc2 - 1 - pp) % (nc2 - 1); }
-        if (a >= nc || b >= nc) continue;
-        if (a > b) { const int t = a; a = b; b = t; }
-        double* ga = cbuf[a];
-        double* gb = cbuf[b];
-        double al = 0.0, be = 0.0, gm = 0.0;
-        for (int i = lane; i < M; i += 64) {
-          const double x = ga[i], y = gb[i];
-          al += x * x; be += y * y; gm += x * y;
-        }
-        al = wave_sum(al); be = wave_sum(be); gm = wave_sum(gm);
-        if (al > zero2 && be > zero2 && fabs(gm) > tol * sqrt(al) * sqrt(be)) {
-          const double zeta = (be - al) / (2.0 * gm);
-          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
-          const double cs = 1.0 / sqrt(1.0 + t * t);
-          const double sn = cs * t;
-          for (int i = lane; i < M; i += 64) {
-            const double x = ga[i], y = gb[i];
-            ga[i] = cs * x - sn * y;
-            gb[i] = sn * x + cs * y;
-          }
-          if (lane < nc) {
-            const double x = Q[lane][a], y = Q[lane][b];
-            Q[lane][a] = cs * x - sn * y;
-            Q[lane][b] = sn * x + cs * y;
-          }
-          if (lane == 0) sweep_rot = 1;
-        }
-      }
-      __syncthreads();
-    }
-    const int any = sweep_rot;
-    __syncthreads();
-    if (!any) break;
-    if (tid == 0) any_rot = 1;
+  if (tid < kJc) {
+    const int blk = tid < kJb ? bi : bj;
+    const int c = blk * kJb + (tid & (kJb - 1));
+    col[tid] = (blk < nb && c < N) ? c : -1;
   }
   __syncthreads();
-  if (kLds) {
-    for (int k = 0; k < nc; ++k) {
-      double* dst = p.G + (int64_t)col[k] * M;
-      for (int i = tid; i < M; i += blockDim.x) dst[i] = cbuf[k][i];
-    }
-  }
-  if (any_rot) {
-    // V[:, col] <- V[:, col] * Q, row by row
-    for (int i = tid; i < N; i += blockDim.x) {
-      double v[kJc];
-      for (int k = 0; k < nc; ++k) v[k] = p.V[(int64_t)col[k] * N + i];
-      for (int k = 0; k < nc; ++k) {
-        double s = 0.0;
-        for (int m = 0; m < nc; ++m) s += v[m] * Q[m][k];
-        p.V[(int64_t)col[k] * N + i] = s;
+  // partial Gram matrices: lane (i, kk) feeds X[row][col_i] as both operands
+  {
+    const int i = lane & 15, kk = lane >> 4;
+    const int ci = col[i];
+    const double* __restrict__ xc = p.G + (int64_t)(ci >= 0 ? ci : 0) * M;
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = wave * 16; k0 < M; k0 += 64) {
+      double v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = k0 + 4 * kk + t;
+        v[t] = (ci >= 0 && r < M) ? xc[r] : 0.0;
       }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t], v[t], acc, 0, 0, 0);
     }
-    if (tid == 0) rotated[pi] = 1;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Gp[wave][kk + 4 * v][i] = acc[v];
   }
+  __syncthreads();
+  {
+    const int r = tid >> 4, c = tid & 15;
+    G[r][c] = (Gp[0][r][c] + Gp[1][r][c]) + (Gp[2][r][c] + Gp[3][r][c]);
+    Q[r][c] = r == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
+    const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
+    const int pp = lane >> 3, i0 = (lane & 7) * 2;
+    int any = 0;
+    for (int r = 0; r < kJc - 1; ++r) {
+      int a, b;
+      if (pp == 0) { a = r; b = kJc - 1; }
+      else { a = (r + pp) % (kJc - 1); b = (r + kJc - 1 - pp) % (kJc - 1); }
+      if (a > b) { const int t = a; a = b; b = t; }
+      const double al = G[a][a], be = G[b][b], gm = G[a][b];
+      double cs = 1.0, sn = 0.0;
+      const bool doit = al > zero2 && be > zero2 && fabs(gm) > tol * sqrt(al) * sqrt(be);
+      if (doit) {
+        const double zeta = (be - al) / (2.0 * gm);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
+        cs = 1.0 / sqrt(1.0 + t * t);
+        sn = cs * t;
+        any = 1;
+      }
+      wave_lds_sync();
+      if (doit) {  // G <- G J, Q <- Q J  (columns a, b)
+        for (int ii = i0; ii < i0 + 2; ++ii) {
+          const double ga = G[ii][a], gb = G[ii][b];
+          G[ii][a] = cs * ga - sn * gb;
+          G[ii][b] = sn * ga + cs * gb;
+          const double qa = Q[ii][a], qb = Q[ii][b];
+          Q[ii][a] = cs * qa - sn * qb;
+          Q[ii][b] = sn * qa + cs * qb;
+        }
+      }
+      wave_lds_sync();
+      if (doit) {  // G <- J^T G  (rows a, b)
+        for (int ii = i0; ii < i0 + 2; ++ii) {
+          const double ga = G[a][ii], gb = G[b][ii];
+          G[a][ii] = cs * ga - sn * gb;
+          G[b][ii] = sn * ga + cs * gb;
+        }
+      }
+      wave_lds_sync();
+    }
+    any = __any(any);
+    if (lane == 0) flag = any;
+  }
+  __syncthreads();
+  if (!flag) return;
+  apply_rotation(p.G, M, M, col, Q, lane, wave);
+  apply_rotation(p.V, N, N, col, Q, lane, wave);
+  if (tid == 0) rotated[pi] = 1;
 }
 
 // singular values and the sign convention, one workgroup per problem
@@ -1229,15 +1260,9 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     ph1.factor(stream);
     if (!p2.empty()) ph2.factor(stream);
   }
-  int maxM = 1;
-  for (const SvdProblem& p : probs) maxM = std::max(maxM, p.M);
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
-  // LDS holds the visit's columns when they fit; otherwise the columns are
-  // rotated in place.
-  const size_t lds = (size_t)kJc * maxM * sizeof(double);
-  const bool use_lds = lds <= 150 * 1024;
   // device ints: rotated[np] | active[np] | prefix[np + 1]
   sweeps_out.ensure((size_t)3 * np + 1);
   int* rotated = sweeps_out.data();
@@ -1264,14 +1289,9 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     h.resize(2 * np, 0);
     h.insert(h.end(), prefix.begin(), prefix.end());
     DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    for (int r = 0; r < rounds; ++r) {
-      if (use_lds)
-        hipLaunchKernelGGL(jacobi_block_round_kernel<true>, dim3(visits), dim3(512), lds, stream,
-                           scratch.data(), dactive, dprefix, na, r, rotated);
-      else
-        hipLaunchKernelGGL(jacobi_block_round_kernel<false>, dim3(visits), dim3(512), 0, stream,
-                           scratch.data(), dactive, dprefix, na, r, rotated);
-    }
+    for (int r = 0; r < rounds; ++r)
+      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(256), 0, stream,
+                         scratch.data(), dactive, dprefix, na, r, rotated);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
