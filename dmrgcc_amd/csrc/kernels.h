@@ -171,16 +171,17 @@ void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGather
 // and are not rotated (their singular values sit at the round-off floor); the
 // squared Frobenius norm is computed on the device into fro2.
 //
-// precond asks for QR preconditioning when N is large enough (both forms
-// return G = U*Sigma and V as above, with about a third of the sweeps):
-//   1: G = Q R (blocked Householder), Jacobi on R^T = W J^T, G <- Q J and
-//      V <- W / sigma.  U = Q J is orthonormal to working precision for every
-//      singular value; V only for singular values well above eps |A| (zero
-//      sigma gives a zero column).  For callers that take the orthonormal
-//      factor from G.
-//   2: second QR R^T = Q2 R2, Jacobi on R2^T = W J^T, V <- Q2 J and
-//      G <- Q [W; 0].  V is orthonormal to working precision (as in the plain
-//      Jacobi); for callers that take the orthonormal factor from V.
+// precond asks for QR preconditioning when N is large enough.  With
+// B_1 = A = Q_1 R_1 and B_k = R_{k-1}^T = Q_k R_k (blocked Householder), the
+// Jacobi runs on X = R_K^T = W J^T, which is closer to diagonal with every
+// QR (a third of the sweeps or less); both forms return G = U*Sigma and V as
+// above:
+//   1 (K = 3): U = Q_1 Q_3 J, G <- U*Sigma, V <- Q_2 W / sigma.  U is
+//      orthonormal to working precision for every singular value; V only for
+//      singular values well above eps |A| (zero sigma gives a zero column).
+//      For callers that take the orthonormal factor from G.
+//   2 (K = 2): V <- Q_2 J (orthonormal to working precision, as in the plain
+//      Jacobi) and G <- Q_1 W.  For callers that take it from V.
 struct SvdProblem {
   double* G;
   double* V;

@@ -1085,8 +1085,8 @@ struct QrPhase {
   bool lds = true;
   size_t lds_bytes = 0;
 
-  // jobs: A, C, S, W, V, sigma, M, N, u_in_v, finish set by the caller; the
-  // plan adds the X, Y, P buffers.
+  // jobs: every field but Y and P set by the caller (X included); the plan
+  // adds the Y, P and Z buffers.
   void plan(std::vector<QrJob> in, hipStream_t stream) {
     jobs = std::move(in);
     maxM = maxN = 0;
@@ -1105,15 +1105,14 @@ struct QrPhase {
     for (const QrJob& j : jobs) {
       const int np = (j.N + b - 1) / b;
       off.push_back(total);
-      total += (int64_t)j.N * j.N + 2 * (int64_t)np * b * j.M + (int64_t)b * j.N;
+      total += 2 * (int64_t)np * b * j.M + (int64_t)b * j.N;
     }
     work.ensure((size_t)std::max<int64_t>(total, 1));
     std::vector<double*> Z;
     for (size_t t = 0; t < jobs.size(); ++t) {
       QrJob& r = jobs[t];
       const int np = (r.N + b - 1) / b;
-      r.X = work.data() + off[t];
-      r.Y = r.X + (int64_t)r.N * r.N;
+      r.Y = work.data() + off[t];
       r.P = r.Y + (int64_t)np * b * r.M;
       Z.push_back(r.P + (int64_t)np * b * r.M);
     }
@@ -1204,61 +1203,65 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
   if (probs_in.empty()) return;
   const int np = (int)probs_in.size();
-  // Jacobi view of every problem.  precond 1: Jacobi on R^T of A = Q R, then
-  // G <- Q J (exact) and V <- W / sigma.  precond 2: second QR R^T = Q2 R2,
-  // Jacobi on R2^T = W J^T, then V <- Q2 J (exact) and G <- Q [W; 0].
+  // Jacobi view of every problem.  With B_1 = A = Q_1 R_1 and
+  // B_k = R_{k-1}^T = Q_k R_k, the Jacobi runs on X = R_K^T (X J = W):
+  //   precond 1 (K = 3): A = (Q_1 Q_3 J) (Q_2 W)^T, G <- Q_1 Q_3 J exact;
+  //   precond 2 (K = 2): A = (Q_1 W) (Q_2 J)^T,    V <- Q_2 J exact.
+  // Each extra QR of the transposed triangle makes X more nearly diagonal
+  // (fewer sweeps).
   std::vector<SvdProblem> probs = probs_in;
-  std::vector<int> mode(np, 0), p1, p2;
+  std::vector<int> mode(np, 0), pq;
   for (int i = 0; i < np; ++i) {
     const SvdProblem& p = probs_in[i];
     require(p.M >= p.N && p.N >= 1, "jacobi_svd: needs M >= N >= 1", kErrInternal);
     if (p.N >= kQrMinN && (p.precond == 1 || p.precond == 2)) mode[i] = p.precond;
-    if (mode[i]) p1.push_back(i);
-    if (mode[i] == 2) p2.push_back(i);
+    if (mode[i]) pq.push_back(i);
   }
-  QrPhase ph1, ph2;
-  std::vector<int> slot1(np, -1);
-  if (!p1.empty()) {
-    std::vector<QrJob> jobs;
-    for (int i : p1) {
+  QrPhase ph1, ph2, ph3;
+  DevBuf<double> xbuf;
+  int nq1 = 0;
+  if (!pq.empty()) {
+    // X_k = R_k^T, N x N each: three per precond-1 problem, two per precond 2
+    int64_t total = 0;
+    for (int i : pq) total += (int64_t)(mode[i] == 1 ? 3 : 2) * probs_in[i].N * probs_in[i].N;
+    xbuf.ensure((size_t)total);
+    std::vector<QrJob> j1, j2, j3;
+    double* x = xbuf.data();
+    for (int i : pq) {
       const SvdProblem& p = probs_in[i];
-      QrJob j{};
-      j.A = p.G; j.C = p.G; j.V = p.V; j.sigma = p.sigma;
-      j.M = p.M; j.N = p.N; j.u_in_v = p.u_in_v; j.finish = mode[i] == 1 ? 1 : 0;
-      slot1[i] = (int)jobs.size();
-      jobs.push_back(j);
-    }
-    ph1.plan(jobs, stream);
-    if (!p2.empty()) {
-      std::vector<QrJob> jobs2;
-      for (int i : p2) {
-        const SvdProblem& p = probs_in[i];
-        QrJob j{};
-        j.A = ph1.jobs[slot1[i]].X;  // R^T, N x N
-        j.C = p.V;                   // holds J after the Jacobi
-        j.M = p.N; j.N = p.N;
-        jobs2.push_back(j);
-      }
-      ph2.plan(jobs2, stream);
-    }
-    // seeds of the first back-transformation, the Jacobi matrices
-    for (size_t t = 0, t2 = 0; t < p1.size(); ++t) {
-      const int i = p1[t];
-      QrJob& j = ph1.jobs[t];
+      const int64_t nn = (int64_t)p.N * p.N;
+      double* X1 = x;
+      double* X2 = x + nn;
+      double* X3 = mode[i] == 1 ? x + 2 * nn : nullptr;
+      x += (mode[i] == 1 ? 3 : 2) * nn;
+      QrJob a{};
+      a.A = p.G; a.X = X1; a.C = p.G; a.V = p.V; a.sigma = p.sigma;
+      a.M = p.M; a.N = p.N; a.u_in_v = p.u_in_v;
+      if (mode[i] == 1) { a.S = p.V; a.W = X3; a.finish = 1; }  // [Q_3 J; 0], Q_2 W
+      else a.S = X2;                                           // [W; 0]
+      j1.push_back(a);
+      QrJob b{};
+      b.A = X1; b.X = X2; b.C = mode[i] == 1 ? X3 : p.V;  // Q_2 W or Q_2 J
+      b.M = p.N; b.N = p.N;
+      j2.push_back(b);
       if (mode[i] == 1) {
-        j.S = probs_in[i].V;  // J
-        j.W = j.X;
-        probs[i].G = j.X;
+        QrJob c{};
+        c.A = X2; c.X = X3; c.C = p.V;  // Q_3 J
+        c.M = p.N; c.N = p.N;
+        j3.push_back(c);
+        probs[i].G = X3;
+        ++nq1;
       } else {
-        j.S = ph2.jobs[t2].X;  // W = R2^T J
-        probs[i].G = ph2.jobs[t2].X;
-        ++t2;
+        probs[i].G = X2;
       }
-      probs[i].M = probs[i].N;
+      probs[i].M = p.N;
     }
-    upload(ph1.d_jobs, ph1.jobs, stream);
+    ph1.plan(j1, stream);
+    ph2.plan(j2, stream);
+    if (!j3.empty()) ph3.plan(j3, stream);
     ph1.factor(stream);
-    if (!p2.empty()) ph2.factor(stream);
+    ph2.factor(stream);
+    if (!j3.empty()) ph3.factor(stream);
   }
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
@@ -1271,7 +1274,7 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   std::vector<int> active;
   for (int i = 0; i < np; ++i)
     if (probs[i].N > 1) active.push_back(i);
-  std::vector<int> hrot(np);
+  std::vector<int> hrot(np), last(np, 0);
   int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
     ++sweeps_done;
@@ -1296,12 +1299,17 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
     std::vector<int> still;
-    for (int i : active)
+    for (int i : active) {
+      last[i] = sweep + 1;
       if (hrot[i]) still.push_back(i);
+    }
     active.swap(still);
   }
-  if (!p2.empty()) ph2.back(stream);  // V = Q2 J
-  if (!p1.empty()) ph1.back(stream);  // G = Q J (then finish) or Q [W; 0]
+  if (nq1 > 0) ph3.back(stream);        // V = Q_3 J             (precond 1)
+  if (!pq.empty()) {
+    ph2.back(stream);                   // X3 = Q_2 W or V = Q_2 J
+    ph1.back(stream);                   // G = Q_1 [Q_3 J; 0] (+ finish) or Q_1 [W; 0]
+  }
   // sigma and sign convention from G = U Sigma and V (plain and precond 2)
   for (int i = 0; i < np; ++i) {
     probs[i] = probs_in[i];
@@ -1311,14 +1319,16 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
   // the phases' buffers are released when this function returns
-  if (!p1.empty()) DMRG_HIP(hipStreamSynchronize(stream));
+  if (!pq.empty()) DMRG_HIP(hipStreamSynchronize(stream));
   if (std::getenv("DMRG_SVD_STATS")) {
     int maxN = 0; int64_t mn2 = 0;
     for (const SvdProblem& p : probs_in) { maxN = std::max(maxN, p.N); mn2 += (int64_t)p.M * p.N * p.N; }
-    std::printf("SVDSTAT np=%d qr=%d qr2=%d maxN=%d sweeps=%d MN2=%lld |", np, (int)p1.size(),
-                (int)p2.size(), maxN,
+    std::printf("SVDSTAT np=%d qr=%d qr3=%d maxN=%d sweeps=%d MN2=%lld |", np, (int)pq.size(),
+                nq1, maxN,
                 sweeps_done, (long long)mn2);
-    for (const SvdProblem& p : probs_in) if ((int64_t)p.M * p.N > 20000) std::printf(" %dx%d", p.M, p.N);
+    for (int i = 0; i < np; ++i)
+      if ((int64_t)probs_in[i].M * probs_in[i].N > 20000)
+        std::printf(" %dx%d:%d:%d", probs_in[i].M, probs_in[i].N, mode[i], last[i]);
     std::printf("\n");
   }
 }
