@@ -59,15 +59,94 @@ struct MfmaAcc<float> {
   __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// Operand staging shared by the GEMM kernels.  One K-step covers KT = 32
+// columns of A (64 x 32 tile) and rows of B (32 x 64 tile); each of the 256
+// threads holds 8 elements of each, prefetched into registers while the MFMAs
+// of the previous step run.  Element q of thread tid:
+//   A, not transposed: m = tid/4,        k = 8*(tid%4) + q   (8 consecutive k)
+//   A, transposed:     k = tid/8,        m = 8*(tid%8) + q   (8 consecutive m)
+//   B, not transposed: k = tid/8,        n = 8*(tid%8) + q
+//   B, transposed:     n = tid/4,        k = 8*(tid%4) + q
+constexpr int kKT = 32;
+
+struct StepRegs {
+  double a[8], b[8];
+};
+
+__device__ inline void load_step(const GemmContrib& cb, const double* __restrict__ in_base, int m0,
+                                 int n0, int M, int N, int k0, int tid, StepRegs& r) {
+  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  const int K = cb.K;
+  const double sc = cb.scale;
+  if (!(cb.flags & kTransA)) {
+    const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+    const double* row = A + (int64_t)gm * cb.lda;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[kb + q] * sc : 0.0;
+  } else {
+    const int gk = k0 + (tid >> 3), mb = m0 + (tid & 7) * 8;
+    const double* row = A + (int64_t)gk * cb.lda;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.a[q] = (gk < K && mb + q < M) ? row[mb + q] * sc : 0.0;
+  }
+  if (!(cb.flags & kTransB)) {
+    const int gk = k0 + (tid >> 3), nb = n0 + (tid & 7) * 8;
+    const double* row = B + (int64_t)gk * cb.ldb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.b[q] = (gk < K && nb + q < N) ? row[nb + q] : 0.0;
+  } else {
+    const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+    const double* row = B + (int64_t)gn * cb.ldb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[kb + q] : 0.0;
+  }
+}
+
+// (contribution, k0) walk of one item: advance to the next K-step
+__device__ inline bool next_step(const GemmContrib* __restrict__ cons, int c1, int& c, int& k0,
+                                 GemmContrib& cb) {
+  k0 += kKT;
+  if (k0 < cb.K) return true;
+  k0 = 0;
+  while (++c < c1) {
+    cb = cons[c];
+    if (cb.K > 0) return true;
+  }
+  return false;
+}
+
+template <typename T>
+__device__ inline void store_step(const GemmContrib& cb, int tid, const StepRegs& r,
+                                  T (*As)[kTile + 2], T (*Bs)[kTile + 2]) {
+  if (!(cb.flags & kTransA)) {
+    const int m = tid >> 2, kb = (tid & 3) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[kb + q][m] = (T)r.a[q];
+  } else {
+    const int k = tid >> 3, mb = (tid & 7) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) As[k][mb + q] = (T)r.a[q];
+  }
+  if (!(cb.flags & kTransB)) {
+    const int k = tid >> 3, nb = (tid & 7) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Bs[k][nb + q] = (T)r.b[q];
+  } else {
+    const int n = tid >> 2, kb = (tid & 3) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Bs[kb + q][n] = (T)r.b[q];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
     double* __restrict__ out_base) {
-  constexpr int KT = 16;            // k-tile
-  constexpr int LD = kTile + 2;     // padded LDS row
-  __shared__ T As[KT][LD];
-  __shared__ T Bs[KT][LD];
+  constexpr int LD = kTile + 2;  // padded LDS row
+  __shared__ T As[kKT][LD];
+  __shared__ T Bs[kKT][LD];
   typedef typename MfmaAcc<T>::V V;
 
   const GemmTile tl = tiles[blockIdx.x];
@@ -85,53 +164,38 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
 
-  for (int c = it.c0; c < it.c1; ++c) {
-    const GemmContrib cb = cons[c];
-    const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
-    const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
-    const bool ta = cb.flags & kTransA;
-    const bool tb = cb.flags & kTransB;
-    const double sc = cb.scale;
-    const int K = cb.K;
-    for (int k0 = 0; k0 < K; k0 += KT) {
-      __syncthreads();
-      // A tile (kTile rows x KT) -> As[k][m]
+  // first non-empty contribution
+  int c = it.c0, k0 = 0;
+  GemmContrib cb{};
+  bool live = false;
+  for (; c < it.c1; ++c) {
+    cb = cons[c];
+    if (cb.K > 0) { live = true; break; }
+  }
+  StepRegs r;
+  if (live) load_step(cb, in_base, m0, n0, M, N, 0, tid, r);
+  while (live) {
+    store_step<T>(cb, tid, r, As, Bs);
+    __syncthreads();
+    GemmContrib nc = cb;
+    int c2 = c, k2 = k0;
+    live = next_step(cons, it.c1, c2, k2, nc);
+    if (live) load_step(nc, in_base, m0, n0, M, N, k2, tid, r);  // in flight during the MFMAs
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        int m, k;
-        if (!ta) { m = tid >> 2; k = (tid & 3) * 4 + q; }
-        else     { k = tid >> 4; m = (tid & 15) * 4 + q; }
-        const int gm = m0 + m, gk = k0 + k;
-        double v = 0.0;
-        if (gm < M && gk < K) v = ta ? A[(int64_t)gk * cb.lda + gm] : A[(int64_t)gm * cb.lda + gk];
-        As[k][m] = (T)(v * sc);
-      }
-      // B tile (KT x kTile) -> Bs[k][n]
+    for (int ks = 0; ks < kKT; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      T af[2], bf[2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        int n, k;
-        if (!tb) { k = tid >> 4; n = (tid & 15) * 4 + q; }
-        else     { n = tid >> 2; k = (tid & 3) * 4 + q; }
-        const int gn = n0 + n, gk = k0 + k;
-        double v = 0.0;
-        if (gn < N && gk < K) v = tb ? B[(int64_t)gn * cb.ldb + gk] : B[(int64_t)gk * cb.ldb + gn];
-        Bs[k][n] = (T)v;
-      }
-      __syncthreads();
+      for (int a = 0; a < 2; ++a) af[a] = As[kr][wm * 32 + a * 16 + (lane & 15)];
 #pragma unroll
-      for (int ks = 0; ks < KT; ks += 4) {
-        const int kr = ks + (lane >> 4);
-        T af[2], bf[2];
+      for (int b = 0; b < 2; ++b) bf[b] = Bs[kr][wn * 32 + b * 16 + (lane & 15)];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) af[a] = As[kr][wm * 32 + a * 16 + (lane & 15)];
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) bf[b] = Bs[kr][wn * 32 + b * 16 + (lane & 15)];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
-      }
+        for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
     }
+    __syncthreads();
+    c = c2; k0 = k2; cb = nc;
   }
   // epilogue: lane holds D[row(lane, v)][lane%16] of each 16x16 block
 #pragma unroll
@@ -144,8 +208,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
           double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
-          const double r = (double)acc[a][b][v];
-          *p = it.accumulate ? (*p + r) : r;
+          const double r2 = (double)acc[a][b][v];
+          *p = it.accumulate ? (*p + r2) : r2;
         }
       }
 }
@@ -156,23 +220,65 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
 // 16-byte read: exactly the v_mfma_f32_16x16x32_bf16 operand layout (lane l
 // supplies A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16]).
 // ---------------------------------------------------------------------------
-__device__ inline unsigned short f2bf(float f) {
-  unsigned int u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-  return (unsigned short)(u >> 16);
+// BF16x3 staging: every thread holds 8 consecutive k of one row of A (m) and
+// of one column of B (n), so each hi/lo plane is written to LDS with one
+// 16-byte store.  Element q of thread tid:
+//   A, not transposed: m = tid/4,  k = 8*(tid%4) + q   (a 64-byte run of a row)
+//   A, transposed:     m = tid%64, k = 8*(tid/64) + q  (64 threads read 64 consecutive m)
+//   B, not transposed: n = tid%64, k = 8*(tid/64) + q
+//   B, transposed:     n = tid/4,  k = 8*(tid%4) + q
+typedef float f8v __attribute__((ext_vector_type(8)));
+
+__device__ inline void load_step_packed(const GemmContrib& cb, const double* __restrict__ in_base,
+                                        int m0, int n0, int M, int N, int k0, int tid,
+                                        StepRegs& r) {
+  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  const int K = cb.K;
+  const double sc = cb.scale;
+  if (!(cb.flags & kTransA)) {
+    const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+    const double* row = A + (int64_t)gm * cb.lda;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[kb + q] * sc : 0.0;
+  } else {
+    const int gm = m0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm] * sc : 0.0;
+  }
+  if (!(cb.flags & kTransB)) {
+    const int gn = n0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn] : 0.0;
+  } else {
+    const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+    const double* row = B + (int64_t)gn * cb.ldb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[kb + q] : 0.0;
+  }
 }
-__device__ inline float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (round to nearest even)
+__device__ inline void split8(const double* v, bf8v& h, bf8v& l) {
+  f8v f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) f[q] = (float)v[q];
+  h = __builtin_convertvector(f, bf8v);
+  const f8v hf = __builtin_convertvector(h, f8v);
+  l = __builtin_convertvector(f - hf, bf8v);
+}
 
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
     double* __restrict__ out_base) {
-  constexpr int KT = 32;
-  constexpr int LD = KT + 8;  // 80-byte rows keep 16-byte alignment
-  __shared__ __attribute__((aligned(16))) unsigned short Ah[kTile][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Al[kTile][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Bh[kTile][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Bl[kTile][LD];
+  constexpr int LD = kKT + 8;  // 80-byte rows keep 16-byte alignment
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[kTile][LD];
 
   const GemmTile tl = tiles[blockIdx.x];
   const GemmItem it = items[tl.item];
@@ -189,66 +295,58 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f4v{0, 0, 0, 0};
 
-  for (int c = it.c0; c < it.c1; ++c) {
-    const GemmContrib cb = cons[c];
-    const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
-    const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
-    const bool ta = cb.flags & kTransA;
-    const bool tb = cb.flags & kTransB;
-    const double sc = cb.scale;
-    const int K = cb.K;
-    for (int k0 = 0; k0 < K; k0 += KT) {
-      __syncthreads();
+  int c = it.c0, k0 = 0;
+  GemmContrib cb{};
+  bool live = false;
+  for (; c < it.c1; ++c) {
+    cb = cons[c];
+    if (cb.K > 0) { live = true; break; }
+  }
+  StepRegs r;
+  if (live) load_step_packed(cb, in_base, m0, n0, M, N, 0, tid, r);
+  while (live) {
+    {
+      const int am = (cb.flags & kTransA) ? (tid & 63) : (tid >> 2);
+      const int ak = (cb.flags & kTransA) ? (tid >> 6) * 8 : (tid & 3) * 8;
+      const int bn = (cb.flags & kTransB) ? (tid >> 2) : (tid & 63);
+      const int bk = (cb.flags & kTransB) ? (tid & 3) * 8 : (tid >> 6) * 8;
+      bf8v h, l;
+      split8(r.a, h, l);
+      *reinterpret_cast<bf8v*>(&Ah[am][ak]) = h;
+      *reinterpret_cast<bf8v*>(&Al[am][ak]) = l;
+      split8(r.b, h, l);
+      *reinterpret_cast<bf8v*>(&Bh[bn][bk]) = h;
+      *reinterpret_cast<bf8v*>(&Bl[bn][bk]) = l;
+    }
+    __syncthreads();
+    GemmContrib nc = cb;
+    int c2 = c, k2 = k0;
+    live = next_step(cons, it.c1, c2, k2, nc);
+    if (live) load_step_packed(nc, in_base, m0, n0, M, N, k2, tid, r);  // in flight during the MFMAs
+    const int kc = 8 * (lane >> 4);
+    bf8v ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        int m, k;
-        if (!ta) { m = tid >> 2; k = (tid & 3) * 8 + q; }
-        else     { k = tid >> 3; m = (tid & 7) * 8 + q; }
-        const int gm = m0 + m, gk = k0 + k;
-        double v = 0.0;
-        if (gm < M && gk < K) v = ta ? A[(int64_t)gk * cb.lda + gm] : A[(int64_t)gm * cb.lda + gk];
-        const float f = (float)(v * sc);
-        const unsigned short h = f2bf(f);
-        Ah[m][k] = h;
-        Al[m][k] = f2bf(f - bf2f(h));
-      }
+    for (int a = 0; a < 2; ++a) {
+      const int rr = wm * 32 + a * 16 + (lane & 15);
+      ah[a] = *reinterpret_cast<const bf8v*>(&Ah[rr][kc]);
+      al[a] = *reinterpret_cast<const bf8v*>(&Al[rr][kc]);
+    }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        int n, k;
-        if (!tb) { k = tid >> 3; n = (tid & 7) * 8 + q; }
-        else     { n = tid >> 2; k = (tid & 3) * 8 + q; }
-        const int gn = n0 + n, gk = k0 + k;
-        double v = 0.0;
-        if (gn < N && gk < K) v = tb ? B[(int64_t)gn * cb.ldb + gk] : B[(int64_t)gk * cb.ldb + gn];
-        const float f = (float)v;
-        const unsigned short h = f2bf(f);
-        Bh[n][k] = h;
-        Bl[n][k] = f2bf(f - bf2f(h));
-      }
-      __syncthreads();
-      const int kc = 8 * (lane >> 4);
-      bf8v ah[2], al[2], bh[2], bl[2];
+    for (int b = 0; b < 2; ++b) {
+      const int rr = wn * 32 + b * 16 + (lane & 15);
+      bh[b] = *reinterpret_cast<const bf8v*>(&Bh[rr][kc]);
+      bl[b] = *reinterpret_cast<const bf8v*>(&Bl[rr][kc]);
+    }
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int r = wm * 32 + a * 16 + (lane & 15);
-        ah[a] = *reinterpret_cast<const bf8v*>(&Ah[r][kc]);
-        al[a] = *reinterpret_cast<const bf8v*>(&Al[r][kc]);
-      }
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const int r = wn * 32 + b * 16 + (lane & 15);
-        bh[b] = *reinterpret_cast<const bf8v*>(&Bh[r][kc]);
-        bl[b] = *reinterpret_cast<const bf8v*>(&Bl[r][kc]);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
       }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        }
-    }
+    __syncthreads();
+    c = c2; k0 = k2; cb = nc;
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -260,8 +358,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
           double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
-          const double r = (double)acc[a][b][v];
-          *p = it.accumulate ? (*p + r) : r;
+          const double r2 = (double)acc[a][b][v];
+          *p = it.accumulate ? (*p + r2) : r2;
         }
       }
 }
