@@ -150,9 +150,11 @@ void Lattice::define_sectors(const std::vector<int>& l, const std::vector<int>& 
 
 Context::Context(hipStream_t s) : stream(s) {
   if (!stream) {
-    // a blocking stream: ordered after work the caller queued on the null
-    // stream (where PyTorch's default stream lives)
-    DMRG_HIP(hipStreamCreate(&stream));
+    // A non-blocking stream (default priority): launches on a stream that
+    // synchronises with the null stream cost tens of microseconds of host
+    // time each.  Callers finish the work producing the inputs before a call;
+    // every entry point waits for its own work before returning.
+    DMRG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     own_stream = true;
   }
   scal.resize(64);

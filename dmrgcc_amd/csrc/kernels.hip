@@ -523,18 +523,31 @@ __global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __re
 }
 
 // Block one-sided Jacobi: the N columns are cut into blocks of kJb; one
-// launch performs one round of the block round-robin, one workgroup (4 waves)
-// per block pair ("visit", up to 16 columns).  A visit forms the 16 x 16 Gram
-// matrix of its columns with FP64 MFMA (the four waves take interleaved
-// 16-row slabs, partial Grams summed in a fixed order), runs one sweep of
-// two-sided Jacobi on it in LDS (wave 0; the rotation angles are those of the
-// one-sided method, computed from the Gram entries), and applies the
-// accumulated rotation Q to its columns of G and V with MFMA.  Pairs whose
-// columns are already orthogonal to tol (or numerically zero) are not
-// rotated; a visit that rotates nothing leaves its columns untouched.
-constexpr int kJb = 8;
+// launch performs one round of the block round-robin, one workgroup of kJw
+// waves per block pair ("visit", up to kJc = 2 kJb columns).  A visit:
+//  1. forms the kJc x kJc Gram matrix of its columns with FP64 MFMA: the waves
+//     take interleaved 16-row slabs and keep partial 16 x 16 blocks, summed in
+//     a fixed order in LDS (deterministic);
+//  2. runs one sweep of two-sided Jacobi on it in LDS (wave 0; the rotation
+//     angles are those of the one-sided method, computed from the Gram
+//     entries), accumulating the rotation Q;
+//  3. applies Q to its columns of G and V with MFMA (slabs over the waves).
+// Pairs whose columns are already orthogonal to tol (or numerically zero) are
+// not rotated; a visit that rotates nothing leaves its columns untouched.
+// The round time is one visit's critical path, so the visit width and the
+// number of waves per visit are compile-time tunables.
+#ifndef DMRG_JACOBI_JB
+#define DMRG_JACOBI_JB 8
+#endif
+#ifndef DMRG_JACOBI_WAVES
+#define DMRG_JACOBI_WAVES 8
+#endif
+constexpr int kJb = DMRG_JACOBI_JB;
 constexpr int kJc = 2 * kJb;
-static_assert(kJc == 16, "a visit is one 16 x 16 MFMA block");
+constexpr int kJw = DMRG_JACOBI_WAVES;
+constexpr int kJn = kJc / 16;  // 16 x 16 MFMA blocks per visit edge
+static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
+static_assert(kJw >= 1 && kJw <= 16, "1 to 16 waves per visit");
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -542,44 +555,66 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Slabs of 16 rows handled together by one wave (their loads in flight at once)
+constexpr int kSlabs = 2;
+
 // rows [0, rows) of the visit's columns: X[:, col] <- X[:, col] * Q
 __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int rows,
                                       const int* col, const double (*Q)[kJc + 1], int lane,
                                       int wave) {
   const int i = lane & 15, kk = lane >> 4;
-  int cs[4];
+  constexpr int kT = kJc / 4;  // MFMA k-steps
+  int cs[kT];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) cs[t] = col[4 * t + kk];
-  const int cout = col[i];
-  double qb[4];
+  for (int t = 0; t < kT; ++t) cs[t] = col[4 * t + kk];
+  int cout[kJn];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) qb[t] = Q[4 * t + kk][i];
-  for (int k0 = wave * 16; k0 < rows; k0 += 64) {
-    d4v acc = {0.0, 0.0, 0.0, 0.0};
-    double a[4];
+  for (int h = 0; h < kJn; ++h) cout[h] = col[16 * h + i];
+  double qb[kJn][kT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-      a[t] = (cs[t] >= 0 && k0 + i < rows) ? X[(int64_t)cs[t] * ld + k0 + i] : 0.0;
+  for (int h = 0; h < kJn; ++h)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], qb[t], acc, 0, 0, 0);
-    if (cout >= 0) {
+    for (int t = 0; t < kT; ++t) qb[h][t] = Q[4 * t + kk][16 * h + i];
+  for (int base = wave * 16; base < rows; base += 16 * kJw * kSlabs) {
+    double a[kSlabs][kT];
+#pragma unroll
+    for (int u = 0; u < kSlabs; ++u)
+#pragma unroll
+      for (int t = 0; t < kT; ++t) {
+        const int r = base + 16 * kJw * u + i;
+        a[u][t] = (cs[t] >= 0 && r < rows) ? X[(int64_t)cs[t] * ld + r] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < kSlabs; ++u) {
+      d4v acc[kJn];
+#pragma unroll
+      for (int h = 0; h < kJn; ++h) {
+        acc[h] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < kT; ++t)
+          acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][t], qb[h][t], acc[h], 0, 0, 0);
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = k0 + kk + 4 * v;
-        if (r < rows) X[(int64_t)cout * ld + r] = acc[v];
+        const int r = base + 16 * kJw * u + kk + 4 * v;
+        if (r < rows) {
+#pragma unroll
+          for (int h = 0; h < kJn; ++h)
+            if (cout[h] >= 0) X[(int64_t)cout[h] * ld + r] = acc[h][v];
+        }
       }
     }
   }
 }
 
-__global__ __launch_bounds__(256) void jacobi_gram_round_kernel(
+__global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ active,
     const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
-  __shared__ double Gp[4][kJc][kJc + 1];
-  __shared__ double G[kJc][kJc + 1];
+  __shared__ double Gp[kJw][kJc][kJc + 1];  // partial Grams, one per wave
   __shared__ double Q[kJc][kJc + 1];
   __shared__ int col[kJc];
   __shared__ int flag;
+  double (*G)[kJc + 1] = Gp[0];  // the summed Gram overwrites wave 0's partial
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = blockIdx.x;
   if (q >= prefix[nactive]) return;
@@ -603,36 +638,65 @@ __global__ __launch_bounds__(256) void jacobi_gram_round_kernel(
     col[tid] = (blk < nb && c < N) ? c : -1;
   }
   __syncthreads();
-  // partial Gram matrices: lane (i, kk) feeds X[row][col_i] as both operands
+  // partial Gram of this wave's slabs: lane (i, kk) feeds X[row][col_{16h+i}]
   {
     const int i = lane & 15, kk = lane >> 4;
-    const int ci = col[i];
-    const double* __restrict__ xc = p.G + (int64_t)(ci >= 0 ? ci : 0) * M;
-    d4v acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = wave * 16; k0 < M; k0 += 64) {
-      double v[4];
+    const double* __restrict__ xc[kJn];
+    bool ok[kJn];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int r = k0 + 4 * kk + t;
-        v[t] = (ci >= 0 && r < M) ? xc[r] : 0.0;
-      }
+    for (int h = 0; h < kJn; ++h) {
+      const int c = col[16 * h + i];
+      ok[h] = c >= 0;
+      xc[h] = p.G + (int64_t)(c >= 0 ? c : 0) * M;
+    }
+    d4v acc[kJn][kJn];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t], v[t], acc, 0, 0, 0);
+    for (int a = 0; a < kJn; ++a)
+#pragma unroll
+      for (int b = 0; b < kJn; ++b) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int base = wave * 16; base < M; base += 16 * kJw * kSlabs) {
+      double v[kSlabs][4][kJn];
+#pragma unroll
+      for (int u = 0; u < kSlabs; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int r = base + 16 * kJw * u + 4 * kk + t;
+#pragma unroll
+          for (int h = 0; h < kJn; ++h) v[u][t][h] = (ok[h] && r < M) ? xc[h][r] : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < kSlabs; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int a = 0; a < kJn; ++a)
+#pragma unroll
+            for (int b = 0; b < kJn; ++b)
+              acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u][t][a], v[u][t][b], acc[a][b], 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Gp[wave][kk + 4 * v][i] = acc[v];
+    for (int a = 0; a < kJn; ++a)
+#pragma unroll
+      for (int b = 0; b < kJn; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Gp[wave][16 * a + kk + 4 * v][16 * b + i] = acc[a][b][v];
   }
   __syncthreads();
-  {
-    const int r = tid >> 4, c = tid & 15;
-    G[r][c] = (Gp[0][r][c] + Gp[1][r][c]) + (Gp[2][r][c] + Gp[3][r][c]);
+  for (int e = tid; e < kJc * kJc; e += blockDim.x) {
+    const int r = e / kJc, c = e % kJc;
+    double sum = Gp[0][r][c];
+    for (int w = 1; w < kJw; ++w) sum += Gp[w][r][c];
+    G[r][c] = sum;
     Q[r][c] = r == c ? 1.0 : 0.0;
   }
   __syncthreads();
   if (wave == 0) {
     const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
+    const double tol2 = tol * tol;
     const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
-    const int pp = lane >> 3, i0 = (lane & 7) * 2;
+    constexpr int kLpp = 128 / kJc;       // lanes per pair
+    constexpr int kRpl = kJc / kLpp;      // rows (and columns) per lane
+    const int pp = lane / kLpp, i0 = (lane % kLpp) * kRpl;
     int any = 0;
     for (int r = 0; r < kJc - 1; ++r) {
       int a, b;
@@ -641,17 +705,31 @@ __global__ __launch_bounds__(256) void jacobi_gram_round_kernel(
       if (a > b) { const int t = a; a = b; b = t; }
       const double al = G[a][a], be = G[b][b], gm = G[a][b];
       double cs = 1.0, sn = 0.0;
-      const bool doit = al > zero2 && be > zero2 && fabs(gm) > tol * sqrt(al) * sqrt(be);
+      const bool doit = al > zero2 && be > zero2 && gm * gm > tol2 * al * be;
       if (doit) {
-        const double zeta = (be - al) / (2.0 * gm);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + hypot(1.0, zeta));
-        cs = 1.0 / sqrt(1.0 + t * t);
-        sn = cs * t;
+        // tan(theta) = sgn(d) g / (|d| + hypot(d, g)) with d = be - al,
+        // g = 2 gm (the one-sided angle); hardware rsq/rcp with one Newton
+        // step each.  c = (1 + t^2)^-1/2 and s = c t keep c^2 + s^2 = 1 to a
+        // few ulp whatever the accuracy of t.
+        const double d = be - al, g = 2.0 * gm;
+        const double w = d * d + g * g;
+        double ri = __builtin_amdgcn_rsq(w);
+        ri = ri * (1.5 - 0.5 * w * ri * ri);
+        const double den = fabs(d) + w * ri;
+        double rc = __builtin_amdgcn_rcp(den);
+        rc = rc * (2.0 - den * rc);
+        const double t = (d >= 0.0 ? g : -g) * rc;
+        const double u = 1.0 + t * t;
+        double ci = __builtin_amdgcn_rsq(u);
+        ci = ci * (1.5 - 0.5 * u * ci * ci);
+        cs = ci;
+        sn = ci * t;
         any = 1;
       }
       wave_lds_sync();
       if (doit) {  // G <- G J, Q <- Q J  (columns a, b)
-        for (int ii = i0; ii < i0 + 2; ++ii) {
+#pragma unroll
+        for (int ii = i0; ii < i0 + kRpl; ++ii) {
           const double ga = G[ii][a], gb = G[ii][b];
           G[ii][a] = cs * ga - sn * gb;
           G[ii][b] = sn * ga + cs * gb;
@@ -662,7 +740,8 @@ __global__ __launch_bounds__(256) void jacobi_gram_round_kernel(
       }
       wave_lds_sync();
       if (doit) {  // G <- J^T G  (rows a, b)
-        for (int ii = i0; ii < i0 + 2; ++ii) {
+#pragma unroll
+        for (int ii = i0; ii < i0 + kRpl; ++ii) {
           const double ga = G[a][ii], gb = G[b][ii];
           G[a][ii] = cs * ga - sn * gb;
           G[b][ii] = sn * ga + cs * gb;
@@ -746,7 +825,6 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
                                                        int b) {
   extern __shared__ double smem[];
   __shared__ double red[8];
-  __shared__ double bcast[2];
   const QrJob p = probs[blockIdx.x];
   const int M = p.M, N = p.N;
   const int c0 = k * b;
@@ -762,9 +840,18 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
     pan = smem;
     ld = m;
     T = smem + (int64_t)m * bw;
-    for (int64_t e = tid; e < (int64_t)m * bw; e += blockDim.x) {
-      const int j = (int)(e / m), i = (int)(e % m);
-      pan[e] = Ag[i + (int64_t)j * M];
+    // one column per pass of each wave group, four loads in flight per thread
+    for (int j = tid / 128; j < bw; j += (int)blockDim.x / 128) {
+      const double* src = Ag + (int64_t)j * M;
+      double* dst = pan + (int64_t)j * m;
+      for (int i0 = tid % 128; i0 < m; i0 += 512) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = i0 + 128 * u < m ? src[i0 + 128 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i0 + 128 * u < m) dst[i0 + 128 * u] = v[u];
+      }
     }
   } else {
     pan = Ag;
@@ -773,6 +860,11 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   }
   double* tau = T + b * b;
   __syncthreads();
+  // Column j: every thread forms the reflector H = I - tau v v^T with
+  // v = (1, x[1:] * scal) from the block-reduced norm (same arithmetic, same
+  // order, so identical values), the trailing panel columns are updated in
+  // parallel (16 threads per column, shuffle-reduced dot), and x is scaled
+  // into v while the next column's norm is formed.  Two barriers per column.
   for (int j = 0; j < bw; ++j) {
     double* x = pan + (int64_t)j * ld + j;  // x[0 .. len-1], len = m - j
     const int len = m - j;
@@ -781,50 +873,71 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
     ss = wave_sum(ss);
     if (lane == 0) red[wave] = ss;
     __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += red[w];
-      const double alpha = x[0];
-      double tj = 0.0, beta = alpha, scal = 0.0;
-      if (t > 0.0) {  // reflector H = I - tau v v^T with v = (1, x[1:] / (alpha - beta))
-        beta = -copysign(sqrt(alpha * alpha + t), alpha);
-        tj = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      tau[j] = tj;
-      bcast[0] = scal;
-      bcast[1] = beta;
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[w];
+    const double alpha = x[0];
+    double tj = 0.0, beta = alpha, scal = 0.0;
+    if (t > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + t), alpha);
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
     }
-    __syncthreads();
-    const double tj = tau[j];
+    if (tid == 0) tau[j] = tj;
     if (tj != 0.0) {
-      const double scal = bcast[0];
-      for (int i = 1 + tid; i < len; i += blockDim.x) x[i] *= scal;
-    }
-    __syncthreads();
-    if (tid == 0) x[0] = bcast[1];
-    if (tj != 0.0) {
-      // remaining panel columns: y -= tau v (v^T y), one wave per column
-      for (int c = j + 1 + wave; c < bw; c += nw) {
+      const int c = j + 1 + (tid >> 4), sub = tid & 15;
+      if (c < bw) {
         double* y = pan + (int64_t)c * ld + j;
         double d = 0.0;
-        for (int i = 1 + lane; i < len; i += 64) d += x[i] * y[i];
-        d = (wave_sum(d) + y[0]) * tj;
-        for (int i = 1 + lane; i < len; i += 64) y[i] -= d * x[i];
-        if (lane == 0) y[0] -= d;
+        for (int i = 1 + sub; i < len; i += 16) d += x[i] * y[i];
+        d += __shfl_xor(d, 8, 64);
+        d += __shfl_xor(d, 4, 64);
+        d += __shfl_xor(d, 2, 64);
+        d += __shfl_xor(d, 1, 64);
+        d = (y[0] + scal * d) * tj;  // tau v^T y
+        const double ds = d * scal;
+        for (int i = 1 + sub; i < len; i += 16) y[i] -= ds * x[i];
+        if (sub == 0) y[0] -= d;
       }
     }
     __syncthreads();
+    if (tj != 0.0)
+      for (int i = 1 + tid; i < len; i += blockDim.x) x[i] *= scal;
+    if (tid == 0) x[0] = beta;
   }
-  // S = Y^T Y (strictly upper part) into the strictly lower triangle of T:
-  // y_l^T y_j = v_l[j] + sum_{i > j} v_l[i] v_j[i]  (l < j)
-  for (int pr = wave; pr < bw * bw; pr += nw) {
-    const int l = pr / bw, j = pr % bw;
-    if (l >= j) continue;
-    double d = 0.0;
-    for (int i = j + 1 + lane; i < m; i += 64) d += pan[(int64_t)l * ld + i] * pan[(int64_t)j * ld + i];
-    d = wave_sum(d);
-    if (lane == 0) T[j * b + l] = d + pan[(int64_t)l * ld + j];
+  __syncthreads();
+  // S = Y^T Y with FP64 MFMA, one wave per upper 16 x 16 block (a <= b);
+  // Y[r][c] = 0 (r < c), 1 (r = c), v_c[r] (r > c).  The strictly upper part
+  // of S goes to the strictly lower triangle of T (T[j][l] = S[l][j], l < j).
+  {
+    const int nbk = (bw + 15) >> 4;
+    int blk = wave, ba = 0, bb = 0;
+    for (int a = 0; a < nbk; ++a)
+      for (int c2 = a; c2 < nbk; ++c2) {
+        if (blk == 0) { ba = a; bb = c2; }
+        --blk;
+      }
+    if (blk < 0) {  // this wave owns block (ba, bb)
+      const int i = lane & 15, kk = lane >> 4;
+      const int cl = ba * 16 + i, cj = bb * 16 + i;
+      d4v acc = {0.0, 0.0, 0.0, 0.0};
+      for (int base = 0; base < m; base += 16) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int r = base + 4 * kk + t;
+          double ya = 0.0, yb = 0.0;
+          if (r < m) {
+            if (cl < bw) ya = r < cl ? 0.0 : (r == cl ? 1.0 : pan[(int64_t)cl * ld + r]);
+            if (cj < bw) yb = r < cj ? 0.0 : (r == cj ? 1.0 : pan[(int64_t)cj * ld + r]);
+          }
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, yb, acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int l = ba * 16 + kk + 4 * v, j = bb * 16 + i;
+        if (l < j && j < bw) T[j * b + l] = acc[v];
+      }
+    }
   }
   __syncthreads();
   // T upper triangular (dlarft): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
@@ -952,7 +1065,7 @@ void GemmBatch::add(const double* A, int64_t lda, const double* B, int64_t ldb, 
 
 int default_split_k() {
   const char* e = std::getenv("DMRGCC_SPLIT_K");
-  return e ? std::max(0, std::atoi(e)) : 512;
+  return e ? std::max(0, std::atoi(e)) : 256;
 }
 
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
@@ -1391,7 +1504,7 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     h.insert(h.end(), prefix.begin(), prefix.end());
     DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     for (int r = 0; r < rounds; ++r)
-      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(256), 0, stream,
+      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(64 * kJw), 0, stream,
                          scratch.data(), dactive, dprefix, na, r, rotated);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -19,7 +19,10 @@
  *    p2), lb*rb*d*d doubles.
  *  - precision: 0 = FP64 (MFMA f64), 1 = FP32 (MFMA f32),
  *               2 = BF16x3 (split bf16 MFMA, FP32 accumulate).
- *  - stream: a hipStream_t, or NULL for the library's own stream.
+ *  - stream: a hipStream_t, or NULL for the library's own stream (a
+ *    non-blocking stream: the caller finishes the work that produces the
+ *    inputs before the call).  Every entry point has completed its device
+ *    work when it returns.
  *  - Not thread-safe per handle.
  */
 #ifndef DMRGCC_HIP_H_
