@@ -40,6 +40,21 @@ def table(maxm, sweeps, niter=10, canpre=1e-10, maxb=400):
     return [(maxm, maxb, niter, canpre, noise[i]) for i in range(2 * sweeps)]
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_stage2_bf16x3_8x4_m1000_traffic.json")
+
+
+def traffic_bytes():
+    """HBM bytes per stage-2 launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE on this workload and scaled by the same counters on a copy
+    kernel of known size (profiles/heff_stage2_bf16x3_8x4_m1000_traffic.json,
+    which gives the commands); None when that measurement is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(run, site, model, flops, budget_s, maxm):
     """Time the oracle's numpy matvec on the state of `site` (bounded)."""
     from oracle.blocks import BlockMat, Lattice
@@ -160,7 +175,9 @@ def main():
         "config": {"workload": "8x4 Bose-Hubbard cylinder (L=32, N=32, d=4, t=1, U=6, periodic y), "
                                "m=%d, centre-site (%d) H_eff matvec" % (args.maxm, site),
                    "sweeps": args.sweeps, "psi_dim": info["psi_dim"], "flops_ref": info["flops_ref"],
-                   "flops_exec": info["flops_exec"], "tiles": info["tiles"]},
+                   "flops_exec": info["flops_exec"], "tiles": info["tiles"],
+                   "tiles_stage1": info["tiles_stage1"], "tiles_stage2": info["tiles_stage2"],
+                   "bytes_min": info["bytes_min"]},
         "fp64": {"tflops": rate("fp64"), "ms": tm["fp64"]["matvec_ms"]},
         "fp32": {"tflops": rate("fp32"), "ms": tm["fp32"]["matvec_ms"]},
         "sweep": {
@@ -173,7 +190,7 @@ def main():
         },
         "roofline": {"bound": "mfma", "kernel": "gemm_bf16x3_kernel (H_eff stage 2)",
                      "achieved": achieved, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["bf16"], "traffic": None,
+                     "frac": achieved / PEAK_TFLOPS["bf16"], "traffic": traffic_bytes(),
                      "stage2_ms": bf["stage2_ms"], "stage2_flops": s2_flops},
         "roofline_fp64": {"bound": "mfma", "kernel": "gemm_mfma_kernel<double> (H_eff stage 2)",
                           "achieved": s2_flops / (tm["fp64"]["stage2_ms"] * 1e-3) / 1e12,

@@ -1634,11 +1634,21 @@ void RealTensorRundmrg::Step(int h, int j) {
 
 bool RealTensorRundmrg::Advance(int stop_h, int stop_j) {
   if (!started_) Start();
+  if (paused_at_ >= 0.0) {  // wall time between Advance calls is not sweep time
+    const double gap = now() - paused_at_;
+    half_t0_ += gap;
+    t0_ += gap;
+    paused_at_ = -1.0;
+  }
   const int nhalf = (int)p_.maxm.size();
   while (cur_h_ < nhalf) {
     const bool l2r = (cur_h_ % 2) == 0;
     if (!l2r && cur_j_ == 0) cur_j_ = L_sites_ - 1;
-    if (cur_h_ == stop_h && cur_j_ == stop_j) return true;
+    if (cur_h_ == stop_h && cur_j_ == stop_j) {
+      cx_.sync();
+      paused_at_ = now();
+      return true;
+    }
     Step(cur_h_, cur_j_);
     if (l2r) {
       if (++cur_j_ == L_sites_ - 1) { ++cur_h_; cur_j_ = 0; }
@@ -1652,7 +1662,7 @@ bool RealTensorRundmrg::Advance(int stop_h, int stop_j) {
 RunResult RealTensorRundmrg::Result() {
   cx_.sync();
   RunResult res = res_;
-  res.total_seconds = now() - t0_;
+  res.total_seconds = (paused_at_ >= 0.0 ? paused_at_ : now()) - t0_;
   res.stats = stats_;
   res.phase = ph_;
   res.rng_draws = cx_.rng.count();

@@ -173,6 +173,7 @@ class RealTensorRundmrg {
   // (DefineTensorSpace + Initialize); Advance(h, j) runs site steps until the
   // Lanczos solve of site j in half sweep h is next (returns true) or the
   // sweep table is exhausted (returns false, pass h = -1 to run to the end).
+  // Wall time spent between Advance calls is excluded from the sweep times.
   void Start();
   bool Advance(int stop_h, int stop_j);
   RunResult Result();
@@ -216,6 +217,7 @@ class RealTensorRundmrg {
   bool started_ = false;
   int cur_h_ = 0, cur_j_ = 0;
   double half_e_ = 0, half_terr_ = 0, half_t0_ = 0, t0_ = 0;
+  double paused_at_ = -1.0;  // set while stopped between Advance calls
   RunResult res_;
 
   const RealTensorSpace& space_;

@@ -26,6 +26,7 @@ namespace dmrg {
 namespace {
 
 constexpr int kTile = 64;     // output tile edge
+constexpr int kXcds = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kThreads = 256; // 4 waves
 
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -150,6 +151,7 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
   typedef typename MfmaAcc<T>::V V;
 
   const GemmTile tl = tiles[blockIdx.x];
+  if (tl.item < 0) return;  // padding of the XCD-interleaved order
   const GemmItem it = items[tl.item];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -281,6 +283,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
   __shared__ __attribute__((aligned(16))) __bf16 Bl[kTile][LD];
 
   const GemmTile tl = tiles[blockIdx.x];
+  if (tl.item < 0) return;  // padding of the XCD-interleaved order
   const GemmItem it = items[tl.item];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -1131,6 +1134,11 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
     for (int c = items[i].c0; c < items[i].c1; ++c) k += contribs[c].K;
     cost[i] = k;
   }
+  // XCD-aware order: workgroup w of a launch runs on XCD w % kXcds, and each
+  // XCD has its own L2.  Every item's tiles go to one XCD queue (greedy by
+  // cost, largest first) so the tiles that share A rows and B columns share
+  // an L2; the queues are interleaved into the launch order, padded with
+  // empty tiles (item -1) where a queue runs short.
   std::vector<int> ends = group_items;
   if (ends.empty()) ends.push_back((int)items.size());
   int first = 0;
@@ -1138,13 +1146,26 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
     std::vector<int> order;
     for (int i = first; i < end; ++i) order.push_back(i);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<GemmTile>> q(kXcds);
+    std::vector<double> load(kXcds, 0.0);
     for (int i : order) {
       const GemmItem& it = items[i];
       if (it.M <= 0 || it.N <= 0) continue;
       if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      int nt = 0;
       for (int m0 = 0; m0 < it.M; m0 += tile_m)
-        for (int n0 = 0; n0 < it.N; n0 += tile_n) tiles.push_back(GemmTile{i, m0, n0, 0});
+        for (int n0 = 0; n0 < it.N; n0 += tile_n, ++nt) q[x].push_back(GemmTile{i, m0, n0, 0});
+      load[x] += (cost[i] + 32.0) * nt;
     }
+    size_t depth = 0;
+    for (const auto& v : q) depth = std::max(depth, v.size());
+    for (size_t k = 0; k < depth; ++k)
+      for (int x = 0; x < kXcds; ++x)
+        tiles.push_back(k < q[x].size() ? q[x][k] : GemmTile{-1, 0, 0, 0});
+    // trailing empty tiles of the last row are not launched
+    while (!tiles.empty() && tiles.back().item < 0 && (int)tiles.size() > (group_tiles.empty() ? 0 : group_tiles.back()))
+      tiles.pop_back();
     group_tiles.push_back((int)tiles.size());
     first = end;
   }
