@@ -405,11 +405,13 @@ class HeffPlan:
                 "flops_stage2": v[7], "tiles_stage1": int(v[8]), "tiles_stage2": int(v[9])}
 
     def time(self, precision=FP64, reps=20):
-        """Mean milliseconds per stage-1 launch, stage-2 launch and whole
-        matvec (HIP events on the plan's stream)."""
-        ms = np.zeros(3)
+        """Mean milliseconds per stage-1 launch, stage-2 launch (with its
+        split-K reduction), whole matvec, and each stage's GEMM kernel alone
+        (HIP events on the plan's stream)."""
+        ms = np.zeros(5)
         _check(lib().dmrgcc_heff_time(self._h, precision, reps, _dp(ms)))
-        return {"stage1_ms": ms[0], "stage2_ms": ms[1], "matvec_ms": ms[2]}
+        return {"stage1_ms": ms[0], "stage2_ms": ms[1], "matvec_ms": ms[2],
+                "stage1_gemm_ms": ms[3], "stage2_gemm_ms": ms[4]}
 
     def lanczos(self, ket, num_iter, precision=FP64, refine=True):
         e = ctypes.c_double()

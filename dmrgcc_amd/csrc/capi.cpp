@@ -630,10 +630,22 @@ int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms) {
     DMRG_HIP(hipEventRecord(e[1], s));
     DMRG_HIP(hipEventSynchronize(e[1]));
     DMRG_HIP(hipEventElapsedTime(&tm, e[0], e[1]));
+    // each stage's GEMM kernel alone, back to back (what a kernel trace
+    // reports per dispatch)
+    float g[2] = {0, 0};
+    for (int st = 1; st <= 2; ++st) {
+      DMRG_HIP(hipEventRecord(e[0], s));
+      for (int r = 0; r < reps; ++r) h->plan.apply_stage(st, x.data(), y.data(), precision, s, true);
+      DMRG_HIP(hipEventRecord(e[1], s));
+      DMRG_HIP(hipEventSynchronize(e[1]));
+      DMRG_HIP(hipEventElapsedTime(&g[st - 1], e[0], e[1]));
+    }
     for (auto& ev : e) (void)hipEventDestroy(ev);
     ms[0] = t1 / reps;
     ms[1] = t2 / reps;
     ms[2] = tm / reps;
+    ms[3] = g[0] / reps;
+    ms[4] = g[1] / reps;
   });
 }
 

@@ -328,9 +328,10 @@ void HeffPlan::apply(const double* x, double* y, int precision, hipStream_t stre
   s2_.launch(stream, precision, nullptr, y);
 }
 
-void HeffPlan::apply_stage(int stage, const double* x, double* y, int precision, hipStream_t stream) {
-  if (stage == 1) s1_.launch(stream, precision, x, nullptr);
-  else s2_.launch(stream, precision, nullptr, y);
+void HeffPlan::apply_stage(int stage, const double* x, double* y, int precision,
+                           hipStream_t stream, bool gemm_only) {
+  if (stage == 1) s1_.launch(stream, precision, x, nullptr, !gemm_only);
+  else s2_.launch(stream, precision, nullptr, y, !gemm_only);
 }
 
 // ===========================================================================

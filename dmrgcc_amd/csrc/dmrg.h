@@ -88,8 +88,9 @@ class HeffPlan {
              hipStream_t stream);
   // y (+)= H x ; x, y in ket-vector layout (psi_dim elements).
   void apply(const double* x, double* y, int precision, hipStream_t stream);
-  // launch one stage only (timing)
-  void apply_stage(int stage, const double* x, double* y, int precision, hipStream_t stream);
+  // launch one stage only (timing); gemm_only skips the split-K reduction
+  void apply_stage(int stage, const double* x, double* y, int precision, hipStream_t stream,
+                   bool gemm_only = false);
   double flops() const { return s1_.flops + s2_.flops; }  // executed
   double fref() const { return fref_; }                      // reference order
   int64_t psi_dim() const { return psi_dim_; }

@@ -101,8 +101,9 @@ struct GemmBatch {
   int end_group() { group_items.push_back((int)items.size()); return (int)group_items.size() - 1; }
   // Finish construction: tile every item and upload the descriptors.
   void finalize(hipStream_t stream, int tile_m = 64, int tile_n = 64);
+  // reduce = false launches the GEMM kernel only (timing of that kernel)
   void launch(hipStream_t stream, int precision, const double* in = nullptr,
-              double* out = nullptr) const;
+              double* out = nullptr, bool reduce = true) const;
   void launch_group(hipStream_t stream, int group, int precision = 0) const;
   bool empty() const { return tiles.empty(); }
 };

@@ -1175,7 +1175,8 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   if (!splits.empty()) upload(d_splits, splits, stream);
 }
 
-void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out) const {
+void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out,
+                       bool reduce) const {
   if (tiles.empty()) return;
   const dim3 grid((unsigned)tiles.size()), block(kThreads);
   if (precision == kFP64)
@@ -1190,7 +1191,7 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   else
     throw Error(kErrArgument, "unknown precision");
   DMRG_HIP(hipGetLastError());
-  if (!splits.empty()) {
+  if (reduce && !splits.empty()) {
     int64_t mx = 0;
     for (const SplitTask& t : splits) mx = std::max<int64_t>(mx, (int64_t)t.M * t.N);
     const unsigned gx = (unsigned)std::min<int64_t>(64, (mx + 255) / 256);

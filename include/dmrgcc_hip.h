@@ -199,8 +199,10 @@ int dmrgcc_dmrg_result(dmrgcc_dmrg* h, int* nhalf_done, double* energies, double
 int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h);
 
 /* Time `reps` H_eff applications (y += H x on internal buffers) with HIP
- * events on the plan's stream.  ms[0] = mean stage-1 (T = K R) launch,
- * ms[1] = mean stage-2 (out += w L T) launch, ms[2] = mean whole matvec. */
+ * events on the plan's stream.  ms (5 entries): [0] mean stage-1 (T = K R)
+ * launch, [1] mean stage-2 (out += w L T) launch including its split-K
+ * reduction, [2] mean whole matvec, back to back, [3] / [4] mean stage-1 /
+ * stage-2 GEMM kernel alone, back to back. */
 int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms);
 
 #ifdef __cplusplus
