@@ -54,6 +54,21 @@ _SIGS = {
                                           _c_int_p, _c_i64_p, _vp,
                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
                                           _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
+    "dmrgcc_zheff_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p,
+                                           ctypes.c_int, ctypes.c_int, _c_int_p,
+                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                           _c_int_p, _c_i64_p, _vp,
+                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                           _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
+    "dmrgcc_zblock_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp), _c_int_p, _c_int_p, _c_int_p,
+                                          ctypes.c_int, _vp]),
+    "dmrgcc_zenv_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _c_dbl_p, ctypes.c_int, ctypes.c_int, _c_int_p,
+                                          ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _vp,
+                                          ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                          _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
     "dmrgcc_heff_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "dmrgcc_heff_info": (ctypes.c_int, [_vp, _c_dbl_p]),
     "dmrgcc_heff_destroy": (ctypes.c_int, [_vp]),
@@ -337,15 +352,25 @@ def _sync_torch():
 
 
 def block_gemm(As, Bs, Cs, precision=FP64, stream=None):
-    """C_i = A_i @ B_i for lists of contiguous float64 device tensors."""
+    """C_i = A_i @ B_i for lists of contiguous float64 (or complex128:
+    ComplexMatrix::MultiplyToMatrix) device tensors."""
     n = len(As)
     arr = lambda ts: (_vp * max(n, 1))(*[_vp(t.data_ptr()) for t in ts])
     M = _i32([a.shape[0] for a in As])
     K = _i32([a.shape[1] for a in As])
     N = _i32([b.shape[1] for b in Bs])
+    fn = lib().dmrgcc_zblock_gemm if n and As[0].is_complex() else lib().dmrgcc_block_gemm
     _sync_torch()
-    _check(lib().dmrgcc_block_gemm(n, arr(As), arr(Bs), arr(Cs), _ip(M), _ip(N), _ip(K), precision,
-                                   _vp(stream) if stream else _vp()))
+    _check(fn(n, arr(As), arr(Bs), arr(Cs), _ip(M), _ip(N), _ip(K), precision,
+              _vp(stream) if stream else _vp()))
+
+
+def _w_arg(W):
+    """MPO site tensor as the ABI's double array: (real, float64) or
+    (complex, interleaved float64 view of complex128)."""
+    if np.iscomplexobj(W):
+        return True, np.ascontiguousarray(W, dtype=np.complex128).view(np.float64)
+    return False, _f64(W)
 
 
 class BlockList:
@@ -364,16 +389,19 @@ class HeffPlan:
     """H_eff matvec plan of one site (MultiplyEffectHamilton)."""
 
     def __init__(self, W, phys, ket_l, ket_r, ket_rows, ket_cols, Lb, L_data, Rb, R_data, stream=None):
-        W = _f64(W)
+        """A complex W (ComplexTensorContraction) makes a complex plan: L_data,
+        R_data and the vectors are then complex128 device tensors."""
         lb, rb, d = W.shape[0], W.shape[1], W.shape[2]
+        self.complex, W = _w_arg(W)
         phys = _i32(phys)
         nl, nr = phys.shape
         self._keep = (W, phys, _i32(ket_l), _i32(ket_r), _i32(ket_rows), _i32(ket_cols), Lb, Rb,
                       L_data, R_data)
         _, _, kl, kr, krows, kcols, _, _, _, _ = self._keep
         h = _vp()
+        fn = lib().dmrgcc_zheff_create if self.complex else lib().dmrgcc_heff_create
         _sync_torch()
-        _check(lib().dmrgcc_heff_create(
+        _check(fn(
             d, lb, rb, _dp(W), nl, nr, _ip(phys), len(kl), _ip(kl), _ip(kr), _ip(krows), _ip(kcols),
             len(Lb), _ip(Lb.o), _ip(Lb.l), _ip(Lb.r), _ip(Lb.rows), _ip(Lb.cols), _lp(Lb.off),
             _ptr(L_data), len(Rb), _ip(Rb.o), _ip(Rb.l), _ip(Rb.r), _ip(Rb.rows), _ip(Rb.cols),
@@ -395,6 +423,7 @@ class HeffPlan:
         self = cls.__new__(cls)
         self._h = h
         self._keep = owner
+        self.complex = False
         return self
 
     def info(self):
@@ -425,23 +454,25 @@ class HeffPlan:
 def env_update(leigh, W, phys, ket_l, ket_r, ket_rows, ket_cols, ket_data, Eb, E_data, stream=None):
     """Left (leigh 0) / right (leigh 1) environment update.  Returns
     (list of (o, l, r, rows, cols, offset), device data pointer holder)."""
-    W = _f64(W)
     lb, rb, d = W.shape[0], W.shape[1], W.shape[2]
+    cplx, W = _w_arg(W)
     phys = _i32(phys)
     nl, nr = phys.shape
     kl, kr, krows, kcols = _i32(ket_l), _i32(ket_r), _i32(ket_rows), _i32(ket_cols)
     h = _vp()
+    fn = lib().dmrgcc_zenv_update if cplx else lib().dmrgcc_env_update
     _sync_torch()
-    _check(lib().dmrgcc_env_update(
+    _check(fn(
         leigh, d, lb, rb, _dp(W), nl, nr, _ip(phys), len(kl), _ip(kl), _ip(kr), _ip(krows),
         _ip(kcols), _ptr(ket_data), len(Eb), _ip(Eb.o), _ip(Eb.l), _ip(Eb.r), _ip(Eb.rows),
         _ip(Eb.cols), _lp(Eb.off), _ptr(E_data), _vp(stream) if stream else _vp(), ctypes.byref(h)))
-    return EnvResult(h)
+    return EnvResult(h, cplx)
 
 
 class EnvResult:
-    def __init__(self, h):
+    def __init__(self, h, cplx=False):
         self._h = h
+        self.complex = cplx
         n = lib().dmrgcc_blocks_count(h)
         self.blocks = []
         o, l, r, rows, cols = (ctypes.c_int() for _ in range(5))
@@ -454,9 +485,11 @@ class EnvResult:
         self.size = sum(b[3] * b[4] for b in self.blocks)
 
     def to_host(self):
-        out = np.zeros(max(self.size, 1))
+        cs = 2 if self.complex else 1
+        out = np.zeros(max(cs * self.size, 1))
         _check(lib().dmrgcc_blocks_download(self._h, _dp(out)))
-        return out[:self.size]
+        out = out[:cs * self.size]
+        return out.view(np.complex128) if self.complex else out
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

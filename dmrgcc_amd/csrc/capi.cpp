@@ -37,6 +37,7 @@ struct dmrgcc_dmrg {
 struct dmrgcc_blocks {
   std::vector<int> o, l, r, rows, cols;
   std::vector<int64_t> off;
+  int cs = 1;  // 2: complex entries, interleaved
   DevBuf<double> data;
 };
 
@@ -56,16 +57,26 @@ int guarded(F&& f) {
   }
 }
 
-SiteOperator make_op(int d, int lb, int rb, const double* W) {
+// cs = 2: W holds interleaved complex entries
+SiteOperator make_op(int d, int lb, int rb, const double* W, int cs = 1) {
   require(W != nullptr && d > 0 && lb > 0 && rb > 0, "invalid MPO site tensor", kErrArgument);
   SiteOperator op;
   op.d = d; op.lb = lb; op.rb = rb;
-  op.w.assign(W, W + (size_t)lb * rb * d * d);
+  const size_t ne = (size_t)lb * rb * d * d;
+  if (cs == 1) {
+    op.w.assign(W, W + ne);
+  } else {
+    op.w.resize(ne);
+    op.wi.resize(ne);
+    for (size_t e = 0; e < ne; ++e) { op.w[e] = W[2 * e]; op.wi[e] = W[2 * e + 1]; }
+  }
   return op;
 }
 
 void make_lattice(Lattice& lat, int d, int nl, int nr, const int* phys, int nket, const int* kl,
-                  const int* kr, const int* krows, const int* kcols, const double* data) {
+                  const int* kr, const int* krows, const int* kcols, const double* data,
+                  int cs = 1) {
+  lat.ket.cs = cs;
   require(nl >= 0 && nr >= 0 && nket >= 0, "negative sizes", kErrArgument);
   lat.d = d;
   lat.lb.resize(nl);
@@ -91,14 +102,72 @@ void make_lattice(Lattice& lat, int d, int nl, int nr, const int* phys, int nket
 }
 
 Env make_env(int nbond, int n, const int* o, const int* l, const int* r, const int* rows,
-             const int* cols, const int64_t* off, const double* data) {
+             const int* cols, const int64_t* off, const double* data, int cs = 1) {
   Env e(nbond);
   for (int i = 0; i < n; ++i) {
     require(o[i] >= 0 && o[i] < nbond, "environment channel out of range");
     e[o[i]].blk.push_back(Block{l[i], r[i], rows[i], cols[i], off[i]});
   }
-  for (auto& b : e) b.ext = data;
+  for (auto& b : e) { b.ext = data; b.cs = cs; }
   return e;
+}
+
+int heff_create(int cs, int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
+                int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                const int* ket_cols, int nL, const int* L_o, const int* L_l, const int* L_r,
+                const int* L_rows, const int* L_cols, const int64_t* L_off, const double* L_data,
+                int nR, const int* R_o, const int* R_l, const int* R_r, const int* R_rows,
+                const int* R_cols, const int64_t* R_off, const double* R_data, void* stream,
+                dmrgcc_heff** out) {
+  return guarded([&] {
+    require(out != nullptr, "null output", kErrArgument);
+    auto h = std::make_unique<dmrgcc_heff>();
+    h->cx = std::make_unique<Context>(static_cast<hipStream_t>(stream));
+    h->W = make_op(d, lb, rb, W, cs);
+    make_lattice(h->lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, nullptr, cs);
+    h->L = make_env(lb, nL, L_o, L_l, L_r, L_rows, L_cols, L_off, L_data, cs);
+    h->R = make_env(rb, nR, R_o, R_l, R_r, R_rows, R_cols, R_off, R_data, cs);
+    h->plan.build(h->lat, h->W, h->L, h->R, h->cx->stream);
+    *out = h.release();
+  });
+}
+
+int env_update(int cs, int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
+               const int* phys, int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+               const int* ket_cols, const double* ket_data, int nE, const int* E_o, const int* E_l,
+               const int* E_r, const int* E_rows, const int* E_cols, const int64_t* E_off,
+               const double* E_data, void* stream, dmrgcc_blocks** out) {
+  return guarded([&] {
+    require(out && (leigh == 0 || leigh == 1), "bad arguments", kErrArgument);
+    Context cx(static_cast<hipStream_t>(stream));
+    SiteOperator op = make_op(d, lb, rb, W, cs);
+    Lattice lat;
+    make_lattice(lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, ket_data, cs);
+    Env e = make_env(leigh == 0 ? lb : rb, nE, E_o, E_l, E_r, E_rows, E_cols, E_off, E_data, cs);
+    Env res = leigh == 0 ? left_env_update(lat, op, e, cx) : right_env_update(lat, op, e, cx);
+    auto b = std::make_unique<dmrgcc_blocks>();
+    b->cs = cs;
+    int64_t total = 0;
+    for (const auto& t : res) total += t.size;
+    b->data.resize((size_t)std::max<int64_t>(cs * total, 1));
+    int64_t acc = 0;
+    for (size_t o = 0; o < res.size(); ++o) {
+      for (const Block& k : res[o].blk) {
+        b->o.push_back((int)o);
+        b->l.push_back(k.l);
+        b->r.push_back(k.r);
+        b->rows.push_back(k.rows);
+        b->cols.push_back(k.cols);
+        b->off.push_back(acc + k.off);
+      }
+      if (res[o].size > 0)
+        DMRG_HIP(hipMemcpyAsync(b->data.data() + cs * acc, res[o].data.data(),
+                                cs * res[o].size * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
+      acc += res[o].size;
+    }
+    cx.sync();
+    *out = b.release();
+  });
 }
 
 }  // namespace
@@ -271,6 +340,24 @@ int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, dou
   });
 }
 
+int dmrgcc_zblock_gemm(int n, const double* const* A, const double* const* B, double* const* C,
+                       const int* M, const int* N, const int* K, int precision, void* stream) {
+  return guarded([&] {
+    require(n >= 0 && (n == 0 || (A && B && C && M && N && K)), "bad arguments", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GemmBatch gb;
+    for (int i = 0; i < n; ++i) {
+      require(M[i] >= 0 && N[i] >= 0 && K[i] >= 0, "negative GEMM size");
+      zgemm_item(gb, C[i], N[i], M[i], N[i], 0, 0,
+                 {ZTerm{A[i], K[i], B[i], N[i], K[i], 0, false, false, 1.0, 0.0}});
+    }
+    gb.finalize(s);
+    gb.launch(s, precision);
+    DMRG_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
                        int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
                        const int* ket_cols, int nL, const int* L_o, const int* L_l,
@@ -279,17 +366,22 @@ int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, c
                        const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
                        const int64_t* R_off, const double* R_data, void* stream,
                        dmrgcc_heff** out) {
-  return guarded([&] {
-    require(out != nullptr, "null output", kErrArgument);
-    auto h = std::make_unique<dmrgcc_heff>();
-    h->cx = std::make_unique<Context>(static_cast<hipStream_t>(stream));
-    h->W = make_op(d, lb, rb, W);
-    make_lattice(h->lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, nullptr);
-    h->L = make_env(lb, nL, L_o, L_l, L_r, L_rows, L_cols, L_off, L_data);
-    h->R = make_env(rb, nR, R_o, R_l, R_r, R_rows, R_cols, R_off, R_data);
-    h->plan.build(h->lat, h->W, h->L, h->R, h->cx->stream);
-    *out = h.release();
-  });
+  return heff_create(1, d, lb, rb, W, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, nL, L_o,
+                     L_l, L_r, L_rows, L_cols, L_off, L_data, nR, R_o, R_l, R_r, R_rows, R_cols,
+                     R_off, R_data, stream, out);
+}
+
+int dmrgcc_zheff_create(int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
+                        int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                        const int* ket_cols, int nL, const int* L_o, const int* L_l,
+                        const int* L_r, const int* L_rows, const int* L_cols,
+                        const int64_t* L_off, const double* L_data, int nR, const int* R_o,
+                        const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
+                        const int64_t* R_off, const double* R_data, void* stream,
+                        dmrgcc_heff** out) {
+  return heff_create(2, d, lb, rb, W, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, nL, L_o,
+                     L_l, L_r, L_rows, L_cols, L_off, L_data, nR, R_o, R_l, R_r, R_rows, R_cols,
+                     R_off, R_data, stream, out);
 }
 
 int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision) {
@@ -356,7 +448,7 @@ int dmrgcc_blocks_download(const dmrgcc_blocks* b, double* host) {
     int64_t total = 0;
     for (size_t i = 0; i < b->rows.size(); ++i) total += (int64_t)b->rows[i] * b->cols[i];
     if (total > 0)
-      DMRG_HIP(hipMemcpy(host, b->data.data(), total * sizeof(double), hipMemcpyDeviceToHost));
+      DMRG_HIP(hipMemcpy(host, b->data.data(), b->cs * total * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 
@@ -371,36 +463,18 @@ int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl,
                       const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
                       const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
                       dmrgcc_blocks** out) {
-  return guarded([&] {
-    require(out && (leigh == 0 || leigh == 1), "bad arguments", kErrArgument);
-    Context cx(static_cast<hipStream_t>(stream));
-    SiteOperator op = make_op(d, lb, rb, W);
-    Lattice lat;
-    make_lattice(lat, d, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols, ket_data);
-    Env e = make_env(leigh == 0 ? lb : rb, nE, E_o, E_l, E_r, E_rows, E_cols, E_off, E_data);
-    Env res = leigh == 0 ? left_env_update(lat, op, e, cx) : right_env_update(lat, op, e, cx);
-    auto b = std::make_unique<dmrgcc_blocks>();
-    int64_t total = 0;
-    for (const auto& t : res) total += t.size;
-    b->data.resize((size_t)std::max<int64_t>(total, 1));
-    int64_t acc = 0;
-    for (size_t o = 0; o < res.size(); ++o) {
-      for (const Block& k : res[o].blk) {
-        b->o.push_back((int)o);
-        b->l.push_back(k.l);
-        b->r.push_back(k.r);
-        b->rows.push_back(k.rows);
-        b->cols.push_back(k.cols);
-        b->off.push_back(acc + k.off);
-      }
-      if (res[o].size > 0)
-        DMRG_HIP(hipMemcpyAsync(b->data.data() + acc, res[o].data.data(), res[o].size * sizeof(double),
-                                hipMemcpyDeviceToDevice, cx.stream));
-      acc += res[o].size;
-    }
-    cx.sync();
-    *out = b.release();
-  });
+  return env_update(1, leigh, d, lb, rb, W, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols,
+                    ket_data, nE, E_o, E_l, E_r, E_rows, E_cols, E_off, E_data, stream, out);
+}
+
+int dmrgcc_zenv_update(int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
+                       const int* phys, int nket, const int* ket_l, const int* ket_r,
+                       const int* ket_rows, const int* ket_cols, const double* ket_data, int nE,
+                       const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
+                       const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
+                       dmrgcc_blocks** out) {
+  return env_update(2, leigh, d, lb, rb, W, nl, nr, phys, nket, ket_l, ket_r, ket_rows, ket_cols,
+                    ket_data, nE, E_o, E_l, E_r, E_rows, E_cols, E_off, E_data, stream, out);
 }
 
 int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N, double* const* U,
@@ -602,7 +676,7 @@ int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h) {
 int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms) {
   return guarded([&] {
     require(h && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
-    const int64_t n = std::max<int64_t>(h->plan.psi_dim(), 1);
+    const int64_t n = std::max<int64_t>(h->plan.doubles(), 1);
     DevBuf<double> x((size_t)n), y((size_t)n);
     fill(x.data(), n, 1e-3, h->cx->stream);
     fill(y.data(), n, 0.0, h->cx->stream);

@@ -123,7 +123,7 @@ void BlockTensor::allocate() {
     acc += (int64_t)b.rows * b.cols;
   }
   size = acc;
-  data.resize((size_t)acc);
+  data.resize((size_t)(cs * acc));
 }
 
 int BlockTensor::find(int l, int r) const {
@@ -228,10 +228,14 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
   s2_.clear();
   const BlockTensor& ket = lat.ket;
   psi_dim_ = ket.size;
+  // complex plans (ComplexTensorContraction::MultiplyEffectHamilton,
+  // complex_tensor_contraction.cc:1244-1312): same task structure, every
+  // complex product expanded into real MFMA terms on interleaved views
+  cplx_ = ket.cs == 2;
   std::vector<std::vector<int>> same_r(lat.nr());
   for (int r = 0; r < lat.nr(); ++r) same_r[r] = ket.same(1, r);
 
-  struct Out { int o1, lp; double w; };
+  struct Out { int o1, lp; double w, wi; };
   struct Task { int o2, ri, kp, jdx; int64_t toff; std::vector<Out> outs; std::vector<int> ops; };
   std::vector<Task> tasks;
   int64_t tsize = 0;
@@ -252,11 +256,10 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
           const int idx = ket.blk[op].l;
           const int p1 = lat.phys[idx][jdx];
           for (int o1 = 0; o1 < W.lb; ++o1) {
-            const double w = W.at(o1, o2, p1, p2);
-            if (w != 0.0) {
+            if (W.used(o1, o2, p1, p2)) {
               const int lp = L[o1].find(idx, kdx);
               if (lp != -1) {
-                t.outs.push_back(Out{o1, lp, w});
+                t.outs.push_back(Out{o1, lp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
                 t.ops.push_back(op);
               }
             }
@@ -270,16 +273,22 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
       }
     }
   }
-  T_.resize((size_t)std::max<int64_t>(tsize, 1));
+  const int cs = cplx_ ? 2 : 1;
+  T_.resize((size_t)std::max<int64_t>(cs * tsize, 1));
   // stage 1: T = K * R  (K relative to the input vector)
   for (const Task& t : tasks) {
     if (t.outs.empty()) continue;
     const Block& kb = ket.blk[t.kp];
     const BlockTensor& Rb = R[t.o2];
     const Block& rbk = Rb.blk[t.ri];
+    const double* krel = reinterpret_cast<const double*>((intptr_t)kb.off);
+    if (cplx_) {
+      zgemm_item(s1_, T_.data() + cs * t.toff, rbk.cols, kb.rows, rbk.cols, 0, 0,
+                 {ZTerm{krel, kb.cols, Rb.ptr(t.ri), rbk.cols, kb.cols, kRelA, false, false, 1.0, 0.0}});
+      continue;
+    }
     s1_.begin_item(T_.data() + t.toff, rbk.cols, kb.rows, rbk.cols, 0);
-    s1_.add(reinterpret_cast<const double*>((intptr_t)kb.off), kb.cols, Rb.ptr(t.ri), rbk.cols,
-            kb.cols, kRelA, 1.0);
+    s1_.add(krel, kb.cols, Rb.ptr(t.ri), rbk.cols, kb.cols, kRelA, 1.0);
   }
   // stage 2: group terms by output block, keeping the reference's term order
   std::vector<std::vector<std::pair<int, int>>> by_out(ket.n());  // (task, term)
@@ -288,7 +297,9 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
   for (int op = 0; op < ket.n(); ++op) {
     if (by_out[op].empty()) continue;
     const Block& ob = ket.blk[op];
-    s2_.begin_item(reinterpret_cast<double*>((intptr_t)ob.off), ob.cols, ob.rows, ob.cols, 1, kRelC);
+    double* orel = reinterpret_cast<double*>((intptr_t)ob.off);
+    std::vector<ZTerm> zt;
+    if (!cplx_) s2_.begin_item(orel, ob.cols, ob.rows, ob.cols, 1, kRelC);
     for (auto [ti, k] : by_out[op]) {
       const Task& t = tasks[ti];
       const Out& o = t.outs[k];
@@ -297,8 +308,13 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
       const Block& rbk = R[t.o2].blk[t.ri];
       require(lbk.rows == ob.rows && lbk.cols == kb.rows && rbk.cols == ob.cols,
               "Matrix indeces do not match in MultiplyToMatrix (L*T)");
-      s2_.add(L[o.o1].ptr(o.lp), lbk.cols, T_.data() + t.toff, rbk.cols, lbk.cols, 0, o.w);
+      if (cplx_)
+        zt.push_back(ZTerm{L[o.o1].ptr(o.lp), lbk.cols, T_.data() + cs * t.toff, rbk.cols, lbk.cols, 0,
+                           false, false, o.w, o.wi});
+      else
+        s2_.add(L[o.o1].ptr(o.lp), lbk.cols, T_.data() + t.toff, rbk.cols, lbk.cols, 0, o.w);
     }
+    if (cplx_) zgemm_item(s2_, orel, ob.cols, ob.rows, ob.cols, 1, kRelC, zt);
   }
   // F_ref counts the reference's K*R products even when no L term uses them.
   double fref = 0.0;
@@ -317,8 +333,9 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
       if (!uL[o.o1][o.lp]) { uL[o.o1][o.lp] = 1; lbytes += (int64_t)lbk.rows * lbk.cols; }
     }
   }
-  fref_ = fref;
-  bytes_min_ = 8 * (2 * psi_dim_ + lbytes + rbytes);
+  // complex: 8 real flops per multiply-add and 16 bytes per entry
+  fref_ = cplx_ ? 4.0 * fref : fref;
+  bytes_min_ = 8 * cs * (2 * psi_dim_ + lbytes + rbytes);
   s1_.finalize(stream);
   s2_.finalize(stream);
 }
@@ -354,7 +371,7 @@ namespace {
 // Output block assembly shared by both environment updates: every block of
 // out[o] collects (F index, ket block, w) terms; its shape comes from the
 // terms (AddToMatrixBlock into an empty block takes the first term's shape).
-struct EnvTerm { int f; int kp; double w; };
+struct EnvTerm { int f; int kp; double w, wi; };
 
 void finish_env(Env& out, std::vector<std::vector<std::vector<EnvTerm>>>& terms,
                 const std::vector<Shape>& fshape, const std::vector<int64_t>& foff,
@@ -374,24 +391,32 @@ void finish_env(Env& out, std::vector<std::vector<std::vector<EnvTerm>>>& terms,
       bt.blk[b].rows = sh.rows;
       bt.blk[b].cols = sh.cols;
     }
+    bt.cs = ket.cs;
     bt.allocate();
+    const bool cplx = ket.cs == 2;
     for (int b = 0; b < bt.n(); ++b) {
       const Block& ob = bt.blk[b];
       if ((int64_t)ob.rows * ob.cols == 0) continue;
-      s2.begin_item(bt.ptr(b), ob.cols, ob.rows, ob.cols, 0);
+      std::vector<ZTerm> zt;
+      if (!cplx) s2.begin_item(bt.ptr(b), ob.cols, ob.rows, ob.cols, 0);
       for (const EnvTerm& t : terms[o][b]) {
         const Block& kb = ket.blk[t.kp];
         const Shape fs = fshape[t.f];
+        double* Ff = F.data() + ket.cs * foff[t.f];
         if (!right) {
           // Lnew += w * F(i,l) * K(l,j)
           require(fs.cols == kb.rows, "Matrix indeces do not match in MultiplyToMatrix (F*K)");
-          s2.add(F.data() + foff[t.f], fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, 0, t.w);
+          if (cplx) zt.push_back(ZTerm{Ff, fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, 0, false, false, t.w, t.wi});
+          else s2.add(Ff, fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, 0, t.w);
         } else {
-          // Rnew += w * F(j,k) * B(i,k)^T
+          // Rnew += w * F(j,k) * B(i,k)^T  (complex: B^H)
           require(fs.cols == kb.cols, "Matrix indeces do not match in MultiplyToMatrix (F*B^T)");
-          s2.add(F.data() + foff[t.f], fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, kTransB, t.w);
+          if (cplx)
+            zt.push_back(ZTerm{Ff, fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, kTransB, false, true, t.w, t.wi});
+          else s2.add(Ff, fs.cols, ket.ptr(t.kp), kb.cols, fs.cols, kTransB, t.w);
         }
       }
+      if (cplx) zgemm_item(s2, bt.ptr(b), ob.cols, ob.rows, ob.cols, 0, 0, zt);
     }
   }
   s1.finalize(cx.stream);
@@ -423,7 +448,7 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
           const int jdx = ket.blk[lp].r;
           const int p2 = lat.phys[ldx][jdx];
           for (int o2 = 0; o2 < W.rb; ++o2)
-            if (W.at(o1, o2, p1, p2) != 0.0) flag[((size_t)o2 * nr + idx) * nr + jdx] = 1;
+            if (W.used(o1, o2, p1, p2)) flag[((size_t)o2 * nr + idx) * nr + jdx] = 1;
         }
       }
     }
@@ -463,24 +488,30 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
           const int jdx = ket.blk[lp].r;
           const int p2 = lat.phys[ldx][jdx];
           for (int o2 = 0; o2 < W.rb; ++o2) {
-            const double w = W.at(o1, o2, p1, p2);
-            if (w != 0.0) {
+            if (W.used(o1, o2, p1, p2)) {
               const int pos = out[o2].find(idx, jdx);
-              terms[o2][pos].push_back(EnvTerm{f, lp, w});
+              terms[o2][pos].push_back(EnvTerm{f, lp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
             }
           }
         }
       }
     }
   }
-  DevBuf<double> F((size_t)std::max<int64_t>(fsize, 1));
+  DevBuf<double> F((size_t)std::max<int64_t>(ket.cs * fsize, 1));
   GemmBatch s1;
   for (size_t f = 0; f < ft.size(); ++f) {
     const Block& kb = ket.blk[ft[f].kp];
     const BlockTensor& Lb = L[ft[f].o1];
     const Block& lbk = Lb.blk[ft[f].a];
     if (fshape[f].size() == 0) continue;
-    s1.begin_item(F.data() + foff[f], fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
+    double* Ff = F.data() + ket.cs * foff[f];
+    if (ket.cs == 2) {  // F = B^H * L  (complex_tensor_contraction.cc:929)
+      zgemm_item(s1, Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0, 0,
+                 {ZTerm{ket.ptr(ft[f].kp), kb.cols, Lb.ptr(ft[f].a), lbk.cols, kb.rows, kTransA, true,
+                        false, 1.0, 0.0}});
+      continue;
+    }
+    s1.begin_item(Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
     s1.add(ket.ptr(ft[f].kp), kb.cols, Lb.ptr(ft[f].a), lbk.cols, kb.rows, kTransA, 1.0);
   }
   finish_env(out, terms, fshape, foff, F, ket, false, s1, cx);
@@ -507,7 +538,7 @@ Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Co
           const int idx = ket.blk[kp].l;
           const int p1 = lat.phys[idx][kdx];
           for (int o1 = 0; o1 < W.lb; ++o1)
-            if (W.at(o1, o2, p1, p2) != 0.0) flag[((size_t)o1 * nl + jdx) * nl + idx] = 1;
+            if (W.used(o1, o2, p1, p2)) flag[((size_t)o1 * nl + jdx) * nl + idx] = 1;
         }
       }
     }
@@ -546,24 +577,30 @@ Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Co
           const int idx = ket.blk[kp].l;
           const int p1 = lat.phys[idx][kdx];
           for (int o1 = 0; o1 < W.lb; ++o1) {
-            const double w = W.at(o1, o2, p1, p2);
-            if (w != 0.0) {
+            if (W.used(o1, o2, p1, p2)) {
               const int pos = out[o1].find(jdx, idx);
-              terms[o1][pos].push_back(EnvTerm{f, kp, w});
+              terms[o1][pos].push_back(EnvTerm{f, kp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
             }
           }
         }
       }
     }
   }
-  DevBuf<double> F((size_t)std::max<int64_t>(fsize, 1));
+  DevBuf<double> F((size_t)std::max<int64_t>(ket.cs * fsize, 1));
   GemmBatch s1;
   for (size_t f = 0; f < ft.size(); ++f) {
     const Block& kb = ket.blk[ft[f].lp];
     const BlockTensor& Rb = R[ft[f].o2];
     const Block& rbk = Rb.blk[ft[f].a];
     if (fshape[f].size() == 0) continue;
-    s1.begin_item(F.data() + foff[f], fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
+    double* Ff = F.data() + ket.cs * foff[f];
+    if (ket.cs == 2) {
+      zgemm_item(s1, Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0, 0,
+                 {ZTerm{ket.ptr(ft[f].lp), kb.cols, Rb.ptr(ft[f].a), rbk.cols, kb.cols, 0, false, false,
+                        1.0, 0.0}});
+      continue;
+    }
+    s1.begin_item(Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
     s1.add(ket.ptr(ft[f].lp), kb.cols, Rb.ptr(ft[f].a), rbk.cols, kb.cols, 0, 1.0);
   }
   finish_env(out, terms, fshape, foff, F, ket, true, s1, cx);
@@ -1017,59 +1054,92 @@ double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
 
 double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, int refine,
                      Context& cx, LanczosStats& st, int* iters) {
+  // Real (RealTensorLanczos::LanczosMethod, real_tensor_lanczos.cc:14-169)
+  // and complex (ComplexTensorLanczos, complex_tensor_lanczos.cc:14-160)
+  // forms.  Complex vectors are n interleaved pairs (nd = 2n doubles): alpha
+  // and the Gram-Schmidt overlaps are complex, beta and the tridiagonal
+  // eigenproblem real (zheevd reads only the real diagonal).
   const int64_t n = plan.psi_dim();
+  const bool cz = plan.complex();
+  const int64_t nd = plan.doubles();
   ++st.solves;
   if (n < 100) {
     // dense H_eff (ComputeEffectHamilton, real_tensor_contraction.cc:1136-1205)
     // built column by column from the FP64 device matvec, then the FP64 host
-    // eigensolver (RealSymMatrixDiag)
+    // eigensolver (RealSymMatrixDiag; a complex Hermitian H through its real
+    // embedding [[Re, -Im], [Im, Re]], whose lowest eigenvector (x, y) gives
+    // the ground state x + i y)
     ++st.dense;
     if (iters) *iters = 0;
     if (n == 0) return 0.0;
-    DevBuf<double> X((size_t)n * n), Y((size_t)n * n);
-    std::vector<double> eye((size_t)n * n, 0.0);
-    for (int64_t i = 0; i < n; ++i) eye[(size_t)i * n + i] = 1.0;
+    DevBuf<double> X((size_t)n * nd), Y((size_t)n * nd);
+    std::vector<double> eye((size_t)n * nd, 0.0);
+    for (int64_t i = 0; i < n; ++i) eye[(size_t)i * nd + (cz ? 2 * i : i)] = 1.0;
     DMRG_HIP(hipMemcpyAsync(X.data(), eye.data(), eye.size() * sizeof(double), hipMemcpyHostToDevice,
                             cx.stream));
-    fill(Y.data(), n * n, 0.0, cx.stream);
-    for (int64_t c = 0; c < n; ++c) plan.apply(X.data() + c * n, Y.data() + c * n, kFP64, cx.stream);
-    std::vector<double> y((size_t)n * n), H((size_t)n * n);
+    fill(Y.data(), n * nd, 0.0, cx.stream);
+    for (int64_t c = 0; c < n; ++c) plan.apply(X.data() + c * nd, Y.data() + c * nd, kFP64, cx.stream);
+    std::vector<double> y((size_t)n * nd);
     DMRG_HIP(hipMemcpyAsync(y.data(), Y.data(), y.size() * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
     cx.sync();
+    const int64_t m = cz ? 2 * n : n;
+    std::vector<double> H((size_t)m * m);
     for (int64_t i = 0; i < n; ++i)
-      for (int64_t c = 0; c < n; ++c) H[(size_t)i * n + c] = y[(size_t)c * n + i];
+      for (int64_t c = 0; c < n; ++c) {
+        if (!cz) {
+          H[(size_t)i * n + c] = y[(size_t)c * n + i];
+        } else {
+          const double re = y[(size_t)c * nd + 2 * i], im = y[(size_t)c * nd + 2 * i + 1];
+          H[(size_t)i * m + c] = re;
+          H[(size_t)(i + n) * m + c + n] = re;
+          H[(size_t)(i + n) * m + c] = im;
+          H[(size_t)i * m + c + n] = -im;
+        }
+      }
     std::vector<double> ev, vec;
-    sym_eig((int)n, H, ev, vec);
-    std::vector<double> g(n);
-    for (int64_t i = 0; i < n; ++i) g[i] = vec[(size_t)i * n];
-    DMRG_HIP(hipMemcpyAsync(ket, g.data(), n * sizeof(double), hipMemcpyHostToDevice, cx.stream));
+    sym_eig((int)m, H, ev, vec);
+    std::vector<double> g(nd);
+    for (int64_t i = 0; i < n; ++i) {
+      if (!cz) {
+        g[i] = vec[(size_t)i * m];
+      } else {
+        g[2 * i] = vec[(size_t)i * m];
+        g[2 * i + 1] = vec[(size_t)(i + n) * m];
+      }
+    }
+    DMRG_HIP(hipMemcpyAsync(ket, g.data(), nd * sizeof(double), hipMemcpyHostToDevice, cx.stream));
     cx.sync();
     return ev[0];
   }
   require(num_iter >= 1 && num_iter <= 200, "num_iter must be in [1, 200]", kErrArgument);
   cx.scal.ensure(256);
-  DevBuf<double> psi((size_t)num_iter * n), h((size_t)n);
+  DevBuf<double> psi((size_t)num_iter * nd), h((size_t)nd);
   double* P = psi.data();
   double* sc = cx.scal.data();
-  DMRG_HIP(hipMemcpyAsync(P, ket, n * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
-  fill(h.data(), n, 0.0, cx.stream);
+  DMRG_HIP(hipMemcpyAsync(P, ket, nd * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
+  fill(h.data(), nd, 0.0, cx.stream);
   std::vector<double> alpha, beta;
   double old_energy = 10000.0, energy = 0.0;
   for (int i = 0; i < num_iter; ++i) {
-    double* pi = P + (int64_t)i * n;
+    double* pi = P + (int64_t)i * nd;
     const double t0 = now();
     plan.apply(pi, h.data(), precision, cx.stream);
-    dot(pi, h.data(), n, sc, cx.partial, cx.stream);             // alpha = <psi|H|psi>
-    axpy_dev(h.data(), pi, n, sc, -1.0, cx.stream);              // h -= alpha psi
-    dot(h.data(), h.data(), n, sc + 1, cx.partial, cx.stream);   // beta^2
-    double ab[2];
-    DMRG_HIP(hipMemcpyAsync(ab, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
+    if (cz) {
+      zdot(pi, h.data(), n, sc, cx.partial, cx.stream);            // alpha = <psi|H|psi>
+      zaxpy_dev(h.data(), pi, n, sc, -1.0, cx.stream);             // h -= alpha psi
+    } else {
+      dot(pi, h.data(), n, sc, cx.partial, cx.stream);             // alpha = <psi|H|psi>
+      axpy_dev(h.data(), pi, n, sc, -1.0, cx.stream);              // h -= alpha psi
+    }
+    dot(h.data(), h.data(), nd, sc + 2, cx.partial, cx.stream);    // beta^2 = Re <h|h>
+    double ab[3];
+    DMRG_HIP(hipMemcpyAsync(ab, sc, 3 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
     cx.sync();
     st.matvec_seconds += now() - t0;
     st.flops += plan.fref();
     ++st.matvecs;
     alpha.push_back(ab[0]);
-    beta.push_back(std::sqrt(ab[1]));
+    beta.push_back(std::sqrt(ab[2]));
     const int k = i + 1;
     std::vector<double> T((size_t)k * k, 0.0), ev, vec;
     for (int j = 0; j < k; ++j) {
@@ -1084,15 +1154,15 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
       std::vector<double> coef(k);
       for (int j = 0; j < k; ++j) coef[j] = vec[(size_t)j * k];
       DMRG_HIP(hipMemcpyAsync(sc + 16, coef.data(), k * sizeof(double), hipMemcpyHostToDevice, cx.stream));
-      combine(ket, P, n, sc + 16, k, n, cx.stream);
+      combine(ket, P, nd, sc + 16, k, nd, cx.stream);
       if (precision != kFP64 && refine) {
         // FP64 Rayleigh-quotient refinement of the mixed-precision Ritz pair
-        fill(h.data(), n, 0.0, cx.stream);
+        fill(h.data(), nd, 0.0, cx.stream);
         plan.apply(ket, h.data(), kFP64, cx.stream);
-        dot(ket, h.data(), n, sc + 2, cx.partial, cx.stream);
-        dot(ket, ket, n, sc + 3, cx.partial, cx.stream);
+        dot(ket, h.data(), nd, sc + 3, cx.partial, cx.stream);   // Re <ket|H ket>
+        dot(ket, ket, nd, sc + 4, cx.partial, cx.stream);
         double rq[2];
-        DMRG_HIP(hipMemcpyAsync(rq, sc + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
+        DMRG_HIP(hipMemcpyAsync(rq, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
         cx.sync();
         st.flops += plan.fref();
         ++st.matvecs;
@@ -1103,15 +1173,21 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     }
     old_energy = e_new;
     if (k < num_iter) {
-      double* pn = P + (int64_t)(i + 1) * n;
-      lanczos_step(pn, h.data(), pi, n, beta[i], cx.stream);
-      // GramSchmidtMethod (real_tensor_lanczos.cc:188-201)
+      double* pn = P + (int64_t)(i + 1) * nd;
+      lanczos_step(pn, h.data(), pi, nd, beta[i], cx.stream);
+      // GramSchmidtMethod (real_tensor_lanczos.cc:188-201,
+      // complex_tensor_lanczos.cc:178-192)
       for (int j = 0; j <= i; ++j) {
-        dot(P + (int64_t)j * n, pn, n, sc + 4, cx.partial, cx.stream);
-        axpy_dev(pn, P + (int64_t)j * n, n, sc + 4, -1.0, cx.stream);
+        if (cz) {
+          zdot(P + (int64_t)j * nd, pn, n, sc + 6, cx.partial, cx.stream);
+          zaxpy_dev(pn, P + (int64_t)j * nd, n, sc + 6, -1.0, cx.stream);
+        } else {
+          dot(P + (int64_t)j * nd, pn, n, sc + 6, cx.partial, cx.stream);
+          axpy_dev(pn, P + (int64_t)j * nd, n, sc + 6, -1.0, cx.stream);
+        }
       }
-      dot(pn, pn, n, sc + 5, cx.partial, cx.stream);
-      scale_inv_sqrt(pn, n, sc + 5, cx.stream, 0);
+      dot(pn, pn, nd, sc + 8, cx.partial, cx.stream);
+      scale_inv_sqrt(pn, nd, sc + 8, cx.stream, 0);
     }
   }
   return energy;

@@ -27,9 +27,12 @@ struct Block {
 
 // RealMatrixBlock: blocks concatenated in block order in one HBM buffer; the
 // flat buffer is exactly the reference's VectorizeMatrixBlock layout.
+// cs = 2 holds complex entries interleaved (re, im) (ComplexMatrixBlock);
+// offsets and `size` count entries, the buffer holds cs * size doubles.
 struct BlockTensor {
   std::vector<Block> blk;
   int64_t size = 0;
+  int cs = 1;
   DevBuf<double> data;
   const double* ext = nullptr;  // non-owning view of caller memory (C ABI)
 
@@ -39,8 +42,9 @@ struct BlockTensor {
   int find(int l, int r) const;
   std::vector<int> same(int leigh, int target) const;
   double* base() const { return ext ? const_cast<double*>(ext) : data.data(); }
-  double* ptr(int i) const { return base() + blk[i].off; }
+  double* ptr(int i) const { return base() + cs * blk[i].off; }
   int64_t elems(int i) const { return (int64_t)blk[i].rows * blk[i].cols; }
+  int64_t doubles() const { return cs * size; }
 };
 
 using Env = std::vector<BlockTensor>;  // one block-sparse matrix per MPO channel
@@ -93,7 +97,9 @@ class HeffPlan {
                    bool gemm_only = false);
   double flops() const { return s1_.flops + s2_.flops; }  // executed
   double fref() const { return fref_; }                      // reference order
-  int64_t psi_dim() const { return psi_dim_; }
+  int64_t psi_dim() const { return psi_dim_; }               // entries (complex: pairs)
+  bool complex() const { return cplx_; }
+  int64_t doubles() const { return (cplx_ ? 2 : 1) * psi_dim_; }
   int64_t tasks() const { return (int64_t)s1_.items.size(); }
   int64_t bytes_min() const { return bytes_min_; }
   const GemmBatch& stage1() const { return s1_; }
@@ -102,6 +108,7 @@ class HeffPlan {
  private:
   GemmBatch s1_, s2_;
   DevBuf<double> T_;
+  bool cplx_ = false;
   int64_t psi_dim_ = 0;
   int64_t bytes_min_ = 0;
   double fref_ = 0;

@@ -25,8 +25,11 @@ enum Precision : int {
 // kRelA/kRelB: the operand pointer is an element offset from the launch's
 // `in` base; kRelC on an item: C is an offset from the launch's `out` base.
 // One uploaded plan then serves any input/output vector pair.
-enum : int { kTransA = 1, kTransB = 2, kRelA = 4, kRelB = 8 };
-enum : int { kRelC = 1 };
+// kStrided2 (term) / kStrided2C (item): consecutive elements along the
+// contiguous dimension sit two doubles apart (one part of an interleaved
+// complex matrix); lda / ldb / ldc and offsets still count doubles.
+enum : int { kTransA = 1, kTransB = 2, kRelA = 4, kRelB = 8, kStrided2 = 16 };
+enum : int { kRelC = 1, kStrided2C = 2 };
 struct GemmContrib {
   const double* A;
   const double* B;
@@ -108,6 +111,26 @@ struct GemmBatch {
   bool empty() const { return tiles.empty(); }
 };
 
+// Complex (interleaved re, im) products on a real batch.  One complex term
+// s * op(A) * op(B), with op = transpose (kTransA/kTransB), optional complex
+// conjugation of each operand, s = sr + i si, expands into up to four real
+// terms per part; zgemm_item adds the real-part and the imaginary-part items
+// of one complex output C (M x N).  Leading dimensions count complex
+// elements; pointers address element (0, 0) (or carry complex offsets with
+// kRelA / kRelB / kRelC).
+struct ZTerm {
+  const double* A;
+  int64_t lda;
+  const double* B;
+  int64_t ldb;
+  int K;
+  int flags;  // kTransA, kTransB, kRelA, kRelB
+  bool conjA, conjB;
+  double sr, si;
+};
+void zgemm_item(GemmBatch& g, double* C, int64_t ldc, int M, int N, int accumulate, int cflags,
+                const std::vector<ZTerm>& terms);
+
 // Strided 2-D copy task: dst[i*dsr + j*dsc] = alpha * src[i*ssr + j*ssc]
 // for i < rows, j < cols.  src == nullptr writes zeros.  The tasks of one
 // launch run concurrently: their destinations must not overlap.
@@ -125,6 +148,14 @@ void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch
 // Deterministic two-pass dot product: result written to *out (device).
 void dot(const double* x, const double* y, int64_t n, double* out,
          DevBuf<double>& partial, hipStream_t stream);
+// Complex forms on n interleaved entries (ComplexTensorLanczos::VectorMultiply
+// / VectorSubtraction, complex_tensor_lanczos.cc:163-176): out[0..1] =
+// <x|y> = sum conj(x) y (deterministic); y += sign * alpha * x with alpha
+// complex on the device.
+void zdot(const double* x, const double* y, int64_t n, double* out,
+          DevBuf<double>& partial, hipStream_t stream);
+void zaxpy_dev(double* y, const double* x, int64_t n, const double* alpha,
+               double sign, hipStream_t stream);
 // y += sign * (*alpha) * x   (alpha on device)
 void axpy_dev(double* y, const double* x, int64_t n, const double* alpha,
               double sign, hipStream_t stream);

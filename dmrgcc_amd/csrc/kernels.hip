@@ -80,27 +80,28 @@ __device__ inline void load_step(const GemmContrib& cb, const double* __restrict
   const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
   const int K = cb.K;
   const double sc = cb.scale;
+  const int es = (cb.flags & kStrided2) ? 2 : 1;
   if (!(cb.flags & kTransA)) {
     const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
     const double* row = A + (int64_t)gm * cb.lda;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[kb + q] * sc : 0.0;
+    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
   } else {
     const int gk = k0 + (tid >> 3), mb = m0 + (tid & 7) * 8;
     const double* row = A + (int64_t)gk * cb.lda;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gk < K && mb + q < M) ? row[mb + q] * sc : 0.0;
+    for (int q = 0; q < 8; ++q) r.a[q] = (gk < K && mb + q < M) ? row[(mb + q) * es] * sc : 0.0;
   }
   if (!(cb.flags & kTransB)) {
     const int gk = k0 + (tid >> 3), nb = n0 + (tid & 7) * 8;
     const double* row = B + (int64_t)gk * cb.ldb;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gk < K && nb + q < N) ? row[nb + q] : 0.0;
+    for (int q = 0; q < 8; ++q) r.b[q] = (gk < K && nb + q < N) ? row[(nb + q) * es] : 0.0;
   } else {
     const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
     const double* row = B + (int64_t)gn * cb.ldb;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[kb + q] : 0.0;
+    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
   }
 }
 
@@ -209,7 +210,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
         const int gm = m0 + wm * 32 + a * 16 + MfmaAcc<T>::row(lane, v);
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
-          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
+          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
+                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
           const double r2 = (double)acc[a][b][v];
           *p = it.accumulate ? (*p + r2) : r2;
         }
@@ -238,27 +240,28 @@ __device__ inline void load_step_packed(const GemmContrib& cb, const double* __r
   const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
   const int K = cb.K;
   const double sc = cb.scale;
+  const int es = (cb.flags & kStrided2) ? 2 : 1;
   if (!(cb.flags & kTransA)) {
     const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
     const double* row = A + (int64_t)gm * cb.lda;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[kb + q] * sc : 0.0;
+    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
   } else {
     const int gm = m0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm] * sc : 0.0;
+      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm * es] * sc : 0.0;
   }
   if (!(cb.flags & kTransB)) {
     const int gn = n0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn] : 0.0;
+      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn * es] : 0.0;
   } else {
     const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
     const double* row = B + (int64_t)gn * cb.ldb;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[kb + q] : 0.0;
+    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
   }
 }
 
@@ -360,7 +363,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
         const int gm = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
-          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc + gn;
+          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
+                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
           const double r2 = (double)acc[a][b][v];
           *p = it.accumulate ? (*p + r2) : r2;
         }
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __re
     double s = 0.0;
     for (int c = 0; c < t.nparts; ++c) s += t.part[c * mn + e];
     const int i = (int)(e / t.N), j = (int)(e % t.N);
-    C[(int64_t)i * t.ldc + j] += s;
+    C[(int64_t)i * t.ldc + (int64_t)j * ((t.flags & kStrided2C) ? 2 : 1)] += s;
   }
 }
 
@@ -429,6 +433,56 @@ __global__ __launch_bounds__(256) void dot_final_kernel(const double* __restrict
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// <x|y> = sum conj(x_i) y_i over n interleaved complex entries: partial
+// (re, im) per block, then a fixed-order final sum (deterministic)
+__global__ __launch_bounds__(256) void zdot_partial_kernel(const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           int64_t n, double* __restrict__ part) {
+  __shared__ double red[2][4];
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+    sr += xr * yr + xi * yi;
+    si += xr * yi - xi * yr;
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = sr; red[1][threadIdx.x >> 6] = si; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    part[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+__global__ __launch_bounds__(256) void zdot_final_kernel(const double* __restrict__ part, int np,
+                                                         double* __restrict__ out) {
+  __shared__ double red[2][4];
+  double sr = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) { sr += part[2 * i]; si += part[2 * i + 1]; }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = sr; red[1][threadIdx.x >> 6] = si; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    out[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// y += sign * alpha * x, alpha = (alpha[0], alpha[1]) complex on the device
+__global__ void zaxpy_dev_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                                 const double* __restrict__ alpha, double sign) {
+  const double ar = sign * alpha[0], ai = sign * alpha[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    y[2 * i] += ar * xr - ai * xi;
+    y[2 * i + 1] += ar * xi + ai * xr;
+  }
 }
 
 __global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n,
@@ -1219,6 +1273,41 @@ void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const
   DMRG_HIP(hipGetLastError());
 }
 
+namespace {
+// complex element offset -> double offset of part p (0 re, 1 im), for real
+// or relative (offset-encoded) pointers alike
+template <typename P>
+P* part_ptr(P* ptr, int p, bool relative) {
+  if (relative) return reinterpret_cast<P*>(2 * (intptr_t)ptr + p);
+  return ptr + p;
+}
+}  // namespace
+
+void zgemm_item(GemmBatch& g, double* C, int64_t ldc, int M, int N, int accumulate, int cflags,
+                const std::vector<ZTerm>& terms) {
+  const bool relc = cflags & kRelC;
+  for (int part = 0; part < 2; ++part) {
+    g.begin_item(part_ptr(C, part, relc), 2 * ldc, M, N, accumulate, cflags | kStrided2C);
+    for (const ZTerm& t : terms) {
+      const double sa = t.conjA ? -1.0 : 1.0, sb = t.conjB ? -1.0 : 1.0;
+      // (A part, B part, scale): P = A'B' with A' = Ar + i sa Ai, B' = Br + i sb Bi
+      struct R { int pa, pb; double s; };
+      R rs[4];
+      if (part == 0)
+        rs[0] = {0, 0, t.sr}, rs[1] = {1, 1, -t.sr * sa * sb}, rs[2] = {0, 1, -t.si * sb},
+        rs[3] = {1, 0, -t.si * sa};
+      else
+        rs[0] = {0, 1, t.sr * sb}, rs[1] = {1, 0, t.sr * sa}, rs[2] = {0, 0, t.si},
+        rs[3] = {1, 1, -t.si * sa * sb};
+      for (const R& r : rs) {
+        if (r.s == 0.0) continue;
+        g.add(part_ptr(t.A, r.pa, t.flags & kRelA), 2 * t.lda, part_ptr(t.B, r.pb, t.flags & kRelB),
+              2 * t.ldb, t.K, t.flags | kStrided2, r.s);
+      }
+    }
+  }
+}
+
 void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
                    hipStream_t stream) {
   if (tasks.empty()) return;
@@ -1251,6 +1340,21 @@ void dot(const double* x, const double* y, int64_t n, double* out, DevBuf<double
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + 255) / 256));
   hipLaunchKernelGGL(dot_partial_kernel, dim3(g), dim3(256), 0, stream, x, y, n, partial.data());
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, stream, partial.data(), g, out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void zdot(const double* x, const double* y, int64_t n, double* out, DevBuf<double>& partial,
+          hipStream_t stream) {
+  partial.ensure(2 * kDotBlocks);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + 255) / 256));
+  hipLaunchKernelGGL(zdot_partial_kernel, dim3(g), dim3(256), 0, stream, x, y, n, partial.data());
+  hipLaunchKernelGGL(zdot_final_kernel, dim3(1), dim3(256), 0, stream, partial.data(), g, out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void zaxpy_dev(double* y, const double* x, int64_t n, const double* alpha, double sign,
+               hipStream_t stream) {
+  hipLaunchKernelGGL(zaxpy_dev_kernel, dim3(grid_for(n)), dim3(256), 0, stream, y, x, n, alpha, sign);
   DMRG_HIP(hipGetLastError());
 }
 

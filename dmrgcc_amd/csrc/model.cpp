@@ -12,19 +12,19 @@ namespace dmrg {
 bool SiteOperator::left_nonzero(int o1, int p1) const {
   for (int r = 0; r < rb; ++r)
     for (int p2 = 0; p2 < d; ++p2)
-      if (std::fabs(at(o1, r, p1, p2)) > 1e-15) return true;
+      if (std::hypot(at(o1, r, p1, p2), im(o1, r, p1, p2)) > 1e-15) return true;
   return false;
 }
 bool SiteOperator::right_nonzero(int o2, int p2) const {
   for (int l = 0; l < lb; ++l)
     for (int p1 = 0; p1 < d; ++p1)
-      if (std::fabs(at(l, o2, p1, p2)) > 1e-15) return true;
+      if (std::hypot(at(l, o2, p1, p2), im(l, o2, p1, p2)) > 1e-15) return true;
   return false;
 }
 bool SiteOperator::mixed_nonzero(int o1, int p2) const {
   for (int r = 0; r < rb; ++r)
     for (int p1 = 0; p1 < d; ++p1)
-      if (std::fabs(at(o1, r, p1, p2)) > 1e-15) return true;
+      if (std::hypot(at(o1, r, p1, p2), im(o1, r, p1, p2)) > 1e-15) return true;
   return false;
 }
 

@@ -17,17 +17,29 @@ using QNTable = std::vector<QN>;
 
 // Dense operator of one MPO site: W[o1][o2](p1, p2), p1 = bra, p2 = ket,
 // stored as ((o1*rb + o2)*d + p1)*d + p2.
+// Complex operators (ComplexTensorOperator) keep the imaginary parts in wi
+// (same layout); wi is empty for a real operator.
 struct SiteOperator {
   int lb = 0, rb = 0, d = 0;
-  std::vector<double> w;
+  std::vector<double> w, wi;
   double at(int o1, int o2, int p1, int p2) const {
     return w[(((size_t)o1 * rb + o2) * d + p1) * d + p2];
   }
   double& at(int o1, int o2, int p1, int p2) {
     return w[(((size_t)o1 * rb + o2) * d + p1) * d + p2];
   }
+  double im(int o1, int o2, int p1, int p2) const {
+    return wi.empty() ? 0.0 : wi[(((size_t)o1 * rb + o2) * d + p1) * d + p2];
+  }
+  bool complex() const { return !wi.empty(); }
+  // an entry the contractions use (real: w != 0; complex: re != 0 or im != 0,
+  // complex_tensor_contraction.cc:1288)
+  bool used(int o1, int o2, int p1, int p2) const {
+    return at(o1, o2, p1, p2) != 0.0 || im(o1, o2, p1, p2) != 0.0;
+  }
   // RealTensorOperator::{Left,Right,Mixed}CheckZero
-  // (real_tensor_operator.cc:363-409): any |entry| > 1e-15 along the
+  // (real_tensor_operator.cc:363-409; complex: Norm > 1e-15,
+  // complex_tensor_operator.cc:363-409): any |entry| > 1e-15 along the
   // remaining indices.
   bool left_nonzero(int o1, int p1) const;
   bool right_nonzero(int o2, int p2) const;

@@ -78,6 +78,12 @@ int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn)
 int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, double* const* C,
                       const int* M, const int* N, const int* K, int precision, void* stream);
 
+/* Complex (ComplexMatrix::MultiplyToMatrix, complex_matrix.cc:294-318):
+ * C_i = A_i * B_i with interleaved complex entries (re, im), so every
+ * matrix of M x N entries is 2*M*N doubles. */
+int dmrgcc_zblock_gemm(int n, const double* const* A, const double* const* B, double* const* C,
+                       const int* M, const int* N, const int* K, int precision, void* stream);
+
 /* ---- effective Hamiltonian (MultiplyEffectHamilton,
  *      src/dmrg/realdmrg/real_tensor_contraction.cc:1223-1291) ---------- */
 typedef struct dmrgcc_heff dmrgcc_heff;
@@ -98,6 +104,21 @@ int dmrgcc_heff_create(int d, int lb, int rb, const double* W, int nl, int nr, c
                        const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
                        const int64_t* R_off, const double* R_data, void* stream,
                        dmrgcc_heff** out);
+/* Complex plan (ComplexTensorContraction::MultiplyEffectHamilton,
+ * src/dmrg/complexdmrg/complex_tensor_contraction.cc:1244-1312): the same
+ * description with W (lb*rb*d*d entries), L_data and R_data holding
+ * interleaved complex entries; offsets and shapes count entries.
+ * dmrgcc_heff_apply / _info / _time / _destroy and dmrgcc_lanczos take the
+ * handle as for a real plan, with vectors of 2*psi_dim doubles; F_ref counts
+ * 8 real flops per complex multiply-add. */
+int dmrgcc_zheff_create(int d, int lb, int rb, const double* W, int nl, int nr, const int* phys,
+                        int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                        const int* ket_cols, int nL, const int* L_o, const int* L_l,
+                        const int* L_r, const int* L_rows, const int* L_cols,
+                        const int64_t* L_off, const double* L_data, int nR, const int* R_o,
+                        const int* R_l, const int* R_r, const int* R_rows, const int* R_cols,
+                        const int64_t* R_off, const double* R_data, void* stream,
+                        dmrgcc_heff** out);
 /* y += H_eff x  (the reference's accumulate-into-state semantics).
  * x, y: device vectors of psi_dim doubles. */
 int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision);
@@ -123,7 +144,8 @@ int dmrgcc_blocks_count(const dmrgcc_blocks* b);
 int dmrgcc_blocks_get(const dmrgcc_blocks* b, int i, int* o, int* l, int* r, int* rows, int* cols,
                       int64_t* off);
 const double* dmrgcc_blocks_data(const dmrgcc_blocks* b);
-/* Copy all block data (sum rows*cols doubles) to host memory. */
+/* Copy all block data (sum rows*cols entries; 2 doubles each for a complex
+ * result) to host memory. */
 int dmrgcc_blocks_download(const dmrgcc_blocks* b, double* host);
 int dmrgcc_blocks_destroy(dmrgcc_blocks* b);
 
@@ -138,6 +160,18 @@ int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl,
                       const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
                       const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
                       dmrgcc_blocks** out);
+
+/* Complex environment update (ComplexTensorContraction::
+ * {Left,Right}ComputeTensorContraction, complex_tensor_contraction.cc:803,
+ * :980; the bra enters as the Hermitian conjugate of the ket): arguments as
+ * dmrgcc_env_update with interleaved complex W, ket and environment data;
+ * the result handle then holds 2 doubles per entry. */
+int dmrgcc_zenv_update(int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
+                       const int* phys, int nket, const int* ket_l, const int* ket_r,
+                       const int* ket_rows, const int* ket_cols, const double* ket_data, int nE,
+                       const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
+                       const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
+                       dmrgcc_blocks** out);
 
 /* ---- sector SVD (RealMatrix::SVDMatrix, real_matrix.cc:588-620) -------- */
 /* Batched thin SVD of n row-major device matrices A_i (M_i x N_i), one-sided

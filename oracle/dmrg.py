@@ -36,7 +36,10 @@ def heff_tasks(lat, W, Lenv, Renv):
     :1223-1291), in the reference's loop order.
 
     Returns a list of (o2, r_pos, k_pos, [(out_pos, o1, l_pos, w), ...]):
-    T = K[k_pos] @ R[o2][r_pos]; out[out_pos] += w * L[o1][l_pos] @ T."""
+    T = K[k_pos] @ R[o2][r_pos]; out[out_pos] += w * L[o1][l_pos] @ T.
+    A complex W (ComplexTensorContraction::MultiplyEffectHamilton,
+    complex_tensor_contraction.cc:1244-1312) keeps complex w; an entry is used
+    when its real or imaginary part is nonzero (:1288)."""
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     same_r = [ket.same(1, r) for r in range(lat.nr)]
@@ -61,13 +64,14 @@ def heff_tasks(lat, W, Lenv, Renv):
                         if w != 0.0:
                             lp = Lenv[o1].find(idx, kdx)
                             if lp != -1:
-                                outs.append((op, o1, lp, float(w)))
+                                outs.append((op, o1, lp, w.item()))
                 tasks.append((o2, ri, kp, outs))
     return tasks
 
 
 def heff_flops(lat, W, Lenv, Renv, tasks=None):
-    """F_ref: real flops of one H_eff application in the reference order."""
+    """F_ref: real flops of one H_eff application in the reference order
+    (complex: 8 real flops per multiply-add)."""
     if tasks is None:
         tasks = heff_tasks(lat, W, Lenv, Renv)
     f = 0
@@ -79,7 +83,7 @@ def heff_flops(lat, W, Lenv, Renv, tasks=None):
         for op, o1, lp, w in outs:
             Lm = Lenv[o1].mats[lp]
             f += 2 * Lm.shape[0] * Lm.shape[1] * R.shape[1]
-    return f
+    return 4 * f if np.iscomplexobj(W) else f
 
 
 def heff_apply(lat, W, Lenv, Renv, x, y, tasks=None):
@@ -105,7 +109,8 @@ def heff_dense(lat, W, Lenv, Renv):
     ket = lat.ket
     dim = ket.dim()
     offs = ket.offsets()
-    H = np.zeros((dim, dim))
+    cplx = np.iscomplexobj(W) or any(np.iscomplexobj(m) for m in ket.mats)
+    H = np.zeros((dim, dim), dtype=complex if cplx else float)
     lb, rb = W.shape[0], W.shape[1]
     for o1 in range(lb):
         Lb = Lenv[o1]
@@ -138,7 +143,8 @@ def heff_dense(lat, W, Lenv, Renv):
 # --------------------------------------------------------------------------
 def left_env(lat, W, Lenv):
     """LeftComputeTensorContraction (real_tensor_contraction.cc:782-940):
-    Lnew[o2](i,j) += w(p1,p2) * B(k,i)^T L[o1](k,l) K(l,j)."""
+    Lnew[o2](i,j) += w(p1,p2) * B(k,i)^T L[o1](k,l) K(l,j); complex
+    (complex_tensor_contraction.cc:803-978) with the bra B^H (:929)."""
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     nr = lat.nr
@@ -177,7 +183,7 @@ def left_env(lat, W, Lenv):
                 p1 = lat.phys[kdx][idx]
                 if not left_check_zero(W, o1, p1):
                     continue
-                F = ket.mats[kp].T @ Lm
+                F = ket.mats[kp].conj().T @ Lm
                 for lp in same_l[ldx]:
                     jdx = ket.right[lp]
                     p2 = lat.phys[ldx][jdx]
@@ -192,7 +198,8 @@ def left_env(lat, W, Lenv):
 
 def right_env(lat, W, Renv):
     """RightComputeTensorContraction (real_tensor_contraction.cc:959-1119):
-    Rnew[o1](j,i) += w(p1,p2) * K(j,l) R[o2](l,k) B(i,k)^T."""
+    Rnew[o1](j,i) += w(p1,p2) * K(j,l) R[o2](l,k) B(i,k)^T; complex
+    (complex_tensor_contraction.cc:980-1158) with B^H (:1113)."""
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     nl = lat.nl
@@ -235,7 +242,7 @@ def right_env(lat, W, Renv):
                 for kp in same_r[kdx]:
                     idx = ket.left[kp]
                     p1 = lat.phys[idx][kdx]
-                    S = F @ ket.mats[kp].T
+                    S = F @ ket.mats[kp].conj().T
                     for o1 in range(lb):
                         w = W[o1, o2, p1, p2]
                         if w != 0.0:
@@ -272,19 +279,27 @@ def fix_signs(vecs):
     reproducible, because the expansion noise contracts a fixed random matrix
     with the gauge-carrying bond index."""
     idx = np.argmax(np.abs(vecs), axis=0)
-    sg = np.where(vecs[idx, np.arange(vecs.shape[1])] < 0, -1.0, 1.0)
+    top = vecs[idx, np.arange(vecs.shape[1])]
+    if np.iscomplexobj(vecs):
+        # complex vectors: the phase that makes that component real positive
+        mag = np.abs(top)
+        return np.where(mag > 0, np.conj(top) / np.where(mag > 0, mag, 1.0), 1.0)
+    sg = np.where(top < 0, -1.0, 1.0)
     return sg
 
 
 def sym_eig(A):
-    """RealSymMatrixDiag (src/util/general.cc:47-64): ascending eigenvalues,
-    eigenvectors in columns (sign convention of fix_signs)."""
+    """RealSymMatrixDiag (src/util/general.cc:47-64; complex Hermitian:
+    ComplexSymMatrixDiag, :66-83): ascending eigenvalues, eigenvectors in
+    columns (sign / phase convention of fix_signs)."""
     w, v = np.linalg.eigh(A)
     return w, v * fix_signs(v)[None, :]
 
 
 def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
-    """LanczosMethod (real_tensor_lanczos.cc:14-169).  Writes the Ritz vector
+    """LanczosMethod (real_tensor_lanczos.cc:14-169; complex
+    complex_tensor_lanczos.cc:14-160: complex alpha subtracted, beta real,
+    the tridiagonal's real diagonal diagonalised).  Writes the Ritz vector
     into lat.ket and returns the energy."""
     psi_dim = lat.ket.dim()
     if psi_dim < 100:
@@ -296,7 +311,10 @@ def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
         return float(ev[0])
     tasks = heff_tasks(lat, W, Lenv, Renv)
     psi = [lat.ket.to_vector()]
-    h = np.zeros(psi_dim)
+    cplx = np.iscomplexobj(psi[0]) or np.iscomplexobj(W)
+    if cplx:
+        psi[0] = psi[0].astype(complex)
+    h = np.zeros(psi_dim, dtype=complex if cplx else float)
     alpha, beta = [], []
     old = 10000.0
     energy = None
@@ -304,9 +322,10 @@ def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
         heff_apply(lat, W, Lenv, Renv, psi[i], h, tasks)
         if stats is not None:
             stats['matvec'] = stats.get('matvec', 0) + 1
-        a = float(np.dot(psi[i], h))
+        a = np.vdot(psi[i], h) if cplx else float(np.dot(psi[i], h))
         h -= a * psi[i]
-        b = math.sqrt(float(np.dot(h, h)))
+        b = math.sqrt(float(np.vdot(h, h).real))
+        a = float(np.real(a))
         alpha.append(a)
         beta.append(b)
         k = i + 1
@@ -315,7 +334,7 @@ def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
         new = float(ev[0])
         if b < 1e-15 or abs(new - old) < 1e-15 or k == num_iter:
             energy = new
-            res = np.zeros(psi_dim)
+            res = np.zeros(psi_dim, dtype=h.dtype)
             for j in range(k):
                 res += psi[j] * vec[j, 0]
             lat.ket.from_vector(res)
@@ -335,9 +354,9 @@ def gram_schmidt(vecs):
     the already-updated vector, then normalised."""
     v = vecs[-1]
     for u in vecs[:-1]:
-        c = float(np.dot(u, v))
+        c = np.vdot(u, v) if np.iscomplexobj(v) else float(np.dot(u, v))
         v -= c * u
-    v /= math.sqrt(float(np.dot(v, v)))
+    v /= math.sqrt(float(np.vdot(v, v).real))
 
 
 # --------------------------------------------------------------------------
