@@ -69,6 +69,9 @@ _SIGS = {
                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _vp,
                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
                                           _c_int_p, _c_i64_p, _vp, _vp, ctypes.POINTER(_vp)]),
+    "dmrgcc_zsector_svd": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, _c_int_p,
+                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp), _vp]),
     "dmrgcc_heff_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "dmrgcc_heff_info": (ctypes.c_int, [_vp, _c_dbl_p]),
     "dmrgcc_heff_destroy": (ctypes.c_int, [_vp]),
@@ -505,13 +508,19 @@ def sector_svd(mats, stream=None, precondition=0):
     factor exactly orthonormal)."""
     import torch
     n = len(mats)
-    U = [torch.empty((m.shape[0], min(m.shape)), dtype=torch.float64, device=m.device) for m in mats]
+    cz = n > 0 and mats[0].is_complex()
+    dt = torch.complex128 if cz else torch.float64
+    U = [torch.empty((m.shape[0], min(m.shape)), dtype=dt, device=m.device) for m in mats]
     S = [torch.empty((min(m.shape),), dtype=torch.float64, device=m.device) for m in mats]
-    Vt = [torch.empty((min(m.shape), m.shape[1]), dtype=torch.float64, device=m.device) for m in mats]
+    Vt = [torch.empty((min(m.shape), m.shape[1]), dtype=dt, device=m.device) for m in mats]
     arr = lambda ts: (_vp * max(n, 1))(*[_vp(t.data_ptr()) for t in ts])
     M = _i32([m.shape[0] for m in mats])
     N = _i32([m.shape[1] for m in mats])
     _sync_torch()
+    if cz:
+        _check(lib().dmrgcc_zsector_svd(n, arr(mats), _ip(M), _ip(N), arr(U), arr(S), arr(Vt),
+                                        _vp(stream) if stream else _vp()))
+        return U, S, Vt
     _check(lib().dmrgcc_sector_svd(n, arr(mats), _ip(M), _ip(N), arr(U), arr(S), arr(Vt),
                                    int(precondition), _vp(stream) if stream else _vp()))
     return U, S, Vt

@@ -570,6 +570,99 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
 }
 
 
+int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N, double* const* U,
+                       double* const* S, double* const* Vt, void* stream) {
+  return guarded([&] {
+    require(n >= 0 && (n == 0 || (A && M && N && U && S && Vt)), "bad arguments", kErrArgument);
+    Context cx(static_cast<hipStream_t>(stream));
+    // G = A (M >= N) or A^T (M < N), column-major complex; G W = U' Sigma
+    struct P { int M, N, m, k; bool trans; int64_t g, v, s; };
+    std::vector<P> ps(n);
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+      require(M[i] > 0 && N[i] > 0, "empty matrix in sector SVD");
+      P& p = ps[i];
+      p.trans = M[i] < N[i];
+      p.m = std::max(M[i], N[i]);
+      p.k = std::min(M[i], N[i]);
+      p.M = M[i]; p.N = N[i];
+      p.g = w; w += 2 * (int64_t)p.m * p.k;
+      p.v = w; w += 2 * (int64_t)p.k * p.k;
+      p.s = w; w += p.k + 1;  // sigma[k] and |A|_F^2
+    }
+    DevBuf<double> work((size_t)std::max<int64_t>(w, 1));
+    std::vector<CopyTask> cps;
+    std::vector<SvdProblem> probs;
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      for (int part = 0; part < 2; ++part) {
+        CopyTask c{A[i] + part, work.data() + p.g + part, p.M, p.N, 2 * (int64_t)p.N, 2, 0, 0, 1.0};
+        if (!p.trans) { c.dsr = 2; c.dsc = 2 * (int64_t)p.m; }
+        else { c.dsr = 2 * (int64_t)p.m; c.dsc = 2; }
+        cps.push_back(c);
+      }
+      probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s,
+                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0, 0});
+    }
+    launch_copies(cps, cx.copy_scratch, cx.stream);
+    zjacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);
+    std::vector<int> hperm;
+    std::vector<double> hsc;
+    std::vector<int64_t> poff(n);
+    std::vector<std::vector<double>> sorted(n);
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      std::vector<double> sv(p.k);
+      DMRG_HIP(hipMemcpyAsync(sv.data(), work.data() + p.s, p.k * sizeof(double), hipMemcpyDeviceToHost,
+                              cx.stream));
+      cx.sync();
+      std::vector<int> ord(p.k);
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
+      poff[i] = hperm.size();
+      for (int j = 0; j < p.k; ++j) {
+        hperm.push_back(ord[j]);
+        sorted[i].push_back(sv[ord[j]]);
+        hsc.push_back(sv[ord[j]] > 0 ? 1.0 / sv[ord[j]] : std::numeric_limits<double>::quiet_NaN());
+      }
+    }
+    const int64_t ones = hsc.size();
+    int kmax = 1;
+    for (const P& p : ps) kmax = std::max(kmax, p.k);
+    for (int j = 0; j < kmax; ++j) hsc.push_back(1.0);
+    upload(cx.ivec, hperm.empty() ? std::vector<int>{0} : hperm, cx.stream);
+    upload(cx.dvec, hsc, cx.stream);
+    std::vector<ColGatherTask> gts;
+    for (int i = 0; i < n; ++i) {
+      const P& p = ps[i];
+      const int* pp = cx.ivec.data() + poff[i];
+      const double* inv = cx.dvec.data() + poff[i];
+      const double* one = cx.dvec.data() + ones;
+      const double* G = work.data() + p.g;
+      const double* V = work.data() + p.v;
+      // U (M x k) and Vt (k x N), row-major complex
+      ColGatherTask u{}, v{};
+      u.perm = v.perm = pp;
+      u.ncol = v.ncol = p.k;
+      u.dst = U[i]; u.dsr = p.k; u.dsc = 1; u.rows = p.M;
+      v.dst = Vt[i]; v.dsr = 1; v.dsc = p.N; v.rows = p.N;
+      if (!p.trans) {  // A = U' Sigma W^H: U = G / sigma, Vt = W^H
+        u.src = G; u.ld = p.m; u.scale = inv; u.flags = kGatherComplex;
+        v.src = V; v.ld = p.k; v.scale = one; v.flags = kGatherComplex | kGatherConj;
+      } else {         // A^T = U' Sigma W^H: U = conj(W), Vt = U'^T = (G / sigma)^T
+        u.src = V; u.ld = p.k; u.scale = one; u.flags = kGatherComplex | kGatherConj;
+        v.src = G; v.ld = p.m; v.scale = inv; v.flags = kGatherComplex;
+      }
+      gts.push_back(u);
+      gts.push_back(v);
+      DMRG_HIP(hipMemcpyAsync(S[i], sorted[i].data(), p.k * sizeof(double), hipMemcpyHostToDevice,
+                              cx.stream));
+    }
+    launch_col_gather(gts, cx.gather_scratch, cx.stream);
+    cx.sync();
+  });
+}
+
 int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                        const int* maxb, const int* niter, const double* canpre, const double* noise,
                        int precision, int refine, int verbose, dmrgcc_dmrg** out) {

@@ -178,10 +178,13 @@ void fill(double* x, int64_t n, double v, hipStream_t stream);
 //   dst[i*dsr + jj*dsc] = scale[jj] * src[(r0 + i) + perm[jj]*ld]
 // (src column-major).  A NaN scale writes the unit vector e_jj instead
 // (completion of a zero singular vector).  perm/scale are device arrays.
+// flags: kGatherComplex = interleaved complex entries (indices count
+// entries), kGatherConj = conjugate what is written.
+enum : int { kGatherComplex = 1, kGatherConj = 2 };
 struct ColGatherTask {
   const double* src;
   int64_t ld;
-  int r0, rows, ncol, pad;
+  int r0, rows, ncol, flags;
   const int* perm;
   const double* scale;
   double* dst;
@@ -225,5 +228,14 @@ struct SvdProblem {
 };
 void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 60);
+
+// Complex form (the SVD of ComplexMatrix::SVDMatrix, complex_matrix.cc:601-637):
+// G and V hold interleaved complex entries (G: M x N column-major, leading
+// dimension M entries); one-sided block Jacobi with complex rotations, G <-
+// G V, V unitary, sigma_j = |G e_j|.  Phase convention: the largest-modulus
+// entry of every left vector is made real positive, the right vector taking
+// the same phase.  precond is ignored (no QR preconditioning).
+void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
+                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 80);
 
 }  // namespace dmrg

@@ -527,13 +527,30 @@ __global__ void lanczos_step_kernel(double* __restrict__ nxt, double* __restrict
 __global__ __launch_bounds__(256) void col_gather_kernel(const ColGatherTask* __restrict__ tasks) {
   const ColGatherTask t = tasks[blockIdx.x];
   const int64_t total = (int64_t)t.rows * t.ncol;
+  const bool cz = t.flags & kGatherComplex;
+  const double cj = (t.flags & kGatherConj) ? -1.0 : 1.0;
   for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
     const int jj = (int)(e / t.rows), i = (int)(e % t.rows);
     const double s = t.scale[jj];
-    double v;
-    if (s != s) v = (t.r0 + i == jj) ? 1.0 : 0.0;
-    else v = s * t.src[(int64_t)(t.r0 + i) + (int64_t)t.perm[jj] * t.ld];
-    t.dst[(int64_t)i * t.dsr + (int64_t)jj * t.dsc] = v;
+    const int64_t di = (int64_t)i * t.dsr + (int64_t)jj * t.dsc;
+    if (!cz) {
+      double v;
+      if (s != s) v = (t.r0 + i == jj) ? 1.0 : 0.0;
+      else v = s * t.src[(int64_t)(t.r0 + i) + (int64_t)t.perm[jj] * t.ld];
+      t.dst[di] = v;
+    } else {
+      double vr, vi;
+      if (s != s) {
+        vr = (t.r0 + i == jj) ? 1.0 : 0.0;
+        vi = 0.0;
+      } else {
+        const int64_t si = (int64_t)(t.r0 + i) + (int64_t)t.perm[jj] * t.ld;
+        vr = s * t.src[2 * si];
+        vi = s * t.src[2 * si + 1];
+      }
+      t.dst[2 * di] = vr;
+      t.dst[2 * di + 1] = cj * vi;
+    }
   }
 }
 
@@ -848,6 +865,279 @@ __global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __
     if (val < 0.0) {
       for (int i = lane; i < M; i += 64) g[i] = -g[i];
       for (int i = lane; i < N; i += 64) v[i] = -v[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Complex one-sided block Jacobi (interleaved complex G, V).  Visits of 16
+// columns (blocks of 8), 8 waves: Hermitian Gram from real MFMA products
+// (Re = Xr^T Xr + Xi^T Xi, Im = Xr^T Xi - Xi^T Xr), one sweep of two-sided
+// complex Jacobi on it in LDS with the unitary
+//   J = [[c, s e^{i phi}], [-s e^{-i phi}, c]],  e^{i phi} = G_ab / |G_ab|
+// (the real rotation on (a, e^{-i phi} b), phase restored), and X <- X Q with
+// four real MFMA chains per slab.
+// ---------------------------------------------------------------------------
+constexpr int kZw = 8;
+constexpr int kZc = 16;
+
+__global__ __launch_bounds__(256) void zjacobi_init_kernel(const SvdProblem* __restrict__ probs) {
+  const SvdProblem p = probs[blockIdx.x];
+  const int64_t nn = (int64_t)p.N * p.N;
+  for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) {
+    p.V[2 * e] = (e % (p.N + 1) == 0) ? 1.0 : 0.0;
+    p.V[2 * e + 1] = 0.0;
+  }
+  __shared__ double red[4];
+  double s = 0.0;
+  const int64_t mn = 2 * (int64_t)p.M * p.N;
+  for (int64_t e = threadIdx.x; e < mn; e += blockDim.x) s += p.G[e] * p.G[e];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *p.fro2 = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// rows [0, rows) of the visit's complex columns: X[:, col] <- X[:, col] * Q
+__device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int rows, const int* col,
+                                       const double (*Qr)[kZc + 1], const double (*Qi)[kZc + 1],
+                                       int lane, int wave) {
+  const int i = lane & 15, kk = lane >> 4;
+  int cs[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) cs[t] = col[4 * t + kk];
+  const int cout = col[i];
+  double qr[4], qi[4], nqi[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    qr[t] = Qr[4 * t + kk][i];
+    qi[t] = Qi[4 * t + kk][i];
+    nqi[t] = -qi[t];
+  }
+  for (int base = wave * 16; base < rows; base += 16 * kZw) {
+    double ar[4], ai[4];
+    const int r = base + i;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool ok = cs[t] >= 0 && r < rows;
+      const int64_t ix = 2 * ((int64_t)(cs[t] >= 0 ? cs[t] : 0) * ld + r);
+      ar[t] = ok ? X[ix] : 0.0;
+      ai[t] = ok ? X[ix + 1] : 0.0;
+    }
+    d4v re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], qr[t], re, 0, 0, 0);
+      re = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], nqi[t], re, 0, 0, 0);
+      im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[t], qi[t], im, 0, 0, 0);
+      im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[t], qr[t], im, 0, 0, 0);
+    }
+    if (cout >= 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = base + kk + 4 * v;
+        if (rr < rows) {
+          const int64_t ix = 2 * ((int64_t)cout * ld + rr);
+          X[ix] = re[v];
+          X[ix + 1] = im[v];
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
+    const SvdProblem* __restrict__ probs, const int* __restrict__ active,
+    const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
+  __shared__ double Pr[kZw][kZc][kZc + 1];
+  __shared__ double Pi[kZw][kZc][kZc + 1];
+  __shared__ double Qr[kZc][kZc + 1];
+  __shared__ double Qi[kZc][kZc + 1];
+  __shared__ int col[kZc];
+  __shared__ int flag;
+  double (*Gr)[kZc + 1] = Pr[0];
+  double (*Gi)[kZc + 1] = Pi[0];
+  constexpr int kB = kZc / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = blockIdx.x;
+  if (q >= prefix[nactive]) return;
+  int lo = 0, hi = nactive - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= q) lo = mid; else hi = mid - 1;
+  }
+  const int pi = active[lo];
+  const SvdProblem p = probs[pi];
+  const int M = p.M, N = p.N;
+  const int nb = (N + kB - 1) / kB, nb2 = nb + (nb & 1);
+  if (round >= nb2 - 1) return;
+  const int pr = q - prefix[lo];
+  int bi, bj;
+  if (pr == 0) { bi = round; bj = nb2 - 1; }
+  else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+  if (tid < kZc) {
+    const int blk = tid < kB ? bi : bj;
+    const int c = blk * kB + (tid & (kB - 1));
+    col[tid] = (blk < nb && c < N) ? c : -1;
+  }
+  __syncthreads();
+  {
+    const int i = lane & 15, kk = lane >> 4;
+    const int ci = col[i];
+    const double* __restrict__ xc = p.G + 2 * (int64_t)(ci >= 0 ? ci : 0) * M;
+    d4v rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
+    for (int base = wave * 16; base < M; base += 16 * kZw) {
+      double vr[4], vi[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = base + 4 * kk + t;
+        const bool ok = ci >= 0 && r < M;
+        vr[t] = ok ? xc[2 * r] : 0.0;
+        vi[t] = ok ? xc[2 * r + 1] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        rr = __builtin_amdgcn_mfma_f64_16x16x4f64(vr[t], vr[t], rr, 0, 0, 0);
+        ii = __builtin_amdgcn_mfma_f64_16x16x4f64(vi[t], vi[t], ii, 0, 0, 0);
+        ri = __builtin_amdgcn_mfma_f64_16x16x4f64(vr[t], vi[t], ri, 0, 0, 0);
+        ir = __builtin_amdgcn_mfma_f64_16x16x4f64(vi[t], vr[t], ir, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Pr[wave][kk + 4 * v][i] = rr[v] + ii[v];
+      Pi[wave][kk + 4 * v][i] = ri[v] - ir[v];
+    }
+  }
+  __syncthreads();
+  {
+    const int r = tid >> 4, c = tid & 15;
+    if (tid < kZc * kZc) {
+      double sr = Pr[0][r][c], si = Pi[0][r][c];
+      for (int w = 1; w < kZw; ++w) { sr += Pr[w][r][c]; si += Pi[w][r][c]; }
+      Gr[r][c] = sr;
+      Gi[r][c] = si;
+      Qr[r][c] = r == c ? 1.0 : 0.0;
+      Qi[r][c] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const double tol = 2.220446049250313e-16 * (double)M;
+    const double tol2 = tol * tol;
+    const double zero2 = 4.930380657631324e-32 * (*p.fro2);
+    const int pp = lane >> 3, i0 = (lane & 7) * 2;
+    int any = 0;
+    for (int r = 0; r < kZc - 1; ++r) {
+      int a, b;
+      if (pp == 0) { a = r; b = kZc - 1; }
+      else { a = (r + pp) % (kZc - 1); b = (r + kZc - 1 - pp) % (kZc - 1); }
+      if (a > b) { const int t = a; a = b; b = t; }
+      const double al = Gr[a][a], be = Gr[b][b], gr = Gr[a][b], gi = Gi[a][b];
+      const double ag2 = gr * gr + gi * gi;
+      double c = 1.0, s = 0.0, ph_r = 1.0, ph_i = 0.0;
+      const bool doit = al > zero2 && be > zero2 && ag2 > tol2 * al * be;
+      if (doit) {
+        double ri = __builtin_amdgcn_rsq(ag2);
+        ri = ri * (1.5 - 0.5 * ag2 * ri * ri);
+        const double ag = ag2 * ri;
+        ph_r = gr * ri;
+        ph_i = gi * ri;
+        const double d = be - al, g = 2.0 * ag;
+        const double w = d * d + g * g;
+        double rw = __builtin_amdgcn_rsq(w);
+        rw = rw * (1.5 - 0.5 * w * rw * rw);
+        const double den = fabs(d) + w * rw;
+        double rc = __builtin_amdgcn_rcp(den);
+        rc = rc * (2.0 - den * rc);
+        const double t = (d >= 0.0 ? g : -g) * rc;
+        const double u = 1.0 + t * t;
+        double ci = __builtin_amdgcn_rsq(u);
+        ci = ci * (1.5 - 0.5 * u * ci * ci);
+        c = ci;
+        s = ci * t;
+        any = 1;
+      }
+      // J = [[c, s e], [-s conj(e)]...]: x_a' = c x_a - s conj(e) x_b,
+      // x_b' = s e x_a + c x_b, with e = (ph_r, ph_i)
+      const double mr = s * ph_r, mi = s * ph_i;  // s e
+      wave_lds_sync();
+      if (doit) {
+        for (int ii = i0; ii < i0 + 2; ++ii) {
+          double ar = Gr[ii][a], ai = Gi[ii][a], br = Gr[ii][b], bi = Gi[ii][b];
+          Gr[ii][a] = c * ar - (mr * br + mi * bi);
+          Gi[ii][a] = c * ai - (mr * bi - mi * br);
+          Gr[ii][b] = (mr * ar - mi * ai) + c * br;
+          Gi[ii][b] = (mr * ai + mi * ar) + c * bi;
+          ar = Qr[ii][a]; ai = Qi[ii][a]; br = Qr[ii][b]; bi = Qi[ii][b];
+          Qr[ii][a] = c * ar - (mr * br + mi * bi);
+          Qi[ii][a] = c * ai - (mr * bi - mi * br);
+          Qr[ii][b] = (mr * ar - mi * ai) + c * br;
+          Qi[ii][b] = (mr * ai + mi * ar) + c * bi;
+        }
+      }
+      wave_lds_sync();
+      if (doit) {  // G <- J^H G: row a' = c a - s e b, row b' = s conj(e) a + c b
+        for (int ii = i0; ii < i0 + 2; ++ii) {
+          const double ar = Gr[a][ii], ai = Gi[a][ii], br = Gr[b][ii], bi = Gi[b][ii];
+          Gr[a][ii] = c * ar - (mr * br - mi * bi);
+          Gi[a][ii] = c * ai - (mr * bi + mi * br);
+          Gr[b][ii] = (mr * ar + mi * ai) + c * br;
+          Gi[b][ii] = (mr * ai - mi * ar) + c * bi;
+        }
+      }
+      wave_lds_sync();
+    }
+    any = __any(any);
+    if (lane == 0) flag = any;
+  }
+  __syncthreads();
+  if (!flag) return;
+  zapply_rotation(p.G, M, M, col, Qr, Qi, lane, wave);
+  zapply_rotation(p.V, N, N, col, Qr, Qi, lane, wave);
+  if (tid == 0) rotated[pi] = 1;
+}
+
+// sigma and the phase convention, one workgroup per problem, wave per column
+__global__ __launch_bounds__(256) void zjacobi_finish_kernel(const SvdProblem* __restrict__ probs) {
+  const SvdProblem p = probs[blockIdx.x];
+  const int M = p.M, N = p.N;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int j = wave; j < N; j += nwaves) {
+    double* g = p.G + 2 * (int64_t)j * M;
+    double* v = p.V + 2 * (int64_t)j * N;
+    double s = 0.0;
+    for (int i = lane; i < 2 * M; i += 64) s += g[i] * g[i];
+    s = wave_sum(s);
+    if (lane == 0) p.sigma[j] = sqrt(s);
+    const double* u = p.u_in_v ? v : g;
+    const int n = p.u_in_v ? N : M;
+    double best = -1.0, vr = 0.0, vi = 0.0;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < n; i += 64) {
+      const double x = u[2 * i] * u[2 * i] + u[2 * i + 1] * u[2 * i + 1];
+      if (x > best) { best = x; bi = i; vr = u[2 * i]; vi = u[2 * i + 1]; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      const double ovr = __shfl_xor(vr, o, 64);
+      const double ovi = __shfl_xor(vi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; vr = ovr; vi = ovi; }
+    }
+    if (best <= 0.0) continue;
+    const double m = sqrt(best);
+    const double pr = vr / m, pim = -vi / m;  // conj(top) / |top|
+    for (int i = lane; i < M; i += 64) {
+      const double a = g[2 * i], b = g[2 * i + 1];
+      g[2 * i] = a * pr - b * pim;
+      g[2 * i + 1] = a * pim + b * pr;
+    }
+    for (int i = lane; i < N; i += 64) {
+      const double a = v[2 * i], b = v[2 * i + 1];
+      v[2 * i] = a * pr - b * pim;
+      v[2 * i + 1] = a * pim + b * pr;
     }
   }
 }
@@ -1668,6 +1958,54 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
         std::printf(" %dx%d:%d:%d", probs_in[i].M, probs_in[i].N, mode[i], last[i]);
     std::printf("\n");
   }
+}
+
+void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
+                 DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
+  if (probs.empty()) return;
+  const int np = (int)probs.size();
+  for (const SvdProblem& p : probs)
+    require(p.M >= p.N && p.N >= 1, "zjacobi_svd: needs M >= N >= 1", kErrInternal);
+  upload(scratch, probs, stream);
+  hipLaunchKernelGGL(zjacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  DMRG_HIP(hipGetLastError());
+  sweeps_out.ensure((size_t)3 * np + 1);
+  int* rotated = sweeps_out.data();
+  int* dactive = rotated + np;
+  int* dprefix = dactive + np;
+  std::vector<int> active;
+  for (int i = 0; i < np; ++i)
+    if (probs[i].N > 1) active.push_back(i);
+  std::vector<int> hrot(np);
+  constexpr int kB = kZc / 2;
+  for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
+    int visits = 0, rounds = 0;
+    std::vector<int> prefix(1, 0);
+    for (int i : active) {
+      const int nb = (probs[i].N + kB - 1) / kB, nb2 = nb + (nb & 1);
+      visits += nb2 / 2;
+      prefix.push_back(visits);
+      rounds = std::max(rounds, nb2 - 1);
+    }
+    const int na = (int)active.size();
+    std::vector<int> h(np, 0);
+    h.insert(h.end(), active.begin(), active.end());
+    h.resize(2 * np, 0);
+    h.insert(h.end(), prefix.begin(), prefix.end());
+    DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    for (int r = 0; r < rounds; ++r)
+      hipLaunchKernelGGL(zjacobi_round_kernel, dim3(visits), dim3(64 * kZw), 0, stream, scratch.data(),
+                         dactive, dprefix, na, r, rotated);
+    DMRG_HIP(hipGetLastError());
+    DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
+    DMRG_HIP(hipStreamSynchronize(stream));
+    std::vector<int> still;
+    for (int i : active)
+      if (hrot[i]) still.push_back(i);
+    active.swap(still);
+  }
+  hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  DMRG_HIP(hipGetLastError());
 }
 
 }  // namespace dmrg

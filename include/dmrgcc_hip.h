@@ -188,6 +188,14 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
                       double* const* U, double* const* S, double* const* Vt, int precondition,
                       void* stream);
 
+/* Complex (ComplexMatrix::SVDMatrix, complex_matrix.cc:601-637; zgesdd in
+ * the reference): A_i, U_i, Vt_i interleaved complex row-major (A = U S Vt,
+ * Vt = V^H), S_i real descending.  Complex one-sided block Jacobi, no QR
+ * preconditioning; the largest-modulus entry of every column of U is real
+ * positive. */
+int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N,
+                       double* const* U, double* const* S, double* const* Vt, void* stream);
+
 /* ---- full sweep (RealTensorRundmrg::Run, real_tensor_rundmrg.cc:115-245) */
 /* Runs the spinless model from DefineTensorSpace through nhalf half sweeps.
  * Per half sweep h: maxm[h], maxb[h], niter[h], canpre[h], noise[h] (the

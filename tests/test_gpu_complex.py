@@ -197,3 +197,36 @@ def test_zlanczos_matches_oracle(name):
         ket3 = cdev(lat.ket.to_vector())
         e_long, _ = plan.lanczos(ket3, 60, dm.FP64)
         assert e2 >= e_long - 1e-9 * abs(e_long)
+
+
+@pytest.mark.parametrize("shapes", [[(1, 1), (5, 5), (40, 12), (12, 40)], [(120, 90), (90, 120)],
+                                    [(300, 64), (33, 257)]])
+def test_zsector_svd(shapes):
+    # complex sector SVD (ComplexMatrix::SVDMatrix) against numpy on graded
+    # spectra down to 1e-12, with the phase convention of the oracle
+    from oracle.dmrg import fix_signs
+    rng = np.random.default_rng(21)
+    mats, spec = [], []
+    for (M, N) in shapes:
+        a = rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))
+        u, s, vh = np.linalg.svd(a, full_matrices=False)
+        s = np.logspace(0, -12, len(s))
+        mats.append((u * s) @ vh)
+        spec.append(s)
+    U, S, Vt = dm.sector_svd([cdev(a) for a in mats])
+    for a, s, u, sg, vt in zip(mats, spec, U, S, Vt):
+        u, sg, vt = u.cpu().numpy(), sg.cpu().numpy(), vt.cpu().numpy()
+        M, N = a.shape
+        k = min(M, N)
+        assert np.all(np.diff(sg) <= 0)
+        assert np.abs(sg - s).max() < 1e-14 * max(M, N)
+        assert np.abs((u * sg) @ vt - a).max() < 1e-14 * max(M, N)
+        assert np.abs(u.conj().T @ u - np.eye(k)).max() < 1e-12
+        assert np.abs(vt @ vt.conj().T - np.eye(k)).max() < 1e-12
+        # phase convention: largest-modulus entry of each left vector real > 0
+        top = u[np.argmax(np.abs(u), axis=0), np.arange(k)]
+        assert np.abs(top.imag).max() < 1e-12 and (top.real > 0).all()
+        ur, _, _ = np.linalg.svd(a, full_matrices=False)
+        ur = ur * fix_signs(ur)[None, :]
+        big = sg > 1e-4
+        assert np.abs(ur[:, big] - u[:, big]).max() < 1e-8
