@@ -39,6 +39,15 @@ _SIGS = {
                                                     ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                     ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_model_zspinless_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                      ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_model_spinful_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
+                                                    _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_model_info": (ctypes.c_int, [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "dmrgcc_model_destroy": (ctypes.c_int, [_vp]),
     "dmrgcc_model_num_site": (ctypes.c_int, [_vp]),
     "dmrgcc_model_space_table": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p]),
@@ -194,18 +203,15 @@ def truncate_dims(svals, max_dim, canpre):
     return trunc, out
 
 
-class SpinlessModel:
-    """RealSpinlessBoseSquareSpace + RealSpinlessBoseSquareHamiltonian."""
+class _Model:
+    """Model handle: MPS QN tables and the deparallelised MPO (real or
+    complex), and the sweep engine on it."""
 
-    def __init__(self, nx, ny, num_boson, physics_dim, hop_x=-1.0, hop_y=-1.0, U=0.0, mu=0.0,
-                 period_x=False, period_y=False):
-        self.nx, self.ny, self.N, self.d = nx, ny, num_boson, physics_dim
-        self.L = nx * ny
-        h = _vp()
-        _check(lib().dmrgcc_model_spinless_create(nx, ny, num_boson, physics_dim, hop_x, hop_y, U,
-                                                  mu, int(period_x), int(period_y),
-                                                  ctypes.byref(h)))
-        self._h = h
+    def _init_info(self):
+        L, d, nq, cz = (ctypes.c_int() for _ in range(4))
+        _check(lib().dmrgcc_model_info(self._h, ctypes.byref(L), ctypes.byref(d), ctypes.byref(nq),
+                                       ctypes.byref(cz)))
+        self.L, self.d, self.nq, self.complex = L.value, d.value, nq.value, bool(cz.value)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -213,40 +219,79 @@ class SpinlessModel:
             self._h = None
 
     def space_table(self, bond):
+        """MPS bond QN table: (count,) for one QN per sector, else (count, nq)."""
         c = ctypes.c_int()
         _check(lib().dmrgcc_model_space_table(self._h, bond, ctypes.byref(c), None))
-        q = np.zeros(c.value, dtype=np.int32)
+        q = np.zeros(c.value * self.nq, dtype=np.int32)
         _check(lib().dmrgcc_model_space_table(self._h, bond, ctypes.byref(c), _ip(q)))
-        return q
+        return q if self.nq == 1 else q.reshape(c.value, self.nq)
 
     def mpo_site(self, site):
         lb, rb = ctypes.c_int(), ctypes.c_int()
         _check(lib().dmrgcc_model_mpo_site(self._h, site, ctypes.byref(lb), ctypes.byref(rb), None))
-        W = np.zeros((lb.value, rb.value, self.d, self.d))
-        _check(lib().dmrgcc_model_mpo_site(self._h, site, ctypes.byref(lb), ctypes.byref(rb), _dp(W)))
+        shape = (lb.value, rb.value, self.d, self.d)
+        W = np.zeros(shape, dtype=np.complex128 if self.complex else np.float64)
+        _check(lib().dmrgcc_model_mpo_site(self._h, site, ctypes.byref(lb), ctypes.byref(rb),
+                                           _dp(W.view(np.float64))))
         return W
 
     def mpo_table(self, bond):
         c = ctypes.c_int()
         _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), None))
-        q = np.zeros(c.value, dtype=np.int32)
+        q = np.zeros(c.value * self.nq, dtype=np.int32)
         _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), _ip(q)))
-        return q
+        return q if self.nq == 1 else q.reshape(c.value, self.nq)
 
     def start(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
         """A resumable run (DmrgRun) of this model over the sweep table."""
         return DmrgRun(self, table, first_block, first_dim, precision, refine, verbose)
 
     def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
-        """RealTensorRundmrg::Run.  table: rows (maxm, maxb, niter, canpre,
-        noise_exponent) per half sweep.  Returns a dict of per-half-sweep
-        energies, discarded weights, seconds and run statistics."""
+        """RealTensorRundmrg::Run (ComplexTensorRundmrg::Run on a complex
+        model).  table: rows (maxm, maxb, niter, canpre, noise_exponent) per
+        half sweep.  Returns a dict of per-half-sweep energies, discarded
+        weights, seconds and run statistics."""
         n, maxm, maxb, niter, canpre, noise = _table_args(table)
         e, t, s, st = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(24)
         _check(lib().dmrgcc_run(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
                                 _dp(canpre), _dp(noise), precision, int(refine), int(verbose), _dp(e),
                                 _dp(t), _dp(s), _dp(st)))
         return _result_dict(e, t, s, st)
+
+
+class SpinlessModel(_Model):
+    """RealSpinlessBoseSquareSpace + RealSpinlessBoseSquareHamiltonian, or with
+    complex=True their complex twins (same MPO on complex storage, complex
+    sweep engine)."""
+
+    def __init__(self, nx, ny, num_boson, physics_dim, hop_x=-1.0, hop_y=-1.0, U=0.0, mu=0.0,
+                 period_x=False, period_y=False, complex=False):
+        self.nx, self.ny, self.N = nx, ny, num_boson
+        h = _vp()
+        create = lib().dmrgcc_model_zspinless_create if complex else lib().dmrgcc_model_spinless_create
+        _check(create(nx, ny, num_boson, physics_dim, hop_x, hop_y, U, mu, int(period_x), int(period_y),
+                      ctypes.byref(h)))
+        self._h = h
+        self._init_info()
+
+
+class SpinfulModel(_Model):
+    """ComplexSpinfulBoseSquareSpace + ComplexSpinfulBoseSquareHamiltonian:
+    two boson species on an nx x ny lattice (MPS sites 2p = up, 2p+1 =
+    down), y-bond phase exp(i 2 pi (x+1) flux).  Scalars apply to both
+    species; U_uu, U_dd, U_ud are the intra/intra/inter interactions."""
+
+    def __init__(self, nx, ny, n_up, n_down, physics_dim, flux=0.0, hop_x=-1.0, hop_y=-1.0,
+                 U_uu=0.0, U_dd=0.0, U_ud=0.0, mu=0.0, period_x=False, period_y=False):
+        self.nx, self.ny, self.N = nx, ny, (n_up, n_down)
+        pair = lambda v: _f64(v if np.ndim(v) else [v, v])
+        h = _vp()
+        _check(lib().dmrgcc_model_spinful_create(nx, ny, n_up, n_down, physics_dim, _dp(pair(flux)),
+                                                 _dp(pair(hop_x)), _dp(pair(hop_y)),
+                                                 _dp(_f64([U_uu, U_dd, U_ud])), _dp(pair(mu)),
+                                                 int(period_x), int(period_y), ctypes.byref(h)))
+        self._h = h
+        self._init_info()
 
 
 def _table_args(table):
@@ -308,7 +353,8 @@ class DmrgRun:
 
     def site_blocks(self, site, which):
         """(blocks [(o, l, r, rows, cols, off)], host data) of the ket (0),
-        left (1) or right (2) environment of `site`."""
+        left (1) or right (2) environment of `site` (complex128 data on a
+        complex model; offsets count entries)."""
         c = ctypes.c_int()
         tot = ctypes.c_int64()
         _check(lib().dmrgcc_dmrg_site_blocks(self._h, site, which, ctypes.byref(c), ctypes.byref(tot),
@@ -316,10 +362,10 @@ class DmrgRun:
         n = c.value
         o, l, r, rows, cols = (np.zeros(max(n, 1), dtype=np.int32) for _ in range(5))
         off = np.zeros(max(n, 1), dtype=np.int64)
-        data = np.zeros(max(tot.value, 1))
+        data = np.zeros(max(tot.value, 1), dtype=np.complex128 if self.model.complex else np.float64)
         _check(lib().dmrgcc_dmrg_site_blocks(self._h, site, which, ctypes.byref(c), ctypes.byref(tot),
                                              _ip(o), _ip(l), _ip(r), _ip(rows), _ip(cols), _lp(off),
-                                             _dp(data)))
+                                             _dp(data.view(np.float64))))
         blocks = [(int(o[i]), int(l[i]), int(r[i]), int(rows[i]), int(cols[i]), int(off[i]))
                   for i in range(n)]
         return blocks, data[:tot.value]
@@ -426,7 +472,7 @@ class HeffPlan:
         self = cls.__new__(cls)
         self._h = h
         self._keep = owner
-        self.complex = False
+        self.complex = bool(getattr(getattr(owner, "model", None), "complex", False))
         return self
 
     def info(self):

@@ -15,8 +15,8 @@
 using namespace dmrg;
 
 struct dmrgcc_model {
-  std::unique_ptr<RealSpinlessBoseSquareSpace> space;
-  std::unique_ptr<RealSpinlessBoseSquareHamiltonian> ham;
+  std::unique_ptr<RealTensorSpace> space;
+  std::unique_ptr<RealTensorHamiltonian> ham;
   std::vector<QNTable> tables;
 };
 
@@ -285,6 +285,51 @@ int dmrgcc_model_spinless_create(int nx, int ny, int num_boson, int physics_dim,
   });
 }
 
+int dmrgcc_model_zspinless_create(int nx, int ny, int num_boson, int physics_dim, double hop_x,
+                                  double hop_y, double U, double mu, int period_x, int period_y,
+                                  dmrgcc_model** out) {
+  return guarded([&] {
+    require(out && nx > 0 && ny > 0 && physics_dim > 1 && num_boson >= 0, "bad model arguments",
+            kErrArgument);
+    auto m = std::make_unique<dmrgcc_model>();
+    m->space = std::make_unique<ComplexSpinlessBoseSquareSpace>(nx, ny, num_boson, physics_dim);
+    m->ham = std::make_unique<ComplexSpinlessBoseSquareHamiltonian>(nx, ny, physics_dim, 0.0, hop_x, hop_y,
+                                                                    U, mu, period_x != 0, period_y != 0);
+    m->ham->DefineTensorHamiltonian();
+    m->tables = m->space->DefineQuantumTable();
+    *out = m.release();
+  });
+}
+
+int dmrgcc_model_spinful_create(int nx, int ny, int n_up, int n_down, int physics_dim,
+                                const double* flux, const double* hop_x, const double* hop_y,
+                                const double* inter, const double* mu, int period_x, int period_y,
+                                dmrgcc_model** out) {
+  return guarded([&] {
+    require(out && flux && hop_x && hop_y && inter && mu, "bad model arguments", kErrArgument);
+    require(nx > 0 && ny > 0 && physics_dim > 1 && n_up >= 0 && n_down >= 0, "bad model arguments",
+            kErrArgument);
+    auto m = std::make_unique<dmrgcc_model>();
+    m->space = std::make_unique<ComplexSpinfulBoseSquareSpace>(nx, ny, n_up, n_down, physics_dim);
+    m->ham = std::make_unique<ComplexSpinfulBoseSquareHamiltonian>(nx, ny, physics_dim, flux, hop_x, hop_y,
+                                                                   inter, mu, period_x != 0, period_y != 0);
+    m->ham->DefineTensorHamiltonian();
+    m->tables = m->space->DefineQuantumTable();
+    *out = m.release();
+  });
+}
+
+int dmrgcc_model_info(const dmrgcc_model* m, int* num_site, int* physics_dim, int* num_quantum,
+                      int* is_complex) {
+  return guarded([&] {
+    require(m, "bad arguments", kErrArgument);
+    if (num_site) *num_site = m->space->num_site();
+    if (physics_dim) *physics_dim = m->space->physics_dim();
+    if (num_quantum) *num_quantum = m->space->num_quantum();
+    if (is_complex) *is_complex = m->ham->complex() ? 1 : 0;
+  });
+}
+
 int dmrgcc_model_destroy(dmrgcc_model* m) {
   delete m;
   return 0;
@@ -296,8 +341,10 @@ int dmrgcc_model_space_table(const dmrgcc_model* m, int bond, int* count, int* q
   return guarded([&] {
     require(m && count && bond >= 0 && bond < (int)m->tables.size(), "bad arguments", kErrArgument);
     *count = (int)m->tables[bond].size();
+    const int nq = m->space->num_quantum();
     if (qn)
-      for (int i = 0; i < *count; ++i) qn[i] = m->tables[bond][i][0];
+      for (int i = 0; i < *count; ++i)
+        for (int q = 0; q < nq; ++q) qn[(size_t)i * nq + q] = m->tables[bond][i][q];
   });
 }
 
@@ -307,7 +354,12 @@ int dmrgcc_model_mpo_site(const dmrgcc_model* m, int site, int* lb, int* rb, dou
     const SiteOperator& op = m->ham->sites()[site];
     *lb = op.lb;
     *rb = op.rb;
-    if (W) std::memcpy(W, op.w.data(), op.w.size() * sizeof(double));
+    if (W && !op.complex()) std::memcpy(W, op.w.data(), op.w.size() * sizeof(double));
+    if (W && op.complex())
+      for (size_t i = 0; i < op.w.size(); ++i) {
+        W[2 * i] = op.w[i];
+        W[2 * i + 1] = op.wi[i];
+      }
   });
 }
 
@@ -317,8 +369,10 @@ int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn)
     const auto& t = m->ham->bond_tables();
     require(bond >= 0 && bond < (int)t.size(), "bond out of range", kErrArgument);
     *count = (int)t[bond].size();
+    const int nq = m->ham->num_quantum();
     if (qn)
-      for (int i = 0; i < *count; ++i) qn[i] = t[bond][i][0];
+      for (int i = 0; i < *count; ++i)
+        for (int q = 0; q < nq; ++q) qn[(size_t)i * nq + q] = t[bond][i][q];
   });
 }
 
@@ -735,7 +789,8 @@ int dmrgcc_dmrg_site_blocks(dmrgcc_dmrg* h, int site, int which, int* count, int
         ++k;
       }
       if (data && t->size > 0)
-        DMRG_HIP(hipMemcpy(data + acc, t->base(), t->size * sizeof(double), hipMemcpyDeviceToHost));
+        DMRG_HIP(hipMemcpy(data + t->cs * acc, t->base(), t->doubles() * sizeof(double),
+                           hipMemcpyDeviceToHost));
       acc += t->size;
     }
   });

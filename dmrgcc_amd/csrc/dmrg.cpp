@@ -61,6 +61,23 @@ struct Shape {
   int64_t size() const { return (int64_t)rows * cols; }
 };
 
+// Push a copy given in entry units (pointers at entry starts, strides in
+// entries) as real copies: one, or the re and im parts of interleaved
+// complex entries (cs = 2).
+void push_copy(std::vector<CopyTask>& v, CopyTask c, int cs) {
+  if (cs == 1) {
+    v.push_back(c);
+    return;
+  }
+  c.ssr *= 2; c.ssc *= 2; c.dsr *= 2; c.dsc *= 2;
+  for (int part = 0; part < 2; ++part) {
+    CopyTask t = c;
+    if (t.src) t.src += part;
+    t.dst += part;
+    v.push_back(t);
+  }
+}
+
 // One piece of a block assembled by the expansion / match fills.
 struct Piece {
   const double* src;  // nullptr: zeros
@@ -354,13 +371,16 @@ void HeffPlan::apply_stage(int stage, const double* x, double* y, int precision,
 // ===========================================================================
 // Environment updates
 // ===========================================================================
-Env boundary_env(int nbond, Context& cx) {
-  // DefineTensorContraction (real_tensor_contraction.cc:111-137)
+Env boundary_env(int nbond, Context& cx, int cs) {
+  // DefineTensorContraction (real_tensor_contraction.cc:111-137; complex
+  // complex_tensor_contraction.cc:111-137: the 1x1 block holds 1 + 0i)
   Env e(nbond);
   for (int i = 0; i < nbond; ++i) {
     e[i].define({0}, {0});
     e[i].blk[0].rows = e[i].blk[0].cols = 1;
+    e[i].cs = cs;
     e[i].allocate();
+    fill(e[i].data.data(), cs, 0.0, cx.stream);
     fill(e[i].data.data(), 1, 1.0, cx.stream);
   }
   return e;
@@ -615,6 +635,7 @@ RealTensorRundmrg::RealTensorRundmrg(const RealTensorSpace& space, const RealTen
     : space_(space), ham_(ham), p_(p), cx_(stream) {
   L_sites_ = space.num_site();
   d_ = space.physics_dim();
+  cs_ = ham.complex() ? 2 : 1;
   require(ham.num_site() == L_sites_, "space and Hamiltonian disagree on the number of sites",
           kErrArgument);
   tables_ = space.DefineQuantumTable();
@@ -661,16 +682,19 @@ void RealTensorRundmrg::tensor_index(int site, const Lattice& lat, std::vector<i
 void RealTensorRundmrg::random_blocks(Lattice& lat, const std::vector<int>& li,
                                       const std::vector<int>& ri, const std::vector<int>& pi,
                                       int rows, int cols) {
-  // DefineTensorLattice (real_tensor_lattice.cc:343-359)
+  // DefineTensorLattice (real_tensor_lattice.cc:343-359; complex
+  // complex_tensor_lattice.cc:343-359)
   lat.ket.define(li, ri);
+  lat.ket.cs = cs_;
   std::vector<double> host;
   for (size_t k = 0; k < li.size(); ++k) {
     lat.phys[li[k]][ri[k]] = pi[k];
     lat.ket.blk[k].rows = rows;
     lat.ket.blk[k].cols = cols;
     const size_t o = host.size();
-    host.resize(o + (size_t)rows * cols);
-    cx_.rng.random_matrix(rows, cols, host.data() + o);
+    host.resize(o + (size_t)cs_ * rows * cols);
+    if (cs_ == 2) cx_.rng.complex_matrix(rows, cols, host.data() + o);
+    else cx_.rng.random_matrix(rows, cols, host.data() + o);
   }
   lat.ket.allocate();
   if (!host.empty())
@@ -713,6 +737,7 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
   Lattice& lat = lat_[site];
   BlockTensor& ket = lat.ket;
   const bool left = leigh == 0;
+  const int cs = cs_;
   const int ns = left ? lat.nr() : lat.nl();
   struct Sec {
     std::vector<int> pos;
@@ -747,8 +772,8 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
     S.trans = S.rows < S.cols;
     S.M = std::max(S.rows, S.cols);
     S.N = std::min(S.rows, S.cols);
-    S.g = wsize; wsize += (int64_t)S.M * S.N;
-    S.v = wsize; wsize += (int64_t)S.N * S.N;
+    S.g = wsize; wsize += (int64_t)cs * S.M * S.N;  // doubles
+    S.v = wsize; wsize += (int64_t)cs * S.N * S.N;
     S.s = wsize; wsize += S.N + 1;  // sigma[N] and |A|_F^2
   }
   cx_.work.ensure((size_t)std::max<int64_t>(wsize, 1));
@@ -771,15 +796,15 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
       // element (i, j) of block p sits at A(acc+i, j) (left) or A(i, acc+j) (right)
       const int64_t M = S.M;
       if (left) {
-        if (!S.trans) { c.dst = w + S.g + acc; c.dsr = 1; c.dsc = M; }
-        else { c.dst = w + S.g + (int64_t)acc * M; c.dsr = M; c.dsc = 1; }
+        if (!S.trans) { c.dst = w + S.g + (int64_t)cs * acc; c.dsr = 1; c.dsc = M; }
+        else { c.dst = w + S.g + (int64_t)cs * acc * M; c.dsr = M; c.dsc = 1; }
         acc += b.rows;
       } else {
-        if (!S.trans) { c.dst = w + S.g + (int64_t)acc * M; c.dsr = 1; c.dsc = M; }
-        else { c.dst = w + S.g + acc; c.dsr = M; c.dsc = 1; }
+        if (!S.trans) { c.dst = w + S.g + (int64_t)cs * acc * M; c.dsr = 1; c.dsc = M; }
+        else { c.dst = w + S.g + (int64_t)cs * acc; c.dsr = M; c.dsc = 1; }
         acc += b.cols;
       }
-      copies.push_back(c);
+      push_copy(copies, c, cs);
     }
     // QR preconditioning that keeps exact the factor read as orthonormal:
     // G (left canonical of a tall sector, right canonical of a wide one) or V
@@ -789,7 +814,9 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
   }
   launch_copies(copies, cx_.copy_scratch, cx_.stream);
   const double tsvd = now();
-  jacobi_svd(probs, cx_.svd_scratch, cx_.svd_sweeps, cx_.stream);
+  // complex: ComplexMatrix::SVDMatrix (complex_matrix.cc:601-637)
+  if (cs == 2) zjacobi_svd(probs, cx_.svd_scratch, cx_.svd_sweeps, cx_.stream);
+  else jacobi_svd(probs, cx_.svd_scratch, cx_.svd_sweeps, cx_.stream);
   cx_.sync();
   ph_.svd += now() - tsvd;
   for (const SvdProblem& q : probs) {
@@ -874,7 +901,9 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
   // new ket and canonical tensors
   BlockTensor nk;
   nk.blk = ket.blk;
+  nk.cs = cs;
   BlockTensor can;
+  can.cs = cs;
   {
     std::vector<int> idx(ns);
     std::iota(idx.begin(), idx.end(), 0);
@@ -910,10 +939,10 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
     if (!S.svd) {
       for (int p : S.pos)
         if (nk.elems(p) > 0)
-          cps.push_back(CopyTask{ket.ptr(p), nk.ptr(p), nk.blk[p].rows, nk.blk[p].cols,
-                                 nk.blk[p].cols, 1, nk.blk[p].cols, 1, 1.0});
+          push_copy(cps, CopyTask{ket.ptr(p), nk.ptr(p), nk.blk[p].rows, nk.blk[p].cols,
+                                  nk.blk[p].cols, 1, nk.blk[p].cols, 1, 1.0}, cs);
       if (can.elems(s) > 0)
-        cps.push_back(CopyTask{nullptr, can.ptr(s), 1, (int)can.elems(s), 0, 0, 0, 1, 0.0});
+        push_copy(cps, CopyTask{nullptr, can.ptr(s), 1, (int)can.elems(s), 0, 0, 0, 1, 0.0}, cs);
       continue;
     }
     const double* G = w + S.g;
@@ -941,6 +970,9 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
         if (!S.trans) { g.src = V; g.ld = S.N; g.scale = one; }
         else { g.src = G; g.ld = S.M; g.scale = su; }
       }
+      // complex: A = U Sigma W^H (G = A) or A = conj(W) Sigma U'^T (G = A^T),
+      // so every factor read from V = W is conjugated
+      if (cs == 2) g.flags = kGatherComplex | (g.src == V ? kGatherConj : 0);
       gts.push_back(g);
       acc += len;
     }
@@ -959,6 +991,7 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
       if (!S.trans) { c.src = G; c.ld = S.M; }
       else { c.src = V; c.ld = S.N; }
     }
+    if (cs == 2) c.flags = kGatherComplex | (c.src == V ? kGatherConj : 0);
     gts.push_back(c);
   }
   launch_copies(cps, cx_.copy_scratch, cx_.stream);
@@ -982,6 +1015,7 @@ void RealTensorRundmrg::merge(int leigh, int site) {
   const BlockTensor& can = nb->can;
   BlockTensor nk;
   nk.blk = ket.blk;
+  nk.cs = ket.cs;
   const int ns = leigh == 0 ? lat->nl() : lat->nr();
   require(can.n() >= ns, "canonical tensor has fewer sectors than the merged site");
   for (int p = 0; p < ket.n(); ++p) {
@@ -1002,6 +1036,18 @@ void RealTensorRundmrg::merge(int leigh, int site) {
     const Block& b = ket.blk[p];
     const Block& o = nk.blk[p];
     if ((int64_t)o.rows * o.cols == 0) continue;
+    if (nk.cs == 2) {  // ComplexTensorLattice merge: complex products
+      if (leigh == 0) {
+        const Block& c = can.blk[b.l];
+        zgemm_item(gb, nk.ptr(p), o.cols, o.rows, o.cols, 0, 0,
+                   {ZTerm{can.ptr(b.l), c.cols, ket.ptr(p), b.cols, c.cols, 0, false, false, 1.0, 0.0}});
+      } else {
+        const Block& c = can.blk[b.r];
+        zgemm_item(gb, nk.ptr(p), o.cols, o.rows, o.cols, 0, 0,
+                   {ZTerm{ket.ptr(p), b.cols, can.ptr(b.r), c.cols, b.cols, 0, false, false, 1.0, 0.0}});
+      }
+      continue;
+    }
     gb.begin_item(nk.ptr(p), o.cols, o.rows, o.cols, 0);
     if (leigh == 0) {
       const Block& c = can.blk[b.l];
@@ -1017,10 +1063,11 @@ void RealTensorRundmrg::merge(int leigh, int site) {
     if (leigh == 0) lat->ld[s] = can.blk[s].rows;
     else lat->rd[s] = can.blk[s].cols;
   }
-  // NormalizeTensorLattice (real_matrix_block.cc:164-171)
+  // NormalizeTensorLattice (real_matrix_block.cc:164-171; complex: sum of
+  // |z|^2 over the interleaved doubles, complex_matrix_block.cc:163-170)
   double* sc = cx_.scal.data();
-  dot(nk.data.data(), nk.data.data(), nk.size, sc, cx_.partial, cx_.stream);
-  scale_inv_sqrt(nk.data.data(), nk.size, sc, cx_.stream, 1);
+  dot(nk.data.data(), nk.data.data(), nk.doubles(), sc, cx_.partial, cx_.stream);
+  scale_inv_sqrt(nk.data.data(), nk.doubles(), sc, cx_.stream, 1);
   cx_.sync();
   ket = std::move(nk);
   nb->has_can = false;
@@ -1030,8 +1077,8 @@ void RealTensorRundmrg::merge(int leigh, int site) {
 
 void RealTensorRundmrg::Initialize() {
   // Initialize (real_tensor_rundmrg.cc:247-287)
-  L_[0] = boundary_env(ham_.sites()[0].lb, cx_);
-  R_[L_sites_ - 1] = boundary_env(ham_.sites()[L_sites_ - 1].rb, cx_);
+  L_[0] = boundary_env(ham_.sites()[0].lb, cx_, cs_);
+  R_[L_sites_ - 1] = boundary_env(ham_.sites()[L_sites_ - 1].rb, cx_, cs_);
   for (int i = L_sites_ - 1; i > 0; --i) {
     canonical(1, i);
     if (p_.verbose >= 2) trace(-1, i, 0.0, "init_canon");
@@ -1237,8 +1284,9 @@ std::vector<std::vector<int>> RealTensorRundmrg::expan_lattice(int leigh, int si
   // DefineTensorLattice(..., 1, 1) draws one rand() per block, then the
   // blocks are reset to empty
   for (size_t k = 0; k < li.size(); ++k) E.phys[li[k]][ri[k]] = pi[k];
-  cx_.rng.skip((int64_t)li.size());
+  cx_.rng.skip((int64_t)cs_ * li.size());
   E.ket.define(li, ri);
+  E.ket.cs = cs_;
   return mapping;
 }
 
@@ -1263,6 +1311,10 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   auto wat = [&](int outer, int inner, int p1, int p2) {
     return left ? W.at(outer, inner, p1, p2) : W.at(inner, outer, p1, p2);
   };
+  auto wim = [&](int outer, int inner, int p1, int p2) {
+    return left ? W.im(outer, inner, p1, p2) : W.im(inner, outer, p1, p2);
+  };
+  const int cs = cs_;
   for (int ob = 0; ob < nouter; ++ob) {
     const BlockTensor& Eb = env[ob];
     for (int a = 0; a < Eb.n(); ++a) {
@@ -1278,7 +1330,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
           const int pos = left ? E.ket.find(idx, ldx) : E.ket.find(ldx, idx);
           if (ldx != -1 && pos != -1) {
             const int p1 = left ? E.phys[idx][ldx] : E.phys[ldx][idx];
-            if (wat(ob, ib, p1, p2) != 0.0) exist[ldx] = 1;
+            if (wat(ob, ib, p1, p2) != 0.0 || wim(ob, ib, p1, p2) != 0.0) exist[ldx] = 1;
           }
         }
       }
@@ -1361,10 +1413,10 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
           const int pos = left ? E.ket.find(idx, ldx) : E.ket.find(ldx, idx);
           if (ldx != -1 && pos != -1) {
             const int p1 = left ? E.phys[idx][ldx] : E.phys[ldx][idx];
-            const double w = wat(ob, ib, p1, p2);
-            if (w != 0.0) {
+            const double w = wat(ob, ib, p1, p2), wi = wim(ob, ib, p1, p2);
+            if (w != 0.0 || wi != 0.0) {
               if (!active) {
-                cx_.rng.skip((int64_t)r_rows * r_cols);
+                cx_.rng.skip((int64_t)cs * r_rows * r_cols);
                 continue;
               }
               Term t;
@@ -1372,11 +1424,19 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
               t.pos = pos;
               t.e0 = r_rows;
               t.e1 = r_cols;
-              t.roff = (int64_t)rnd.size();
-              rnd.resize(rnd.size() + (size_t)r_rows * r_cols);
+              t.roff = (int64_t)rnd.size();  // doubles
+              rnd.resize(rnd.size() + (size_t)cs * r_rows * r_cols);
               double* R = rnd.data() + t.roff;
-              cx_.rng.random_matrix(r_rows, r_cols, R);
-              for (int q = 0; q < std::min(r_rows, r_cols); ++q) R[(int64_t)q * r_cols + q] = 0.1 * w;
+              if (cs == 2) {
+                cx_.rng.complex_matrix(r_rows, r_cols, R);
+                for (int q = 0; q < std::min(r_rows, r_cols); ++q) {
+                  R[2 * ((int64_t)q * r_cols + q)] = 0.1 * w;
+                  R[2 * ((int64_t)q * r_cols + q) + 1] = 0.1 * wi;
+                }
+              } else {
+                cx_.rng.random_matrix(r_rows, r_cols, R);
+                for (int q = 0; q < std::min(r_rows, r_cols); ++q) R[(int64_t)q * r_cols + q] = 0.1 * w;
+              }
               // product shape and its place in the (concatenated) block
               Shape ps = left ? Shape{f.sh.rows, r_cols} : Shape{r_rows, f.sh.cols};
               Shape& es = eshape[pos];
@@ -1411,10 +1471,17 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
       for (int l = 0; l < E.nl(); ++l) {
         const int pos = E.ket.find(l, r);
         if (pos != -1 && eshape[pos].rows == 0) {
-          // the reference sizes this zero block by left_dim_[r] (:448)
           std::printf("row of expan_tensor is zero in LeftExpanTensorContraction\n");
+          if (cs == 2) {  // complex_tensor_contraction.cc:446-451
+            eshape[pos] = Shape{E.ld[l], E.rd[r]};
+            continue;
+          }
+          // the reference sizes this zero block by left_dim_[r] (:448)
           require(r < E.nl(), "zero expansion block with r >= num_left_block (reference reads out of range)");
           eshape[pos] = Shape{E.ld[r], E.rd[r]};
+        } else if (pos != -1 && cs == 2) {
+          // complex pads short blocks with zero columns (complex_tensor_contraction.cc:453-462)
+          eshape[pos].cols = std::max(eshape[pos].cols, E.rd[r]);
         }
       }
     }
@@ -1429,8 +1496,15 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
         const int pos = E.ket.find(l, r);
         if (pos != -1 && eshape[pos].rows == 0) {
           std::printf("row of expan_tensor is zero in RightExpanTensorContraction\n");
+          if (cs == 2) {  // complex_tensor_contraction.cc:762-767
+            eshape[pos] = Shape{E.ld[l], E.rd[r]};
+            continue;
+          }
           require(r < E.nl(), "zero expansion block with r >= num_left_block (reference reads out of range)");
           eshape[pos] = Shape{E.ld[r], E.rd[r]};
+        } else if (pos != -1 && cs == 2) {
+          // complex pads short blocks with zero rows (complex_tensor_contraction.cc:769-778)
+          eshape[pos].rows = std::max(eshape[pos].rows, E.ld[l]);
         }
       }
     }
@@ -1439,10 +1513,11 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     E.ket.blk[p].rows = eshape[p].rows;
     E.ket.blk[p].cols = eshape[p].cols;
   }
+  E.ket.cs = cs;
   E.ket.allocate();
-  fill(E.ket.data.data(), E.ket.size, 0.0, cx_.stream);
+  fill(E.ket.data.data(), E.ket.doubles(), 0.0, cx_.stream);
   // ---- device: first = L*K (or K*R), then noise * first * random ----
-  DevBuf<double> F((size_t)std::max<int64_t>(fsize, 1)), Rd((size_t)std::max<size_t>(rnd.size(), 1));
+  DevBuf<double> F((size_t)std::max<int64_t>(cs * fsize, 1)), Rd((size_t)std::max<size_t>(rnd.size(), 1));
   if (!rnd.empty())
     DMRG_HIP(hipMemcpyAsync(Rd.data(), rnd.data(), rnd.size() * sizeof(double), hipMemcpyHostToDevice,
                             cx_.stream));
@@ -1456,6 +1531,12 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     const BlockTensor& Eb = env[f.ob];
     const Block& eb = Eb.blk[f.a];
     const Block& kb = ket.blk[f.kp];
+    if (cs == 2) {
+      const ZTerm zt = left ? ZTerm{Eb.ptr(f.a), eb.cols, ket.ptr(f.kp), kb.cols, eb.cols, 0, false, false, 1.0, 0.0}
+                            : ZTerm{ket.ptr(f.kp), kb.cols, Eb.ptr(f.a), eb.cols, kb.cols, 0, false, false, 1.0, 0.0};
+      zgemm_item(s1, F.data() + cs * f.off, f.sh.cols, f.sh.rows, f.sh.cols, 0, 0, {zt});
+      continue;
+    }
     s1.begin_item(F.data() + f.off, f.sh.cols, f.sh.rows, f.sh.cols, 0);
     if (left) s1.add(Eb.ptr(f.a), eb.cols, ket.ptr(f.kp), kb.cols, eb.cols, 0, 1.0);
     else s1.add(ket.ptr(f.kp), kb.cols, Eb.ptr(f.a), eb.cols, kb.cols, 0, 1.0);
@@ -1465,10 +1546,22 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     const Block& eb = E.ket.blk[t.pos];
     if (left) {  // cols [at, at+e1) of the block: first (Di x Dj) * rnd (Dj x e1)
       if ((int64_t)f.sh.rows * t.e1 == 0) continue;
+      if (cs == 2) {
+        zgemm_item(s2, E.ket.ptr(t.pos) + cs * t.at, eb.cols, f.sh.rows, t.e1, 0, 0,
+                   {ZTerm{F.data() + cs * f.off, f.sh.cols, Rd.data() + t.roff, t.e1, f.sh.cols, 0, false,
+                          false, noise_, 0.0}});
+        continue;
+      }
       s2.begin_item(E.ket.ptr(t.pos) + t.at, eb.cols, f.sh.rows, t.e1, 0);
       s2.add(F.data() + f.off, f.sh.cols, Rd.data() + t.roff, t.e1, f.sh.cols, 0, noise_);
     } else {     // rows [at, at+e0): rnd (e0 x Dj) * first (Dj x Di)
       if ((int64_t)t.e0 * f.sh.cols == 0) continue;
+      if (cs == 2) {
+        zgemm_item(s2, E.ket.ptr(t.pos) + cs * t.at * eb.cols, eb.cols, t.e0, f.sh.cols, 0, 0,
+                   {ZTerm{Rd.data() + t.roff, t.e1, F.data() + cs * f.off, f.sh.cols, t.e1, 0, false, false,
+                          noise_, 0.0}});
+        continue;
+      }
       s2.begin_item(E.ket.ptr(t.pos) + t.at * eb.cols, eb.cols, t.e0, f.sh.cols, 0);
       s2.add(Rd.data() + t.roff, t.e1, F.data() + f.off, f.sh.cols, t.e1, 0, noise_);
     }
@@ -1485,13 +1578,13 @@ namespace {
 // Assemble a new ket from pieces (expansion / match fills).
 void assemble(BlockTensor& nk, const std::vector<std::vector<Piece>>& pieces, Context& cx) {
   nk.allocate();
-  fill(nk.data.data(), nk.size, 0.0, cx.stream);
+  fill(nk.data.data(), nk.doubles(), 0.0, cx.stream);
   std::vector<CopyTask> cps;
   for (int p = 0; p < nk.n(); ++p)
     for (const Piece& pc : pieces[p]) {
       if (!pc.src || (int64_t)pc.rows * pc.cols == 0) continue;
-      cps.push_back(CopyTask{pc.src, nk.ptr(p) + (int64_t)pc.r0 * nk.blk[p].cols + pc.c0, pc.rows, pc.cols,
-                             pc.ssr, 1, nk.blk[p].cols, 1, 1.0});
+      push_copy(cps, CopyTask{pc.src, nk.ptr(p) + nk.cs * ((int64_t)pc.r0 * nk.blk[p].cols + pc.c0), pc.rows,
+                              pc.cols, pc.ssr, 1, nk.blk[p].cols, 1, 1.0}, nk.cs);
     }
   launch_copies(cps, cx.copy_scratch, cx.stream);
 }
@@ -1503,6 +1596,7 @@ void assemble(BlockTensor& nk, const std::vector<std::vector<Piece>>& pieces, Co
 void fill_expanded(Lattice& lat, const Lattice& origin, const Lattice* E, bool left_fill, Context& cx) {
   BlockTensor nk;
   nk.blk = lat.ket.blk;
+  nk.cs = origin.ket.cs;
   std::vector<std::vector<Piece>> pieces(nk.n());
   for (int p = 0; p < nk.n(); ++p) {
     const int l = nk.blk[p].l, r = nk.blk[p].r;
@@ -1594,7 +1688,7 @@ void RealTensorRundmrg::expand(int leigh, int site, Lattice& E) {
   std::vector<int> li, ri, pi;
   tensor_index(site, lat, li, ri, pi);
   for (size_t k = 0; k < li.size(); ++k) lat.phys[li[k]][ri[k]] = pi[k];
-  cx_.rng.skip((int64_t)li.size());  // DefineTensorLattice(..., 1, 1)
+  cx_.rng.skip((int64_t)cs_ * li.size());  // DefineTensorLattice(..., 1, 1)
   lat.ket.define(li, ri);
   fill_expanded(lat, origin, &E, leigh == 0, cx_);
 }
@@ -1620,7 +1714,7 @@ void RealTensorRundmrg::match(int leigh, int site) {
   std::vector<int> li, ri, pi;
   tensor_index(site, lat, li, ri, pi);
   for (size_t k = 0; k < li.size(); ++k) lat.phys[li[k]][ri[k]] = pi[k];
-  cx_.rng.skip((int64_t)li.size());
+  cx_.rng.skip((int64_t)cs_ * li.size());
   lat.ket.define(li, ri);
   if (leigh == 0 && site > 0) fill_expanded(lat, origin, nullptr, false, cx_);
   else if (leigh == 1 && site < L_sites_ - 1) fill_expanded(lat, origin, nullptr, true, cx_);
@@ -1636,7 +1730,8 @@ void RealTensorRundmrg::trace(int h, int j, double e, const char* what) {
   for (int v : lat.rd) rd += v;
   double nrm = 0.0;
   if (lat.ket.size > 0) {
-    dot(lat.ket.data.data(), lat.ket.data.data(), lat.ket.size, cx_.scal.data() + 30, cx_.partial, cx_.stream);
+    dot(lat.ket.data.data(), lat.ket.data.data(), lat.ket.doubles(), cx_.scal.data() + 30, cx_.partial,
+        cx_.stream);
     DMRG_HIP(hipMemcpyAsync(&nrm, cx_.scal.data() + 30, sizeof(double), hipMemcpyDeviceToHost, cx_.stream));
     cx_.sync();
   }

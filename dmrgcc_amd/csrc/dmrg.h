@@ -118,7 +118,8 @@ class HeffPlan {
 // RightComputeTensorContraction :959-1119).
 Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx);
 Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx);
-Env boundary_env(int nbond, Context& cx);
+// cs = 2: complex (1 + 0i) boundary blocks
+Env boundary_env(int nbond, Context& cx, int cs = 1);
 
 // Dense symmetric eigen-decomposition (cyclic Jacobi), ascending eigenvalues,
 // eigenvectors in columns of the row-major n x n `vec`.  Used for the Lanczos
@@ -172,6 +173,9 @@ struct RunResult {
 };
 
 // RealTensorRundmrg: owns the MPS, the environments and drives the sweeps.
+// On a complex Hamiltonian it is ComplexTensorRundmrg
+// (src/dmrg/complexdmrg/complex_tensor_rundmrg.cc): complex MPS and
+// environments, complex random draws, complex SVD and expansion padding.
 class RealTensorRundmrg {
  public:
   RealTensorRundmrg(const RealTensorSpace& space, const RealTensorHamiltonian& ham,
@@ -186,6 +190,7 @@ class RealTensorRundmrg {
   bool Advance(int stop_h, int stop_j);
   RunResult Result();
   int num_site() const { return L_sites_; }
+  bool complex() const { return cs_ == 2; }
   void set_precision(int precision, int refine) { p_.precision = precision; p_.refine = refine; }
   // H_eff plan on the current state of `site` (valid when the site is next).
   void BuildHeff(int site, HeffPlan& plan) {
@@ -233,6 +238,7 @@ class RealTensorRundmrg {
   RunParams p_;
   Context cx_;
   int L_sites_ = 0, d_ = 0;
+  int cs_ = 1;  // 2: complex MPS and environments (ComplexTensorRundmrg)
   std::vector<QNTable> tables_;
   std::vector<Lattice> lat_;
   std::vector<Env> L_, R_;

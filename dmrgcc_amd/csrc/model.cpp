@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 
 #include "common.h"
 #include "quicksort.h"
@@ -28,22 +29,42 @@ bool SiteOperator::mixed_nonzero(int o1, int p2) const {
   return false;
 }
 
-// isParallelElement (src/util/general.h:146-180) applied along a whole
-// column: returns true (and the prefactor) when `col` is parallel to `ref`.
-static bool parallel_to(const double* col, const double* ref, int n, int stride,
-                        double& prefactor) {
+namespace {
+
+using zd = std::complex<double>;
+
+// Norm of an entry (complex: Norm, src/tensor/complex.h:112-115) and the
+// quotient of isParallelElement (complex: operator/ of complex.h, written
+// out so that the prefactors, which enter the merged MPO, are the same bits).
+inline double mag(double x) { return std::fabs(x); }
+inline double mag(const zd& z) { return std::sqrt(z.real() * z.real() + z.imag() * z.imag()); }
+inline double quot(double a, double b) { return a / b; }
+inline zd quot(const zd& a, const zd& b) {
+  const double k = 1.0 / (b.real() * b.real() + b.imag() * b.imag());
+  return zd((a.real() * b.real() + a.imag() * b.imag()) * k, (a.imag() * b.real() - a.real() * b.imag()) * k);
+}
+inline zd cmul(const zd& a, const zd& b) {
+  return zd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+inline double cmul(double a, double b) { return a * b; }
+
+// isParallelElement (src/util/general.h:146-180, complex :182-207) applied
+// along a whole column: returns true (and the prefactor) when `col` is
+// parallel to `ref`.
+template <typename T>
+bool parallel_to(const T* col, const T* ref, int n, int stride, T& prefactor) {
   bool zero = true;
-  prefactor = 0.0;
+  prefactor = T(0);
   for (int i = 0; i < n; ++i) {
-    const double e0 = col[(size_t)i * stride], e1 = ref[(size_t)i * stride];
-    if (std::fabs(e1) <= 1e-10) {
-      if (std::fabs(e0) > 1e-10) return false;
+    const T e0 = col[(size_t)i * stride], e1 = ref[(size_t)i * stride];
+    if (mag(e1) <= 1e-10) {
+      if (mag(e0) > 1e-10) return false;
     } else {
-      const double f = e0 / e1;
+      const T f = quot(e0, e1);
       if (zero) {
         prefactor = f;
         zero = false;
-      } else if (std::fabs(f - prefactor) > 1e-8) {
+      } else if (mag(f - prefactor) > 1e-8) {
         return false;
       }
     }
@@ -51,35 +72,149 @@ static bool parallel_to(const double* col, const double* ref, int n, int stride,
   return true;
 }
 
-std::vector<int> parallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
-                                 std::vector<double>& transfer) {
+template <typename T>
+std::vector<int> parallel_t(const std::vector<T>& a, int rows, int cols, int leigh, std::vector<T>& transfer) {
   // Work on "vectors": columns (leigh 0) or rows (leigh 1).
   const int nv = leigh == 0 ? cols : rows;
   const int len = leigh == 0 ? rows : cols;
   const int stride = leigh == 0 ? cols : 1;
   auto vec = [&](int v) { return leigh == 0 ? &a[v] : &a[(size_t)v * cols]; };
   std::vector<int> pos{0};
-  transfer.assign((size_t)nv * nv, 0.0);
-  auto T = [&](int k, int v) -> double& {
+  transfer.assign((size_t)nv * nv, T(0));
+  auto T_ = [&](int k, int v) -> T& {
     // leigh 0: transfer(k, v); leigh 1: transfer(v, k)
     return leigh == 0 ? transfer[(size_t)k * nv + v] : transfer[(size_t)v * nv + k];
   };
-  T(0, 0) = 1.0;
+  T_(0, 0) = T(1);
   for (int v = 0; v < nv; ++v) {
     bool found = false;
     for (size_t k = 0; k < pos.size(); ++k) {
-      double pref;
+      T pref;
       if (parallel_to(vec(v), vec(pos[k]), len, stride, pref)) {
-        T((int)k, v) = pref;
+        T_((int)k, v) = pref;
         found = true;
         break;
       }
     }
     if (!found) {
       pos.push_back(v);
-      T((int)pos.size() - 1, v) = 1.0;
+      T_((int)pos.size() - 1, v) = T(1);
     }
   }
+  return pos;
+}
+
+std::vector<zd> to_z(const std::vector<double>& a) {
+  std::vector<zd> z(a.size() / 2);
+  for (size_t i = 0; i < z.size(); ++i) z[i] = zd(a[2 * i], a[2 * i + 1]);
+  return z;
+}
+std::vector<double> from_z(const std::vector<zd>& z) {
+  std::vector<double> a(2 * z.size());
+  for (size_t i = 0; i < z.size(); ++i) { a[2 * i] = z[i].real(); a[2 * i + 1] = z[i].imag(); }
+  return a;
+}
+
+// Site tensor entries as T (complex: w + i wi).
+template <typename T> std::vector<T> load(const SiteOperator& W);
+template <> std::vector<double> load<double>(const SiteOperator& W) { return W.w; }
+template <> std::vector<zd> load<zd>(const SiteOperator& W) {
+  std::vector<zd> z(W.w.size());
+  for (size_t i = 0; i < z.size(); ++i) z[i] = zd(W.w[i], W.wi[i]);
+  return z;
+}
+void store(SiteOperator& W, const std::vector<double>& v) { W.w = v; W.wi.clear(); }
+void store(SiteOperator& W, const std::vector<zd>& v) {
+  W.w.resize(v.size());
+  W.wi.resize(v.size());
+  for (size_t i = 0; i < v.size(); ++i) { W.w[i] = v[i].real(); W.wi[i] = v[i].imag(); }
+}
+
+// ParallelTensorHamiltonian on entries of type T.
+template <typename T>
+void parallel_sweep(std::vector<SiteOperator>& Ws, std::vector<QNTable>& qt, int num_site, int d) {
+  const int dd = d * d;
+  for (int i = 0; i < num_site - 1; ++i) {
+    // LeftParallelTensorOperator (real_tensor_operator.cc:199-236): rows (l,p1,p2)
+    SiteOperator& W = Ws[i];
+    const std::vector<T> w = load<T>(W);
+    const int rows = W.lb * dd, cols = W.rb;
+    std::vector<T> resh((size_t)rows * cols);
+    for (int l = 0; l < W.lb; ++l)
+      for (int r = 0; r < W.rb; ++r)
+        for (int p = 0; p < dd; ++p) resh[((size_t)l * dd + p) * cols + r] = w[((size_t)l * W.rb + r) * dd + p];
+    std::vector<T> Tm;
+    std::vector<int> pos = parallel_t(resh, rows, cols, 0, Tm);
+    const int np = (int)pos.size();
+    std::vector<T> nw((size_t)W.lb * np * dd, T(0));
+    for (int l = 0; l < W.lb; ++l)
+      for (int r = 0; r < np; ++r)
+        for (int p = 0; p < dd; ++p) nw[((size_t)l * np + r) * dd + p] = resh[((size_t)l * dd + p) * cols + pos[r]];
+    W.rb = np;
+    store(W, nw);
+    // LeftMergeTensorOperator (real_tensor_operator.cc:277-318): W'[l] = sum_m T[l][m] W[m]
+    SiteOperator& X = Ws[i + 1];
+    require(cols == X.lb, "merge dimension is not match in LeftMergeTensorOperator");
+    const std::vector<T> x = load<T>(X);
+    std::vector<T> m((size_t)np * X.rb * dd, T(0));
+    for (int l = 0; l < np; ++l)
+      for (int r = 0; r < X.rb; ++r)
+        for (int p = 0; p < dd; ++p) {
+          T e = T(0);
+          for (int k = 0; k < cols; ++k) e += cmul(Tm[(size_t)l * cols + k], x[((size_t)k * X.rb + r) * dd + p]);
+          m[((size_t)l * X.rb + r) * dd + p] = e;
+        }
+    X.lb = np;
+    store(X, m);
+    QNTable q;
+    for (int p : pos) q.push_back(qt[i + 1][p]);
+    qt[i + 1] = q;
+  }
+  for (int i = num_site - 1; i > 0; --i) {
+    // RightParallelTensorOperator (real_tensor_operator.cc:238-275): columns (r,p1,p2)
+    SiteOperator& W = Ws[i];
+    const int rows = W.lb, cols = W.rb * dd;
+    const std::vector<T> resh = load<T>(W);  // (l, r, p1, p2) row-major == rows x cols
+    std::vector<T> Tm;
+    std::vector<int> pos = parallel_t(resh, rows, cols, 1, Tm);
+    const int np = (int)pos.size();
+    std::vector<T> nw((size_t)np * W.rb * dd, T(0));
+    for (int l = 0; l < np; ++l)
+      for (int c = 0; c < cols; ++c) nw[(size_t)l * cols + c] = resh[(size_t)pos[l] * cols + c];
+    W.lb = np;
+    store(W, nw);
+    // RightMergeTensorOperator (real_tensor_operator.cc:320-361): W'[:, r] = sum_m T[m][r] W[:, m]
+    SiteOperator& X = Ws[i - 1];
+    require(rows == X.rb, "merge dimension is not match in RightMergeTensorOperator");
+    const std::vector<T> x = load<T>(X);
+    std::vector<T> m((size_t)X.lb * np * dd, T(0));
+    for (int l = 0; l < X.lb; ++l)
+      for (int r = 0; r < np; ++r)
+        for (int p = 0; p < dd; ++p) {
+          T e = T(0);
+          for (int k = 0; k < rows; ++k) e += cmul(Tm[(size_t)k * rows + r], x[((size_t)l * X.rb + k) * dd + p]);
+          m[((size_t)l * np + r) * dd + p] = e;
+        }
+    X.rb = np;
+    store(X, m);
+    QNTable q;
+    for (int p : pos) q.push_back(qt[i][p]);
+    qt[i] = q;
+  }
+}
+
+}  // namespace
+
+std::vector<int> parallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
+                                 std::vector<double>& transfer) {
+  return parallel_t(a, rows, cols, leigh, transfer);
+}
+
+std::vector<int> zparallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
+                                  std::vector<double>& transfer) {
+  std::vector<zd> t;
+  std::vector<int> pos = parallel_t(to_z(a), rows, cols, leigh, t);
+  transfer = from_z(t);
   return pos;
 }
 
@@ -96,14 +231,15 @@ void RealTensorHamiltonian::Init(int num_site, int physics_dim, int num_quantum,
   qt_[num_site] = QNTable{QN(nq_, 0)};
 }
 
-void RealTensorHamiltonian::ExpanTensorHamiltonian(int site, int op_index, double coefficient,
+void RealTensorHamiltonian::ExpanTensorHamiltonian(int site, int op_index, double cr, double ci,
                                                    const QN& table) {
-  // RealTensorOperator::ExpanTensorOperator (real_tensor_operator.cc:141-197)
+  // RealTensorOperator::ExpanTensorOperator (real_tensor_operator.cc:141-197;
+  // complex: complex_tensor_operator.cc:141-197, coefficient * basic)
   int leigh = -1;
   if (site == 0) leigh = 0;
   if (site == num_site_ - 1) leigh = 1;
   SiteOperator& W = W_[site];
-  const int d = d_;
+  const int d = d_, dd = d_ * d_;
   SiteOperator n;
   n.d = d;
   int e0 = 1, e1 = 1;
@@ -116,87 +252,28 @@ void RealTensorHamiltonian::ExpanTensorHamiltonian(int site, int op_index, doubl
     n.lb = W.lb + e0;
     n.rb = W.rb + e1;
   }
-  n.w.assign((size_t)n.lb * n.rb * d * d, 0.0);
+  n.w.assign((size_t)n.lb * n.rb * dd, 0.0);
+  if (cplx_) n.wi.assign(n.w.size(), 0.0);
   for (int l = 0; l < W.lb; ++l)
     for (int r = 0; r < W.rb; ++r)
-      for (int p = 0; p < d * d; ++p) n.w[((size_t)l * n.rb + r) * d * d + p] = W.w[((size_t)l * W.rb + r) * d * d + p];
+      for (int p = 0; p < dd; ++p) {
+        const size_t to = ((size_t)l * n.rb + r) * dd + p, from = ((size_t)l * W.rb + r) * dd + p;
+        n.w[to] = W.w[from];
+        if (cplx_) n.wi[to] = W.wi[from];
+      }
   const int il = e0 * W.lb, ir = e1 * W.rb;
-  for (int p = 0; p < d * d; ++p)
-    n.w[((size_t)il * n.rb + ir) * d * d + p] = coefficient * basic_[op_index][p];
+  for (int p = 0; p < dd; ++p) {
+    const size_t to = ((size_t)il * n.rb + ir) * dd + p;
+    n.w[to] = cr * basic_[op_index][p];
+    if (cplx_) n.wi[to] = ci * basic_[op_index][p];
+  }
   W = std::move(n);
   if (site < num_site_ - 1) qt_[site + 1].push_back(table);
 }
 
 void RealTensorHamiltonian::ParallelTensorHamiltonian() {
-  const int d = d_, dd = d_ * d_;
-  for (int i = 0; i < num_site_ - 1; ++i) {
-    // LeftParallelTensorOperator (real_tensor_operator.cc:199-236): rows (l,p1,p2)
-    SiteOperator& W = W_[i];
-    const int rows = W.lb * dd, cols = W.rb;
-    std::vector<double> resh((size_t)rows * cols);
-    for (int l = 0; l < W.lb; ++l)
-      for (int r = 0; r < W.rb; ++r)
-        for (int p = 0; p < dd; ++p) resh[((size_t)l * dd + p) * cols + r] = W.w[((size_t)l * W.rb + r) * dd + p];
-    std::vector<double> T;
-    std::vector<int> pos = parallel_matrix(resh, rows, cols, 0, T);
-    const int np = (int)pos.size();
-    SiteOperator n;
-    n.d = d; n.lb = W.lb; n.rb = np;
-    n.w.assign((size_t)n.lb * np * dd, 0.0);
-    for (int l = 0; l < W.lb; ++l)
-      for (int r = 0; r < np; ++r)
-        for (int p = 0; p < dd; ++p) n.w[((size_t)l * np + r) * dd + p] = resh[((size_t)l * dd + p) * cols + pos[r]];
-    W = std::move(n);
-    // LeftMergeTensorOperator (real_tensor_operator.cc:277-318): W'[l] = sum_m T[l][m] W[m]
-    SiteOperator& X = W_[i + 1];
-    require(cols == X.lb, "merge dimension is not match in LeftMergeTensorOperator");
-    SiteOperator m;
-    m.d = d; m.lb = np; m.rb = X.rb;
-    m.w.assign((size_t)np * X.rb * dd, 0.0);
-    for (int l = 0; l < np; ++l)
-      for (int r = 0; r < X.rb; ++r)
-        for (int p = 0; p < dd; ++p) {
-          double e = 0.0;
-          for (int k = 0; k < cols; ++k) e += T[(size_t)l * cols + k] * X.w[((size_t)k * X.rb + r) * dd + p];
-          m.w[((size_t)l * X.rb + r) * dd + p] = e;
-        }
-    X = std::move(m);
-    QNTable q;
-    for (int p : pos) q.push_back(qt_[i + 1][p]);
-    qt_[i + 1] = q;
-  }
-  for (int i = num_site_ - 1; i > 0; --i) {
-    // RightParallelTensorOperator (real_tensor_operator.cc:238-275): columns (r,p1,p2)
-    SiteOperator& W = W_[i];
-    const int rows = W.lb, cols = W.rb * dd;
-    std::vector<double> resh(W.w);  // (l, r, p1, p2) row-major == rows x cols
-    std::vector<double> T;
-    std::vector<int> pos = parallel_matrix(resh, rows, cols, 1, T);
-    const int np = (int)pos.size();
-    SiteOperator n;
-    n.d = d; n.lb = np; n.rb = W.rb;
-    n.w.assign((size_t)np * W.rb * dd, 0.0);
-    for (int l = 0; l < np; ++l)
-      for (int c = 0; c < cols; ++c) n.w[(size_t)l * cols + c] = resh[(size_t)pos[l] * cols + c];
-    W = std::move(n);
-    // RightMergeTensorOperator (real_tensor_operator.cc:320-361): W'[:, r] = sum_m T[m][r] W[:, m]
-    SiteOperator& X = W_[i - 1];
-    require(rows == X.rb, "merge dimension is not match in RightMergeTensorOperator");
-    SiteOperator m;
-    m.d = d; m.lb = X.lb; m.rb = np;
-    m.w.assign((size_t)X.lb * np * dd, 0.0);
-    for (int l = 0; l < X.lb; ++l)
-      for (int r = 0; r < np; ++r)
-        for (int p = 0; p < dd; ++p) {
-          double e = 0.0;
-          for (int k = 0; k < rows; ++k) e += T[(size_t)k * rows + r] * X.w[((size_t)l * X.rb + k) * dd + p];
-          m.w[((size_t)l * np + r) * dd + p] = e;
-        }
-    X = std::move(m);
-    QNTable q;
-    for (int p : pos) q.push_back(qt_[i][p]);
-    qt_[i] = q;
-  }
+  if (cplx_) parallel_sweep<zd>(W_, qt_, num_site_, d_);
+  else parallel_sweep<double>(W_, qt_, num_site_, d_);
 }
 
 std::vector<QNTable> RealSpinlessBoseSquareSpace::DefineQuantumTable() const {
@@ -300,6 +377,165 @@ void RealSpinlessBoseSquareHamiltonian::DefineTensorHamiltonian() {
         else if (ny_ > 2 && py_) { Hop1(p1, p0, hop_y_); Hop2(p1, p0, hop_y_); }
       }
       if (mu_ != 0 || U_ != 0) OnSite(p0, 1.0);
+      ParallelTensorHamiltonian();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex spinful Bose-Hubbard
+// ---------------------------------------------------------------------------
+std::vector<QNTable> ComplexSpinfulBoseSquareSpace::DefineQuantumTable() const {
+  // complex_spinful_bose_square_space.cc:28-96: species 0 (up) sits on even
+  // MPS sites, so bond i has seen (i+1)/2 up and i/2 down sites; the table is
+  // the rectangle of reachable (N_up, N_down), N_up fastest, ordered by
+  // ReorderQuantumTable (complex_tensor_space.cc:719-741).
+  const int L = num_site(), half = L / 2;
+  std::vector<QNTable> tabs;
+  for (int i = 0; i <= L; ++i) {
+    int lo[2] = {0, 0}, hi[2] = {0, 0};
+    double mean[2] = {0.0, 0.0};
+    if (i > 0) {
+      const int pos[2] = {(i + 1) / 2, i / 2};
+      for (int t = 0; t < 2; ++t) {
+        lo[t] = std::max(0, n_[t] - (half - pos[t]) * (d_ - 1));
+        hi[t] = std::min((d_ - 1) * pos[t], n_[t]);
+        mean[t] = (double)n_[t] / half * pos[t];
+      }
+    }
+    const int n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1;
+    require(n0 > 0 && n1 > 0, "empty quantum table (particle numbers out of range)", kErrArgument);
+    QNTable t((size_t)n0 * n1);
+    for (int t0 = lo[0]; t0 <= hi[0]; ++t0)
+      for (int t1 = lo[1]; t1 <= hi[1]; ++t1) t[(t0 - lo[0]) + (size_t)(t1 - lo[1]) * n0] = QN{t0, t1};
+    std::vector<double> dev;
+    std::vector<int> idx;
+    for (const QN& q : t) {
+      dev.push_back(std::pow(q[0] - mean[0], 2) + std::pow(q[1] - mean[1], 2));
+      idx.push_back((int)idx.size());
+    }
+    quicksort(dev, &idx, 1);
+    QNTable r;
+    for (int k : idx) r.push_back(t[k]);
+    tabs.push_back(r);
+  }
+  return tabs;
+}
+
+int ComplexSpinfulBoseSquareSpace::CheckQuantumTable(int site, const QN& left, const QN& right) const {
+  // complex_spinful_bose_square_space.cc:105-125: site parity picks the species
+  const int shift = site % 2;
+  for (int i = 0; i < d_; ++i) {
+    const int a0 = shift == 0 ? i : 0, a1 = shift == 1 ? i : 0;
+    if (left[0] + a0 == right[0] && left[1] + a1 == right[1]) return i;
+  }
+  return -1;
+}
+
+ComplexSpinfulBoseSquareHamiltonian::ComplexSpinfulBoseSquareHamiltonian(
+    int nx, int ny, int d, const double flux[2], const double hop_x[2], const double hop_y[2],
+    const double inter[3], const double mu[2], bool period_x, bool period_y)
+    : nx_(nx), ny_(ny), px_(period_x), py_(period_y) {
+  for (int t = 0; t < 2; ++t) {
+    flux_[t] = flux[t];
+    hop_x_[t] = hop_x[t];
+    hop_y_[t] = hop_y[t];
+    mu_[t] = mu[t];
+  }
+  for (int t = 0; t < 3; ++t) inter_[t] = inter[t];
+  // DefineBasicTensor (complex_spinful_bose_square_hamiltonian.cc:45-70):
+  // 0=I, 1=a^+, 2=a^-, 3=n, 4=mu n + U_uu/2 n(n-1), 5=mu n + U_dd/2 n(n-1).
+  // Both on-site operators take the species-0 chemical potential, as the
+  // reference does (:61).
+  std::vector<std::vector<double>> ops(6, std::vector<double>((size_t)d * d, 0.0));
+  for (int i = 0; i < d; ++i) {
+    ops[0][i * d + i] = 1.0;
+    ops[3][i * d + i] = i;
+    ops[4][i * d + i] = mu_[0] * i + 0.5 * inter_[0] * i * (i - 1);
+    ops[5][i * d + i] = mu_[0] * i + 0.5 * inter_[1] * i * (i - 1);
+  }
+  for (int i = 1; i < d; ++i) {
+    ops[1][i * d + (i - 1)] = std::sqrt((double)i);
+    ops[2][(i - 1) * d + i] = std::sqrt((double)i);
+  }
+  cplx_ = true;
+  Init(2 * nx * ny, d, 2, ops);
+}
+
+void ComplexSpinfulBoseSquareHamiltonian::Hop1(int p1, int p2, double cr, double ci) {
+  // DefineHamiltonian1 (:73-126): a^+_{p1} a^-_{p2}, channel QN +1 of the species of p1
+  require(p1 < p2, "position1 >= position2 in DefineHamiltonian1", kErrArgument);
+  const QN q = p1 % 2 == 0 ? QN{1, 0} : QN{0, 1};
+  for (int i = 0; i < num_site_; ++i) {
+    if (i == p1) ExpanTensorHamiltonian(i, 1, 1.0, 0.0, q);
+    else if (i > p1 && i < p2) ExpanTensorHamiltonian(i, 0, 1.0, 0.0, q);
+    else if (i == p2) ExpanTensorHamiltonian(i, 2, cr, ci, QN{0, 0});
+    else ExpanTensorHamiltonian(i, 0, 1.0, 0.0, QN{0, 0});
+  }
+}
+
+void ComplexSpinfulBoseSquareHamiltonian::Hop2(int p1, int p2, double cr, double ci) {
+  // DefineHamiltonian2 (:129-181): a^-_{p1} a^+_{p2}, channel QN -1
+  require(p1 < p2, "position1 >= position2 in DefineHamiltonian2", kErrArgument);
+  const QN q = p1 % 2 == 0 ? QN{-1, 0} : QN{0, -1};
+  for (int i = 0; i < num_site_; ++i) {
+    if (i == p1) ExpanTensorHamiltonian(i, 2, 1.0, 0.0, q);
+    else if (i > p1 && i < p2) ExpanTensorHamiltonian(i, 0, 1.0, 0.0, q);
+    else if (i == p2) ExpanTensorHamiltonian(i, 1, cr, ci, QN{0, 0});
+    else ExpanTensorHamiltonian(i, 0, 1.0, 0.0, QN{0, 0});
+  }
+}
+
+void ComplexSpinfulBoseSquareHamiltonian::OnSite(int p, double c) {
+  // DefineHamiltonian3 (:184-210): operator 4 on up sites, 5 on down sites
+  for (int i = 0; i < num_site_; ++i) {
+    if (i == p) ExpanTensorHamiltonian(i, p % 2 == 0 ? 4 : 5, c, 0.0, QN{0, 0});
+    else ExpanTensorHamiltonian(i, 0, 1.0, 0.0, QN{0, 0});
+  }
+}
+
+void ComplexSpinfulBoseSquareHamiltonian::Inter(int p1, int p2, double c) {
+  // DefineHamiltonian4 (:213-240): n_{p1} n_{p2} c
+  for (int i = 0; i < num_site_; ++i) {
+    if (i == p1) ExpanTensorHamiltonian(i, 3, 1.0, 0.0, QN{0, 0});
+    else if (i == p2) ExpanTensorHamiltonian(i, 3, c, 0.0, QN{0, 0});
+    else ExpanTensorHamiltonian(i, 0, 1.0, 0.0, QN{0, 0});
+  }
+}
+
+void ComplexSpinfulBoseSquareHamiltonian::DefineTensorHamiltonian() {
+  // complex_spinful_bose_square_hamiltonian.cc:242-350; physical site
+  // p = j + i*ny, MPS sites 2p (up) and 2p+1 (down).  y bonds carry
+  // phase = exp(i 2 pi (i+1) flux): a^+_p a_q conj(phase) + a_p a^+_q phase.
+  for (int i = 0; i < nx_; ++i)
+    for (int j = 0; j < ny_; ++j) {
+      const int p0 = j + i * ny_;
+      for (int t = 0; t < 2; ++t) {
+        if (hop_x_[t] == 0.0) continue;
+        const int p1 = j + ((i + 1) % nx_) * ny_;
+        if (i + 1 < nx_) {
+          Hop1(2 * p0 + t, 2 * p1 + t, hop_x_[t], 0.0);
+          Hop2(2 * p0 + t, 2 * p1 + t, hop_x_[t], 0.0);
+        } else if (nx_ > 2 && px_) {
+          Hop1(2 * p1 + t, 2 * p0 + t, hop_x_[t], 0.0);
+          Hop2(2 * p1 + t, 2 * p0 + t, hop_x_[t], 0.0);
+        }
+      }
+      for (int t = 0; t < 2; ++t) {
+        if (hop_y_[t] == 0.0) continue;
+        const int p1 = (j + 1) % ny_ + i * ny_;
+        const double theta = 2 * M_PI * (i + 1) * flux_[t];
+        const double c = std::cos(theta), s = std::sin(theta);
+        if (j + 1 < ny_) {
+          Hop1(2 * p0 + t, 2 * p1 + t, c * hop_y_[t], -s * hop_y_[t]);
+          Hop2(2 * p0 + t, 2 * p1 + t, c * hop_y_[t], s * hop_y_[t]);
+        } else if (ny_ > 2 && py_) {
+          Hop2(2 * p1 + t, 2 * p0 + t, c * hop_y_[t], -s * hop_y_[t]);
+          Hop1(2 * p1 + t, 2 * p0 + t, c * hop_y_[t], s * hop_y_[t]);
+        }
+      }
+      if (mu_[0] != 0.0 || inter_[0] != 0.0) OnSite(2 * p0, 1.0);
+      if (mu_[1] != 0.0 || inter_[1] != 0.0) OnSite(2 * p0 + 1, 1.0);
+      if (inter_[2] != 0.0) Inter(2 * p0, 2 * p0 + 1, inter_[2]);
       ParallelTensorHamiltonian();
     }
 }

@@ -2,9 +2,13 @@
 // incremental MPO builder, mirroring the reference's abstract
 // RealTensorSpace / RealTensorHamiltonian interface
 // (src/dmrg/realdmrg/real_tensor_space.h:137-150,
-//  src/dmrg/realdmrg/real_tensor_hamiltonian.h:6-78) and its spinless
-// Bose-Hubbard subclasses (model/space/real_spinless_bose_square_space.*,
-// model/hamiltonian/real_spinless_bose_square_hamiltonian.*).
+//  src/dmrg/realdmrg/real_tensor_hamiltonian.h:6-78) and its Bose-Hubbard
+// subclasses: real spinless (model/space/real_spinless_bose_square_space.*,
+// model/hamiltonian/real_spinless_bose_square_hamiltonian.*), complex
+// spinless and complex spinful (model/*/complex_spin*_bose_square_*).  The
+// complex twins (ComplexTensorSpace / ComplexTensorHamiltonian) have the same
+// interface; a Hamiltonian built complex() stores the imaginary parts of its
+// MPO in SiteOperator::wi.
 #pragma once
 
 #include <cstddef>
@@ -69,16 +73,24 @@ class RealTensorHamiltonian {
   virtual void DefineTensorHamiltonian() = 0;
 
   int num_site() const { return num_site_; }
+  int num_quantum() const { return nq_; }
+  bool complex() const { return cplx_; }
   const std::vector<SiteOperator>& sites() const { return W_; }
   const std::vector<QNTable>& bond_tables() const { return qt_; }
 
  protected:
   void Init(int num_site, int physics_dim, int num_quantum, std::vector<std::vector<double>> basic);
-  // ExpanTensorHamiltonian (real_tensor_hamiltonian.cc:151-187)
-  void ExpanTensorHamiltonian(int site, int op_index, double coefficient, const QN& table);
-  // ParallelTensorHamiltonian (real_tensor_hamiltonian.cc:189-214)
+  // ExpanTensorHamiltonian (real_tensor_hamiltonian.cc:151-187; complex
+  // coefficient cr + i ci: complex_tensor_hamiltonian.cc:151-187)
+  void ExpanTensorHamiltonian(int site, int op_index, double coefficient, const QN& table) {
+    ExpanTensorHamiltonian(site, op_index, coefficient, 0.0, table);
+  }
+  void ExpanTensorHamiltonian(int site, int op_index, double cr, double ci, const QN& table);
+  // ParallelTensorHamiltonian (real_tensor_hamiltonian.cc:189-214,
+  // complex_tensor_hamiltonian.cc:189-214)
   void ParallelTensorHamiltonian();
 
+  bool cplx_ = false;  // set by complex models before DefineTensorHamiltonian
   int num_site_ = 0, d_ = 0, nq_ = 0;
   std::vector<std::vector<double>> basic_;  // d*d row-major operators
   std::vector<SiteOperator> W_;
@@ -90,6 +102,10 @@ class RealTensorHamiltonian {
 // positions; transfer is n x n row-major (n = cols or rows).
 std::vector<int> parallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
                                  std::vector<double>& transfer);
+// ComplexMatrix::ParallelMatrix (complex_matrix.cc:451-540) on interleaved
+// (re, im) entries; transfer is n x n interleaved complex.
+std::vector<int> zparallel_matrix(const std::vector<double>& a, int rows, int cols, int leigh,
+                                  std::vector<double>& transfer);
 
 class RealSpinlessBoseSquareSpace : public RealTensorSpace {
  public:
@@ -119,6 +135,68 @@ class RealSpinlessBoseSquareHamiltonian : public RealTensorHamiltonian {
   void OnSite(int p, double c);
   int nx_, ny_;
   double hop_x_, hop_y_, U_, mu_;
+  bool px_, py_;
+};
+
+// The complex spinless model is the real one on Complex storage
+// (model/space/complex_spinless_bose_square_space.cc and
+// model/hamiltonian/complex_spinless_bose_square_hamiltonian.cc are the real
+// files with Complex coefficients; the flux argument is stored and unused).
+using ComplexSpinlessBoseSquareSpace = RealSpinlessBoseSquareSpace;
+class ComplexSpinlessBoseSquareHamiltonian : public RealSpinlessBoseSquareHamiltonian {
+ public:
+  ComplexSpinlessBoseSquareHamiltonian(int nx, int ny, int physics_dim, double flux, double hop_x,
+                                       double hop_y, double U, double mu, bool period_x,
+                                       bool period_y)
+      : RealSpinlessBoseSquareHamiltonian(nx, ny, physics_dim, flux, hop_x, hop_y, U, mu, period_x,
+                                          period_y) {
+    cplx_ = true;
+  }
+};
+
+// Two-component bosons (model/space/complex_spinful_bose_square_space.cc):
+// physical site p becomes MPS sites 2p (up) and 2p+1 (down), each of local
+// dimension physics_dim; U(1) x U(1) quantum numbers (N_up, N_down).
+class ComplexSpinfulBoseSquareSpace : public RealTensorSpace {
+ public:
+  ComplexSpinfulBoseSquareSpace(int nx, int ny, int n_up, int n_down, int physics_dim)
+      : nx_(nx), ny_(ny), d_(physics_dim) {
+    n_[0] = n_up;
+    n_[1] = n_down;
+  }
+  int num_site() const override { return 2 * nx_ * ny_; }
+  int physics_dim() const override { return d_; }
+  int num_quantum() const override { return 2; }
+  std::vector<QNTable> DefineQuantumTable() const override;
+  QN MergeQuantumTable(const QN& op, const QN& space) const override {
+    return QN{op[0] + space[0], op[1] + space[1]};
+  }
+  int CheckQuantumTable(int site, const QN& left, const QN& right) const override;
+
+ private:
+  int nx_, ny_, d_;
+  int n_[2];
+};
+
+// model/hamiltonian/complex_spinful_bose_square_hamiltonian.cc: per species
+// t (0 = up, 1 = down) hopping hop_x[t], hop_y[t] with the Landau-gauge phase
+// exp(i 2 pi (x+1) flux[t]) on y bonds, on-site mu n + U_tt/2 n(n-1), and
+// the inter-species U_ud n_up n_down.  inter = {U_uu, U_dd, U_ud}.
+class ComplexSpinfulBoseSquareHamiltonian : public RealTensorHamiltonian {
+ public:
+  ComplexSpinfulBoseSquareHamiltonian(int nx, int ny, int physics_dim, const double flux[2],
+                                      const double hop_x[2], const double hop_y[2],
+                                      const double inter[3], const double mu[2], bool period_x,
+                                      bool period_y);
+  void DefineTensorHamiltonian() override;
+
+ private:
+  void Hop1(int p1, int p2, double cr, double ci);
+  void Hop2(int p1, int p2, double cr, double ci);
+  void OnSite(int p, double c);
+  void Inter(int p1, int p2, double c);
+  int nx_, ny_;
+  double flux_[2], hop_x_[2], hop_y_[2], inter_[3], mu_[2];
   bool px_, py_;
 };
 

@@ -44,6 +44,20 @@ class GlibcRand {
     for (int j = 0; j < cols; ++j)
       for (int i = 0; i < rows; ++i) out[(int64_t)i * cols + j] = 0.01 * (next() % 100);
   }
+  // ComplexMatrix::RandomMatrix (complex_matrix.cc:201-208): row-major
+  // interleaved (re, im), filled column-major, two draws per entry from
+  // Complex(0.01*(rand()%100), 0.01*(rand()%100)).  C++ leaves the order of
+  // the two calls unspecified; g++ on x86-64 evaluates the second argument
+  // first, so the first draw of a pair is the imaginary part.
+  void complex_matrix(int rows, int cols, double* out) {
+    for (int j = 0; j < cols; ++j)
+      for (int i = 0; i < rows; ++i) {
+        const double im = 0.01 * (next() % 100);
+        const double re = 0.01 * (next() % 100);
+        out[2 * ((int64_t)i * cols + j)] = re;
+        out[2 * ((int64_t)i * cols + j) + 1] = im;
+      }
+  }
   void skip(int64_t n) {
     for (int64_t i = 0; i < n; ++i) next();
   }

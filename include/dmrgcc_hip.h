@@ -64,13 +64,36 @@ typedef struct dmrgcc_model dmrgcc_model;
 int dmrgcc_model_spinless_create(int nx, int ny, int num_boson, int physics_dim, double hop_x,
                                  double hop_y, double U, double mu, int period_x, int period_y,
                                  dmrgcc_model** out);
+/* The complex twin (ComplexSpinlessBoseSquareSpace /
+ * ComplexSpinlessBoseSquareHamiltonian, model/space/complex_spinless_bose_square_space.cc,
+ * model/hamiltonian/complex_spinless_bose_square_hamiltonian.cc): the same
+ * MPO on complex storage; runs on it use the complex sweep engine
+ * (ComplexTensorRundmrg). */
+int dmrgcc_model_zspinless_create(int nx, int ny, int num_boson, int physics_dim, double hop_x,
+                                  double hop_y, double U, double mu, int period_x, int period_y,
+                                  dmrgcc_model** out);
+/* Complex two-component bosons (ComplexSpinfulBoseSquareSpace /
+ * ComplexSpinfulBoseSquareHamiltonian, model/space/complex_spinful_bose_square_space.cc:3-125,
+ * model/hamiltonian/complex_spinful_bose_square_hamiltonian.cc:3-350):
+ * L = 2*nx*ny MPS sites (up, down per lattice site), QNs (N_up, N_down).
+ * flux, hop_x, hop_y, mu: two entries (per species); inter: U_uu, U_dd, U_ud.
+ * y bonds carry the phase exp(i*2*pi*(x+1)*flux). */
+int dmrgcc_model_spinful_create(int nx, int ny, int n_up, int n_down, int physics_dim,
+                                const double* flux, const double* hop_x, const double* hop_y,
+                                const double* inter, const double* mu, int period_x, int period_y,
+                                dmrgcc_model** out);
 int dmrgcc_model_destroy(dmrgcc_model* m);
 int dmrgcc_model_num_site(const dmrgcc_model* m);
-/* MPS bond QN table `bond` (0..L): count, then the first QN of each entry. */
+/* Number of sites, local dimension, QNs per sector and whether the MPO is
+ * complex (any pointer may be NULL). */
+int dmrgcc_model_info(const dmrgcc_model* m, int* num_site, int* physics_dim, int* num_quantum,
+                      int* is_complex);
+/* MPS bond QN table `bond` (0..L): count, then num_quantum ints per entry. */
 int dmrgcc_model_space_table(const dmrgcc_model* m, int bond, int* count, int* qn);
-/* MPO site tensor: bond dims, then W (lb*rb*d*d doubles) if W != NULL. */
+/* MPO site tensor: bond dims, then W (lb*rb*d*d entries; doubles, or
+ * interleaved (re, im) pairs for a complex model) if W != NULL. */
 int dmrgcc_model_mpo_site(const dmrgcc_model* m, int site, int* lb, int* rb, double* W);
-/* MPO bond QN table (0..L): count, then the channel QNs. */
+/* MPO bond QN table (0..L): count, then num_quantum ints per channel. */
 int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn);
 
 /* ---- block GEMM (RealMatrix::MultiplyToMatrix, real_matrix.cc:291-315) -- */
@@ -196,8 +219,11 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
 int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N,
                        double* const* U, double* const* S, double* const* Vt, void* stream);
 
-/* ---- full sweep (RealTensorRundmrg::Run, real_tensor_rundmrg.cc:115-245) */
-/* Runs the spinless model from DefineTensorSpace through nhalf half sweeps.
+/* ---- full sweep (RealTensorRundmrg::Run, real_tensor_rundmrg.cc:115-245;
+ *      ComplexTensorRundmrg::Run, complex_tensor_rundmrg.cc:115-245) ---- */
+/* Runs the model from DefineTensorSpace through nhalf half sweeps; a complex
+ * model runs the complex engine (complex MPS, environments, SVD and random
+ * draws; complex_matrix.cc:201-208 draws two rand() per entry).
  * Per half sweep h: maxm[h], maxb[h], niter[h], canpre[h], noise[h] (the
  * table exponent; 0.1^noise is applied as in the reference).  Outputs per
  * half sweep (host, nhalf entries): energies, discarded weight, seconds.
@@ -231,7 +257,9 @@ int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
 /* Export the current blocks of `site` to host memory (tests, CPU baseline):
  * which 0 = ket, 1 = left environment, 2 = right environment.  With
- * o == NULL only *count and *total (elements) are written. */
+ * o == NULL only *count and *total (elements) are written.  Offsets count
+ * entries; on a complex model data receives 2 * total doubles (re, im)
+ * and the ket vector holds interleaved pairs. */
 int dmrgcc_dmrg_site_blocks(dmrgcc_dmrg* h, int site, int which, int* count, int64_t* total, int* o,
                             int* l, int* r, int* rows, int* cols, int64_t* off, double* data);
 /* Physical-index table of `site` (nl x nr, -1 where no block). */

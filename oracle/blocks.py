@@ -58,11 +58,17 @@ class BlockMat:
     def from_vector(self, v):
         acc = 0
         for i, m in enumerate(self.mats):
-            self.mats[i] = np.asarray(v[acc:acc + m.size], dtype=m.dtype).reshape(m.shape).copy()
+            dt = np.result_type(m.dtype, np.asarray(v).dtype)
+            self.mats[i] = np.asarray(v[acc:acc + m.size], dtype=dt).reshape(m.shape).copy()
             acc += m.size
 
     def normalize(self):
-        s = sum(float(np.sum(m * m)) for m in self.mats)
+        """NormalizeMatrixBlock (real_matrix_block.cc:164-171; complex:
+        sum of SquareNorm, complex_matrix_block.cc:163-170)."""
+        if any(np.iscomplexobj(m) for m in self.mats):
+            s = sum(float(np.sum(m.real * m.real + m.imag * m.imag)) for m in self.mats)
+        else:
+            s = sum(float(np.sum(m * m)) for m in self.mats)
         f = np.sqrt(1.0 / s)
         self.mats = [m * f for m in self.mats]
 
@@ -110,11 +116,12 @@ class Lattice:
         self.right_dim = list(right_dim)
         self.phys = [[-1] * len(right_block) for _ in left_block]
 
-    def define_blocks(self, li, ri, pi, rows, cols, rng):
+    def define_blocks(self, li, ri, pi, rows, cols, rng, cplx=False):
         """DefineTensorLattice (real_tensor_lattice.cc:343-359): every block
-        gets a fresh RandomMatrix(rows, cols) (consumes rand())."""
+        gets a fresh RandomMatrix(rows, cols) (consumes rand(); complex
+        blocks, complex_tensor_lattice.cc:343-359, two draws per entry)."""
         mats = []
         for k in range(len(li)):
             self.phys[li[k]][ri[k]] = pi[k]
-            mats.append(rng.random_matrix(rows, cols))
+            mats.append(rng.complex_matrix(rows, cols) if cplx else rng.random_matrix(rows, cols))
         self.ket = BlockMat(li, ri, mats)

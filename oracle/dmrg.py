@@ -1,4 +1,7 @@
-"""Single-site DMRG with subspace expansion, oracle copy (real FP64).
+"""Single-site DMRG with subspace expansion, oracle copy (real FP64 and
+complex128: the complex twins in src/dmrg/complexdmrg/ run the same steps on
+Complex entries, with the Hermitian conjugate bra in the environment updates,
+two rand() draws per random entry, and complex-specific expansion padding).
 
 Restates src/dmrg/realdmrg/:
 * real_tensor_contraction.cc  -- H_eff matvec (:1223-1291), dense H_eff
@@ -388,16 +391,22 @@ def truncate_dims(max_dim, canpre, svals):
 
 
 def svd(A):
-    """RealMatrix::SVDMatrix (real_matrix.cc:588-620): thin SVD; the right
-    factor is returned padded to n rows (rows >= min(m,n) are zero)."""
+    """RealMatrix::SVDMatrix (real_matrix.cc:588-620; complex
+    ComplexMatrix::SVDMatrix, complex_matrix.cc:601-637, right factor V^H):
+    thin SVD; the right factor is returned padded to n rows (rows >=
+    min(m,n) are zero)."""
     m, n = A.shape
     U, s, Vt = np.linalg.svd(A, full_matrices=False)
     sg = fix_signs(U)
     U = U * sg[None, :]
-    Vt = Vt * sg[:, None]
-    Vfull = np.zeros((n, n))
+    Vt = Vt * np.conj(sg)[:, None]
+    Vfull = np.zeros((n, n), dtype=A.dtype)
     Vfull[:Vt.shape[0]] = Vt
     return U, s.copy(), Vfull
+
+
+def _dtype(bm):
+    return np.complex128 if any(np.iscomplexobj(m) for m in bm.mats) else np.float64
 
 
 def left_canonical(lat, max_dim, canpre, info=None):
@@ -414,7 +423,7 @@ def left_canonical(lat, max_dim, canpre, info=None):
             S[r] = np.zeros(max(cols, 1))
             S[r] = S[r][:1]
         else:
-            A = np.zeros((rows, cols))
+            A = np.zeros((rows, cols), dtype=_dtype(ket))
             acc = 0
             for p in pos:
                 m = ket.mats[p]
@@ -455,7 +464,7 @@ def right_canonical(lat, max_dim, canpre, info=None):
             Us[l] = np.zeros((rows, rows))
             S[l] = np.zeros(1)
         else:
-            A = np.zeros((rows, cols))
+            A = np.zeros((rows, cols), dtype=_dtype(ket))
             acc = 0
             for p in pos:
                 m = ket.mats[p]
@@ -478,7 +487,7 @@ def right_canonical(lat, max_dim, canpre, info=None):
                 acc += cols
         lat.left_dim[l] = t[l]
         lat.match_dim[l] = Us[l].shape[0]
-        Ut = np.zeros((Us[l].shape[0], t[l]))
+        Ut = np.zeros((Us[l].shape[0], t[l]), dtype=Us[l].dtype)
         kk = min(t[l], Us[l].shape[1])
         Ut[:, :kk] = Us[l][:, :kk] * S[l][:kk][None, :]
         can.append(Ut)
@@ -541,6 +550,7 @@ class Space:
         self.noise = 10.0
         self.expan = None
         self.tracer = None
+        self.cplx = bool(getattr(model, "complex", False))
 
     def set_params(self, max_block, max_dim, canpre, noise):
         """DefineTensorSpaceParameter (real_tensor_space.cc:59-66)."""
@@ -593,7 +603,7 @@ class Space:
             li, ri, pi = self.tensor_index(i)
             rows = 1 if i == 0 else first_dim
             cols = 1 if i == L - 1 else first_dim
-            self.lat[i].define_blocks(li, ri, pi, rows, cols, self.rng)
+            self.lat[i].define_blocks(li, ri, pi, rows, cols, self.rng, self.cplx)
             if self.tracer:
                 self.tracer(-2, i, 0.0, "define_random")
             self.canonical(0, i)
@@ -659,7 +669,7 @@ class Space:
             left, right = list(lat.left_block), blocks
         E.define_sectors(left, right, [1] * len(left), [1] * len(right))
         li, ri, pi = self.tensor_index(site, E)
-        E.define_blocks(li, ri, pi, 1, 1, self.rng)
+        E.define_blocks(li, ri, pi, 1, 1, self.rng, self.cplx)
         E.ket.reset()
         self.expan = E
         return mapping
@@ -682,7 +692,7 @@ class Space:
             left, ld = combine(origin.left_block, origin.left_dim, E.left_block, E.left_dim)
         lat.define_sectors(left, right, ld, rd)
         li, ri, pi = self.tensor_index(site)
-        lat.define_blocks(li, ri, pi, 1, 1, self.rng)
+        lat.define_blocks(li, ri, pi, 1, 1, self.rng, self.cplx)
         lat.ket.reset()
         if leigh == 0:
             left_expan_fill(lat, origin, E)
@@ -708,7 +718,7 @@ class Space:
             left, ld, right, rd = [], [], [], []
         lat.define_sectors(left, right, ld, rd)
         li, ri, pi = self.tensor_index(site)
-        lat.define_blocks(li, ri, pi, 1, 1, self.rng)
+        lat.define_blocks(li, ri, pi, 1, 1, self.rng, self.cplx)
         lat.ket.reset()
         if leigh == 0 and site > 0:
             right_expan_fill(lat, origin, None)
@@ -831,8 +841,9 @@ def _expan_width(D, d):
 
 
 def left_expan_contraction(lat, W, Lenv, E, mapping, noise, rng):
-    """LeftExpanTensorContraction (real_tensor_contraction.cc:169-459)."""
-
+    """LeftExpanTensorContraction (real_tensor_contraction.cc:169-459;
+    complex_tensor_contraction.cc:169-470)."""
+    cplx = np.iscomplexobj(W)
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     same_l = [ket.same(0, l) for l in range(lat.nl)]
@@ -905,7 +916,7 @@ def left_expan_contraction(lat, W, Lenv, E, mapping, noise, rng):
                         p1 = E.phys[idx][ldx]
                         w = W[o1, o2, p1, p2]
                         if w != 0.0:
-                            rnd = rng.random_matrix(e0, e1)
+                            rnd = rng.complex_matrix(e0, e1) if cplx else rng.random_matrix(e0, e1)
                             for t in range(min(e0, e1)):
                                 rnd[t, t] = 0.1 * w
                             sec = _mm(first, rnd)
@@ -920,15 +931,26 @@ def left_expan_contraction(lat, W, Lenv, E, mapping, noise, rng):
         for l in range(E.nl):
             pos = E.ket.find(l, r)
             if pos != -1 and E.ket.mats[pos].shape[0] == 0:
+                if cplx:  # complex_tensor_contraction.cc:446-451
+                    E.ket.mats[pos] = np.zeros((E.left_dim[l], E.right_dim[r]), dtype=complex)
+                    continue
                 # the reference sizes this zero block by left_dim_[r] (:448)
                 if r >= len(E.left_dim):
                     raise IndexError("zero expansion block with r >= num_left_block (reference reads out of range)")
                 E.ket.mats[pos] = _add(E.ket.mats[pos], np.zeros((E.left_dim[r], E.right_dim[r])))
+            elif pos != -1 and cplx:
+                # complex pads short blocks with zero columns (:453-462)
+                m = E.ket.mats[pos]
+                extra = E.right_dim[r] - m.shape[1]
+                if extra > 0:
+                    E.ket.mats[pos] = _concat(m, np.zeros((m.shape[0], extra), dtype=complex), 1)
     E.ket.mats = [m * noise for m in E.ket.mats]
 
 
 def right_expan_contraction(lat, W, Renv, E, mapping, noise, rng):
-    """RightExpanTensorContraction (real_tensor_contraction.cc:477-765)."""
+    """RightExpanTensorContraction (real_tensor_contraction.cc:477-765;
+    complex_tensor_contraction.cc:488-786)."""
+    cplx = np.iscomplexobj(W)
     lb, rb = W.shape[0], W.shape[1]
     ket = lat.ket
     same_r = [ket.same(1, r) for r in range(lat.nr)]
@@ -1001,7 +1023,7 @@ def right_expan_contraction(lat, W, Renv, E, mapping, noise, rng):
                         p1 = E.phys[ldx][idx]
                         w = W[o1, o2, p1, p2]
                         if w != 0.0:
-                            rnd = rng.random_matrix(e0, e1)
+                            rnd = rng.complex_matrix(e0, e1) if cplx else rng.random_matrix(e0, e1)
                             for t in range(min(e0, e1)):
                                 rnd[t, t] = 0.1 * w
                             sec = _mm(rnd, first)
@@ -1016,10 +1038,19 @@ def right_expan_contraction(lat, W, Renv, E, mapping, noise, rng):
         for r in range(E.nr):
             pos = E.ket.find(l, r)
             if pos != -1 and E.ket.mats[pos].shape[0] == 0:
+                if cplx:  # complex_tensor_contraction.cc:762-767
+                    E.ket.mats[pos] = np.zeros((E.left_dim[l], E.right_dim[r]), dtype=complex)
+                    continue
                 # the reference sizes this zero block by left_dim_[r] (:754)
                 if r >= len(E.left_dim):
                     raise IndexError("zero expansion block with r >= num_left_block (reference reads out of range)")
                 E.ket.mats[pos] = _add(E.ket.mats[pos], np.zeros((E.left_dim[r], E.right_dim[r])))
+            elif pos != -1 and cplx:
+                # complex pads short blocks with zero rows (:769-778)
+                m = E.ket.mats[pos]
+                extra = E.left_dim[l] - m.shape[0]
+                if extra > 0:
+                    E.ket.mats[pos] = _concat(m, np.zeros((extra, m.shape[1]), dtype=complex), 0)
     E.ket.mats = [m * noise for m in E.ket.mats]
 
 
@@ -1090,7 +1121,7 @@ class DMRG:
         v = lat.ket.to_vector()
         print("TRACE %d %d %-17s E=%.12f nl=%d nr=%d ld=%d rd=%d nblk=%d size=%d rng=%d nrm2=%.12e" % (
             h, j, what, e if e is not None else 0.0, lat.nl, lat.nr, sum(lat.left_dim), sum(lat.right_dim),
-            lat.ket.n, lat.ket.dim(), self.rng.count, float(np.dot(v, v))), flush=True)
+            lat.ket.n, lat.ket.dim(), self.rng.count, float(np.vdot(v, v).real)), flush=True)
 
     def _discarded(self, info):
         s2 = 0.0

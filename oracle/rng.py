@@ -57,3 +57,13 @@ class GlibcRand:
         """RealMatrix::RandomMatrix (real_matrix.cc:201-208): column-major fill."""
         vals = self.draws(rows * cols)
         return (0.01 * (vals % 100)).reshape(cols, rows).T.copy()
+
+    def complex_matrix(self, rows, cols):
+        """ComplexMatrix::RandomMatrix (complex_matrix.cc:201-208): column-major
+        fill, two draws per entry, Complex(0.01*(rand()%100), 0.01*(rand()%100)).
+        C++ leaves the order of the two calls unspecified; g++ on x86-64
+        evaluates the second argument first, so the first draw of a pair is
+        the imaginary part and the second the real part."""
+        vals = 0.01 * (self.draws(2 * rows * cols) % 100)
+        z = vals[1::2] + 1j * vals[0::2]
+        return z.reshape(cols, rows).T.copy()

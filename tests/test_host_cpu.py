@@ -9,7 +9,7 @@ import pytest
 
 import dmrgcc_amd as dm
 from oracle.dmrg import truncate_dims as o_truncate
-from oracle.model import SpinlessBoseSquare
+from oracle.model import SpinfulBoseSquare, SpinlessBoseSquare
 from oracle.rng import GlibcRand
 from oracle.sort import quicksort as o_quicksort
 
@@ -97,3 +97,41 @@ def test_qn_tables_and_mpo_match_oracle(args):
         assert np.array_equal(pm.mpo_site(s), W[s]), s
     for b in range(om.L + 1):
         assert list(pm.mpo_table(b)) == [q[0] for q in qt[b]]
+
+
+SPINFUL_MODELS = [
+    # nx, ny, n_up, n_down, d, flux, jx, jy, (U_uu, U_dd, U_ud), mu, px, py
+    (2, 2, 2, 2, 3, 0.25, -1.0, -1.0, (4.0, 4.0, 2.0), 0.0, False, False),
+    (4, 4, 8, 8, 3, 0.25, -1.0, -1.0, (4.0, 4.0, 2.0), 0.0, False, False),  # 4x4, local dim 9
+    (3, 3, 2, 4, 3, 0.2, -0.7, -1.1, (1.0, 2.0, 1.5), 0.3, True, True),
+]
+
+
+@pytest.mark.parametrize("args", SPINFUL_MODELS)
+def test_spinful_tables_and_mpo_match_oracle(args):
+    nx, ny, nu, nd, d, flux, jx, jy, (uu, udd, ud), mu, px, py = args
+    om = SpinfulBoseSquare(nx, ny, nu, nd, d, flux=flux, jx=jx, jy=jy, U_uu=uu, U_dd=udd, U_ud=ud,
+                           mu=mu, period_x=px, period_y=py)
+    pm = dm.SpinfulModel(nx, ny, nu, nd, d, flux=flux, hop_x=jx, hop_y=jy, U_uu=uu, U_dd=udd,
+                         U_ud=ud, mu=mu, period_x=px, period_y=py)
+    assert pm.complex and pm.nq == 2 and pm.L == 2 * nx * ny
+    tabs = om.space_tables()
+    for b in range(om.L + 1):
+        assert [tuple(q) for q in pm.space_table(b)] == list(tabs[b])
+    W, qt = om.mpo()
+    for s in range(om.L):
+        assert np.array_equal(pm.mpo_site(s), W[s]), s
+    for b in range(om.L + 1):
+        assert [tuple(q) for q in pm.mpo_table(b)] == list(qt[b])
+
+
+def test_complex_spinless_mpo_is_real_mpo():
+    pr = dm.SpinlessModel(4, 2, 8, 3, U=2.0)
+    pz = dm.SpinlessModel(4, 2, 8, 3, U=2.0, complex=True)
+    om = SpinlessBoseSquare(4, 2, 8, 3, U=2.0, complex=True)
+    W, _ = om.mpo()
+    assert pz.complex and not pr.complex
+    for s in range(8):
+        assert np.array_equal(pz.mpo_site(s), W[s])
+        assert np.array_equal(pz.mpo_site(s).real, pr.mpo_site(s))
+        assert not np.any(pz.mpo_site(s).imag)

@@ -7,7 +7,7 @@ import pytest
 
 from oracle import ed
 from oracle.dmrg import DMRG, heff_apply, heff_dense, heff_tasks, svd, truncate_dims
-from oracle.model import SpinlessBoseSquare, parallel_matrix
+from oracle.model import SpinfulBoseSquare, SpinlessBoseSquare, parallel_matrix
 from oracle.rng import GlibcRand
 from oracle.sort import quicksort
 from tests._support import oracle_state, sweep_rows
@@ -126,4 +126,62 @@ def test_dmrg_reaches_exact_ground_energy(nx, ny, N, d, U, py):
     e0 = ed.ground_energy(m.L, d, N, m.bonds(), U)
     assert abs(E[-1] - e0) < 1e-10 * abs(e0)
     # variational: no half sweep goes below the exact energy
+    assert min(E) > e0 - 1e-10 * abs(e0)
+
+
+@pytest.mark.parametrize("nx,ny,d,flux,py", [(2, 2, 2, 0.25, False), (1, 3, 2, 0.2, True),
+                                             (3, 1, 3, 0.1, False), (2, 2, 3, 0.125, False)])
+def test_spinful_mpo_equals_bruteforce(nx, ny, d, flux, py):
+    # complex MPO with flux phases against the Fock-space H built from the
+    # lattice geometry (Landau gauge on y bonds)
+    m = SpinfulBoseSquare(nx, ny, 1, 1, d, flux=flux, jx=-1.0, jy=-0.8, U_uu=4.0, U_dd=3.0,
+                          U_ud=2.0, mu=0.3, period_y=py)
+    W, _ = m.mpo()
+    Hb = ed.fock_full(m.L, d, m.hops(), ed.spinful_diag(m))
+    H = contract_mpo(W)
+    assert np.abs(H - Hb).max() < 1e-12
+    assert np.abs(H - H.conj().T).max() < 1e-12
+    if ny > 1:  # y bonds carry the flux: H is genuinely complex
+        assert np.abs(Hb.imag).max() > 0.1
+
+
+def test_spinful_qn_rule_and_tables():
+    m = SpinfulBoseSquare(2, 2, 2, 3, 3)
+    tabs = m.space_tables()
+    assert tabs[0] == [(0, 0)] and tabs[-1] == [(2, 3)]
+    for i in range(1, m.L + 1):
+        up, dn = (i + 1) // 2, i // 2
+        full = {(a, b) for a in range(0, 2 * up + 1) for b in range(0, 2 * dn + 1)
+                if 2 - a <= 2 * (4 - up) and 3 - b <= 2 * (4 - dn) and a <= 2 and b <= 3}
+        assert set(tabs[i]) == full
+    # even sites carry species 0, odd sites species 1
+    assert m.check(0, (0, 0), (2, 0)) == 2 and m.check(0, (0, 0), (0, 1)) == -1
+    assert m.check(1, (1, 0), (1, 2)) == 2 and m.check(1, (0, 0), (1, 0)) == -1
+
+
+def test_complex_random_matrix_draw_order():
+    # two draws per entry, column-major; the first draw of a pair is the
+    # imaginary part (g++ evaluates the second constructor argument first)
+    a, b = GlibcRand(1), GlibcRand(1)
+    z = a.complex_matrix(2, 3)
+    v = 0.01 * (b.draws(12) % 100)
+    for j in range(3):
+        for i in range(2):
+            k = 2 * (j * 2 + i)
+            assert z[i, j] == complex(v[k + 1], v[k])
+
+
+@pytest.mark.parametrize("which", ["spinful", "zspinless"])
+def test_complex_dmrg_reaches_exact_ground_energy(which):
+    if which == "spinful":
+        m = SpinfulBoseSquare(2, 2, 2, 2, 3, flux=0.25, U_uu=4.0, U_dd=4.0, U_ud=2.0)
+        e0 = ed.spinful_ground_energy(m)
+        rows = sweep_rows(100, 6, niter=10, canpre=1e-12)
+    else:
+        m = SpinlessBoseSquare(3, 2, 6, 3, U=2.0, complex=True)
+        e0 = ed.ground_energy(m.L, 3, 6, m.bonds(), 2.0)
+        rows = sweep_rows(80, 6, niter=10, canpre=1e-12)
+    dmr = DMRG(m, rows, 3, first_block=20, first_dim=4)
+    E = dmr.run()
+    assert abs(E[-1] - e0) < 1e-10 * abs(e0)
     assert min(E) > e0 - 1e-10 * abs(e0)
