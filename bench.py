@@ -17,8 +17,17 @@ One "step" is one application of the centre-site effective Hamiltonian
   sweep        wall time per sweep for the FP64 path and for the mixed path
                (mixed H_eff inside Lanczos + FP64 Rayleigh-quotient
                refinement), and |dE|/|E| of the mixed energies vs FP64;
+  site_lanczos the centre site's Lanczos energy from the same start vector in
+               each arithmetic (the precision effect without the sweep
+               trajectory's own divergence);
   roofline     stage-2 kernel (the dominant launch) against the MFMA peak;
   cpu_baseline the oracle's numpy FP64 matvec on the identical state.
+
+--workload spinful measures the complex engine instead (not the default
+line): two-component Bose-Hubbard on a 4x4 open lattice with flux 1/4 per
+plaquette (32 MPS sites, N_up = N_down = 8, max occupation 2 per species,
+t = 1, U = 4, U' = 2), m = 2000 by default, complex128 plain vs complex
+BF16x3 contractions with FP64 Lanczos.
 """
 import argparse
 import json
@@ -100,13 +109,50 @@ def cpu_baseline(run, site, model, flops, budget_s, maxm):
                       "m=%d state, %d matvecs, %.3f s each" % (cores, maxm, reps, dt)}
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def site_lanczos(run, plan, site, niter, dm):
+    """Lanczos energy of one site's H_eff from the same start vector (the
+    paused run's ket) in each precision: isolates the arithmetic's effect on
+    the energy from the sweep trajectory.  Mixed forms end with the FP64
+    Rayleigh-quotient refinement."""
+    import torch
+    _, kd = run.site_blocks(site, 0)
+    x0 = torch.from_numpy(np.ascontiguousarray(kd).view(np.float64).copy()).cuda()
+    e, it = {}, {}
+    for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3)):
+        v = x0.clone()
+        e[name], it[name] = plan.lanczos(v, niter, prec, refine=True)
+    ref = e["fp64"]
+    return {"niter": niter, "energy": e, "iterations": it,
+            "dE_rel": {k: abs(e[k] - ref) / abs(ref) for k in ("fp32", "bf16x3")}}
+
+
+def drive(run, L, nhalf, name, stop=None):
+    """Advance `run` half sweep by half sweep (a progress line after each),
+    to the Lanczos solve of site stop[1] in half sweep stop[0], or to the
+    end of the table."""
+    last = stop[0] if stop else nhalf
+    for h in range(1, last + 1):
+        if h < nhalf:
+            run.advance(h, 0 if h % 2 == 0 else L - 1)
+            log("%s: half sweep %d done" % (name, h - 1))
+    if stop:
+        return run.advance(stop[0], stop[1])
+    return run.advance()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50, help="timed H_eff applications")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=8, help="sweeps of the m=1000 table")
-    ap.add_argument("--maxm", type=int, default=1000)
+    ap.add_argument("--maxm", type=int, default=None, help="bond dimension (1000; spinful 2000)")
+    ap.add_argument("--workload", choices=("cylinder", "spinful"), default="cylinder")
     ap.add_argument("--no-mixed-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -118,10 +164,20 @@ def main():
         sys.exit(2)
 
     import dmrgcc_amd as dm
-    nx, ny, N, d = 8, 4, 32, 4
-    U_std = 6.0
-    model = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U_std / 2, period_y=True)
-    L = nx * ny
+    if args.workload == "cylinder":
+        args.maxm = args.maxm or 1000
+        nx, ny, N, d = 8, 4, 32, 4
+        U_std = 6.0
+        model = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U_std / 2, period_y=True)
+        workload = ("8x4 Bose-Hubbard cylinder (L=32, N=32, d=4, t=1, U=6, periodic y), "
+                    "m=%d, centre-site (%d) H_eff matvec")
+    else:
+        args.maxm = args.maxm or 2000
+        model = dm.SpinfulModel(4, 4, 8, 8, 3, flux=0.25, hop_x=-1.0, hop_y=-1.0, U_uu=4.0, U_dd=4.0,
+                                U_ud=2.0)
+        workload = ("4x4 two-component Bose-Hubbard, flux 1/4 (32 MPS sites, N_up=N_down=8, d=3 per "
+                    "species, t=1, U=4, U'=2, open), complex, m=%d, centre-site (%d) H_eff matvec")
+    L = model.L
     site = L // 2
     tab = table(args.maxm, args.sweeps)
     nhalf = len(tab)
@@ -129,28 +185,35 @@ def main():
     # ---- FP64 run, stopped at the centre site of the last half sweep ----
     run64 = model.start(tab, precision=dm.FP64)
     t0 = time.time()
-    reached = run64.advance(nhalf - 1, site)
+    reached = drive(run64, L, nhalf, "FP64", (nhalf - 1, site))
     t_to_site = time.time() - t0
     assert reached
+    log("FP64 run reached site %d of half sweep %d in %.1f s" % (site, nhalf - 1, t_to_site))
     plan = run64.heff(site)
     info = plan.info()
     for prec in (dm.FP64, dm.FP32, dm.BF16X3):
         plan.time(prec, max(args.warmup, 1))
     tm = {name: plan.time(prec, args.steps)
           for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3))}
+    log("matvec timing done: " + ", ".join("%s %.3f ms" % (k, v["matvec_ms"]) for k, v in tm.items()))
+    site_e = site_lanczos(run64, plan, site, tab[-1][2], dm)
+    log("site Lanczos dE_rel: %s" % site_e["dE_rel"])
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(run64, site, model, info["flops_ref"], args.cpu_seconds, args.maxm)
+        log("CPU baseline done: %s" % cpu["sample"])
     t1 = time.time()
-    run64.advance()
+    run64.advance()  # the rest of the last half sweep
     t_rest = time.time() - t1
     r64 = run64.result()
+    log("FP64 run done: %.1f s" % r64["total_seconds"])
 
     rmx = None
     if not args.no_mixed_sweep:
         runmx = model.start(tab, precision=dm.BF16X3, refine=True)
-        runmx.advance()
+        drive(runmx, L, nhalf, "mixed")
         rmx = runmx.result()
+        log("mixed run done: %.1f s" % rmx["total_seconds"])
 
     def rate(name):
         return info["flops_ref"] / (tm[name]["matvec_ms"] * 1e-3) / 1e12
@@ -171,16 +234,17 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "bf16x3",
+        "dtype": "bf16x3" if not model.complex else "complex bf16x3",
         "data": "synthetic: reference seeded rand() initial MPS, model Hamiltonian (no dataset)",
-        "config": {"workload": "8x4 Bose-Hubbard cylinder (L=32, N=32, d=4, t=1, U=6, periodic y), "
-                               "m=%d, centre-site (%d) H_eff matvec" % (args.maxm, site),
+        "config": {"workload": workload % (args.maxm, site),
                    "sweeps": args.sweeps, "psi_dim": info["psi_dim"], "flops_ref": info["flops_ref"],
                    "flops_exec": info["flops_exec"], "tiles": info["tiles"],
                    "tiles_stage1": info["tiles_stage1"], "tiles_stage2": info["tiles_stage2"],
                    "bytes_min": info["bytes_min"]},
         "fp64": {"tflops": rate("fp64"), "ms": tm["fp64"]["matvec_ms"]},
         "fp32": {"tflops": rate("fp32"), "ms": tm["fp32"]["matvec_ms"]},
+        # same H_eff, same start vector: energy of each arithmetic vs FP64
+        "site_lanczos": site_e,
         "sweep": {
             "fp64_seconds_per_sweep": sweeps64,
             "fp64_total_seconds": r64["total_seconds"],
@@ -193,7 +257,8 @@ def main():
         # with HIP events; algorithmic flops = the stage-2 share of F_ref
         "roofline": {"bound": "mfma", "kernel": "gemm_bf16x3_kernel (H_eff stage 2)",
                      "achieved": achieved, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["bf16"], "traffic": traffic_bytes(),
+                     "frac": achieved / PEAK_TFLOPS["bf16"],
+                     "traffic": traffic_bytes() if args.workload == "cylinder" else None,
                      "kernel_ms": bf["stage2_gemm_ms"], "stage2_launch_ms": bf["stage2_ms"],
                      "stage2_flops": s2_flops, "grid_workgroups": info["tiles_stage2"]},
         "roofline_fp64": {"bound": "mfma", "kernel": "gemm_mfma_kernel<double> (H_eff stage 2)",

@@ -130,6 +130,15 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libdmrgcc_amd.so is not built; run python dmrgcc_amd/build.py")
+        # torch ships its own libamdhip64.so.7.  Loaded first, it is the one
+        # HIP runtime of the process and the library binds to it (same
+        # soname); loaded after the system copy it would start a second
+        # runtime that cannot see the GPU.  So torch, when installed, loads
+        # first, and device tensors and library calls share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(l, name)
