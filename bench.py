@@ -252,6 +252,7 @@ def main():
             "fp64_trunc_err": [float(e) for e in r64["trunc_err"]],
             "fp64_phase_seconds": r64["phase_seconds"],
             "max_bond": r64["max_bond"],
+            "rand_draws": r64["rand_draws"],
         },
         # dominant kernel: the stage-2 GEMM launch (out += w L T), timed alone
         # with HIP events; algorithmic flops = the stage-2 share of F_ref

@@ -32,6 +32,8 @@ _SIGS = {
     "dmrgcc_last_error": (ctypes.c_char_p, []),
     "dmrgcc_version": (ctypes.c_int, []),
     "dmrgcc_rand_draws": (ctypes.c_int, [ctypes.c_uint, ctypes.c_int64, ctypes.c_int, _c_int_p]),
+    "dmrgcc_random_matrices": (ctypes.c_int, [ctypes.c_uint, ctypes.c_int64, ctypes.c_int, _c_int_p,
+                                              _c_int_p, ctypes.c_int, _c_dbl_p, _c_dbl_p]),
     "dmrgcc_quicksort": (ctypes.c_int, [ctypes.c_int, _c_dbl_p, _c_int_p, ctypes.c_int]),
     "dmrgcc_truncate_dims": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_dbl_p, ctypes.c_int,
                                             ctypes.c_double, _c_int_p]),
@@ -189,6 +191,29 @@ def rand_draws(n, seed=1, skip=0):
     out = np.zeros(n, dtype=np.int32)
     _check(lib().dmrgcc_rand_draws(seed, skip, n, _ip(out)))
     return out
+
+
+def random_matrices(shapes, seed=1, skip=0, complex=False, w=None):
+    """RandomMatrix for each (rows, cols) in draw order; returns a list of
+    arrays (complex128 when complex).  w: per-matrix diagonal value source
+    (diagonal := 0.1 * w), or None for the raw draws."""
+    rows = _i32([r for r, _ in shapes])
+    cols = _i32([c for _, c in shapes])
+    cs = 2 if complex else 1
+    out = np.zeros(int(cs * np.sum(rows.astype(np.int64) * cols)))
+    wv = None
+    if w is not None:
+        wa = np.asarray(w, dtype=np.complex128 if complex else np.float64)
+        wv = _f64(wa.view(np.float64) if complex else wa)
+    _check(lib().dmrgcc_random_matrices(seed, skip, len(shapes), _ip(rows), _ip(cols), int(complex),
+                                        _dp(wv) if wv is not None else None, _dp(out)))
+    res, acc = [], 0
+    for r, c in shapes:
+        k = cs * r * c
+        blk = out[acc:acc + k]
+        res.append(blk.view(np.complex128).reshape(r, c) if complex else blk.reshape(r, c))
+        acc += k
+    return res
 
 
 def quicksort(vals, index=None, flag=0):

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["common.cpp", "kernels.hip", "model.cpp", "dmrg.cpp", "capi.cpp"]
 
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread",
     "-Wno-unused-result",
     # keep the compiler from forming packed FP32 ops on its own
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -48,7 +48,7 @@ def build(verbose=False):
             if msg:
                 print(msg)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + [o for o, _ in results]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-pthread", "-o", tmp] + [o for o, _ in results]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % (r.stdout + r.stderr))

@@ -236,6 +236,16 @@ int dmrgcc_rand_draws(unsigned seed, int64_t skip, int n, int* out) {
   });
 }
 
+int dmrgcc_random_matrices(unsigned seed, int64_t skip, int n, const int* rows, const int* cols,
+                           int is_complex, const double* w, double* out) {
+  return guarded([&] {
+    require(skip >= 0, "bad arguments", kErrArgument);
+    GlibcRand g(seed);
+    g.skip(skip);
+    random_matrices(g, n, rows, cols, is_complex ? 2 : 1, w, out);
+  });
+}
+
 int dmrgcc_quicksort(int n, double* vals, int* index, int flag) {
   return guarded([&] {
     require(vals != nullptr && n >= 0 && (flag == 0 || flag == 1), "bad arguments", kErrArgument);

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <thread>
 
 #include "quicksort.h"
 
@@ -78,6 +79,62 @@ void push_copy(std::vector<CopyTask>& v, CopyTask c, int cs) {
   }
 }
 
+// One expansion random matrix: RandomMatrix(rows, cols) (complex: two draws
+// per entry) at rnd[roff], then its diagonal set to 0.1 * (w + i wi)
+// (real_tensor_contraction.cc:397-399, complex_tensor_contraction.cc:397-399).
+struct RandomDraw {
+  int64_t roff;
+  int rows, cols;
+  double w, wi;
+  bool diag = true;  // false: leave the raw draws on the diagonal
+};
+
+// Fill the random matrices in draw order from `rng`, which then stands after
+// the last draw.  Large sets are split over host threads, each starting from
+// a jump-ahead copy of the stream, so the values are those of one sequential
+// pass.
+void fill_random(const std::vector<RandomDraw>& ds, std::vector<double>& rnd, int cs, GlibcRand& rng) {
+  std::vector<int64_t> first(ds.size() + 1, 0);
+  for (size_t k = 0; k < ds.size(); ++k) first[k + 1] = first[k] + (int64_t)cs * ds[k].rows * ds[k].cols;
+  const int64_t total = first.back();
+  auto work = [&](size_t k0, size_t k1) {
+    GlibcRand g = rng;
+    g.skip(first[k0]);
+    for (size_t k = k0; k < k1; ++k) {
+      const RandomDraw& d = ds[k];
+      double* R = rnd.data() + d.roff;
+      if (cs == 2) {
+        g.complex_matrix(d.rows, d.cols, R);
+        for (int q = 0; q < (d.diag ? std::min(d.rows, d.cols) : 0); ++q) {
+          R[2 * ((int64_t)q * d.cols + q)] = 0.1 * d.w;
+          R[2 * ((int64_t)q * d.cols + q) + 1] = 0.1 * d.wi;
+        }
+      } else {
+        g.random_matrix(d.rows, d.cols, R);
+        for (int q = 0; q < (d.diag ? std::min(d.rows, d.cols) : 0); ++q) R[(int64_t)q * d.cols + q] = 0.1 * d.w;
+      }
+    }
+  };
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<int64_t>(std::min<int64_t>(hw, 16), total / (1 << 18));
+  if (nt <= 1) {
+    work(0, ds.size());
+  } else {
+    std::vector<std::thread> th;
+    size_t k0 = 0;
+    for (int t = 0; t < nt && k0 < ds.size(); ++t) {
+      const int64_t target = total * (t + 1) / nt;
+      size_t k1 = k0 + 1;
+      while (k1 < ds.size() && first[k1] < target) ++k1;
+      if (t == nt - 1) k1 = ds.size();
+      th.emplace_back(work, k0, k1);
+      k0 = k1;
+    }
+    for (auto& x : th) x.join();
+  }
+  rng.skip(total);
+}
+
 // One piece of a block assembled by the expansion / match fills.
 struct Piece {
   const double* src;  // nullptr: zeros
@@ -87,6 +144,23 @@ struct Piece {
 };
 
 }  // namespace
+
+void random_matrices(GlibcRand& rng, int n, const int* rows, const int* cols, int cs, const double* w,
+                     double* out) {
+  require(n >= 0 && (cs == 1 || cs == 2) && (n == 0 || (rows && cols && out)), "bad arguments",
+          kErrArgument);
+  std::vector<RandomDraw> ds;
+  int64_t off = 0;
+  for (int k = 0; k < n; ++k) {
+    require(rows[k] >= 0 && cols[k] >= 0, "bad matrix shape", kErrArgument);
+    ds.push_back(RandomDraw{off, rows[k], cols[k], w ? w[cs * k] : 0.0, (w && cs == 2) ? w[cs * k + 1] : 0.0,
+                            w != nullptr});
+    off += (int64_t)cs * rows[k] * cols[k];
+  }
+  std::vector<double> buf((size_t)off);
+  fill_random(ds, buf, cs, rng);
+  std::copy(buf.begin(), buf.end(), out);
+}
 
 // ComputeTruncateDim (real_tensor_lattice.cc:501-555).  `sv` holds each
 // sector's singular values in descending order; a sector left with no state
@@ -1380,6 +1454,8 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   std::vector<First> firsts;
   std::vector<Term> terms;
   std::vector<double> rnd;
+  std::vector<RandomDraw> draws;  // the random matrices, in draw order
+  int64_t pending = 0;            // draws of an inactive expansion
   std::vector<Shape> eshape(E.ket.n());
   int64_t fsize = 0;
   for (int ob = 0; ob < nouter; ++ob) {
@@ -1416,7 +1492,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
             const double w = wat(ob, ib, p1, p2), wi = wim(ob, ib, p1, p2);
             if (w != 0.0 || wi != 0.0) {
               if (!active) {
-                cx_.rng.skip((int64_t)cs * r_rows * r_cols);
+                pending += (int64_t)cs * r_rows * r_cols;
                 continue;
               }
               Term t;
@@ -1426,17 +1502,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
               t.e1 = r_cols;
               t.roff = (int64_t)rnd.size();  // doubles
               rnd.resize(rnd.size() + (size_t)cs * r_rows * r_cols);
-              double* R = rnd.data() + t.roff;
-              if (cs == 2) {
-                cx_.rng.complex_matrix(r_rows, r_cols, R);
-                for (int q = 0; q < std::min(r_rows, r_cols); ++q) {
-                  R[2 * ((int64_t)q * r_cols + q)] = 0.1 * w;
-                  R[2 * ((int64_t)q * r_cols + q) + 1] = 0.1 * wi;
-                }
-              } else {
-                cx_.rng.random_matrix(r_rows, r_cols, R);
-                for (int q = 0; q < std::min(r_rows, r_cols); ++q) R[(int64_t)q * r_cols + q] = 0.1 * w;
-              }
+              draws.push_back(RandomDraw{t.roff, r_rows, r_cols, w, wi});
               // product shape and its place in the (concatenated) block
               Shape ps = left ? Shape{f.sh.rows, r_cols} : Shape{r_rows, f.sh.cols};
               Shape& es = eshape[pos];
@@ -1459,7 +1525,11 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
       }
     }
   }
-  if (!active) return;  // ExpanTensorSpace returns early: E is never used
+  if (!active) {  // ExpanTensorSpace returns early: E is never used
+    cx_.rng.skip(pending);
+    return;
+  }
+  fill_random(draws, rnd, cs, cx_.rng);
   // ---- dims of the expansion sectors; zero blocks for empty ones ----
   if (left) {
     for (int r = 0; r < E.nr(); ++r) {

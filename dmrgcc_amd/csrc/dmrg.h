@@ -118,6 +118,15 @@ class HeffPlan {
 // RightComputeTensorContraction :959-1119).
 Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx);
 Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx);
+// Random matrices of the expansion in draw order (RandomMatrix,
+// real_matrix.cc:201-208; complex_matrix.cc:201-208 with cs = 2): matrix k
+// is rows[k] x cols[k], row-major (complex: interleaved), concatenated in
+// `out`; with w != nullptr its diagonal is then set to 0.1 * w[k] (complex:
+// w holds re, im pairs), as the expansion does (real_tensor_contraction.cc:
+// 397-399).  Large sets are generated by several host threads from
+// jump-ahead copies of `rng`, which ends after the last draw.
+void random_matrices(GlibcRand& rng, int n, const int* rows, const int* cols, int cs, const double* w,
+                     double* out);
 // cs = 2: complex (1 + 0i) boundary blocks
 Env boundary_env(int nbond, Context& cx, int cs = 1);
 

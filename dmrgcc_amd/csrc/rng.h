@@ -58,12 +58,59 @@ class GlibcRand {
         out[2 * ((int64_t)i * cols + j) + 1] = im;
       }
   }
+  // Advance by n draws.  Short skips step; long ones jump: the raw outputs
+  // obey r[k] = r[k-3] + r[k-31] (mod 2^32), i.e. the characteristic
+  // polynomial P(x) = x^31 - x^28 - 1, so with x^n = sum_j c_j x^j (mod P)
+  // every later window follows from the current one:
+  // r[m+n+t] = sum_j c_j r[m+j+t].
   void skip(int64_t n) {
-    for (int64_t i = 0; i < n; ++i) next();
+    if (n < 4096) {
+      for (int64_t i = 0; i < n; ++i) next();
+      return;
+    }
+    // window w[0..60]: the 31 current values, oldest first, then 30 more
+    uint32_t w[61];
+    for (int j = 0; j < 31; ++j) w[j] = st_[(f_ + j) % 31];
+    for (int j = 31; j < 61; ++j) w[j] = w[j - 3] + w[j - 31];
+    uint32_t c[31];
+    pow_mod(n, c);
+    for (int t = 0; t < 31; ++t) {
+      uint32_t v = 0;
+      for (int j = 0; j < 31; ++j) v += c[j] * w[j + t];
+      st_[t] = v;
+    }
+    f_ = 0;
+    r_ = 28;
+    count_ += (uint64_t)n;
   }
   uint64_t count() const { return count_; }
 
  private:
+  // a = a * b mod P(x) on degree-30 coefficient vectors (mod 2^32)
+  static void mul_mod(uint32_t* a, const uint32_t* b) {
+    uint32_t p[61] = {0};
+    for (int i = 0; i < 31; ++i)
+      for (int j = 0; j < 31; ++j) p[i + j] += a[i] * b[j];
+    for (int k = 60; k >= 31; --k) {  // x^k = x^(k-3) + x^(k-31)
+      p[k - 3] += p[k];
+      p[k - 31] += p[k];
+      p[k] = 0;
+    }
+    for (int i = 0; i < 31; ++i) a[i] = p[i];
+  }
+  // c = x^n mod P(x)
+  static void pow_mod(int64_t n, uint32_t* c) {
+    uint32_t base[31] = {0};
+    base[1] = 1;
+    for (int i = 0; i < 31; ++i) c[i] = 0;
+    c[0] = 1;
+    while (n > 0) {
+      if (n & 1) mul_mod(c, base);
+      n >>= 1;
+      if (n) mul_mod(base, base);
+    }
+  }
+
   uint32_t st_[31];
   int f_ = 3, r_ = 0;
   uint64_t count_ = 0;

@@ -43,6 +43,17 @@ int dmrgcc_version(void);
 /* glibc rand() stream from `seed` (the reference never seeds: seed 1),
  * src/tensor/real_matrix.cc:201-208.  Writes n successive values. */
 int dmrgcc_rand_draws(unsigned seed, int64_t skip, int n, int* out);
+/* n random matrices in draw order from the stream after `skip` draws:
+ * RealMatrix::RandomMatrix (src/tensor/real_matrix.cc:201-208) or, with
+ * is_complex, ComplexMatrix::RandomMatrix (src/tensor/complex_matrix.cc:
+ * 201-208, two draws per entry, the first one the imaginary part as g++
+ * evaluates the constructor arguments); row-major, concatenated in out
+ * (interleaved re, im when complex).  With w != NULL the diagonal of matrix
+ * k is set to 0.1 * w[k] (complex: w holds re, im pairs), as the subspace
+ * expansion does (real_tensor_contraction.cc:397-399).  Long skips jump
+ * ahead in O(log skip); large sets are filled by several host threads. */
+int dmrgcc_random_matrices(unsigned seed, int64_t skip, int n, const int* rows, const int* cols,
+                           int is_complex, const double* w, double* out);
 
 /* QuickSort (src/util/general.ih:6-65), flag 0 descending / 1 ascending,
  * sorting `vals` in place together with `index`. */

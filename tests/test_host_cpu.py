@@ -135,3 +135,30 @@ def test_complex_spinless_mpo_is_real_mpo():
         assert np.array_equal(pz.mpo_site(s), W[s])
         assert np.array_equal(pz.mpo_site(s).real, pr.mpo_site(s))
         assert not np.any(pz.mpo_site(s).imag)
+
+
+def test_random_matrices_match_oracle_stream():
+    # the threaded jump-ahead fill reproduces one sequential pass of the
+    # stream: real raw draws, then complex with the expansion's diagonal
+    shapes = [(900, 25)] * 30 + [(3, 7), (0, 5), (17, 1), (25, 40)]
+    M = dm.random_matrices(shapes, skip=12345)
+    g = GlibcRand(1)
+    g.skip(12345)
+    for m, (r, c) in zip(M, shapes):
+        assert np.array_equal(m, g.random_matrix(r, c))
+    w = [complex(0.3 * k, -0.2) for k in range(len(shapes))]
+    Mz = dm.random_matrices(shapes, skip=777, complex=True, w=w)
+    g = GlibcRand(1)
+    g.skip(777)
+    for k, (m, (r, c)) in enumerate(zip(Mz, shapes)):
+        z = g.complex_matrix(r, c)
+        for q in range(min(r, c)):
+            z[q, q] = 0.1 * w[k]
+        assert np.array_equal(m, z), k
+
+
+@pytest.mark.parametrize("skip", [4095, 4096, 100003, 300000])
+def test_rand_jump_ahead(skip):
+    g = GlibcRand(1)
+    g.skip(skip)
+    assert list(dm.rand_draws(40, skip=skip)) == list(g.draws(40))
