@@ -50,6 +50,10 @@ _SIGS = {
                                                     _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                                     ctypes.c_int, ctypes.POINTER(_vp)]),
     "dmrgcc_model_info": (ctypes.c_int, [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "dmrgcc_input_model_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                                 ctypes.POINTER(_vp)]),
+    "dmrgcc_input_run_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
+                                               _c_int_p, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_int_p]),
     "dmrgcc_model_destroy": (ctypes.c_int, [_vp]),
     "dmrgcc_model_num_site": (ctypes.c_int, [_vp]),
     "dmrgcc_model_space_table": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p]),
@@ -326,6 +330,46 @@ class SpinfulModel(_Model):
                                                  int(period_x), int(period_y), ctypes.byref(h)))
         self._h = h
         self._init_info()
+
+
+class InputModel(_Model):
+    """The model a reference driver builds from its input file (group
+    "input"): driver is "real_spinless_bose_square",
+    "complex_spinless_bose_square" or "complex_spinful_bose_square"."""
+
+    def __init__(self, path, driver, echo=False):
+        h = _vp()
+        _check(lib().dmrgcc_input_model_create(os.fsencode(path), driver.encode(), int(echo), ctypes.byref(h)))
+        self._h = h
+        self._init_info()
+
+
+def input_run_config(path, echo=False):
+    """The run section of an input file: first_block, first_dim, the sweep
+    table as rows (maxm, maxb, niter, canpre, noise exponent) per half sweep,
+    and the cache flags."""
+    n, fb, fd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().dmrgcc_input_run_config(os.fsencode(path), int(echo), ctypes.byref(fb), ctypes.byref(fd),
+                                         ctypes.byref(n), None, None, None, None, None, None))
+    k = n.value
+    maxm, maxb, niter = (np.zeros(k, dtype=np.int32) for _ in range(3))
+    canpre, noise = np.zeros(k), np.zeros(k)
+    flags = np.zeros(4, dtype=np.int32)
+    _check(lib().dmrgcc_input_run_config(os.fsencode(path), 0, ctypes.byref(fb), ctypes.byref(fd),
+                                         ctypes.byref(n), _ip(maxm), _ip(maxb), _ip(niter), _dp(canpre),
+                                         _dp(noise), _ip(flags)))
+    rows = [(int(maxm[i]), int(maxb[i]), int(niter[i]), float(canpre[i]), float(noise[i])) for i in range(k)]
+    return {"first_block": fb.value, "first_dim": fd.value, "table": rows,
+            "disk_cache": bool(flags[0]), "cache_record": bool(flags[1]),
+            "cache_resume": bool(flags[2]), "record_process": bool(flags[3])}
+
+
+def run_input(path, driver, precision=FP64, refine=True, verbose=False, echo=False):
+    """A reference model driver end to end: the model and the sweep table
+    from the input file, then Run()."""
+    m = InputModel(path, driver, echo)
+    c = input_run_config(path, echo)
+    return m.run(c["table"], c["first_block"], c["first_dim"], precision, refine, verbose)
 
 
 def _table_args(table):

@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdmrgcc_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["common.cpp", "kernels.hip", "model.cpp", "dmrg.cpp", "capi.cpp"]
+SOURCES = ["common.cpp", "kernels.hip", "model.cpp", "input.cpp", "dmrg.cpp", "capi.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread",

@@ -2,6 +2,7 @@
 // piece of the engine, converting exceptions into status codes.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -9,6 +10,7 @@
 
 #include "../../include/dmrgcc_hip.h"
 #include "dmrg.h"
+#include "input.h"
 #include "quicksort.h"
 #include "rng.h"
 
@@ -337,6 +339,47 @@ int dmrgcc_model_info(const dmrgcc_model* m, int* num_site, int* physics_dim, in
     if (physics_dim) *physics_dim = m->space->physics_dim();
     if (num_quantum) *num_quantum = m->space->num_quantum();
     if (is_complex) *is_complex = m->ham->complex() ? 1 : 0;
+  });
+}
+
+int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dmrgcc_model** out) {
+  return guarded([&] {
+    require(path && driver && out, "bad arguments", kErrArgument);
+    InputGroup::echo = echo != 0;
+    InputGroup input(path, "input");
+    auto m = std::make_unique<dmrgcc_model>();
+    read_model(input, driver, m->space, m->ham);
+    m->ham->DefineTensorHamiltonian();
+    m->tables = m->space->DefineQuantumTable();
+    *out = m.release();
+    std::fflush(stdout);
+  });
+}
+
+int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
+                            int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags) {
+  return guarded([&] {
+    require(path && nhalf, "bad arguments", kErrArgument);
+    InputGroup::echo = echo != 0;
+    InputGroup input(path, "input");
+    const RunConfig c = read_run_config(input);
+    *nhalf = (int)c.maxm.size();
+    if (first_block) *first_block = c.first_block;
+    if (first_dim) *first_dim = c.first_dim;
+    for (size_t i = 0; i < c.maxm.size(); ++i) {
+      if (maxm) maxm[i] = c.maxm[i];
+      if (maxb) maxb[i] = c.maxb[i];
+      if (niter) niter[i] = c.niter[i];
+      if (canpre) canpre[i] = c.canpre[i];
+      if (noise) noise[i] = c.noise[i];
+    }
+    if (flags) {
+      flags[0] = c.disk_cache;
+      flags[1] = c.cache_record;
+      flags[2] = c.cache_resume;
+      flags[3] = c.record_process;
+    }
+    std::fflush(stdout);
   });
 }
 

@@ -1870,7 +1870,9 @@ void RealTensorRundmrg::Step(int h, int j) {
     res_.energies.push_back(half_e_);
     res_.trunc_err.push_back(half_terr_);
     res_.half_seconds.push_back(now() - half_t0_);
-    if (p_.verbose) std::printf("%s energy = %.12f\n", l2r ? "right" : "left", half_e_), std::fflush(stdout);
+    // cout fixed, precision 12 (real_tensor_rundmrg.cc:122-123) or 14 (complex_tensor_rundmrg.cc:123)
+    if (p_.verbose)
+      std::printf("%s energy = %.*f\n", l2r ? "right" : "left", cs_ == 2 ? 14 : 12, half_e_), std::fflush(stdout);
   }
 }
 
@@ -1920,7 +1922,9 @@ RunResult RealTensorRundmrg::Result() {
 RunResult RealTensorRundmrg::Run() {
   Start();
   Advance(-1, -1);
-  return Result();
+  RunResult r = Result();
+  if (p_.verbose) std::printf("Total Time : %.*f s\n", cs_ == 2 ? 14 : 12, r.total_seconds), std::fflush(stdout);
+  return r;
 }
 
 }  // namespace dmrg

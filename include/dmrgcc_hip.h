@@ -93,6 +93,23 @@ int dmrgcc_model_spinful_create(int nx, int ny, int n_up, int n_down, int physic
                                 const double* flux, const double* hop_x, const double* hop_y,
                                 const double* inter, const double* mu, int period_x, int period_y,
                                 dmrgcc_model** out);
+/* Model from a driver's input file: InputGroup(path, "input")
+ * (src/util/input.cc:98-331) read as the reference's model drivers read it.
+ * driver: "real_spinless_bose_square" (model/real_spinless_bose_square_model.cc:3-43),
+ * "complex_spinless_bose_square" (model/complex_spinless_bose_square_model.cc:3-43) or
+ * "complex_spinful_bose_square" (model/complex_spinful_bose_square_model.cc:3-56).
+ * echo != 0 prints the reference's "Got/Def input.<key> = value" lines.
+ * A missing required key fails with "there is no item: <key>". */
+int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dmrgcc_model** out);
+/* The run section of the same file, as the RealTensorRundmrg constructor
+ * reads it (real_tensor_rundmrg.cc:3-42) with Sweep (:70-113): first_block,
+ * first_dim and the sweep table, one row per half sweep (nhalf =
+ * 2*num_sweep) after the reference's fill-in rule; noise holds the table's
+ * exponents (dmrgcc_run applies 0.1^noise).  Pass maxm == NULL to get
+ * nhalf first.  flags (4 ints, may be NULL): disk_cache, cache_record,
+ * cache_resume, record_process. */
+int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
+                            int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags);
 int dmrgcc_model_destroy(dmrgcc_model* m);
 int dmrgcc_model_num_site(const dmrgcc_model* m);
 /* Number of sites, local dimension, QNs per sector and whether the MPO is
