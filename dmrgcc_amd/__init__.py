@@ -52,6 +52,8 @@ _SIGS = {
     "dmrgcc_model_info": (ctypes.c_int, [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "dmrgcc_input_model_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.POINTER(_vp)]),
+    "dmrgcc_input_string": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                           _c_int_p]),
     "dmrgcc_input_run_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
                                                _c_int_p, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_int_p]),
     "dmrgcc_model_destroy": (ctypes.c_int, [_vp]),
@@ -109,6 +111,11 @@ _SIGS = {
     "dmrgcc_dmrg_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
                                           _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_dmrg_create_files": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
+                                                _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                                ctypes.POINTER(_vp)]),
+    "dmrgcc_dmrg_write_space": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "dmrgcc_dmrg_advance": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p]),
     "dmrgcc_dmrg_set_precision": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "dmrgcc_dmrg_heff": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -123,6 +130,10 @@ _SIGS = {
     "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                   _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+    "dmrgcc_run_files": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                        _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, _c_dbl_p, _c_dbl_p,
+                                        _c_dbl_p, _c_dbl_p]),
 }
 
 
@@ -280,20 +291,24 @@ class _Model:
         _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), _ip(q)))
         return q if self.nq == 1 else q.reshape(c.value, self.nq)
 
-    def start(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
+    def start(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False,
+              resume=None, record=None):
         """A resumable run (DmrgRun) of this model over the sweep table."""
-        return DmrgRun(self, table, first_block, first_dim, precision, refine, verbose)
+        return DmrgRun(self, table, first_block, first_dim, precision, refine, verbose, resume, record)
 
-    def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False):
+    def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False,
+            resume=None, record=None):
         """RealTensorRundmrg::Run (ComplexTensorRundmrg::Run on a complex
         model).  table: rows (maxm, maxb, niter, canpre, noise_exponent) per
-        half sweep.  Returns a dict of per-half-sweep energies, discarded
+        half sweep.  resume: tensor-space file to start from instead of the
+        random initial MPS; record: file the tensor space is written to after
+        every sweep.  Returns a dict of per-half-sweep energies, discarded
         weights, seconds and run statistics."""
         n, maxm, maxb, niter, canpre, noise = _table_args(table)
         e, t, s, st = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(24)
-        _check(lib().dmrgcc_run(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
-                                _dp(canpre), _dp(noise), precision, int(refine), int(verbose), _dp(e),
-                                _dp(t), _dp(s), _dp(st)))
+        _check(lib().dmrgcc_run_files(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
+                                      _dp(canpre), _dp(noise), precision, int(refine), int(verbose),
+                                      _path(resume), _path(record), _dp(e), _dp(t), _dp(s), _dp(st)))
         return _result_dict(e, t, s, st)
 
 
@@ -359,17 +374,39 @@ def input_run_config(path, echo=False):
                                          ctypes.byref(n), _ip(maxm), _ip(maxb), _ip(niter), _dp(canpre),
                                          _dp(noise), _ip(flags)))
     rows = [(int(maxm[i]), int(maxb[i]), int(niter[i]), float(canpre[i]), float(noise[i])) for i in range(k)]
-    return {"first_block": fb.value, "first_dim": fd.value, "table": rows,
-            "disk_cache": bool(flags[0]), "cache_record": bool(flags[1]),
-            "cache_resume": bool(flags[2]), "record_process": bool(flags[3])}
+    out = {"first_block": fb.value, "first_dim": fd.value, "table": rows,
+           "disk_cache": bool(flags[0]), "cache_record": bool(flags[1]),
+           "cache_resume": bool(flags[2]), "record_process": bool(flags[3])}
+    for key in ("cache_name", "record_name", "resume_name", "process_name"):
+        out[key] = input_string(path, key)
+    return out
+
+
+def input_string(path, key):
+    """String value of `key` in the file's "input" group (None if absent)."""
+    buf = ctypes.create_string_buffer(4096)
+    found = ctypes.c_int()
+    _check(lib().dmrgcc_input_string(os.fsencode(path), key.encode(), buf, len(buf), ctypes.byref(found)))
+    return buf.value.decode() if found.value else None
 
 
 def run_input(path, driver, precision=FP64, refine=True, verbose=False, echo=False):
     """A reference model driver end to end: the model and the sweep table
-    from the input file, then Run()."""
+    from the input file, then Run().  cache_resume starts from the file
+    resume_name; cache_record writes cache_name/record_name after every
+    sweep (the folder is created), as the reference's Run does."""
     m = InputModel(path, driver, echo)
     c = input_run_config(path, echo)
-    return m.run(c["table"], c["first_block"], c["first_dim"], precision, refine, verbose)
+    resume = c["resume_name"] if c["cache_resume"] else None
+    record = None
+    if c["cache_record"]:
+        os.makedirs(c["cache_name"], exist_ok=True)
+        record = os.path.join(c["cache_name"], c["record_name"])
+    return m.run(c["table"], c["first_block"], c["first_dim"], precision, refine, verbose, resume, record)
+
+
+def _path(p):
+    return None if p is None else os.fsencode(p)
 
 
 def _table_args(table):
@@ -394,14 +431,19 @@ class DmrgRun:
     """Resumable RealTensorRundmrg over a sweep table (see SpinlessModel.run)."""
 
     def __init__(self, model, table, first_block=20, first_dim=10, precision=FP64, refine=True,
-                 verbose=False):
+                 verbose=False, resume=None, record=None):
         self.model = model
         self.nhalf, maxm, maxb, niter, canpre, noise = _table_args(table)
         h = _vp()
-        _check(lib().dmrgcc_dmrg_create(model._h, first_block, first_dim, self.nhalf, _ip(maxm),
-                                        _ip(maxb), _ip(niter), _dp(canpre), _dp(noise), precision,
-                                        int(refine), int(verbose), ctypes.byref(h)))
+        _check(lib().dmrgcc_dmrg_create_files(model._h, first_block, first_dim, self.nhalf, _ip(maxm),
+                                              _ip(maxb), _ip(niter), _dp(canpre), _dp(noise), precision,
+                                              int(refine), int(verbose), _path(resume), _path(record),
+                                              ctypes.byref(h)))
         self._h = h
+
+    def write_space(self, path):
+        """RealTensorSpace::WriteTensorSpace of the current state."""
+        _check(lib().dmrgcc_dmrg_write_space(self._h, os.fsencode(path)))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

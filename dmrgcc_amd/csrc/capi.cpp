@@ -356,6 +356,21 @@ int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dm
   });
 }
 
+int dmrgcc_input_string(const char* path, const char* key, char* buf, int buflen, int* found) {
+  return guarded([&] {
+    require(path && key && buf && buflen > 0, "bad arguments", kErrArgument);
+    const bool echo = InputGroup::echo;
+    InputGroup::echo = false;
+    InputGroup input(path, "input");
+    std::string v;
+    const int got = input.GetString(key, v);
+    InputGroup::echo = echo;
+    if (found) *found = got;
+    require((int)v.size() < buflen, "buffer too small for input value", kErrArgument);
+    std::memcpy(buf, v.c_str(), v.size() + 1);
+  });
+}
+
 int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
                             int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags) {
   return guarded([&] {
@@ -773,15 +788,32 @@ int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N
 int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                        const int* maxb, const int* niter, const double* canpre, const double* noise,
                        int precision, int refine, int verbose, dmrgcc_dmrg** out) {
+  return dmrgcc_dmrg_create_files(m, first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise,
+                                  precision, refine, verbose, nullptr, nullptr, out);
+}
+
+int dmrgcc_dmrg_create_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                             const int* maxb, const int* niter, const double* canpre, const double* noise,
+                             int precision, int refine, int verbose, const char* resume_path,
+                             const char* record_path, dmrgcc_dmrg** out) {
   return guarded([&] {
     require(m && out, "null argument", kErrArgument);
     RunParams p = make_params(first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
                               refine, verbose);
+    if (resume_path) p.resume_path = resume_path;
+    if (record_path) p.record_path = record_path;
     auto h = std::make_unique<dmrgcc_dmrg>();
     h->model = m;
     h->run = std::make_unique<RealTensorRundmrg>(*m->space, *m->ham, p);
     h->run->Start();
     *out = h.release();
+  });
+}
+
+int dmrgcc_dmrg_write_space(dmrgcc_dmrg* h, const char* path) {
+  return guarded([&] {
+    require(h && path, "null argument", kErrArgument);
+    h->run->WriteTensorSpace(path);
   });
 }
 
@@ -928,10 +960,21 @@ int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
                double* seconds, double* stats) {
+  return dmrgcc_run_files(m, first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
+                          refine, verbose, nullptr, nullptr, energies, trunc, seconds, stats);
+}
+
+int dmrgcc_run_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                     const int* maxb, const int* niter, const double* canpre, const double* noise,
+                     int precision, int refine, int verbose, const char* resume_path,
+                     const char* record_path, double* energies, double* trunc, double* seconds,
+                     double* stats) {
   return guarded([&] {
     require(m != nullptr, "null model", kErrArgument);
     RunParams p = make_params(first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
                               refine, verbose);
+    if (resume_path) p.resume_path = resume_path;
+    if (record_path) p.record_path = record_path;
     RealTensorRundmrg run(*m->space, *m->ham, p);
     fill_result(run.Run(), nullptr, energies, trunc, seconds, stats);
   });

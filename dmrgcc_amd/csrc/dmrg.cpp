@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <numeric>
 #include <thread>
@@ -1816,7 +1817,8 @@ void RealTensorRundmrg::Start() {
   stats_ = LanczosStats{};
   ph_ = PhaseTimes{};
   res_ = RunResult{};
-  DefineTensorSpace(p_.first_block, p_.first_dim);
+  if (!p_.resume_path.empty()) ReadTensorSpace(p_.resume_path);
+  else DefineTensorSpace(p_.first_block, p_.first_dim);
   Initialize();
   cur_h_ = 0;
   cur_j_ = 0;
@@ -1873,6 +1875,7 @@ void RealTensorRundmrg::Step(int h, int j) {
     // cout fixed, precision 12 (real_tensor_rundmrg.cc:122-123) or 14 (complex_tensor_rundmrg.cc:123)
     if (p_.verbose)
       std::printf("%s energy = %.*f\n", l2r ? "right" : "left", cs_ == 2 ? 14 : 12, half_e_), std::fflush(stdout);
+    if (!l2r && !p_.record_path.empty()) WriteTensorSpace(p_.record_path);
   }
 }
 
@@ -1917,6 +1920,182 @@ RunResult RealTensorRundmrg::Result() {
     res.final_bond_dims.push_back(s);
   }
   return res;
+}
+
+// --------------------------------------------------------------------------
+// Tensor-space files
+// --------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+void put(std::ostream& os, const T& v) {
+  os.write(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+template <typename T>
+T get(std::istream& is) {
+  T v{};
+  is.read(reinterpret_cast<char*>(&v), sizeof(T));
+  require(bool(is), "tensor space file ends early", kErrArgument);
+  return v;
+}
+
+// RealMatrixBlock::WriteMatrixBlock (real_matrix_block.cc:227-240): block
+// count, then per block (left, right, RealMatrix::WriteMatrix: rows, cols,
+// rows*cols, the entries row-major)
+void write_blocks(std::ostream& os, const BlockTensor& t, hipStream_t stream) {
+  std::vector<double> h((size_t)t.doubles());
+  if (!h.empty()) {
+    DMRG_HIP(hipMemcpyAsync(h.data(), t.base(), h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DMRG_HIP(hipStreamSynchronize(stream));
+  }
+  put<int>(os, t.n());
+  for (const Block& b : t.blk) {
+    put<int>(os, b.l);
+    put<int>(os, b.r);
+    put<int>(os, b.rows);
+    put<int>(os, b.cols);
+    put<int>(os, b.rows * b.cols);
+    os.write(reinterpret_cast<const char*>(h.data() + t.cs * b.off),
+             (std::streamsize)(sizeof(double) * t.cs * b.rows * b.cols));
+  }
+}
+
+// RealMatrixBlock::ReadMatrixBlock (real_matrix_block.cc:254-283)
+void read_blocks(std::istream& is, BlockTensor& t, int cs, hipStream_t stream) {
+  const int n = get<int>(is);
+  require(n >= 0, "bad block count in tensor space file", kErrArgument);
+  std::vector<int> l(n), r(n);
+  std::vector<std::vector<double>> data(n);
+  std::vector<std::pair<int, int>> shape(n);
+  for (int k = 0; k < n; ++k) {
+    l[k] = get<int>(is);
+    r[k] = get<int>(is);
+    const int rows = get<int>(is), cols = get<int>(is), total = get<int>(is);
+    require(rows >= 0 && cols >= 0 && total == rows * cols, "bad matrix header in tensor space file",
+            kErrArgument);
+    shape[k] = {rows, cols};
+    data[k].resize((size_t)cs * total);
+    is.read(reinterpret_cast<char*>(data[k].data()), (std::streamsize)(sizeof(double) * data[k].size()));
+    require(bool(is), "tensor space file ends early", kErrArgument);
+  }
+  t = BlockTensor{};
+  t.cs = cs;
+  t.define(l, r);
+  for (int k = 0; k < n; ++k) {
+    t.blk[k].rows = shape[k].first;
+    t.blk[k].cols = shape[k].second;
+  }
+  t.allocate();
+  std::vector<double> h((size_t)t.doubles());
+  for (int k = 0; k < n; ++k) std::copy(data[k].begin(), data[k].end(), h.begin() + cs * t.blk[k].off);
+  if (!h.empty()) {
+    DMRG_HIP(hipMemcpyAsync(t.data.data(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    DMRG_HIP(hipStreamSynchronize(stream));
+  }
+}
+
+// RealTensorLattice::WriteTensorLattice (real_tensor_lattice.cc:218-239)
+void write_lattice(std::ostream& os, const Lattice& lat, hipStream_t stream) {
+  put<int>(os, lat.d);
+  put<int>(os, lat.nl());
+  for (int i = 0; i < lat.nl(); ++i) {
+    put<int>(os, lat.lb[i]);
+    put<int>(os, lat.ld[i]);
+  }
+  put<int>(os, lat.nr());
+  for (int j = 0; j < lat.nr(); ++j) {
+    put<int>(os, lat.rb[j]);
+    put<int>(os, lat.rd[j]);
+  }
+  for (int i = 0; i < lat.nl(); ++i)
+    for (int j = 0; j < lat.nr(); ++j) put<int>(os, lat.phys[i][j]);
+  write_blocks(os, lat.ket, stream);
+}
+
+// RealTensorLattice::ReadTensorLattice (real_tensor_lattice.cc:252-298);
+// the canonical tensor and match dims are not stored
+void read_lattice(std::istream& is, Lattice& lat, int cs, hipStream_t stream) {
+  lat = Lattice{};
+  lat.d = get<int>(is);
+  const int nl = get<int>(is);
+  require(nl >= 0, "bad sector count in tensor space file", kErrArgument);
+  std::vector<int> lb(nl), ld(nl);
+  for (int i = 0; i < nl; ++i) {
+    lb[i] = get<int>(is);
+    ld[i] = get<int>(is);
+  }
+  const int nr = get<int>(is);
+  require(nr >= 0, "bad sector count in tensor space file", kErrArgument);
+  std::vector<int> rb(nr), rd(nr);
+  for (int j = 0; j < nr; ++j) {
+    rb[j] = get<int>(is);
+    rd[j] = get<int>(is);
+  }
+  lat.define_sectors(lb, rb, ld, rd);
+  for (int i = 0; i < nl; ++i)
+    for (int j = 0; j < nr; ++j) lat.phys[i][j] = get<int>(is);
+  read_blocks(is, lat.ket, cs, stream);
+}
+
+}  // namespace
+
+void RealTensorRundmrg::WriteTensorSpace(const std::string& path) {
+  // RealTensorSpace::WriteTensorSpace (real_tensor_space.cc:182-208)
+  cx_.sync();
+  std::ofstream os(path, std::ios::binary | std::ios::out);
+  require(os.good(), "can't open tensor space file for writing: " + path, kErrArgument);
+  const int nq = space_.num_quantum();
+  put<int>(os, L_sites_);
+  put<int>(os, L_sites_ + 1);
+  put<int>(os, L_sites_ - 1);
+  put<int>(os, nq);
+  for (int i = 0; i <= L_sites_; ++i) put<int>(os, (int)tables_[i].size());
+  for (int i = 0; i <= L_sites_; ++i)
+    for (const QN& q : tables_[i])
+      for (int k = 0; k < nq; ++k) put<int>(os, q[k]);
+  for (int i = 0; i < L_sites_; ++i) write_lattice(os, lat_[i], cx_.stream);
+  put<int>(os, max_block_);
+  put<int>(os, max_dim_);
+  put<double>(os, canpre_);
+  put<double>(os, noise_);
+  os.flush();
+  require(os.good(), "write error on tensor space file: " + path, kErrInternal);
+}
+
+void RealTensorRundmrg::ReadTensorSpace(const std::string& path) {
+  // RealTensorSpace::ReadTensorSpace (real_tensor_space.cc:221-277).  The
+  // reference sizes two of its arrays by num_quantum where num_site+1 is
+  // meant (:245, :248); this reads the layout its writer produces.
+  std::ifstream is(path, std::ios::binary | std::ios::in);
+  require(is.good(), "can't open tensor space file: " + path, kErrArgument);
+  const int L = get<int>(is), Lpp = get<int>(is), Lmm = get<int>(is), nq = get<int>(is);
+  require(L == L_sites_ && Lpp == L + 1 && Lmm == L - 1, "tensor space file has another number of sites",
+          kErrArgument);
+  require(nq == space_.num_quantum(), "tensor space file has another number of quantum numbers",
+          kErrArgument);
+  std::vector<int> nt(Lpp);
+  for (int i = 0; i < Lpp; ++i) {
+    nt[i] = get<int>(is);
+    require(nt[i] >= 0, "bad quantum table size in tensor space file", kErrArgument);
+  }
+  std::vector<QNTable> tabs(Lpp);
+  for (int i = 0; i < Lpp; ++i)
+    for (int j = 0; j < nt[i]; ++j) {
+      QN q(nq);
+      for (int k = 0; k < nq; ++k) q[k] = get<int>(is);
+      tabs[i].push_back(q);
+    }
+  std::vector<Lattice> lats(L);
+  for (int i = 0; i < L; ++i) {
+    read_lattice(is, lats[i], cs_, cx_.stream);
+    require(lats[i].d == d_, "tensor space file has another physical dimension", kErrArgument);
+  }
+  max_block_ = get<int>(is);
+  max_dim_ = get<int>(is);
+  canpre_ = get<double>(is);
+  noise_ = get<double>(is);
+  tables_ = tabs;
+  lat_ = std::move(lats);
 }
 
 RunResult RealTensorRundmrg::Run() {

@@ -160,6 +160,10 @@ struct RunParams {
   int precision = kFP64;  // H_eff arithmetic inside Lanczos
   int refine = 1;         // mixed precision: final FP64 Rayleigh quotient
   int verbose = 0;
+  // cache_resume: start from this tensor-space file instead of
+  // DefineTensorSpace (real_tensor_rundmrg.cc:44-52); cache_record: write
+  // the tensor space to this file after every sweep (:231-235)
+  std::string resume_path, record_path;
 };
 
 // Wall time per sweep phase (seconds, host clock around device-synchronised
@@ -205,6 +209,16 @@ class RealTensorRundmrg {
   void BuildHeff(int site, HeffPlan& plan) {
     plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
   }
+
+  // Tensor-space files (RealTensorSpace::Write/ReadTensorSpace,
+  // real_tensor_space.cc:171-277): the bond QN tables, every site lattice
+  // (real_tensor_lattice.cc:207-298) with its ket blocks
+  // (real_matrix_block.cc:216-283, real_matrix.cc:128-175; complex entries
+  // are the 16-byte Complex, complex_matrix.cc:139-175) and the current
+  // max_block, max_dim, canonical precision and noise factor.  Native
+  // byte order, as the reference writes it.
+  void WriteTensorSpace(const std::string& path);
+  void ReadTensorSpace(const std::string& path);
 
   // pieces exposed for tests
   void DefineTensorSpace(int first_block, int first_dim);

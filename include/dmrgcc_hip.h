@@ -110,6 +110,10 @@ int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dm
  * cache_resume, record_process. */
 int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
                             int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags);
+/* One string value of the "input" group (InputGroup::GetString,
+ * input.cc:223-244; a key with nothing after '=' is found and empty):
+ * the value into buf (NUL-terminated, buflen bytes), *found = 0/1. */
+int dmrgcc_input_string(const char* path, const char* key, char* buf, int buflen, int* found);
 int dmrgcc_model_destroy(dmrgcc_model* m);
 int dmrgcc_model_num_site(const dmrgcc_model* m);
 /* Number of sites, local dimension, QNs per sector and whether the MPO is
@@ -265,6 +269,17 @@ int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
                double* seconds, double* stats);
+/* dmrgcc_run with the reference's checkpoint files
+ * (real_tensor_rundmrg.cc:44-52, :231-235): resume_path != NULL starts from
+ * that tensor-space file instead of DefineTensorSpace (cache_resume);
+ * record_path != NULL writes the tensor space there after every sweep
+ * (cache_record).  The file format is RealTensorSpace::WriteTensorSpace's
+ * (real_tensor_space.cc:182-208; see dmrgcc_dmrg_write_space). */
+int dmrgcc_run_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                     const int* maxb, const int* niter, const double* canpre, const double* noise,
+                     int precision, int refine, int verbose, const char* resume_path,
+                     const char* record_path, double* energies, double* trunc, double* seconds,
+                     double* stats);
 
 /* ---- resumable sweep (RealTensorRundmrg, real_tensor_rundmrg.cc:3-287) -- */
 /* A handle owns the MPS, environments and rand() stream of one run.
@@ -279,6 +294,19 @@ typedef struct dmrgcc_dmrg dmrgcc_dmrg;
 int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                        const int* maxb, const int* niter, const double* canpre, const double* noise,
                        int precision, int refine, int verbose, dmrgcc_dmrg** out);
+int dmrgcc_dmrg_create_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                             const int* maxb, const int* niter, const double* canpre, const double* noise,
+                             int precision, int refine, int verbose, const char* resume_path,
+                             const char* record_path, dmrgcc_dmrg** out);
+/* RealTensorSpace::WriteTensorSpace (real_tensor_space.cc:171-208) of the
+ * current state: num_site, num_site+1, num_site-1, num_quantum, the table
+ * sizes and quantum tables, then per site WriteTensorLattice
+ * (real_tensor_lattice.cc:218-239: physics_dim, left sectors (index, dim),
+ * right sectors, physics_index, ket blocks as WriteMatrixBlock /
+ * WriteMatrix), then max_block, max_dim (int) and canonical precision,
+ * noise factor (double).  int is 4 bytes, entries 8-byte doubles or 16-byte
+ * Complex, native byte order. */
+int dmrgcc_dmrg_write_space(dmrgcc_dmrg* h, const char* path);
 int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached);
 int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine);
 int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);

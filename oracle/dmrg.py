@@ -1081,7 +1081,8 @@ def sweep_table(rows, num_sweep):
 class DMRG:
     """RealTensorRundmrg + RealTensorNetwork + RealTensorLanczos."""
 
-    def __init__(self, model, sweeps, num_sweep, first_block=20, first_dim=10, seed=1, trace=False):
+    def __init__(self, model, sweeps, num_sweep, first_block=20, first_dim=10, seed=1, trace=False,
+                 resume=None):
         self.model = model
         self.rng = GlibcRand(seed)
         self.space = Space(model, self.rng)
@@ -1091,7 +1092,11 @@ class DMRG:
         self.trace = trace
         if trace:
             self.space.tracer = self._trace
-        self.space.define(first_block, first_dim)
+        if resume is not None:  # cache_resume (real_tensor_rundmrg.cc:44-52)
+            from .io import read_space
+            read_space(resume, self.space)
+        else:
+            self.space.define(first_block, first_dim)
         # tensor_contraction_[site] holds L (left_bond of W[site]) and R
         # (right_bond of W[site]) (real_tensor_network.cc:3-33)
         self.Lenv = [None] * (self.L + 1)
