@@ -58,6 +58,9 @@ struct GemmTile {
 // Summed-K threshold above which finalize() splits an item: 512, or the
 // value of the environment variable DMRGCC_SPLIT_K (0 disables splitting).
 int default_split_k();
+// Tile shape of finalize(): the environment variable DMRGCC_TILE (32 or 64;
+// 0 when unset, which means 64).
+int default_tile_mode();
 
 // Split-K of one heavy output item: partial sums of chunks 1..n-1 land in
 // `part` (n-1 consecutive M x N buffers) and are added into C afterwards in
@@ -86,6 +89,10 @@ struct GemmBatch {
   DevBuf<SplitTask> d_splits;
   DevBuf<double> part;
   int split_k = default_split_k();
+  // 32: 32x32 wave tiles (four independent tiles per workgroup); otherwise
+  // 64x64 workgroup tiles (environment DMRGCC_TILE)
+  int tile_mode = default_tile_mode();
+  bool wave_tiles = false;  // set by finalize: four 32x32 tiles per workgroup
   double flops = 0;  // executed real flops (2*M*N*K summed)
   // Groups: end_group() closes the items added since the previous call into
   // one group that launch_group() runs on its own (a chain of dependent

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <numeric>
 
 #include "kernels.h"
 
@@ -371,6 +372,250 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
       }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-tile GEMM kernels: every wave of the 256-thread workgroup owns one
+// 32x32 output tile (tiles[4*blockIdx.x + wave]) with its own K walk and its
+// own LDS slice, so the four tiles of a workgroup may belong to different
+// items and run different K lengths.  No block-level barrier: a wave stores
+// its staged operands, fences its own LDS traffic and reads them back.  For
+// block-sparse operands with 50-600 rows this wastes less of every tile than
+// 64x64 tiles and gives a launch four times as many independent tiles.
+// Staging of one K-step (32 x 32 of A, 32 x 32 of B, 16 elements each per
+// lane), element q of lane l:
+//   A, not transposed: m = l/2,  k = 16*(l%2) + q      (a row's 16 k)
+//   A, transposed:     m = l%32, k = 16*(l/32) + q     (32 lanes read 32 consecutive m)
+//   B, not transposed: n = l%32, k = 16*(l/32) + q
+//   B, transposed:     n = l/2,  k = 16*(l%2) + q
+// ---------------------------------------------------------------------------
+constexpr int kWT = 32;  // wave tile edge
+
+struct WaveRegs {
+  double a[16], b[16];
+};
+
+__device__ inline void wave_load(const GemmContrib& cb, const double* __restrict__ in_base, int m0,
+                                 int n0, int M, int N, int k0, int lane, WaveRegs& r) {
+  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  const int K = cb.K;
+  const double sc = cb.scale;
+  const int es = (cb.flags & kStrided2) ? 2 : 1;
+  if (!(cb.flags & kTransA)) {
+    const int gm = m0 + (lane >> 1), kb = k0 + (lane & 1) * 16;
+    const double* row = A + (int64_t)gm * cb.lda;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
+  } else {
+    const int gm = m0 + (lane & 31), kb = k0 + (lane >> 5) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm * es] * sc : 0.0;
+  }
+  if (!(cb.flags & kTransB)) {
+    const int gn = n0 + (lane & 31), kb = k0 + (lane >> 5) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn * es] : 0.0;
+  } else {
+    const int gn = n0 + (lane >> 1), kb = k0 + (lane & 1) * 16;
+    const double* row = B + (int64_t)gn * cb.ldb;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
+  }
+}
+
+// LDS traffic of this wave ordered before what follows (other lanes' stores
+// visible to this lane's loads, and loads done before the next stores)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ inline void wave_store_out(const GemmItem& it, double* __restrict__ out_base, int m0, int n0,
+                                      int lane, const typename MfmaAcc<T>::V (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + a * 16 + MfmaAcc<T>::row(lane, v);
+        const int gn = n0 + b * 16 + (lane & 15);
+        if (gm < it.M && gn < it.N) {
+          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
+                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
+          const double r2 = (double)acc[a][b][v];
+          *p = it.accumulate ? (*p + r2) : r2;
+        }
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemm_wave_kernel(
+    const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
+    const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
+    double* __restrict__ out_base) {
+  constexpr int LD = kWT + 2;
+  __shared__ T As[4][kKT][LD];  // k-major, per wave
+  __shared__ T Bs[4][kKT][LD];
+  typedef typename MfmaAcc<T>::V V;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const GemmTile tl = tiles[blockIdx.x * 4 + wave];
+  if (tl.item < 0) return;  // no block barriers: a wave without a tile leaves
+  const GemmItem it = items[tl.item];
+  const int M = it.M, N = it.N, m0 = tl.m0, n0 = tl.n0;
+  T (*as)[LD] = As[wave];
+  T (*bs)[LD] = Bs[wave];
+
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = V{0, 0, 0, 0};
+  int c = it.c0, k0 = 0;
+  GemmContrib cb{};
+  bool live = false;
+  for (; c < it.c1; ++c) {
+    cb = cons[c];
+    if (cb.K > 0) { live = true; break; }
+  }
+  WaveRegs r;
+  if (live) wave_load(cb, in_base, m0, n0, M, N, 0, lane, r);
+  while (live) {
+    if (!(cb.flags & kTransA)) {
+      const int m = lane >> 1, kb = (lane & 1) * 16;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) as[kb + q][m] = (T)r.a[q];
+    } else {
+      const int m = lane & 31, kb = (lane >> 5) * 16;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) as[kb + q][m] = (T)r.a[q];
+    }
+    if (!(cb.flags & kTransB)) {
+      const int n = lane & 31, kb = (lane >> 5) * 16;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) bs[kb + q][n] = (T)r.b[q];
+    } else {
+      const int n = lane >> 1, kb = (lane & 1) * 16;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) bs[kb + q][n] = (T)r.b[q];
+    }
+    wave_lds_sync();
+    GemmContrib nc = cb;
+    int c2 = c, k2 = k0;
+    live = next_step(cons, it.c1, c2, k2, nc);
+    if (live) wave_load(nc, in_base, m0, n0, M, N, k2, lane, r);  // in flight during the MFMAs
+#pragma unroll
+    for (int ks = 0; ks < kKT; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      T af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = as[kr][a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bf[b] = bs[kr][b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = MfmaAcc<T>::mma(af[a], bf[b], acc[a][b]);
+    }
+    wave_lds_sync();
+    c = c2; k0 = k2; cb = nc;
+  }
+  wave_store_out<T>(it, out_base, m0, n0, lane, acc);
+}
+
+// BF16x3 wave-tile kernel: per-wave hi/lo planes m-major with k contiguous,
+// as in gemm_bf16x3_kernel; each lane writes its 16 k as two 16-byte stores.
+__global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
+    const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
+    const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
+    double* __restrict__ out_base) {
+  constexpr int LD = kKT + 8;
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[4][kWT][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[4][kWT][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[4][kWT][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[4][kWT][LD];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const GemmTile tl = tiles[blockIdx.x * 4 + wave];
+  if (tl.item < 0) return;
+  const GemmItem it = items[tl.item];
+  const int M = it.M, N = it.N, m0 = tl.m0, n0 = tl.n0;
+  __bf16 (*ah_)[LD] = Ah[wave];
+  __bf16 (*al_)[LD] = Al[wave];
+  __bf16 (*bh_)[LD] = Bh[wave];
+  __bf16 (*bl_)[LD] = Bl[wave];
+
+  f4v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4v{0, 0, 0, 0};
+  int c = it.c0, k0 = 0;
+  GemmContrib cb{};
+  bool live = false;
+  for (; c < it.c1; ++c) {
+    cb = cons[c];
+    if (cb.K > 0) { live = true; break; }
+  }
+  WaveRegs r;
+  if (live) wave_load(cb, in_base, m0, n0, M, N, 0, lane, r);
+  while (live) {
+    {
+      const int am = (cb.flags & kTransA) ? (lane & 31) : (lane >> 1);
+      const int ak = (cb.flags & kTransA) ? (lane >> 5) * 16 : (lane & 1) * 16;
+      const int bn = (cb.flags & kTransB) ? (lane >> 1) : (lane & 31);
+      const int bk = (cb.flags & kTransB) ? (lane & 1) * 16 : (lane >> 5) * 16;
+      bf8v h, l;
+      split8(r.a, h, l);
+      *reinterpret_cast<bf8v*>(&ah_[am][ak]) = h;
+      *reinterpret_cast<bf8v*>(&al_[am][ak]) = l;
+      split8(r.a + 8, h, l);
+      *reinterpret_cast<bf8v*>(&ah_[am][ak + 8]) = h;
+      *reinterpret_cast<bf8v*>(&al_[am][ak + 8]) = l;
+      split8(r.b, h, l);
+      *reinterpret_cast<bf8v*>(&bh_[bn][bk]) = h;
+      *reinterpret_cast<bf8v*>(&bl_[bn][bk]) = l;
+      split8(r.b + 8, h, l);
+      *reinterpret_cast<bf8v*>(&bh_[bn][bk + 8]) = h;
+      *reinterpret_cast<bf8v*>(&bl_[bn][bk + 8]) = l;
+    }
+    wave_lds_sync();
+    GemmContrib nc = cb;
+    int c2 = c, k2 = k0;
+    live = next_step(cons, it.c1, c2, k2, nc);
+    if (live) wave_load(nc, in_base, m0, n0, M, N, k2, lane, r);  // in flight during the MFMAs
+    const int kc = 8 * (lane >> 4);
+    bf8v ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int rr = a * 16 + (lane & 15);
+      ah[a] = *reinterpret_cast<const bf8v*>(&ah_[rr][kc]);
+      al[a] = *reinterpret_cast<const bf8v*>(&al_[rr][kc]);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int rr = b * 16 + (lane & 15);
+      bh[b] = *reinterpret_cast<const bf8v*>(&bh_[rr][kc]);
+      bl[b] = *reinterpret_cast<const bf8v*>(&bl_[rr][kc]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    wave_lds_sync();
+    c = c2; k0 = k2; cb = nc;
+  }
+  wave_store_out<float>(it, out_base, m0, n0, lane, acc);
+}
+
 // split-K reduction: C (+)= sum of the partial buffers, in chunk order
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __restrict__ tasks,
                                                            double* __restrict__ out_base) {
@@ -622,12 +867,6 @@ constexpr int kJw = DMRG_JACOBI_WAVES;
 constexpr int kJn = kJc / 16;  // 16 x 16 MFMA blocks per visit edge
 static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
 static_assert(kJw >= 1 && kJw <= 16, "1 to 16 waves per visit");
-
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Slabs of 16 rows handled together by one wave (their loads in flight at once)
 constexpr int kSlabs = 2;
@@ -1415,6 +1654,13 @@ int default_split_k() {
   return e ? std::max(0, std::atoi(e)) : 256;
 }
 
+int default_tile_mode() {
+  const char* e = std::getenv("DMRGCC_TILE");
+  if (!e) return 0;
+  const int v = std::atoi(e);
+  return (v == 32 || v == 64) ? v : 0;
+}
+
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   tiles.clear();
   splits.clear();
@@ -1483,9 +1729,48 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
   // cost, largest first) so the tiles that share A rows and B columns share
   // an L2; the queues are interleaved into the launch order, padded with
   // empty tiles (item -1) where a queue runs short.
+  // Tile shape: 64x64 workgroup tiles, or 32x32 wave tiles (four per
+  // workgroup, DMRGCC_TILE = 32).  Wave tiles waste less of each padded tile
+  // on small blocks but read twice the operand bytes per flop; on the
+  // H_eff launches measured here the 64x64 form is the faster (BF16x3:
+  // 0.105 vs 0.158 ms per C2 matvec), so it is the default.
+  wave_tiles = group_items.empty() && tile_m == kTile && tile_n == kTile && tile_mode == 32;
   std::vector<int> ends = group_items;
   if (ends.empty()) ends.push_back((int)items.size());
   int first = 0;
+  if (wave_tiles) {
+    // 32x32 tiles in per-XCD queues as below, cut into runs of four (one
+    // workgroup); workgroup w runs on XCD w % kXcds
+    std::vector<int> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<GemmTile>> q(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    for (int i : order) {
+      const GemmItem& it = items[i];
+      if (it.M <= 0 || it.N <= 0) continue;
+      if (it.c0 == it.c1 && it.accumulate) continue;
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      int nt = 0;
+      for (int m0 = 0; m0 < it.M; m0 += kWT)
+        for (int n0 = 0; n0 < it.N; n0 += kWT, ++nt) q[x].push_back(GemmTile{i, m0, n0, 0});
+      load[x] += (cost[i] + 32.0) * nt;
+    }
+    size_t depth = 0;
+    for (auto& v : q) {
+      while (v.size() % 4) v.push_back(GemmTile{-1, 0, 0, 0});
+      depth = std::max(depth, v.size() / 4);
+    }
+    for (size_t k = 0; k < depth; ++k)
+      for (int x = 0; x < kXcds; ++x)
+        for (int w = 0; w < 4; ++w)
+          tiles.push_back(4 * k + w < q[x].size() ? q[x][4 * k + w] : GemmTile{-1, 0, 0, 0});
+    while (tiles.size() >= 4 && tiles[tiles.size() - 1].item < 0 && tiles[tiles.size() - 2].item < 0 &&
+           tiles[tiles.size() - 3].item < 0 && tiles[tiles.size() - 4].item < 0)
+      tiles.resize(tiles.size() - 4);
+    group_tiles.push_back((int)tiles.size());
+    ends.clear();
+  }
   for (int end : ends) {
     std::vector<int> order;
     for (int i = first; i < end; ++i) order.push_back(i);
@@ -1522,18 +1807,31 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
 void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out,
                        bool reduce) const {
   if (tiles.empty()) return;
-  const dim3 grid((unsigned)tiles.size()), block(kThreads);
-  if (precision == kFP64)
-    hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, d_tiles.data(),
-                       d_items.data(), d_contribs.data(), in, out);
-  else if (precision == kFP32)
-    hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, d_tiles.data(),
-                       d_items.data(), d_contribs.data(), in, out);
-  else if (precision == kBF16x3)
-    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
-                       d_contribs.data(), in, out);
-  else
-    throw Error(kErrArgument, "unknown precision");
+  require(precision >= kFP64 && precision <= kBF16x3, "unknown precision", kErrArgument);
+  const dim3 block(kThreads);
+  if (wave_tiles) {
+    const dim3 grid((unsigned)(tiles.size() / 4));
+    if (precision == kFP64)
+      hipLaunchKernelGGL(gemm_wave_kernel<double>, grid, block, 0, stream, d_tiles.data(), d_items.data(),
+                         d_contribs.data(), in, out);
+    else if (precision == kFP32)
+      hipLaunchKernelGGL(gemm_wave_kernel<float>, grid, block, 0, stream, d_tiles.data(), d_items.data(),
+                         d_contribs.data(), in, out);
+    else
+      hipLaunchKernelGGL(gemm_bf16x3_wave_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
+                         d_contribs.data(), in, out);
+  } else {
+    const dim3 grid((unsigned)tiles.size());
+    if (precision == kFP64)
+      hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, d_tiles.data(),
+                         d_items.data(), d_contribs.data(), in, out);
+    else if (precision == kFP32)
+      hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, d_tiles.data(),
+                         d_items.data(), d_contribs.data(), in, out);
+    else
+      hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
+                         d_contribs.data(), in, out);
+  }
   DMRG_HIP(hipGetLastError());
   if (reduce && !splits.empty()) {
     int64_t mx = 0;

@@ -15,8 +15,11 @@ SHAPES = [(1, 1, 1), (3, 5, 2), (16, 16, 4), (17, 33, 65), (64, 64, 64), (65, 1,
 TOL = {dm.FP64: 1e-13, dm.FP32: 1e-5, dm.BF16X3: 5e-5}
 
 
+@pytest.mark.parametrize("tile", ["32", "64"])
 @pytest.mark.parametrize("prec", [dm.FP64, dm.FP32, dm.BF16X3])
-def test_block_gemm_matches_numpy(prec):
+def test_block_gemm_matches_numpy(monkeypatch, prec, tile):
+    # tile: 32x32 wave tiles or 64x64 workgroup tiles (DMRGCC_TILE)
+    monkeypatch.setenv("DMRGCC_TILE", tile)
     rng = np.random.default_rng(5)
     As, Bs, ref = [], [], []
     for (M, N, K) in SHAPES:
@@ -36,9 +39,11 @@ def test_block_gemm_matches_numpy(prec):
         assert err < TOL[prec], (M, N, K, err)
 
 
-def test_block_gemm_fp64_is_exact_on_integers():
+@pytest.mark.parametrize("tile", ["32", "64"])
+def test_block_gemm_fp64_is_exact_on_integers(monkeypatch, tile):
     # small integers: every product and partial sum is exact in FP64, so the
     # MFMA result must equal numpy bit for bit (catches any lane/layout slip)
+    monkeypatch.setenv("DMRGCC_TILE", tile)
     rng = np.random.default_rng(1)
     for (M, N, K) in SHAPES:
         a = rng.integers(-3, 4, (M, K)).astype(float)

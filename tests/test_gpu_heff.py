@@ -59,13 +59,16 @@ def test_heff_fp64_matches_oracle(name):
     assert info["flops_ref"] == heff_flops(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site])
 
 
+@pytest.mark.parametrize("tile", ["32", "64"])
 @pytest.mark.parametrize("split_k", [0, 1, 40])
 @pytest.mark.parametrize("prec", [dm.FP64, dm.BF16X3])
-def test_heff_split_k(monkeypatch, split_k, prec):
+def test_heff_split_k(monkeypatch, split_k, prec, tile):
     # split-K cuts heavy output blocks into contribution chunks that are summed
-    # by a second kernel; any threshold must give the unsplit result
+    # by a second kernel; any threshold must give the unsplit result, with
+    # 32x32 wave tiles or 64x64 workgroup tiles
     dmo, site = state("4x3_d4_left")
     monkeypatch.setenv("DMRGCC_SPLIT_K", str(split_k))
+    monkeypatch.setenv("DMRGCC_TILE", tile)
     plan, lat = make_plan(dmo, site)
     n = lat.ket.dim()
     x = np.random.default_rng(6).standard_normal(n)
