@@ -23,6 +23,9 @@ One "step" is one application of the centre-site effective Hamiltonian
   roofline     stage-2 kernel (the dominant launch) against the MFMA peak;
   cpu_baseline the oracle's numpy FP64 matvec on the identical state.
 
+--workload large runs the 10x6 cylinder (L = 60, N = 60, d = 4, t = 1,
+U = 8, periodic y) at m = 4000, where sector blocks reach ~1000 x 1000.
+
 --workload spinful measures the complex engine instead (not the default
 line): two-component Bose-Hubbard on a 4x4 open lattice with flux 1/4 per
 plaquette (32 MPS sites, N_up = N_down = 8, max occupation 2 per species,
@@ -152,7 +155,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--sweeps", type=int, default=8, help="sweeps of the m=1000 table")
     ap.add_argument("--maxm", type=int, default=None, help="bond dimension (1000; spinful 2000)")
-    ap.add_argument("--workload", choices=("cylinder", "spinful"), default="cylinder")
+    ap.add_argument("--workload", choices=("cylinder", "spinful", "large"), default="cylinder")
     ap.add_argument("--no-mixed-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -170,6 +173,11 @@ def main():
         U_std = 6.0
         model = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U_std / 2, period_y=True)
         workload = ("8x4 Bose-Hubbard cylinder (L=32, N=32, d=4, t=1, U=6, periodic y), "
+                    "m=%d, centre-site (%d) H_eff matvec")
+    elif args.workload == "large":
+        args.maxm = args.maxm or 4000
+        model = dm.SpinlessModel(10, 6, 60, 4, hop_x=-1.0, hop_y=-1.0, U=8.0 / 2, period_y=True)
+        workload = ("10x6 Bose-Hubbard cylinder (L=60, N=60, d=4, t=1, U=8, periodic y), "
                     "m=%d, centre-site (%d) H_eff matvec")
     else:
         args.maxm = args.maxm or 2000
