@@ -116,6 +116,8 @@ _SIGS = {
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                                 ctypes.POINTER(_vp)]),
     "dmrgcc_dmrg_write_space": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "dmrgcc_dmrg_heff2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_heff_state": (ctypes.c_int, [_vp, _vp]),
     "dmrgcc_dmrg_advance": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p]),
     "dmrgcc_dmrg_set_precision": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "dmrgcc_dmrg_heff": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -464,6 +466,18 @@ class DmrgRun:
         h = _vp()
         _check(lib().dmrgcc_dmrg_heff(self._h, site, ctypes.byref(h)))
         return HeffPlan._from_handle(h, self)
+
+    def heff2(self, site):
+        """Two-site H_eff of (site, site+1) on the current state (a HeffPlan)
+        and its two-site state Theta = A_site B_site+1 as a device tensor."""
+        import torch
+        h = _vp()
+        _check(lib().dmrgcc_dmrg_heff2(self._h, site, ctypes.byref(h)))
+        plan = HeffPlan._from_handle(h, self)
+        n = plan.info()["psi_dim"]
+        theta = torch.empty(n * (2 if plan.complex else 1), dtype=torch.float64, device="cuda")
+        _check(lib().dmrgcc_heff_state(h, _ptr(theta)))
+        return plan, theta
 
     def ket(self, site):
         p = _vp()

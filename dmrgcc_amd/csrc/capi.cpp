@@ -29,6 +29,7 @@ struct dmrgcc_heff {
   Env L, R;
   HeffPlan plan;
   LanczosStats st;
+  TwoSite two;  // the two-site state of a dmrgcc_dmrg_heff2 plan
 };
 
 struct dmrgcc_dmrg {
@@ -839,6 +840,26 @@ int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     e->cx = std::make_unique<Context>(nullptr);
     h->run->BuildHeff(site, e->plan);
     *out = e.release();
+  });
+}
+
+int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
+  return guarded([&] {
+    require(h && out && site >= 0 && site + 1 < h->run->num_site(), "bad arguments", kErrArgument);
+    auto e = std::make_unique<dmrgcc_heff>();
+    e->cx = std::make_unique<Context>(nullptr);
+    h->run->BuildTwoSite(site, e->two, e->plan);
+    *out = e.release();
+  });
+}
+
+int dmrgcc_heff_state(const dmrgcc_heff* h, double* dst) {
+  return guarded([&] {
+    require(h && dst, "null argument", kErrArgument);
+    require(h->two.theta.size > 0, "no two-site state on this plan", kErrArgument);
+    DMRG_HIP(hipMemcpyAsync(dst, h->two.theta.base(), h->two.theta.doubles() * sizeof(double),
+                            hipMemcpyDeviceToDevice, h->cx->stream));
+    DMRG_HIP(hipStreamSynchronize(h->cx->stream));
   });
 }
 

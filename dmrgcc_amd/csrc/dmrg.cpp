@@ -316,16 +316,24 @@ void sym_eig(int n, std::vector<double> a, std::vector<double>& eval, std::vecto
 // ===========================================================================
 void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, const Env& R,
                      hipStream_t stream) {
+  std::vector<int> bphys(lat.ket.n());
+  for (int k = 0; k < lat.ket.n(); ++k) bphys[k] = lat.phys[lat.ket.blk[k].l][lat.ket.blk[k].r];
+  build_blocks(lat.ket, bphys, lat.nr(), W, L, R, stream);
+}
+
+void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphys, int nr,
+                            const SiteOperator& W, const Env& L, const Env& R, hipStream_t stream) {
+  // Blocks (l, r) of the state with physical index bphys[k]; several blocks
+  // may share a sector pair (the two-site state, one block per middle sector).
   s1_.clear();
   s2_.clear();
-  const BlockTensor& ket = lat.ket;
   psi_dim_ = ket.size;
   // complex plans (ComplexTensorContraction::MultiplyEffectHamilton,
   // complex_tensor_contraction.cc:1244-1312): same task structure, every
   // complex product expanded into real MFMA terms on interleaved views
   cplx_ = ket.cs == 2;
-  std::vector<std::vector<int>> same_r(lat.nr());
-  for (int r = 0; r < lat.nr(); ++r) same_r[r] = ket.same(1, r);
+  std::vector<std::vector<int>> same_r(nr);
+  for (int r = 0; r < nr; ++r) same_r[r] = ket.same(1, r);
 
   struct Out { int o1, lp; double w, wi; };
   struct Task { int o2, ri, kp, jdx; int64_t toff; std::vector<Out> outs; std::vector<int> ops; };
@@ -340,13 +348,13 @@ void HeffPlan::build(const Lattice& lat, const SiteOperator& W, const Env& L, co
       const int ldx = rbk.l, jdx = rbk.r;
       for (int kp : same_r[ldx]) {
         const int kdx = ket.blk[kp].l;
-        const int p2 = lat.phys[kdx][ldx];
+        const int p2 = bphys[kp];
         if (!W.right_nonzero(o2, p2)) continue;
         Task t;
         t.o2 = o2; t.ri = ri; t.kp = kp; t.jdx = jdx;
         for (int op : same_r[jdx]) {
           const int idx = ket.blk[op].l;
-          const int p1 = lat.phys[idx][jdx];
+          const int p1 = bphys[op];
           for (int o1 = 0; o1 < W.lb; ++o1) {
             if (W.used(o1, o2, p1, p2)) {
               const int lp = L[o1].find(idx, kdx);
@@ -1920,6 +1928,92 @@ RunResult RealTensorRundmrg::Result() {
     res.final_bond_dims.push_back(s);
   }
   return res;
+}
+
+// --------------------------------------------------------------------------
+// Two-site effective Hamiltonian
+// --------------------------------------------------------------------------
+SiteOperator fuse_sites(const SiteOperator& a, const SiteOperator& b) {
+  require(a.rb == b.lb && a.d == b.d, "MPO bond dimensions do not match in fuse_sites", kErrShape);
+  const bool cz = a.complex() || b.complex();
+  const int d = a.d, D = d * d;
+  SiteOperator w;
+  w.lb = a.lb;
+  w.rb = b.rb;
+  w.d = D;
+  w.w.assign((size_t)w.lb * w.rb * D * D, 0.0);
+  if (cz) w.wi.assign(w.w.size(), 0.0);
+  for (int o1 = 0; o1 < a.lb; ++o1)
+    for (int o3 = 0; o3 < b.rb; ++o3)
+      for (int p1 = 0; p1 < d; ++p1)
+        for (int q1 = 0; q1 < d; ++q1)
+          for (int p2 = 0; p2 < d; ++p2)
+            for (int q2 = 0; q2 < d; ++q2) {
+              double re = 0.0, im = 0.0;
+              for (int o2 = 0; o2 < a.rb; ++o2) {
+                const double ar = a.at(o1, o2, p1, q1), ai = a.im(o1, o2, p1, q1);
+                const double br = b.at(o2, o3, p2, q2), bi = b.im(o2, o3, p2, q2);
+                re += ar * br - ai * bi;
+                im += ar * bi + ai * br;
+              }
+              const size_t k = (((size_t)o1 * w.rb + o3) * D + (p1 * d + p2)) * D + (q1 * d + q2);
+              w.w[k] = re;
+              if (cz) w.wi[k] = im;
+            }
+  return w;
+}
+
+void RealTensorRundmrg::BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan) {
+  require(site >= 0 && site + 1 < L_sites_, "two-site pair out of range", kErrArgument);
+  const Lattice& A = lat_[site];
+  const Lattice& B = lat_[site + 1];
+  const int d = d_;
+  require(A.nr() == B.nl(), "middle bond sectors differ between the two sites", kErrShape);
+  // Theta blocks: for each block (l, m) of A in its order, the blocks (m, r)
+  // of B in B's order; physical index p1 * d + p2
+  BlockTensor& th = ts.theta;
+  th = BlockTensor{};
+  th.cs = cs_;
+  std::vector<int> li, ri;
+  std::vector<std::pair<int, int>> src;
+  ts.bphys.clear();
+  for (int a = 0; a < A.ket.n(); ++a) {
+    const Block& ab = A.ket.blk[a];
+    for (int b : B.ket.same(0, ab.r)) {
+      const Block& bb = B.ket.blk[b];
+      require(ab.cols == bb.rows, "Matrix indeces do not match in MultiplyToMatrix (A*B)");
+      li.push_back(ab.l);
+      ri.push_back(bb.r);
+      src.push_back({a, b});
+      ts.bphys.push_back(A.phys[ab.l][ab.r] * d + B.phys[bb.l][bb.r]);
+    }
+  }
+  th.define(li, ri);
+  for (size_t k = 0; k < src.size(); ++k) {
+    th.blk[k].rows = A.ket.blk[src[k].first].rows;
+    th.blk[k].cols = B.ket.blk[src[k].second].cols;
+  }
+  th.allocate();
+  GemmBatch g;
+  for (size_t k = 0; k < src.size(); ++k) {
+    const Block& ab = A.ket.blk[src[k].first];
+    const Block& bb = B.ket.blk[src[k].second];
+    const Block& tb = th.blk[k];
+    if ((int64_t)tb.rows * tb.cols == 0) continue;
+    if (cs_ == 2) {
+      zgemm_item(g, th.ptr((int)k), tb.cols, tb.rows, tb.cols, 0, 0,
+                 {ZTerm{A.ket.ptr(src[k].first), ab.cols, B.ket.ptr(src[k].second), bb.cols, ab.cols, 0, false,
+                        false, 1.0, 0.0}});
+    } else {
+      g.begin_item(th.ptr((int)k), tb.cols, tb.rows, tb.cols, 0);
+      g.add(A.ket.ptr(src[k].first), ab.cols, B.ket.ptr(src[k].second), bb.cols, ab.cols, 0, 1.0);
+    }
+  }
+  g.finalize(cx_.stream);
+  g.launch(cx_.stream, kFP64);
+  ts.W2 = fuse_sites(ham_.sites()[site], ham_.sites()[site + 1]);
+  plan.build_blocks(th, ts.bphys, B.nr(), ts.W2, L_[site], R_[site + 1], cx_.stream);
+  cx_.sync();
 }
 
 // --------------------------------------------------------------------------

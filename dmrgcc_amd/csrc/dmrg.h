@@ -90,6 +90,11 @@ class HeffPlan {
  public:
   void build(const Lattice& lat, const SiteOperator& W, const Env& L, const Env& R,
              hipStream_t stream);
+  // General form: state blocks (l, r) with physical index bphys[k] (nr right
+  // sectors); blocks may share a sector pair.  The two-site plan uses it
+  // with the fused MPO (fuse_sites) and p = p1 * d + p2.
+  void build_blocks(const BlockTensor& ket, const std::vector<int>& bphys, int nr, const SiteOperator& W,
+                    const Env& L, const Env& R, hipStream_t stream);
   // y (+)= H x ; x, y in ket-vector layout (psi_dim elements).
   void apply(const double* x, double* y, int precision, hipStream_t stream);
   // launch one stage only (timing); gemm_only skips the split-K reduction
@@ -112,6 +117,19 @@ class HeffPlan {
   int64_t psi_dim_ = 0;
   int64_t bytes_min_ = 0;
   double fref_ = 0;
+};
+
+// Two-site MPO W_j W_{j+1} contracted over the shared bond:
+// W[o1][o3](p1 d + p2, q1 d + q2) = sum_o2 W_j[o1][o2](p1, q1) W_{j+1}[o2][o3](p2, q2)
+// (physical dimension d^2; complex when either factor is).
+SiteOperator fuse_sites(const SiteOperator& a, const SiteOperator& b);
+
+// Two-site state of sites (j, j+1): Theta = A_j * B_{j+1} per middle sector,
+// its blocks' physical indices and the fused MPO.
+struct TwoSite {
+  BlockTensor theta;
+  std::vector<int> bphys;
+  SiteOperator W2;
 };
 
 // Environment updates (LeftComputeTensorContraction :782-940,
@@ -209,6 +227,10 @@ class RealTensorRundmrg {
   void BuildHeff(int site, HeffPlan& plan) {
     plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
   }
+  // Two-site H_eff of (site, site+1) on the current state: L_site x W_site
+  // W_{site+1} x Theta x R_{site+1}, with Theta = A_site B_{site+1} built on
+  // the device (valid when `site` is next in a left-to-right half sweep).
+  void BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan);
 
   // Tensor-space files (RealTensorSpace::Write/ReadTensorSpace,
   // real_tensor_space.cc:171-277): the bond QN tables, every site lattice

@@ -311,6 +311,18 @@ int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached);
 int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine);
 int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
+/* Two-site effective Hamiltonian of (site, site+1) on the current state:
+ * L_site x W_site W_site+1 x Theta x R_site+1, the matvec the reference's
+ * single-site sweep does not have (it is the two-site form the hot path is
+ * specified by).  Theta = A_site * B_site+1 is built on the device, one block
+ * (l, r) per middle sector m, in A's block order then B's, physical index
+ * p1 * d + p2; the fused MPO contracts W_site W_site+1 over their shared
+ * bond.  The plan takes dmrgcc_heff_apply / _info / _time / dmrgcc_lanczos
+ * like a one-site plan; dmrgcc_heff_state copies Theta (psi_dim entries)
+ * into dst (device).  Valid when `site` is next in a left-to-right half
+ * sweep (L_site and R_site+1 current). */
+int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
+int dmrgcc_heff_state(const dmrgcc_heff* h, double* dst);
 /* Export the current blocks of `site` to host memory (tests, CPU baseline):
  * which 0 = ket, 1 = left environment, 2 = right environment.  With
  * o == NULL only *count and *total (elements) are written.  Offsets count
