@@ -134,6 +134,20 @@ def site_lanczos(run, plan, site, niter, dm):
             "dE_rel": {k: abs(e[k] - ref) / abs(ref) for k in ("fp32", "bf16x3")}}
 
 
+def two_site(run, site, steps, dm):
+    """The two-site H_eff of (site, site+1) on the same state: F_ref and
+    matvec time per arithmetic (the same timing as the one-site plan)."""
+    plan2, _ = run.heff2(site)
+    info = plan2.info()
+    out = {}
+    for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3)):
+        plan2.time(prec, 3)
+        ms = plan2.time(prec, steps)["matvec_ms"]
+        out[name] = {"ms": ms, "tflops": info["flops_ref"] / (ms * 1e-3) / 1e12}
+    return {"sites": [site, site + 1], "psi_dim": info["psi_dim"], "flops_ref": info["flops_ref"],
+            "matvec": out}
+
+
 def drive(run, L, nhalf, name, stop=None):
     """Advance `run` half sweep by half sweep (a progress line after each),
     to the Lanczos solve of site stop[1] in half sweep stop[0], or to the
@@ -158,6 +172,7 @@ def main():
     ap.add_argument("--workload", choices=("cylinder", "spinful", "large"), default="cylinder")
     ap.add_argument("--no-mixed-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-two-site", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -206,6 +221,10 @@ def main():
     log("matvec timing done: " + ", ".join("%s %.3f ms" % (k, v["matvec_ms"]) for k, v in tm.items()))
     site_e = site_lanczos(run64, plan, site, tab[-1][2], dm)
     log("site Lanczos dE_rel: %s" % site_e["dE_rel"])
+    two = None
+    if not args.no_two_site:
+        two = two_site(run64, site, args.steps, dm)
+        log("two-site matvec: %s" % {k: v["ms"] for k, v in two["matvec"].items()})
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(run64, site, model, info["flops_ref"], args.cpu_seconds, args.maxm)
@@ -253,6 +272,8 @@ def main():
         "fp32": {"tflops": rate("fp32"), "ms": tm["fp32"]["matvec_ms"]},
         # same H_eff, same start vector: energy of each arithmetic vs FP64
         "site_lanczos": site_e,
+        # two-site H_eff (L x W W x Theta x R) of the centre pair, same state
+        "two_site": two,
         "sweep": {
             "fp64_seconds_per_sweep": sweeps64,
             "fp64_total_seconds": r64["total_seconds"],

@@ -229,7 +229,8 @@ class RealTensorRundmrg {
   }
   // Two-site H_eff of (site, site+1) on the current state: L_site x W_site
   // W_{site+1} x Theta x R_{site+1}, with Theta = A_site B_{site+1} built on
-  // the device (valid when `site` is next in a left-to-right half sweep).
+  // the device (valid when `site` is the next Lanczos site: site holds the
+  // centre, site+1 is right-canonical, L_site and R_site+1 are current).
   void BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan);
 
   // Tensor-space files (RealTensorSpace::Write/ReadTensorSpace,

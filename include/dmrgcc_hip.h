@@ -319,8 +319,9 @@ int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
  * p1 * d + p2; the fused MPO contracts W_site W_site+1 over their shared
  * bond.  The plan takes dmrgcc_heff_apply / _info / _time / dmrgcc_lanczos
  * like a one-site plan; dmrgcc_heff_state copies Theta (psi_dim entries)
- * into dst (device).  Valid when `site` is next in a left-to-right half
- * sweep (L_site and R_site+1 current). */
+ * into dst (device).  Valid when the run is paused before the Lanczos solve
+ * of `site` (site holds the centre, site+1 is right-canonical, L_site and
+ * R_site+1 are current). */
 int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_heff_state(const dmrgcc_heff* h, double* dst);
 /* Export the current blocks of `site` to host memory (tests, CPU baseline):
