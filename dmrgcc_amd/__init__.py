@@ -132,6 +132,9 @@ _SIGS = {
     "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                   _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
+    "dmrgcc_run_two_site": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                  _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
     "dmrgcc_run_files": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                         _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, _c_dbl_p, _c_dbl_p,
@@ -299,15 +302,23 @@ class _Model:
         return DmrgRun(self, table, first_block, first_dim, precision, refine, verbose, resume, record)
 
     def run(self, table, first_block=20, first_dim=10, precision=FP64, refine=True, verbose=False,
-            resume=None, record=None):
+            resume=None, record=None, two_site=False):
         """RealTensorRundmrg::Run (ComplexTensorRundmrg::Run on a complex
         model).  table: rows (maxm, maxb, niter, canpre, noise_exponent) per
         half sweep.  resume: tensor-space file to start from instead of the
         random initial MPS; record: file the tensor space is written to after
-        every sweep.  Returns a dict of per-half-sweep energies, discarded
+        every sweep.  two_site: two-site sweeps instead (no expansion; no
+        files).  Returns a dict of per-half-sweep energies, discarded
         weights, seconds and run statistics."""
         n, maxm, maxb, niter, canpre, noise = _table_args(table)
         e, t, s, st = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(24)
+        if two_site:
+            if resume is not None or record is not None:
+                raise ValueError("two-site runs take no tensor-space files")
+            _check(lib().dmrgcc_run_two_site(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb),
+                                             _ip(niter), _dp(canpre), _dp(noise), precision, int(refine),
+                                             int(verbose), _dp(e), _dp(t), _dp(s), _dp(st)))
+            return _result_dict(e, t, s, st)
         _check(lib().dmrgcc_run_files(self._h, first_block, first_dim, n, _ip(maxm), _ip(maxb), _ip(niter),
                                       _dp(canpre), _dp(noise), precision, int(refine), int(verbose),
                                       _path(resume), _path(record), _dp(e), _dp(t), _dp(s), _dp(st)))

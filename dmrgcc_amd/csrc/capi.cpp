@@ -985,6 +985,20 @@ int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const
                           refine, verbose, nullptr, nullptr, energies, trunc, seconds, stats);
 }
 
+int dmrgcc_run_two_site(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                        const int* maxb, const int* niter, const double* canpre, const double* noise,
+                        int precision, int refine, int verbose, double* energies, double* trunc,
+                        double* seconds, double* stats) {
+  return guarded([&] {
+    require(m != nullptr, "null model", kErrArgument);
+    RunParams p = make_params(first_block, first_dim, nhalf, maxm, maxb, niter, canpre, noise, precision,
+                              refine, verbose);
+    p.two_site = true;
+    RealTensorRundmrg run(*m->space, *m->ham, p);
+    fill_result(run.Run(), nullptr, energies, trunc, seconds, stats);
+  });
+}
+
 int dmrgcc_run_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                      const int* maxb, const int* niter, const double* canpre, const double* noise,
                      int precision, int refine, int verbose, const char* resume_path,

@@ -1887,6 +1887,157 @@ void RealTensorRundmrg::Step(int h, int j) {
   }
 }
 
+// One two-site step of half sweep h: the pair (j, j+1) left to right, the
+// pair (j-1, j) right to left, with the centre on j; the SVD split moves the
+// centre to the pair's other site.
+void RealTensorRundmrg::Step2(int h, int j) {
+  const auto& W = ham_.sites();
+  const bool l2r = (h % 2) == 0;
+  if ((l2r && j == 0) || (!l2r && j == L_sites_ - 1)) {
+    set_params(p_.maxb[h], p_.maxm[h], p_.canpre[h], p_.noise[h]);
+    half_terr_ = 0;
+    half_t0_ = now();
+  }
+  const int a = l2r ? j : j - 1;
+  double tp = now();
+  TwoSite ts;
+  HeffPlan plan;
+  BuildTwoSite(a, ts, plan);
+  const double energy = lanczos_solve(plan, ts.theta.data.data(), p_.niter[h], p_.precision, p_.refine, cx_,
+                                      stats_);
+  half_e_ = energy;
+  ph_.lanczos += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, a, energy, "lanczos2");
+  half_terr_ = std::max(half_terr_, split_two_site(a, ts, l2r));
+  ph_.canonical += now() - tp; tp = now();
+  if (p_.verbose >= 2) trace(h, a, energy, "split");
+  if (l2r) L_[a + 1] = left_env_update(lat_[a], W[a], L_[a], cx_);
+  else R_[a] = right_env_update(lat_[a + 1], W[a + 1], R_[a + 1], cx_);
+  ph_.env += now() - tp;
+  if ((l2r && j == L_sites_ - 2) || (!l2r && j == 1)) {
+    res_.energies.push_back(half_e_);
+    res_.trunc_err.push_back(half_terr_);
+    res_.half_seconds.push_back(now() - half_t0_);
+    if (p_.verbose)
+      std::printf("%s energy = %.*f\n", l2r ? "right" : "left", cs_ == 2 ? 14 : 12, half_e_), std::fflush(stdout);
+    if (!l2r && !p_.record_path.empty()) WriteTensorSpace(p_.record_path);
+  }
+}
+
+double RealTensorRundmrg::split_two_site(int a, const TwoSite& ts, bool l2r) {
+  // Theta's blocks regrouped into one site tensor whose other bond is the far
+  // bond of the pair (left to right: blocks (l, m) holding [Theta(l,m,r) ...]
+  // side by side over B's blocks r; right to left: blocks (m, r) holding
+  // Theta(l,m,r) stacked over A's blocks l), then the canonical form of that
+  // tensor (per-sector SVD, ComputeTruncateDim) keeps the orthonormal factor
+  // on one site and its canonical tensor (Sigma V^H or U Sigma) is cut back
+  // into the blocks of the other.
+  Lattice& A = lat_[a];
+  Lattice& B = lat_[a + 1];
+  const int cs = cs_;
+  const BlockTensor& th = ts.theta;
+  const int site = l2r ? a : a + 1;
+  Lattice& keep = l2r ? A : B;  // receives the orthonormal factor
+  Lattice& other = l2r ? B : A;
+  const BlockTensor& kk = keep.ket;  // block list of the kept site (its order)
+  // fat blocks: one per block of the kept site
+  std::vector<int> fl, fr, frows, fcols;
+  std::vector<std::vector<int>> parts(kk.n());  // Theta blocks of each fat block, in order
+  for (size_t t = 0; t < ts.src.size(); ++t) parts[l2r ? ts.src[t].first : ts.src[t].second].push_back((int)t);
+  if (!l2r) {  // stacked over A's blocks in A's order
+    for (auto& v : parts)
+      std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return ts.src[x].first < ts.src[y].first; });
+  }
+  BlockTensor F;
+  F.cs = cs;
+  for (int k = 0; k < kk.n(); ++k) { fl.push_back(kk.blk[k].l); fr.push_back(kk.blk[k].r); }
+  F.define(fl, fr);
+  std::vector<int> fdim(l2r ? keep.nr() : keep.nl(), 0);  // far-side extent per sector
+  for (int k = 0; k < kk.n(); ++k) {
+    int ext = 0, other_ext = -1;
+    for (int t : parts[k]) {
+      const Block& b = th.blk[t];
+      ext += l2r ? b.cols : b.rows;
+      const int o = l2r ? b.rows : b.cols;
+      require(other_ext < 0 || other_ext == o, "two-site blocks do not line up", kErrInternal);
+      other_ext = o;
+    }
+    if (other_ext < 0) other_ext = l2r ? kk.blk[k].rows : kk.blk[k].cols;
+    F.blk[k].rows = l2r ? other_ext : ext;
+    F.blk[k].cols = l2r ? ext : other_ext;
+    const int sec = l2r ? kk.blk[k].r : kk.blk[k].l;
+    fdim[sec] = std::max(fdim[sec], ext);
+  }
+  F.allocate();
+  fill(F.data.data(), F.doubles(), 0.0, cx_.stream);
+  std::vector<CopyTask> cps;
+  for (int k = 0; k < kk.n(); ++k) {
+    int at = 0;
+    for (int t : parts[k]) {
+      const Block& b = th.blk[t];
+      if ((int64_t)b.rows * b.cols > 0) {
+        double* dst = F.ptr(k) + (int64_t)cs * (l2r ? at : (int64_t)at * F.blk[k].cols);
+        push_copy(cps, CopyTask{th.ptr(t), dst, b.rows, b.cols, b.cols, 1, F.blk[k].cols, 1, 1.0}, cs);
+      }
+      at += l2r ? b.cols : b.rows;
+    }
+  }
+  launch_copies(cps, cx_.copy_scratch, cx_.stream);
+  // canonical form of the fat tensor on the kept site
+  std::vector<int> far_keep = l2r ? keep.rd : keep.ld;
+  keep.ket = std::move(F);
+  if (l2r) keep.rd = fdim; else keep.ld = fdim;
+  (void)far_keep;
+  const double discarded = canonical(l2r ? 0 : 1, site);
+  // the canonical tensor, per sector of the shared bond, cut into the other
+  // site's blocks (in its own order); the shared bond's dims follow
+  const BlockTensor& can = keep.can;
+  BlockTensor nk;
+  nk.cs = cs;
+  nk.blk = other.ket.blk;
+  const int nsec = l2r ? keep.nr() : keep.nl();
+  std::vector<int> tdim(nsec);
+  for (int m = 0; m < nsec; ++m) tdim[m] = l2r ? can.blk[m].rows : can.blk[m].cols;
+  // offset of each of the other site's blocks inside its sector's fat extent
+  std::vector<int> off(other.ket.n(), 0);
+  {
+    std::vector<int> acc(nsec, 0);
+    // the extent order is the order of the parts of any fat block of that
+    // sector: B's block order (left to right) or A's block order (right to left)
+    for (int k = 0; k < other.ket.n(); ++k) {
+      const Block& ob = other.ket.blk[k];
+      const int m = l2r ? ob.l : ob.r;
+      off[k] = acc[m];
+      acc[m] += l2r ? ob.cols : ob.rows;
+    }
+  }
+  for (int k = 0; k < nk.n(); ++k) {
+    const int m = l2r ? nk.blk[k].l : nk.blk[k].r;
+    if (l2r) nk.blk[k].rows = tdim[m];
+    else nk.blk[k].cols = tdim[m];
+  }
+  nk.allocate();
+  cps.clear();
+  for (int k = 0; k < nk.n(); ++k) {
+    const Block& b = nk.blk[k];
+    if ((int64_t)b.rows * b.cols == 0) continue;
+    const int m = l2r ? b.l : b.r;
+    const Block& cb = can.blk[m];
+    // left to right: can[m] is t_m x extent, columns [off, off + cols);
+    // right to left: can[m] is extent x t_m, rows [off, off + rows)
+    const double* src = can.ptr(m) + (int64_t)cs * (l2r ? off[k] : (int64_t)off[k] * cb.cols);
+    push_copy(cps, CopyTask{src, nk.ptr(k), b.rows, b.cols, cb.cols, 1, b.cols, 1, 1.0}, cs);
+  }
+  launch_copies(cps, cx_.copy_scratch, cx_.stream);
+  cx_.sync();
+  other.ket = std::move(nk);
+  if (l2r) other.ld = tdim; else other.rd = tdim;
+  keep.has_can = false;
+  keep.can = BlockTensor{};
+  keep.match_dim.clear();
+  return discarded;
+}
+
 bool RealTensorRundmrg::Advance(int stop_h, int stop_j) {
   if (!started_) Start();
   if (paused_at_ >= 0.0) {  // wall time between Advance calls is not sweep time
@@ -1904,7 +2055,8 @@ bool RealTensorRundmrg::Advance(int stop_h, int stop_j) {
       paused_at_ = now();
       return true;
     }
-    Step(cur_h_, cur_j_);
+    if (p_.two_site) Step2(cur_h_, cur_j_);
+    else Step(cur_h_, cur_j_);
     if (l2r) {
       if (++cur_j_ == L_sites_ - 1) { ++cur_h_; cur_j_ = 0; }
     } else {
@@ -1977,6 +2129,7 @@ void RealTensorRundmrg::BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan) {
   std::vector<int> li, ri;
   std::vector<std::pair<int, int>> src;
   ts.bphys.clear();
+  ts.src.clear();
   for (int a = 0; a < A.ket.n(); ++a) {
     const Block& ab = A.ket.blk[a];
     for (int b : B.ket.same(0, ab.r)) {
@@ -1985,6 +2138,7 @@ void RealTensorRundmrg::BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan) {
       li.push_back(ab.l);
       ri.push_back(bb.r);
       src.push_back({a, b});
+      ts.src.push_back({a, b});
       ts.bphys.push_back(A.phys[ab.l][ab.r] * d + B.phys[bb.l][bb.r]);
     }
   }

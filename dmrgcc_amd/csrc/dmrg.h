@@ -129,6 +129,7 @@ SiteOperator fuse_sites(const SiteOperator& a, const SiteOperator& b);
 struct TwoSite {
   BlockTensor theta;
   std::vector<int> bphys;
+  std::vector<std::pair<int, int>> src;  // (A block, B block) of each Theta block
   SiteOperator W2;
 };
 
@@ -182,6 +183,9 @@ struct RunParams {
   // DefineTensorSpace (real_tensor_rundmrg.cc:44-52); cache_record: write
   // the tensor space to this file after every sweep (:231-235)
   std::string resume_path, record_path;
+  // Two-site sweeps: Lanczos on the two-site H_eff of each pair, then the
+  // SVD split of Theta moves the centre (no subspace expansion).
+  bool two_site = false;
 };
 
 // Wall time per sweep phase (seconds, host clock around device-synchronised
@@ -273,6 +277,8 @@ class RealTensorRundmrg {
   void match(int leigh, int site);
   void trace(int h, int j, double e, const char* what);
   void Step(int h, int j);
+  void Step2(int h, int j);
+  double split_two_site(int a, const TwoSite& ts, bool l2r);  // returns discarded weight
   bool started_ = false;
   int cur_h_ = 0, cur_j_ = 0;
   double half_e_ = 0, half_terr_ = 0, half_t0_ = 0, t0_ = 0;

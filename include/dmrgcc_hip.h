@@ -269,6 +269,16 @@ int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
                double* seconds, double* stats);
+/* Two-site sweeps (the reference sweeps one site with subspace expansion;
+ * this is the two-site form): per pair, Lanczos on the two-site H_eff
+ * (dmrgcc_dmrg_heff2), then the SVD split of Theta per middle sector with
+ * ComputeTruncateDim (maxm, canpre of the table row) moves the centre; no
+ * expansion noise (the table's noise column is ignored).  Same arguments
+ * and outputs as dmrgcc_run; the initial MPS is the reference's. */
+int dmrgcc_run_two_site(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
+                        const int* maxb, const int* niter, const double* canpre, const double* noise,
+                        int precision, int refine, int verbose, double* energies, double* trunc,
+                        double* seconds, double* stats);
 /* dmrgcc_run with the reference's checkpoint files
  * (real_tensor_rundmrg.cc:44-52, :231-235): resume_path != NULL starts from
  * that tensor-space file instead of DefineTensorSpace (cache_resume);

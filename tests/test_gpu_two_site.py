@@ -117,3 +117,28 @@ def test_two_site_lanczos_energy(name, mk, om):
 def _ket(run, site, cplx):
     _, data = run.site_blocks(site, 0)
     return data.view(np.float64).copy() if cplx else data.copy()
+
+
+@pytest.mark.parametrize("name,mk,om", CASES)
+def test_two_site_sweeps_reach_exact_energy(name, mk, om):
+    model = mk()
+    e0 = ed.spinful_ground_energy(om) if name == "spinful" else ed.ground_energy(om.L, 3, 6, om.bonds(), 2.0)
+    rows = sweep_rows(100, 6, niter=10, canpre=1e-12)
+    r = model.run(rows, first_block=20, first_dim=4, two_site=True)
+    assert abs(r["energies"][-1] - e0) < 1e-9 * abs(e0)
+    assert min(r["energies"]) > e0 - 1e-9 * abs(e0)
+    # mixed-precision H_eff with the FP64 refinement reaches it too
+    rm = mk().run(rows, first_block=20, first_dim=4, two_site=True, precision=dm.BF16X3)
+    assert abs(rm["energies"][-1] - e0) < 1e-9 * abs(e0)
+
+
+def test_two_site_truncated_sweeps_are_variational():
+    om = SpinlessBoseSquare(4, 2, 8, 3, U=2.0)
+    e0 = ed.ground_energy(om.L, 3, 8, om.bonds(), 2.0)
+    r = dm.SpinlessModel(4, 2, 8, 3, U=2.0).run(sweep_rows(8, 8, niter=10, canpre=1e-12), first_block=20,
+                                                 first_dim=4, two_site=True)
+    e = np.array(r["energies"])
+    assert (e > e0 - 1e-10).all()
+    assert max(r["trunc_err"][2:]) > 0          # m = 8 truncates
+    assert e[-1] <= e[1] + 1e-10                # sweeps do not lose energy
+    assert abs(e[-1] - e0) < 1e-2 * abs(e0)
