@@ -117,6 +117,12 @@ _SIGS = {
                                                 ctypes.POINTER(_vp)]),
     "dmrgcc_dmrg_write_space": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "dmrgcc_dmrg_heff2": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "dmrgcc_heff_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "dmrgcc_heff_batch_apply": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "dmrgcc_heff_batch_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
+    "dmrgcc_heff_batch_info": (ctypes.c_int, [_vp, _c_dbl_p]),
+    "dmrgcc_heff_batch_destroy": (ctypes.c_int, [_vp]),
     "dmrgcc_heff_state": (ctypes.c_int, [_vp, _vp]),
     "dmrgcc_dmrg_advance": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p]),
     "dmrgcc_dmrg_set_precision": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -643,6 +649,42 @@ class HeffPlan:
         _check(lib().dmrgcc_lanczos(self._h, _ptr(ket), num_iter, precision, int(refine),
                                     ctypes.byref(e), ctypes.byref(it)))
         return e.value, it.value
+
+
+class HeffBatch:
+    """y_i += H_i x_i for many plans (independent instances) in one grouped
+    launch per stage; x_i / y_i are device tensors bound at creation."""
+
+    def __init__(self, plans, xs, ys):
+        n = len(plans)
+        self._keep = (list(plans), list(xs), list(ys))
+        arr = lambda v: (_vp * n)(*v)
+        h = _vp()
+        _sync_torch()
+        _check(lib().dmrgcc_heff_batch_create(n, arr([p._h for p in plans]), arr([_ptr(x) for x in xs]),
+                                              arr([_ptr(y) for y in ys]), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.dmrgcc_heff_batch_destroy(self._h)
+            self._h = None
+
+    def apply(self, precision=FP64, graph=True):
+        _sync_torch()
+        _check(lib().dmrgcc_heff_batch_apply(self._h, precision, int(graph)))
+
+    def time(self, precision=FP64, reps=20):
+        """Mean ms of the whole batch as one graph replay, and of the same
+        products plan by plan."""
+        ms = np.zeros(2)
+        _check(lib().dmrgcc_heff_batch_time(self._h, precision, reps, _dp(ms)))
+        return {"batch_ms": ms[0], "each_ms": ms[1]}
+
+    def info(self):
+        v = np.zeros(3)
+        _check(lib().dmrgcc_heff_batch_info(self._h, _dp(v)))
+        return {"flops_ref": v[0], "tiles": int(v[1]), "plans": int(v[2])}
 
 
 def env_update(leigh, W, phys, ket_l, ket_r, ket_rows, ket_cols, ket_data, Eb, E_data, stream=None):

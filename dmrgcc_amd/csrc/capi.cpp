@@ -37,6 +37,12 @@ struct dmrgcc_dmrg {
   std::unique_ptr<RealTensorRundmrg> run;
 };
 
+struct dmrgcc_heff_batch {
+  std::unique_ptr<Context> cx;
+  HeffBatch batch;
+  std::vector<dmrgcc_heff*> plans;
+};
+
 struct dmrgcc_blocks {
   std::vector<int> o, l, r, rows, cols;
   std::vector<int64_t> off;
@@ -841,6 +847,77 @@ int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     h->run->BuildHeff(site, e->plan);
     *out = e.release();
   });
+}
+
+int dmrgcc_heff_batch_create(int n, dmrgcc_heff* const* plans, const double* const* x, double* const* y,
+                             dmrgcc_heff_batch** out) {
+  return guarded([&] {
+    require(n > 0 && plans && x && y && out, "bad arguments", kErrArgument);
+    auto b = std::make_unique<dmrgcc_heff_batch>();
+    b->cx = std::make_unique<Context>(nullptr);
+    std::vector<HeffPlan*> ps;
+    std::vector<const double*> xs;
+    std::vector<double*> ys;
+    for (int i = 0; i < n; ++i) {
+      require(plans[i] && x[i] && y[i], "null plan or vector in batch", kErrArgument);
+      ps.push_back(&plans[i]->plan);
+      xs.push_back(x[i]);
+      ys.push_back(y[i]);
+      b->plans.push_back(plans[i]);
+    }
+    b->batch.build(ps, xs, ys, b->cx->stream);
+    *out = b.release();
+  });
+}
+
+int dmrgcc_heff_batch_apply(dmrgcc_heff_batch* b, int precision, int use_graph) {
+  return guarded([&] {
+    require(b && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    if (use_graph) b->batch.apply_graph(precision, b->cx->stream);
+    else b->batch.apply(precision, b->cx->stream);
+    DMRG_HIP(hipStreamSynchronize(b->cx->stream));
+  });
+}
+
+int dmrgcc_heff_batch_time(dmrgcc_heff_batch* b, int precision, int reps, double* ms) {
+  return guarded([&] {
+    require(b && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    hipStream_t s = b->cx->stream;
+    hipEvent_t e0, e1;
+    DMRG_HIP(hipEventCreate(&e0));
+    DMRG_HIP(hipEventCreate(&e1));
+    float t = 0;
+    // [0] graph replay of the whole batch, [1] the plans one after another
+    b->batch.apply_graph(precision, s);
+    DMRG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) b->batch.apply_graph(precision, s);
+    DMRG_HIP(hipEventRecord(e1, s));
+    DMRG_HIP(hipEventSynchronize(e1));
+    DMRG_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[0] = t / reps;
+    DMRG_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) b->batch.apply_each(precision, s);
+    DMRG_HIP(hipEventRecord(e1, s));
+    DMRG_HIP(hipEventSynchronize(e1));
+    DMRG_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[1] = t / reps;
+    DMRG_HIP(hipEventDestroy(e0));
+    DMRG_HIP(hipEventDestroy(e1));
+  });
+}
+
+int dmrgcc_heff_batch_info(const dmrgcc_heff_batch* b, double* info) {
+  return guarded([&] {
+    require(b && info, "null argument", kErrArgument);
+    info[0] = b->batch.fref();
+    info[1] = (double)b->batch.tiles();
+    info[2] = (double)b->plans.size();
+  });
+}
+
+int dmrgcc_heff_batch_destroy(dmrgcc_heff_batch* b) {
+  delete b;
+  return 0;
 }
 
 int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {

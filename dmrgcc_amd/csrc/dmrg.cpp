@@ -2083,6 +2083,60 @@ RunResult RealTensorRundmrg::Result() {
 }
 
 // --------------------------------------------------------------------------
+// Batched H_eff of independent instances
+// --------------------------------------------------------------------------
+HeffBatch::~HeffBatch() {
+  for (int p = 0; p < 3; ++p) {
+    if (exec_[p]) (void)hipGraphExecDestroy(exec_[p]);
+    if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
+  }
+}
+
+void HeffBatch::build(const std::vector<HeffPlan*>& plans, const std::vector<const double*>& x,
+                      const std::vector<double*>& y, hipStream_t stream) {
+  require(plans.size() == x.size() && plans.size() == y.size(), "one input and output vector per plan",
+          kErrArgument);
+  std::vector<const GemmBatch*> b1, b2;
+  std::vector<const double*> none_in(plans.size(), nullptr);
+  std::vector<double*> none_out(plans.size(), nullptr);
+  fref_ = 0;
+  for (HeffPlan* p : plans) {
+    b1.push_back(&p->stage1());
+    b2.push_back(&p->stage2());
+    fref_ += p->fref();
+  }
+  // stage 1 reads x (relative A), writes each plan's own stage buffer;
+  // stage 2 reads that buffer, accumulates into y (relative C)
+  s1_.merge(b1, x, none_out, stream);
+  s2_.merge(b2, none_in, y, stream);
+  plans_ = plans;
+  x_ = x;
+  y_ = y;
+  DMRG_HIP(hipStreamSynchronize(stream));
+}
+
+void HeffBatch::apply_each(int precision, hipStream_t stream) {
+  for (size_t i = 0; i < plans_.size(); ++i) plans_[i]->apply(x_[i], y_[i], precision, stream);
+}
+
+void HeffBatch::apply(int precision, hipStream_t stream) {
+  s1_.launch(stream, precision);
+  s2_.launch(stream, precision);
+}
+
+void HeffBatch::apply_graph(int precision, hipStream_t stream) {
+  require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+  if (!exec_[precision]) {
+    // kernels only (no memset nodes); captured on this thread's stream
+    DMRG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    apply(precision, stream);
+    DMRG_HIP(hipStreamEndCapture(stream, &graph_[precision]));
+    DMRG_HIP(hipGraphInstantiate(&exec_[precision], graph_[precision], nullptr, nullptr, 0));
+  }
+  DMRG_HIP(hipGraphLaunch(exec_[precision], stream));
+}
+
+// --------------------------------------------------------------------------
 // Two-site effective Hamiltonian
 // --------------------------------------------------------------------------
 SiteOperator fuse_sites(const SiteOperator& a, const SiteOperator& b) {

@@ -119,6 +119,33 @@ class HeffPlan {
   double fref_ = 0;
 };
 
+// Many H_eff plans (independent instances, any mix of sizes) applied in one
+// grouped launch per stage: y_i += H_i x_i for every plan, with x_i / y_i
+// bound at build time.  apply_graph captures those launches once per
+// arithmetic into a HIP graph and replays it.  The plans must outlive the
+// batch.
+class HeffBatch {
+ public:
+  ~HeffBatch();
+  void build(const std::vector<HeffPlan*>& plans, const std::vector<const double*>& x,
+             const std::vector<double*>& y, hipStream_t stream);
+  void apply(int precision, hipStream_t stream);
+  void apply_graph(int precision, hipStream_t stream);
+  // the same products plan by plan, two launches each (reference timing)
+  void apply_each(int precision, hipStream_t stream);
+  double fref() const { return fref_; }
+  int64_t tiles() const { return (int64_t)(s1_.tiles.size() + s2_.tiles.size()); }
+
+ private:
+  GemmBatch s1_, s2_;
+  std::vector<HeffPlan*> plans_;
+  std::vector<const double*> x_;
+  std::vector<double*> y_;
+  double fref_ = 0;
+  hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};
+};
+
 // Two-site MPO W_j W_{j+1} contracted over the shared bond:
 // W[o1][o3](p1 d + p2, q1 d + q2) = sum_o2 W_j[o1][o2](p1, q1) W_{j+1}[o2][o3](p2, q2)
 // (physical dimension d^2; complex when either factor is).

@@ -115,6 +115,12 @@ struct GemmBatch {
   void launch(hipStream_t stream, int precision, const double* in = nullptr,
               double* out = nullptr, bool reduce = true) const;
   void launch_group(hipStream_t stream, int group, int precision = 0) const;
+  // Concatenate finalized batches into this one (one launch for all): the
+  // relative operands and outputs of batch i are bound to ins[i] / outs[i]
+  // (made absolute), split tasks kept; the source batches' buffers (stage
+  // buffers, split-K partials) must outlive this batch.
+  void merge(const std::vector<const GemmBatch*>& bs, const std::vector<const double*>& ins,
+             const std::vector<double*>& outs, hipStream_t stream);
   bool empty() const { return tiles.empty(); }
 };
 

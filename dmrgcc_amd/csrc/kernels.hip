@@ -1843,6 +1843,41 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   }
 }
 
+void GemmBatch::merge(const std::vector<const GemmBatch*>& bs, const std::vector<const double*>& ins,
+                      const std::vector<double*>& outs, hipStream_t stream) {
+  require(bs.size() == ins.size() && bs.size() == outs.size(), "merge: one input and output per batch",
+          kErrArgument);
+  clear();
+  std::vector<SplitTask> sp;
+  for (size_t i = 0; i < bs.size(); ++i) {
+    const GemmBatch& b = *bs[i];
+    require(b.group_items.empty(), "merge: grouped batches cannot be merged", kErrArgument);
+    const int cbase = (int)contribs.size();
+    for (GemmContrib c : b.contribs) {
+      if (c.flags & kRelA) { c.A = ins[i] + (intptr_t)c.A; c.flags &= ~kRelA; }
+      if (c.flags & kRelB) { c.B = ins[i] + (intptr_t)c.B; c.flags &= ~kRelB; }
+      contribs.push_back(c);
+    }
+    for (GemmItem it : b.items) {
+      if (it.flags & kRelC) { it.C = outs[i] + (intptr_t)it.C; it.flags &= ~kRelC; }
+      it.c0 += cbase;
+      it.c1 += cbase;
+      items.push_back(it);
+    }
+    for (SplitTask t : b.splits) {
+      if (t.flags & kRelC) { t.C = outs[i] + (intptr_t)t.C; t.flags &= ~kRelC; }
+      sp.push_back(t);
+    }
+    flops += b.flops;
+  }
+  const int sk = split_k;
+  split_k = 0;  // the sources are split already
+  finalize(stream);
+  split_k = sk;
+  splits = sp;
+  if (!splits.empty()) upload(d_splits, splits, stream);
+}
+
 void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const {
   require(group >= 0 && group < (int)group_tiles.size(), "GemmBatch: no such group", kErrInternal);
   const int t0 = group == 0 ? 0 : group_tiles[group - 1], t1 = group_tiles[group];

@@ -332,6 +332,21 @@ int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
  * into dst (device).  Valid when the run is paused before the Lanczos solve
  * of `site` (site holds the centre, site+1 is right-canonical, L_site and
  * R_site+1 are current). */
+/* Batched H_eff of independent instances (no reference counterpart: the
+ * reference runs one instance per process): y_i += H_i x_i for n plans in
+ * one grouped launch per stage.  x[i], y[i] are device vectors of plan i,
+ * bound at creation; the plans must outlive the batch.  apply with
+ * use_graph != 0 captures the launches once per arithmetic into a HIP
+ * graph and replays it.  time: ms[0] mean graph replay of the whole batch,
+ * ms[1] mean of the same products plan by plan.  info: [0] sum of F_ref,
+ * [1] tiles, [2] plans. */
+typedef struct dmrgcc_heff_batch dmrgcc_heff_batch;
+int dmrgcc_heff_batch_create(int n, dmrgcc_heff* const* plans, const double* const* x, double* const* y,
+                             dmrgcc_heff_batch** out);
+int dmrgcc_heff_batch_apply(dmrgcc_heff_batch* b, int precision, int use_graph);
+int dmrgcc_heff_batch_time(dmrgcc_heff_batch* b, int precision, int reps, double* ms);
+int dmrgcc_heff_batch_info(const dmrgcc_heff_batch* b, double* info);
+int dmrgcc_heff_batch_destroy(dmrgcc_heff_batch* b);
 int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_heff_state(const dmrgcc_heff* h, double* dst);
 /* Export the current blocks of `site` to host memory (tests, CPU baseline):
