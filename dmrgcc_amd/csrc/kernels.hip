@@ -75,34 +75,66 @@ struct StepRegs {
   double a[8], b[8];
 };
 
+// Operands live in device (global) memory.  The descriptors carry generic
+// pointers, which the compiler would load through FLAT instructions; FLAT
+// loads count against the LDS wait counter too, so every LDS read of a K-step
+// would also wait for the next step's prefetch.  Loads go through global
+// pointers instead.
+typedef const double __attribute__((address_space(1)))* gcdptr;
+typedef double __attribute__((address_space(1)))* gdptr;
+__device__ inline gcdptr gload_ptr(const double* p) { return (gcdptr)p; }
+__device__ inline gdptr gstore_ptr(double* p) { return (gdptr)p; }
+
+// v[q] = sc * p[q * step] for q < nv, 0 for q >= nv (nv may be <= 0).  A
+// partial run reads clamped addresses (its last valid element) without
+// per-element branches; a run with no valid element issues no loads.  p must
+// address a valid element whenever nv > 0.
+__device__ inline void load_run8(gcdptr p, int64_t step, int nv, double sc, double (&v)[8]) {
+  if (nv >= 8) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = p[q * step] * sc;
+  } else if (nv > 0) {
+    const int last = nv - 1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double x = p[(q < last ? q : last) * step];
+      v[q] = q < nv ? x * sc : 0.0;
+    }
+  } else {  // the whole run lies outside the operand: no loads
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = 0.0;
+  }
+}
+
+// Element (r, c) runs of the K-step: the run starts at row r (clamped into
+// [0, R)) and column c (clamped into [0, C)); it holds nv valid elements.
+__device__ inline int run_len(int r, int R, int c, int C) { return r < R ? min(C - c, 8) : 0; }
+__device__ inline int clamp_hi(int x, int n) { return x < n ? x : n - 1; }
+
 __device__ inline void load_step(const GemmContrib& cb, const double* __restrict__ in_base, int m0,
                                  int n0, int M, int N, int k0, int tid, StepRegs& r) {
-  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
-  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  gcdptr A = gload_ptr((cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A);
+  gcdptr B = gload_ptr((cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B);
   const int K = cb.K;
   const double sc = cb.scale;
   const int es = (cb.flags & kStrided2) ? 2 : 1;
   if (!(cb.flags & kTransA)) {
     const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    const double* row = A + (int64_t)gm * cb.lda;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
+    load_run8(A + (int64_t)clamp_hi(gm, M) * cb.lda + (int64_t)clamp_hi(kb, K) * es, es,
+              run_len(gm, M, kb, K), sc, r.a);
   } else {
     const int gk = k0 + (tid >> 3), mb = m0 + (tid & 7) * 8;
-    const double* row = A + (int64_t)gk * cb.lda;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gk < K && mb + q < M) ? row[(mb + q) * es] * sc : 0.0;
+    load_run8(A + (int64_t)clamp_hi(gk, K) * cb.lda + (int64_t)clamp_hi(mb, M) * es, es,
+              run_len(gk, K, mb, M), sc, r.a);
   }
   if (!(cb.flags & kTransB)) {
     const int gk = k0 + (tid >> 3), nb = n0 + (tid & 7) * 8;
-    const double* row = B + (int64_t)gk * cb.ldb;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gk < K && nb + q < N) ? row[(nb + q) * es] : 0.0;
+    load_run8(B + (int64_t)clamp_hi(gk, K) * cb.ldb + (int64_t)clamp_hi(nb, N) * es, es,
+              run_len(gk, K, nb, N), 1.0, r.b);
   } else {
     const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    const double* row = B + (int64_t)gn * cb.ldb;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
+    load_run8(B + (int64_t)clamp_hi(gn, N) * cb.ldb + (int64_t)clamp_hi(kb, K) * es, es,
+              run_len(gn, N, kb, K), 1.0, r.b);
   }
 }
 
@@ -211,8 +243,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
         const int gm = m0 + wm * 32 + a * 16 + MfmaAcc<T>::row(lane, v);
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
-          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
-                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
+          gdptr p = gstore_ptr((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) +
+                    (int64_t)gm * it.ldc + (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
           const double r2 = (double)acc[a][b][v];
           *p = it.accumulate ? (*p + r2) : r2;
         }
@@ -237,32 +269,28 @@ typedef float f8v __attribute__((ext_vector_type(8)));
 __device__ inline void load_step_packed(const GemmContrib& cb, const double* __restrict__ in_base,
                                         int m0, int n0, int M, int N, int k0, int tid,
                                         StepRegs& r) {
-  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
-  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  gcdptr A = gload_ptr((cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A);
+  gcdptr B = gload_ptr((cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B);
   const int K = cb.K;
   const double sc = cb.scale;
   const int es = (cb.flags & kStrided2) ? 2 : 1;
   if (!(cb.flags & kTransA)) {
     const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    const double* row = A + (int64_t)gm * cb.lda;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
+    load_run8(A + (int64_t)clamp_hi(gm, M) * cb.lda + (int64_t)clamp_hi(kb, K) * es, es,
+              run_len(gm, M, kb, K), sc, r.a);
   } else {
     const int gm = m0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm * es] * sc : 0.0;
+    load_run8(A + (int64_t)clamp_hi(kb, K) * cb.lda + (int64_t)clamp_hi(gm, M) * es, cb.lda,
+              run_len(gm, M, kb, K), sc, r.a);
   }
   if (!(cb.flags & kTransB)) {
     const int gn = n0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn * es] : 0.0;
+    load_run8(B + (int64_t)clamp_hi(kb, K) * cb.ldb + (int64_t)clamp_hi(gn, N) * es, cb.ldb,
+              run_len(gn, N, kb, K), 1.0, r.b);
   } else {
     const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    const double* row = B + (int64_t)gn * cb.ldb;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
+    load_run8(B + (int64_t)clamp_hi(gn, N) * cb.ldb + (int64_t)clamp_hi(kb, K) * es, es,
+              run_len(gn, N, kb, K), 1.0, r.b);
   }
 }
 
@@ -364,8 +392,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
         const int gm = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + v;
         const int gn = n0 + wn * 32 + b * 16 + (lane & 15);
         if (gm < M && gn < N) {
-          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
-                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
+          gdptr p = gstore_ptr((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) +
+                    (int64_t)gm * it.ldc + (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
           const double r2 = (double)acc[a][b][v];
           *p = it.accumulate ? (*p + r2) : r2;
         }
@@ -444,8 +472,8 @@ __device__ inline void wave_store_out(const GemmItem& it, double* __restrict__ o
         const int gm = m0 + a * 16 + MfmaAcc<T>::row(lane, v);
         const int gn = n0 + b * 16 + (lane & 15);
         if (gm < it.M && gn < it.N) {
-          double* p = ((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) + (int64_t)gm * it.ldc +
-                      (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
+          gdptr p = gstore_ptr((it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C) +
+                    (int64_t)gm * it.ldc + (int64_t)gn * ((it.flags & kStrided2C) ? 2 : 1);
           const double r2 = (double)acc[a][b][v];
           *p = it.accumulate ? (*p + r2) : r2;
         }
