@@ -65,10 +65,14 @@ struct MfmaAcc<float> {
 // columns of A (64 x 32 tile) and rows of B (32 x 64 tile); each of the 256
 // threads holds 8 elements of each, prefetched into registers while the MFMAs
 // of the previous step run.  Element q of thread tid:
-//   A, not transposed: m = tid/4,        k = 8*(tid%4) + q   (8 consecutive k)
-//   A, transposed:     k = tid/8,        m = 8*(tid%8) + q   (8 consecutive m)
-//   B, not transposed: k = tid/8,        n = 8*(tid%8) + q
-//   B, transposed:     n = tid/4,        k = 8*(tid%4) + q
+//   A, not transposed: m = tid/16 + 16*(q/2), k = 2*(tid%16) + q%2
+//   A, transposed:     k = tid/8,             m = 8*(tid%8) + q   (8 consecutive m)
+//   B, not transposed: k = tid/8,             n = 8*(tid%8) + q
+//   B, transposed:     n = tid/16 + 16*(q/2), k = 2*(tid%16) + q%2
+// For the operands stored with k contiguous (A not transposed, B transposed)
+// sixteen consecutive lanes read one row's 32 k, so one load instruction
+// covers four whole 256-byte row segments; a thread-per-row run of 8 k would
+// touch 32 cache lines per instruction for 512 bytes.
 constexpr int kKT = 32;
 
 struct StepRegs {
@@ -106,6 +110,27 @@ __device__ inline void load_run8(gcdptr p, int64_t step, int nv, double sc, doub
   }
 }
 
+// Row-major operand X (rows r, k contiguous with element stride es): v[2i],
+// v[2i+1] = sc * X[r0 + tid/16 + 16 i][k0 + 2*(tid%16) + {0, 1}], zero outside
+// R rows and K columns; rows outside R issue no loads.
+__device__ inline void load_rows_k(gcdptr X, int64_t ld, int es, int r0, int R, int k0, int K,
+                                   int tid, double sc, double (&v)[8]) {
+  const int kk = k0 + 2 * (tid & 15);
+  const int ka = kk < K ? kk : K - 1, kb = kk + 1 < K ? kk + 1 : K - 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = r0 + (tid >> 4) + 16 * i;
+    double x0 = 0.0, x1 = 0.0;
+    if (r < R) {
+      const gcdptr row = X + (int64_t)r * ld;
+      x0 = row[(int64_t)ka * es];
+      x1 = row[(int64_t)kb * es];
+    }
+    v[2 * i] = (r < R && kk < K) ? x0 * sc : 0.0;
+    v[2 * i + 1] = (r < R && kk + 1 < K) ? x1 * sc : 0.0;
+  }
+}
+
 // Element (r, c) runs of the K-step: the run starts at row r (clamped into
 // [0, R)) and column c (clamped into [0, C)); it holds nv valid elements.
 __device__ inline int run_len(int r, int R, int c, int C) { return r < R ? min(C - c, 8) : 0; }
@@ -119,9 +144,7 @@ __device__ inline void load_step(const GemmContrib& cb, const double* __restrict
   const double sc = cb.scale;
   const int es = (cb.flags & kStrided2) ? 2 : 1;
   if (!(cb.flags & kTransA)) {
-    const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    load_run8(A + (int64_t)clamp_hi(gm, M) * cb.lda + (int64_t)clamp_hi(kb, K) * es, es,
-              run_len(gm, M, kb, K), sc, r.a);
+    load_rows_k(A, cb.lda, es, m0, M, k0, K, tid, sc, r.a);
   } else {
     const int gk = k0 + (tid >> 3), mb = m0 + (tid & 7) * 8;
     load_run8(A + (int64_t)clamp_hi(gk, K) * cb.lda + (int64_t)clamp_hi(mb, M) * es, es,
@@ -132,16 +155,15 @@ __device__ inline void load_step(const GemmContrib& cb, const double* __restrict
     load_run8(B + (int64_t)clamp_hi(gk, K) * cb.ldb + (int64_t)clamp_hi(nb, N) * es, es,
               run_len(gk, K, nb, N), 1.0, r.b);
   } else {
-    const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    load_run8(B + (int64_t)clamp_hi(gn, N) * cb.ldb + (int64_t)clamp_hi(kb, K) * es, es,
-              run_len(gn, N, kb, K), 1.0, r.b);
+    load_rows_k(B, cb.ldb, es, n0, N, k0, K, tid, 1.0, r.b);
   }
 }
 
 // (contribution, k0) walk of one item: advance to the next K-step
+template <int KT = kKT>
 __device__ inline bool next_step(const GemmContrib* __restrict__ cons, int c1, int& c, int& k0,
                                  GemmContrib& cb) {
-  k0 += kKT;
+  k0 += KT;
   if (k0 < cb.K) return true;
   k0 = 0;
   while (++c < c1) {
@@ -155,9 +177,12 @@ template <typename T>
 __device__ inline void store_step(const GemmContrib& cb, int tid, const StepRegs& r,
                                   T (*As)[kTile + 2], T (*Bs)[kTile + 2]) {
   if (!(cb.flags & kTransA)) {
-    const int m = tid >> 2, kb = (tid & 3) * 8;
+    const int m = tid >> 4, k = 2 * (tid & 15);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) As[kb + q][m] = (T)r.a[q];
+    for (int i = 0; i < 4; ++i) {
+      As[k][m + 16 * i] = (T)r.a[2 * i];
+      As[k + 1][m + 16 * i] = (T)r.a[2 * i + 1];
+    }
   } else {
     const int k = tid >> 3, mb = (tid & 7) * 8;
 #pragma unroll
@@ -168,9 +193,12 @@ __device__ inline void store_step(const GemmContrib& cb, int tid, const StepRegs
 #pragma unroll
     for (int q = 0; q < 8; ++q) Bs[k][nb + q] = (T)r.b[q];
   } else {
-    const int n = tid >> 2, kb = (tid & 3) * 8;
+    const int n = tid >> 4, k = 2 * (tid & 15);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Bs[kb + q][n] = (T)r.b[q];
+    for (int i = 0; i < 4; ++i) {
+      Bs[k][n + 16 * i] = (T)r.b[2 * i];
+      Bs[k + 1][n + 16 * i] = (T)r.b[2 * i + 1];
+    }
   }
 }
 
@@ -260,37 +288,45 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
 // BF16x3 staging: every thread holds 8 consecutive k of one row of A (m) and
 // of one column of B (n), so each hi/lo plane is written to LDS with one
 // 16-byte store.  Element q of thread tid:
-//   A, not transposed: m = tid/4,  k = 8*(tid%4) + q   (a 64-byte run of a row)
+//   A, not transposed: m = tid/16 + 16*(q/2), k = 2*(tid%16) + q%2 (pairs; as above)
 //   A, transposed:     m = tid%64, k = 8*(tid/64) + q  (64 threads read 64 consecutive m)
 //   B, not transposed: n = tid%64, k = 8*(tid/64) + q
-//   B, transposed:     n = tid/4,  k = 8*(tid%4) + q
+//   B, transposed:     n = tid/16 + 16*(q/2), k = 2*(tid%16) + q%2
 typedef float f8v __attribute__((ext_vector_type(8)));
 
+// KT = 32 * NR: every thread loads NR runs of 8 k per operand, the runs 32 k
+// apart (run j covers k0 + 32 j + [the offsets above]).
+template <int NR>
+struct PackedRegs {
+  double a[NR][8], b[NR][8];
+};
+
+template <int NR>
 __device__ inline void load_step_packed(const GemmContrib& cb, const double* __restrict__ in_base,
                                         int m0, int n0, int M, int N, int k0, int tid,
-                                        StepRegs& r) {
+                                        PackedRegs<NR>& r) {
   gcdptr A = gload_ptr((cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A);
   gcdptr B = gload_ptr((cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B);
   const int K = cb.K;
   const double sc = cb.scale;
   const int es = (cb.flags & kStrided2) ? 2 : 1;
-  if (!(cb.flags & kTransA)) {
-    const int gm = m0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    load_run8(A + (int64_t)clamp_hi(gm, M) * cb.lda + (int64_t)clamp_hi(kb, K) * es, es,
-              run_len(gm, M, kb, K), sc, r.a);
-  } else {
-    const int gm = m0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
-    load_run8(A + (int64_t)clamp_hi(kb, K) * cb.lda + (int64_t)clamp_hi(gm, M) * es, cb.lda,
-              run_len(gm, M, kb, K), sc, r.a);
-  }
-  if (!(cb.flags & kTransB)) {
-    const int gn = n0 + (tid & 63), kb = k0 + (tid >> 6) * 8;
-    load_run8(B + (int64_t)clamp_hi(kb, K) * cb.ldb + (int64_t)clamp_hi(gn, N) * es, cb.ldb,
-              run_len(gn, N, kb, K), 1.0, r.b);
-  } else {
-    const int gn = n0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
-    load_run8(B + (int64_t)clamp_hi(gn, N) * cb.ldb + (int64_t)clamp_hi(kb, K) * es, es,
-              run_len(gn, N, kb, K), 1.0, r.b);
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int kj = k0 + 32 * j;
+    if (!(cb.flags & kTransA)) {
+      load_rows_k(A, cb.lda, es, m0, M, kj, K, tid, sc, r.a[j]);
+    } else {
+      const int gm = m0 + (tid & 63), kb = kj + (tid >> 6) * 8;
+      load_run8(A + (int64_t)clamp_hi(kb, K) * cb.lda + (int64_t)clamp_hi(gm, M) * es, cb.lda,
+                run_len(gm, M, kb, K), sc, r.a[j]);
+    }
+    if (!(cb.flags & kTransB)) {
+      const int gn = n0 + (tid & 63), kb = kj + (tid >> 6) * 8;
+      load_run8(B + (int64_t)clamp_hi(kb, K) * cb.ldb + (int64_t)clamp_hi(gn, N) * es, cb.ldb,
+                run_len(gn, N, kb, K), 1.0, r.b[j]);
+    } else {
+      load_rows_k(B, cb.ldb, es, n0, N, kj, K, tid, 1.0, r.b[j]);
+    }
   }
 }
 
@@ -304,11 +340,52 @@ __device__ inline void split8(const double* v, bf8v& h, bf8v& l) {
   l = __builtin_convertvector(f - hf, bf8v);
 }
 
+// K-step of the BF16x3 kernel: 32 or 64 (compile-time)
+#ifndef DMRG_BF16X3_KT
+#define DMRG_BF16X3_KT 32
+#endif
+constexpr int kBKT = DMRG_BF16X3_KT;
+static_assert(kBKT == 32 || kBKT == 64, "BF16x3 K-step of 32 or 64");
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+
+// the same split for a pair
+__device__ inline void split2(double x0, double x1, bf2v& h, bf2v& l) {
+  const f2v f{(float)x0, (float)x1};
+  h = __builtin_convertvector(f, bf2v);
+  const f2v hf = __builtin_convertvector(h, f2v);
+  l = __builtin_convertvector(f - hf, bf2v);
+}
+
+// one operand of a K-step into its hi/lo planes X[row][k] (row = m for A, n
+// for B); `kcont` selects the pair layout of the k-contiguous operands
+template <int LD>
+__device__ inline void store_planes(bool kcont, int tid, int kofs, const double (&v)[8],
+                                    __bf16 (*Xh)[LD], __bf16 (*Xl)[LD], int row_run, int k_run) {
+  if (kcont) {
+    const int row = tid >> 4, k = kofs + 2 * (tid & 15);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      bf2v h, l;
+      split2(v[2 * i], v[2 * i + 1], h, l);
+      *reinterpret_cast<bf2v*>(&Xh[row + 16 * i][k]) = h;
+      *reinterpret_cast<bf2v*>(&Xl[row + 16 * i][k]) = l;
+    }
+  } else {
+    bf8v h, l;
+    split8(v, h, l);
+    *reinterpret_cast<bf8v*>(&Xh[row_run][kofs + k_run]) = h;
+    *reinterpret_cast<bf8v*>(&Xl[row_run][kofs + k_run]) = l;
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
     double* __restrict__ out_base) {
-  constexpr int LD = kKT + 8;  // 80-byte rows keep 16-byte alignment
+  constexpr int KT = kBKT, NR = KT / 32;
+  constexpr int LD = KT + 8;  // rows of 80 or 144 bytes keep 16-byte alignment
   __shared__ __attribute__((aligned(16))) __bf16 Ah[kTile][LD];
   __shared__ __attribute__((aligned(16))) __bf16 Al[kTile][LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[kTile][LD];
@@ -337,49 +414,45 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
     cb = cons[c];
     if (cb.K > 0) { live = true; break; }
   }
-  StepRegs r;
-  if (live) load_step_packed(cb, in_base, m0, n0, M, N, 0, tid, r);
+  PackedRegs<NR> r;
+  if (live) load_step_packed<NR>(cb, in_base, m0, n0, M, N, 0, tid, r);
   while (live) {
-    {
-      const int am = (cb.flags & kTransA) ? (tid & 63) : (tid >> 2);
-      const int ak = (cb.flags & kTransA) ? (tid >> 6) * 8 : (tid & 3) * 8;
-      const int bn = (cb.flags & kTransB) ? (tid >> 2) : (tid & 63);
-      const int bk = (cb.flags & kTransB) ? (tid & 3) * 8 : (tid >> 6) * 8;
-      bf8v h, l;
-      split8(r.a, h, l);
-      *reinterpret_cast<bf8v*>(&Ah[am][ak]) = h;
-      *reinterpret_cast<bf8v*>(&Al[am][ak]) = l;
-      split8(r.b, h, l);
-      *reinterpret_cast<bf8v*>(&Bh[bn][bk]) = h;
-      *reinterpret_cast<bf8v*>(&Bl[bn][bk]) = l;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      // runs of the transposed A and the plain B: row tid%64, k 8*(tid/64)
+      store_planes<LD>(!(cb.flags & kTransA), tid, 32 * j, r.a[j], Ah, Al, tid & 63, (tid >> 6) * 8);
+      store_planes<LD>((cb.flags & kTransB) != 0, tid, 32 * j, r.b[j], Bh, Bl, tid & 63, (tid >> 6) * 8);
     }
     __syncthreads();
     GemmContrib nc = cb;
     int c2 = c, k2 = k0;
-    live = next_step(cons, it.c1, c2, k2, nc);
-    if (live) load_step_packed(nc, in_base, m0, n0, M, N, k2, tid, r);  // in flight during the MFMAs
-    const int kc = 8 * (lane >> 4);
-    bf8v ah[2], al[2], bh[2], bl[2];
+    live = next_step<KT>(cons, it.c1, c2, k2, nc);
+    if (live) load_step_packed<NR>(nc, in_base, m0, n0, M, N, k2, tid, r);  // in flight during the MFMAs
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int rr = wm * 32 + a * 16 + (lane & 15);
-      ah[a] = *reinterpret_cast<const bf8v*>(&Ah[rr][kc]);
-      al[a] = *reinterpret_cast<const bf8v*>(&Al[rr][kc]);
-    }
+    for (int j = 0; j < NR; ++j) {
+      const int kc = 32 * j + 8 * (lane >> 4);
+      bf8v ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int rr = wn * 32 + b * 16 + (lane & 15);
-      bh[b] = *reinterpret_cast<const bf8v*>(&Bh[rr][kc]);
-      bl[b] = *reinterpret_cast<const bf8v*>(&Bl[rr][kc]);
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 2; ++a) {
+        const int rr = wm * 32 + a * 16 + (lane & 15);
+        ah[a] = *reinterpret_cast<const bf8v*>(&Ah[rr][kc]);
+        al[a] = *reinterpret_cast<const bf8v*>(&Al[rr][kc]);
+      }
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        const int rr = wn * 32 + b * 16 + (lane & 15);
+        bh[b] = *reinterpret_cast<const bf8v*>(&Bh[rr][kc]);
+        bl[b] = *reinterpret_cast<const bf8v*>(&Bl[rr][kc]);
       }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+    }
     __syncthreads();
     c = c2; k0 = k2; cb = nc;
   }
@@ -648,14 +721,18 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __restrict__ tasks,
                                                            double* __restrict__ out_base) {
   const SplitTask t = tasks[blockIdx.y];
-  double* C = (t.flags & kRelC) ? out_base + (intptr_t)t.C : t.C;
+  gdptr C = gstore_ptr((t.flags & kRelC) ? out_base + (intptr_t)t.C : t.C);
+  gcdptr part = gload_ptr(t.part);
   const int64_t mn = (int64_t)t.M * t.N;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < t.nparts; ++c) s += t.part[c * mn + e];
-    const int i = (int)(e / t.N), j = (int)(e % t.N);
-    C[(int64_t)i * t.ldc + (int64_t)j * ((t.flags & kStrided2C) ? 2 : 1)] += s;
+  const int es = (t.flags & kStrided2C) ? 2 : 1;
+  // rows over the workgroups, columns over the threads
+  for (int i = blockIdx.x; i < t.M; i += gridDim.x) {
+    for (int j = threadIdx.x; j < t.N; j += blockDim.x) {
+      const int64_t e = (int64_t)i * t.N + j;
+      double s = 0.0;
+      for (int c = 0; c < t.nparts; ++c) s += part[c * mn + e];
+      C[(int64_t)i * t.ldc + (int64_t)j * es] += s;
+    }
   }
 }
 
@@ -1862,9 +1939,9 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   }
   DMRG_HIP(hipGetLastError());
   if (reduce && !splits.empty()) {
-    int64_t mx = 0;
-    for (const SplitTask& t : splits) mx = std::max<int64_t>(mx, (int64_t)t.M * t.N);
-    const unsigned gx = (unsigned)std::min<int64_t>(64, (mx + 255) / 256);
+    int mx = 0;
+    for (const SplitTask& t : splits) mx = std::max(mx, t.M);
+    const unsigned gx = (unsigned)std::max(1, std::min(mx, 64));
     hipLaunchKernelGGL(split_reduce_kernel, dim3(gx, (unsigned)splits.size()), dim3(256), 0, stream,
                        d_splits.data(), out);
     DMRG_HIP(hipGetLastError());
