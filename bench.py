@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--no-mixed-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-two-site", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuner)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,12 +293,19 @@ def main():
     assert reached
     log("FP64 run reached site %d of half sweep %d in %.1f s" % (site, nhalf - 1, t_to_site))
     plan = run64.heff(site)
-    info = plan.info()
-    for prec in (dm.FP64, dm.FP32, dm.BF16X3):
+    # per arithmetic: the tile autotuner picks the tiling of this plan, then
+    # the matvec is timed with it (the stage-2 roofline uses the BF16x3 tiling)
+    tm, tuned, infos = {}, {}, {}
+    for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3)):
+        if not args.no_tune:
+            tuned[name] = plan.tune(prec, reps=10)
         plan.time(prec, max(args.warmup, 1))
-    tm = {name: plan.time(prec, args.steps)
-          for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3))}
+        tm[name] = plan.time(prec, args.steps)
+        infos[name] = plan.info()
+    info = infos["bf16x3"]
     log("matvec timing done: " + ", ".join("%s %.3f ms" % (k, v["matvec_ms"]) for k, v in tm.items()))
+    if tuned:
+        log("autotuned tiling (tile, split-K): %s" % {k: v["best"] for k, v in tuned.items()})
     site_e = site_lanczos(run64, plan, site, tab[-1][2], dm)
     log("site Lanczos dE_rel: %s" % site_e["dE_rel"])
     two = None
@@ -346,7 +354,9 @@ def main():
                    "sweeps": args.sweeps, "psi_dim": info["psi_dim"], "flops_ref": info["flops_ref"],
                    "flops_exec": info["flops_exec"], "tiles": info["tiles"],
                    "tiles_stage1": info["tiles_stage1"], "tiles_stage2": info["tiles_stage2"],
-                   "bytes_min": info["bytes_min"]},
+                   "bytes_min": info["bytes_min"],
+                   "tiling": {k: list(v["best"]) for k, v in tuned.items()} or "default 64x64, split-K 256"},
+        "autotune": {k: v["candidates"] for k, v in tuned.items()} or None,
         "fp64": {"tflops": rate("fp64"), "ms": tm["fp64"]["matvec_ms"]},
         "fp32": {"tflops": rate("fp32"), "ms": tm["fp32"]["matvec_ms"]},
         # same H_eff, same start vector: energy of each arithmetic vs FP64

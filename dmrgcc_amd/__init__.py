@@ -135,6 +135,10 @@ _SIGS = {
     "dmrgcc_dmrg_result": (ctypes.c_int, [_vp, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
     "dmrgcc_dmrg_destroy": (ctypes.c_int, [_vp]),
     "dmrgcc_heff_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
+    "dmrgcc_heff_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                        _c_int_p, _c_dbl_p]),
+    "dmrgcc_heff_config": (ctypes.c_int, [_vp, _c_int_p, _c_int_p]),
+    "dmrgcc_gemm_config": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                   _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
@@ -642,6 +646,26 @@ class HeffPlan:
         return {"stage1_ms": ms[0], "stage2_ms": ms[1], "matvec_ms": ms[2],
                 "stage1_gemm_ms": ms[3], "stage2_gemm_ms": ms[4]}
 
+    def tune(self, precision=BF16X3, reps=10):
+        """Tile autotuner: time every (tile mode, split-K) candidate on this
+        plan and keep the fastest.  Returns {"best": (tile_mode, split_k),
+        "candidates": [(tile_mode, split_k, ms), ...]}."""
+        cap = 16
+        n = ctypes.c_int()
+        tm = np.zeros(cap, dtype=np.int32)
+        sk = np.zeros(cap, dtype=np.int32)
+        ms = np.zeros(cap)
+        _check(lib().dmrgcc_heff_tune(self._h, precision, reps, cap, ctypes.byref(n), _ip(tm), _ip(sk),
+                                      _dp(ms)))
+        cands = [(int(tm[i]), int(sk[i]), float(ms[i])) for i in range(n.value)]
+        return {"best": self.config(), "candidates": cands}
+
+    def config(self):
+        """(tile_mode, split_k) of the plan's current tiling."""
+        t, k = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dmrgcc_heff_config(self._h, ctypes.byref(t), ctypes.byref(k)))
+        return t.value, k.value
+
     def lanczos(self, ket, num_iter, precision=FP64, refine=True):
         e = ctypes.c_double()
         it = ctypes.c_int()
@@ -649,6 +673,13 @@ class HeffPlan:
         _check(lib().dmrgcc_lanczos(self._h, _ptr(ket), num_iter, precision, int(refine),
                                     ctypes.byref(e), ctypes.byref(it)))
         return e.value, it.value
+
+
+def gemm_config(tile_mode=-1, split_k=-1):
+    """Process-wide tiling (tile mode 32 or 64, split-K threshold) for GEMM
+    batches built from now on, e.g. a tuned plan's config(); -1 restores the
+    default of that field."""
+    _check(lib().dmrgcc_gemm_config(int(tile_mode), int(split_k)))
 
 
 class HeffBatch:

@@ -543,6 +543,38 @@ int dmrgcc_heff_destroy(dmrgcc_heff* h) {
   return 0;
 }
 
+int dmrgcc_heff_tune(dmrgcc_heff* h, int precision, int reps, int cap, int* ncand, int* tile_mode,
+                     int* split_k, double* ms) {
+  return guarded([&] {
+    require(h && ncand && tile_mode && split_k && ms, "null argument", kErrArgument);
+    require(precision >= 0 && precision <= 2 && reps > 0, "bad arguments", kErrArgument);
+    const std::vector<HeffPlan::Tuning> t = h->plan.tune(precision, reps, h->cx->stream);
+    require(cap >= (int)t.size(), "tune: output arrays too short", kErrArgument);
+    *ncand = (int)t.size();
+    for (size_t i = 0; i < t.size(); ++i) {
+      tile_mode[i] = t[i].tile_mode;
+      split_k[i] = t[i].split_k;
+      ms[i] = t[i].ms;
+    }
+  });
+}
+
+int dmrgcc_heff_config(const dmrgcc_heff* h, int* tile_mode, int* split_k) {
+  return guarded([&] {
+    require(h && tile_mode && split_k, "null argument", kErrArgument);
+    *tile_mode = h->plan.tile_mode() == 32 ? 32 : 64;
+    *split_k = h->plan.split_k();
+  });
+}
+
+int dmrgcc_gemm_config(int tile_mode, int split_k) {
+  return guarded([&] {
+    require(tile_mode < 0 || tile_mode == 0 || tile_mode == 32 || tile_mode == 64, "tile mode: 32 or 64",
+            kErrArgument);
+    set_gemm_config(tile_mode, split_k);
+  });
+}
+
 int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int refine,
                    double* energy, int* iters_out) {
   return guarded([&] {

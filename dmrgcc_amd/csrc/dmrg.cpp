@@ -327,6 +327,8 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   // may share a sector pair (the two-site state, one block per middle sector).
   s1_.clear();
   s2_.clear();
+  s1_.split_k = s2_.split_k = default_split_k();
+  s1_.tile_mode = s2_.tile_mode = default_tile_mode();
   psi_dim_ = ket.size;
   // complex plans (ComplexTensorContraction::MultiplyEffectHamilton,
   // complex_tensor_contraction.cc:1244-1312): same task structure, every
@@ -436,8 +438,63 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   // complex: 8 real flops per multiply-add and 16 bytes per entry
   fref_ = cplx_ ? 4.0 * fref : fref;
   bytes_min_ = 8 * cs * (2 * psi_dim_ + lbytes + rbytes);
+  items1_ = s1_.items;
+  terms1_ = s1_.contribs;
+  items2_ = s2_.items;
+  terms2_ = s2_.contribs;
   s1_.finalize(stream);
   s2_.finalize(stream);
+}
+
+void HeffPlan::retile(int tile_mode, int split_k, hipStream_t stream) {
+  // the previous launches may still read the descriptors being replaced
+  DMRG_HIP(hipStreamSynchronize(stream));
+  GemmBatch* st[2] = {&s1_, &s2_};
+  const std::vector<GemmItem>* it[2] = {&items1_, &items2_};
+  const std::vector<GemmContrib>* tm[2] = {&terms1_, &terms2_};
+  for (int k = 0; k < 2; ++k) {
+    GemmBatch& g = *st[k];
+    g.items = *it[k];
+    g.contribs = *tm[k];
+    g.tile_mode = tile_mode == 32 ? 32 : 64;
+    g.split_k = std::max(0, split_k);
+    g.finalize(stream);
+  }
+}
+
+std::vector<HeffPlan::Tuning> HeffPlan::tune(int precision, int reps, hipStream_t stream) {
+  require(reps > 0, "tune: reps must be positive", kErrArgument);
+  static const int kCand[][2] = {{64, 128}, {64, 256}, {64, 512}, {64, 1024}, {32, 256}, {32, 512}};
+  const int64_t n = std::max<int64_t>(doubles(), 1);
+  DevBuf<double> x((size_t)n), y((size_t)n);
+  fill(x.data(), n, 1e-3, stream);
+  fill(y.data(), n, 0.0, stream);
+  hipEvent_t e0, e1;
+  DMRG_HIP(hipEventCreate(&e0));
+  DMRG_HIP(hipEventCreate(&e1));
+  std::vector<Tuning> out;
+  for (const auto& c : kCand) {
+    retile(c[0], c[1], stream);
+    apply(x.data(), y.data(), precision, stream);  // warm-up
+    apply(x.data(), y.data(), precision, stream);
+    DMRG_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) apply(x.data(), y.data(), precision, stream);
+    DMRG_HIP(hipEventRecord(e1, stream));
+    DMRG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DMRG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    out.push_back(Tuning{c[0], c[1], ms / reps});
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the default tiling (64x64, split-K 256) stays unless a candidate beats it
+  // by more than 3 %, so that close calls do not flip between runs
+  size_t best = 1;
+  for (size_t i = 0; i < out.size(); ++i)
+    if (out[i].ms < out[best].ms) best = i;
+  if (out[best].ms > 0.97 * out[1].ms) best = 1;
+  retile(out[best].tile_mode, out[best].split_k, stream);
+  return out;
 }
 
 void HeffPlan::apply(const double* x, double* y, int precision, hipStream_t stream) {

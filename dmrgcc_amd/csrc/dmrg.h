@@ -110,8 +110,23 @@ class HeffPlan {
   const GemmBatch& stage1() const { return s1_; }
   const GemmBatch& stage2() const { return s2_; }
 
+  // Tile autotuner: re-tile both stages with every (tile mode, split-K)
+  // candidate, time `reps` back-to-back matvecs of `precision` each, keep
+  // the fastest (the default 64x64 / split-K 256 unless another is more
+  // than 3 % faster).  Returns the timing of every candidate (in order); the
+  // plan ends up tiled with the chosen one.
+  struct Tuning { int tile_mode, split_k; double ms; };
+  std::vector<Tuning> tune(int precision, int reps, hipStream_t stream);
+  // re-tile both stages (tile_mode 32 or 64, split-K threshold)
+  void retile(int tile_mode, int split_k, hipStream_t stream);
+  int tile_mode() const { return s1_.tile_mode; }
+  int split_k() const { return s1_.split_k; }
+
  private:
   GemmBatch s1_, s2_;
+  // items and terms of both stages before tiling, kept for retile()
+  std::vector<GemmItem> items1_, items2_;
+  std::vector<GemmContrib> terms1_, terms2_;
   DevBuf<double> T_;
   bool cplx_ = false;
   int64_t psi_dim_ = 0;

@@ -55,12 +55,17 @@ struct GemmTile {
   int item, m0, n0, pad;
 };
 
-// Summed-K threshold above which finalize() splits an item: 512, or the
-// value of the environment variable DMRGCC_SPLIT_K (0 disables splitting).
+// Summed-K threshold above which finalize() splits an item: 256, or the
+// value of the environment variable DMRGCC_SPLIT_K (0 disables splitting),
+// or the value set by set_gemm_config.
 int default_split_k();
-// Tile shape of finalize(): the environment variable DMRGCC_TILE (32 or 64;
-// 0 when unset, which means 64).
+// Tile shape of finalize(): 32 or 64 (0 means 64), from set_gemm_config, else
+// the environment variable DMRGCC_TILE.
 int default_tile_mode();
+// Process-wide GEMM configuration for batches built from now on (the tile
+// autotuner's choice); a negative value restores the environment / built-in
+// default of that field.
+void set_gemm_config(int tile_mode, int split_k);
 
 // Split-K of one heavy output item: partial sums of chunks 1..n-1 land in
 // `part` (n-1 consecutive M x N buffers) and are added into C afterwards in

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1754,16 +1755,29 @@ void GemmBatch::add(const double* A, int64_t lda, const double* B, int64_t ldb, 
   flops += 2.0 * items.back().M * (double)items.back().N * K;
 }
 
+namespace {
+std::atomic<int> g_split_k{-1}, g_tile_mode{-1};
+}  // namespace
+
 int default_split_k() {
+  const int g = g_split_k.load();
+  if (g >= 0) return g;
   const char* e = std::getenv("DMRGCC_SPLIT_K");
   return e ? std::max(0, std::atoi(e)) : 256;
 }
 
 int default_tile_mode() {
+  const int g = g_tile_mode.load();
+  if (g >= 0) return g;
   const char* e = std::getenv("DMRGCC_TILE");
   if (!e) return 0;
   const int v = std::atoi(e);
   return (v == 32 || v == 64) ? v : 0;
+}
+
+void set_gemm_config(int tile_mode, int split_k) {
+  g_tile_mode = tile_mode < 0 ? -1 : (tile_mode == 32 ? 32 : 64);
+  g_split_k = split_k < 0 ? -1 : split_k;
 }
 
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {

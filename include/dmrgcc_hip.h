@@ -184,6 +184,22 @@ int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision)
 int dmrgcc_heff_info(const dmrgcc_heff* h, double* info);
 int dmrgcc_heff_destroy(dmrgcc_heff* h);
 
+/* Tile autotuner (no counterpart in the reference, whose MKL calls have no
+ * tiles): re-tiles the plan with every candidate (tile mode 64 = 64x64
+ * workgroup tiles or 32 = 32x32 wave tiles; split-K threshold), times `reps`
+ * back-to-back matvecs of `precision` each with HIP events and keeps the
+ * fastest (the default 64 / 256 unless another candidate is more than 3 %
+ * faster).  Writes *ncand entries (cap >= 8) of tile_mode / split_k / ms.
+ * Re-tiling invalidates H_eff batches built from this plan. */
+int dmrgcc_heff_tune(dmrgcc_heff* h, int precision, int reps, int cap, int* ncand, int* tile_mode,
+                     int* split_k, double* ms);
+/* The plan's current tiling. */
+int dmrgcc_heff_config(const dmrgcc_heff* h, int* tile_mode, int* split_k);
+/* Process-wide tiling for GEMM batches built from now on (the autotuner's
+ * choice applied to later sites and sweeps); -1 restores the default of
+ * that field (environment DMRGCC_TILE / DMRGCC_SPLIT_K, else 64 / 256). */
+int dmrgcc_gemm_config(int tile_mode, int split_k);
+
 /* Lanczos ground state (RealTensorLanczos::LanczosMethod,
  * src/dmrg/realdmrg/real_tensor_lanczos.cc:14-169) on the plan's site:
  * `ket` (device, psi_dim) is the start vector and receives the Ritz vector.

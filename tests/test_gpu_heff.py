@@ -80,6 +80,43 @@ def test_heff_split_k(monkeypatch, split_k, prec, tile):
     assert err < (1e-12 if prec == dm.FP64 else 2e-4), err
 
 
+@pytest.mark.parametrize("prec", [dm.FP64, dm.BF16X3])
+def test_heff_tune_keeps_results(prec):
+    # the autotuner re-tiles the plan with every candidate and keeps the
+    # fastest; the tuned plan still gives the oracle's matvec, and the chosen
+    # tiling can be made the default of later plans
+    dmo, site = state("4x3_d4_left")
+    plan, lat = make_plan(dmo, site)
+    t = plan.tune(prec, reps=3)
+    cands = t["candidates"]
+    assert len(cands) >= 4 and {c[0] for c in cands} == {32, 64}
+    best = min(cands, key=lambda c: c[2])
+    dflt = [c for c in cands if c[:2] == (64, 256)][0]
+    if best[2] > 0.97 * dflt[2]:
+        best = dflt
+    assert t["best"] == (best[0], best[1]) == plan.config()
+    n = lat.ket.dim()
+    x = np.random.default_rng(8).standard_normal(n)
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, np.zeros(n))
+    for tile, sk, _ in cands[:2] + [best]:
+        dm.gemm_config(tile, sk)
+        try:
+            p2, _ = make_plan(dmo, site)
+            assert p2.config() == (tile, sk)
+        finally:
+            dm.gemm_config(-1, -1)
+        yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        p2.apply(torch.tensor(x, device="cuda"), yd, prec)
+        err = np.abs(yd.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < (1e-12 if prec == dm.FP64 else 2e-4), (tile, sk, err)
+    yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    plan.apply(torch.tensor(x, device="cuda"), yd, prec)
+    err = np.abs(yd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < (1e-12 if prec == dm.FP64 else 2e-4), err
+    p3, _ = make_plan(dmo, site)
+    assert p3.config() == (64, 256)  # defaults restored
+
+
 @pytest.mark.parametrize("prec,tol", [(dm.FP32, 1e-5), (dm.BF16X3, 2e-4)])
 def test_heff_mixed_within_tolerance(prec, tol):
     dmo, site = state("4x3_d4_left")
