@@ -1580,6 +1580,9 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       if (c < bw) {
         double* y = pan + (int64_t)c * ld + j;
         double d = 0.0;
+        // unrolled so that the LDS reads of four iterations are in flight
+        // together (same accumulation order)
+#pragma unroll 4
         for (int i = 1 + sub; i < len; i += 16) d += x[i] * y[i];
         d += __shfl_xor(d, 8, 64);
         d += __shfl_xor(d, 4, 64);
@@ -1587,6 +1590,7 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
         d += __shfl_xor(d, 1, 64);
         d = (y[0] + scal * d) * tj;  // tau v^T y
         const double ds = d * scal;
+#pragma unroll 4
         for (int i = 1 + sub; i < len; i += 16) y[i] -= ds * x[i];
         if (sub == 0) y[0] -= d;
       }
