@@ -334,6 +334,21 @@ def main():
 
     bf = tm["bf16x3"]
     s2_flops = info["flops_stage2"]
+
+    def tiling(name):
+        return tuned[name]["best"] if name in tuned else (64, 256)
+
+    def s2_kernel(name):
+        wave = tiling(name)[0] == 32
+        base = {"fp64": "gemm_%s<double>", "bf16x3": "gemm_bf16x3%s"}[name]
+        if name == "fp64":
+            return base % ("wave_kernel" if wave else "mfma_kernel")
+        return base % ("_wave_kernel" if wave else "_kernel")
+
+    def s2_grid(name):
+        # 32x32 wave tiles run four tiles per workgroup
+        t = infos[name]["tiles_stage2"]
+        return t // 4 if tiling(name)[0] == 32 else t
     achieved = s2_flops / (bf["stage2_gemm_ms"] * 1e-3) / 1e12
     achieved64 = s2_flops / (tm["fp64"]["stage2_gemm_ms"] * 1e-3) / 1e12
     sweeps64 = [float(r64["seconds"][2 * i] + r64["seconds"][2 * i + 1]) for i in range(len(r64["seconds"]) // 2)]
@@ -374,17 +389,17 @@ def main():
         },
         # dominant kernel: the stage-2 GEMM launch (out += w L T), timed alone
         # with HIP events; algorithmic flops = the stage-2 share of F_ref
-        "roofline": {"bound": "mfma", "kernel": "gemm_bf16x3_kernel (H_eff stage 2)",
+        "roofline": {"bound": "mfma", "kernel": s2_kernel("bf16x3") + " (H_eff stage 2)",
                      "achieved": achieved, "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["bf16"],
                      "traffic": traffic_bytes() if args.workload == "cylinder" else None,
                      "kernel_ms": bf["stage2_gemm_ms"], "stage2_launch_ms": bf["stage2_ms"],
-                     "stage2_flops": s2_flops, "grid_workgroups": info["tiles_stage2"]},
-        "roofline_fp64": {"bound": "mfma", "kernel": "gemm_mfma_kernel<double> (H_eff stage 2)",
+                     "stage2_flops": s2_flops, "grid_workgroups": s2_grid("bf16x3")},
+        "roofline_fp64": {"bound": "mfma", "kernel": s2_kernel("fp64") + " (H_eff stage 2)",
                           "achieved": achieved64, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
                           "frac": achieved64 / PEAK_TFLOPS["fp64"],
                           "kernel_ms": tm["fp64"]["stage2_gemm_ms"],
-                          "stage2_launch_ms": tm["fp64"]["stage2_ms"]},
+                          "stage2_launch_ms": tm["fp64"]["stage2_ms"], "grid_workgroups": s2_grid("fp64")},
         "cpu_baseline": cpu,
     }
     if rmx is not None:

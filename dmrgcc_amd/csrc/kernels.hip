@@ -482,12 +482,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
 // its staged operands, fences its own LDS traffic and reads them back.  For
 // block-sparse operands with 50-600 rows this wastes less of every tile than
 // 64x64 tiles and gives a launch four times as many independent tiles.
-// Staging of one K-step (32 x 32 of A, 32 x 32 of B, 16 elements each per
-// lane), element q of lane l:
-//   A, not transposed: m = l/2,  k = 16*(l%2) + q      (a row's 16 k)
-//   A, transposed:     m = l%32, k = 16*(l/32) + q     (32 lanes read 32 consecutive m)
-//   B, not transposed: n = l%32, k = 16*(l/32) + q
-//   B, transposed:     n = l/2,  k = 16*(l%2) + q
+// One K-step stages 32 x 32 of A and 32 x 32 of B, 16 elements of each per
+// lane (mapping at wave_load).
 // ---------------------------------------------------------------------------
 constexpr int kWT = 32;  // wave tile edge
 
@@ -495,34 +491,94 @@ struct WaveRegs {
   double a[16], b[16];
 };
 
+// The wave-tile staging of a K-step, element q of lane l:
+//   k contiguous (A not transposed, B transposed): row l/16 + 4*(q/2),
+//     k = 2*(l%16) + q%2 (four 256-byte row segments per load instruction);
+//   k-major (A transposed, B not transposed): row-run index l%32,
+//     k = 16*(l/32) + q (32 lanes read 32 consecutive elements).
+__device__ inline void wave_rows_k(gcdptr X, int64_t ld, int es, int r0, int R, int k0, int K, int lane,
+                                   double sc, double (&v)[16]) {
+  const int kk = k0 + 2 * (lane & 15);
+  const int ka = kk < K ? kk : K - 1, kb = kk + 1 < K ? kk + 1 : K - 1;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = r0 + (lane >> 4) + 4 * i;
+    double x0 = 0.0, x1 = 0.0;
+    if (r < R) {
+      const gcdptr row = X + (int64_t)r * ld;
+      x0 = row[(int64_t)ka * es];
+      x1 = row[(int64_t)kb * es];
+    }
+    v[2 * i] = (r < R && kk < K) ? x0 * sc : 0.0;
+    v[2 * i + 1] = (r < R && kk + 1 < K) ? x1 * sc : 0.0;
+  }
+}
+
+__device__ inline void wave_k_major(gcdptr X, int64_t ld, int es, int c0, int C, int k0, int K, int lane,
+                                    double sc, double (&v)[16]) {
+  const int gc = c0 + (lane & 31);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int kb = k0 + (lane >> 5) * 16 + 8 * j;
+    double t[8];
+    load_run8(X + (int64_t)clamp_hi(kb, K) * ld + (int64_t)clamp_hi(gc, C) * es, ld, run_len(gc, C, kb, K),
+              sc, t);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[8 * j + q] = t[q];
+  }
+}
+
 __device__ inline void wave_load(const GemmContrib& cb, const double* __restrict__ in_base, int m0,
                                  int n0, int M, int N, int k0, int lane, WaveRegs& r) {
-  const double* __restrict__ A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
-  const double* __restrict__ B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  gcdptr A = gload_ptr((cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A);
+  gcdptr B = gload_ptr((cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B);
   const int K = cb.K;
-  const double sc = cb.scale;
   const int es = (cb.flags & kStrided2) ? 2 : 1;
-  if (!(cb.flags & kTransA)) {
-    const int gm = m0 + (lane >> 1), kb = k0 + (lane & 1) * 16;
-    const double* row = A + (int64_t)gm * cb.lda;
+  if (!(cb.flags & kTransA)) wave_rows_k(A, cb.lda, es, m0, M, k0, K, lane, cb.scale, r.a);
+  else wave_k_major(A, cb.lda, es, m0, M, k0, K, lane, cb.scale, r.a);
+  if (!(cb.flags & kTransB)) wave_k_major(B, cb.ldb, es, n0, N, k0, K, lane, 1.0, r.b);
+  else wave_rows_k(B, cb.ldb, es, n0, N, k0, K, lane, 1.0, r.b);
+}
+
+// one operand of a wave K-step into a k-major tile X[k][row] (FP64 / FP32)
+template <typename T, int LD>
+__device__ inline void wave_store_kmajor(bool kcont, int lane, const double (&v)[16], T (*X)[LD]) {
+  if (kcont) {
+    const int row = lane >> 4, k = 2 * (lane & 15);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) r.a[q] = (gm < M && kb + q < K) ? row[(kb + q) * es] * sc : 0.0;
+    for (int i = 0; i < 8; ++i) {
+      X[k][row + 4 * i] = (T)v[2 * i];
+      X[k + 1][row + 4 * i] = (T)v[2 * i + 1];
+    }
   } else {
-    const int gm = m0 + (lane & 31), kb = k0 + (lane >> 5) * 16;
+    const int row = lane & 31, kb = (lane >> 5) * 16;
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      r.a[q] = (gm < M && kb + q < K) ? A[(int64_t)(kb + q) * cb.lda + gm * es] * sc : 0.0;
+    for (int q = 0; q < 16; ++q) X[kb + q][row] = (T)v[q];
   }
-  if (!(cb.flags & kTransB)) {
-    const int gn = n0 + (lane & 31), kb = k0 + (lane >> 5) * 16;
+}
+
+// one operand of a wave K-step into hi/lo planes X[row][k] (BF16x3)
+template <int LD>
+__device__ inline void wave_store_planes(bool kcont, int lane, const double (&v)[16], __bf16 (*Xh)[LD],
+                                         __bf16 (*Xl)[LD]) {
+  if (kcont) {
+    const int row = lane >> 4, k = 2 * (lane & 15);
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      r.b[q] = (gn < N && kb + q < K) ? B[(int64_t)(kb + q) * cb.ldb + gn * es] : 0.0;
+    for (int i = 0; i < 8; ++i) {
+      bf2v h, l;
+      split2(v[2 * i], v[2 * i + 1], h, l);
+      *reinterpret_cast<bf2v*>(&Xh[row + 4 * i][k]) = h;
+      *reinterpret_cast<bf2v*>(&Xl[row + 4 * i][k]) = l;
+    }
   } else {
-    const int gn = n0 + (lane >> 1), kb = k0 + (lane & 1) * 16;
-    const double* row = B + (int64_t)gn * cb.ldb;
+    const int row = lane & 31, kb = (lane >> 5) * 16;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) r.b[q] = (gn < N && kb + q < K) ? row[(kb + q) * es] : 0.0;
+    for (int j = 0; j < 2; ++j) {
+      bf8v h, l;
+      split8(v + 8 * j, h, l);
+      *reinterpret_cast<bf8v*>(&Xh[row][kb + 8 * j]) = h;
+      *reinterpret_cast<bf8v*>(&Xl[row][kb + 8 * j]) = l;
+    }
   }
 }
 
@@ -587,24 +643,8 @@ __global__ __launch_bounds__(kThreads) void gemm_wave_kernel(
   WaveRegs r;
   if (live) wave_load(cb, in_base, m0, n0, M, N, 0, lane, r);
   while (live) {
-    if (!(cb.flags & kTransA)) {
-      const int m = lane >> 1, kb = (lane & 1) * 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) as[kb + q][m] = (T)r.a[q];
-    } else {
-      const int m = lane & 31, kb = (lane >> 5) * 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) as[kb + q][m] = (T)r.a[q];
-    }
-    if (!(cb.flags & kTransB)) {
-      const int n = lane & 31, kb = (lane >> 5) * 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) bs[kb + q][n] = (T)r.b[q];
-    } else {
-      const int n = lane >> 1, kb = (lane & 1) * 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) bs[kb + q][n] = (T)r.b[q];
-    }
+    wave_store_kmajor<T, LD>(!(cb.flags & kTransA), lane, r.a, as);
+    wave_store_kmajor<T, LD>((cb.flags & kTransB) != 0, lane, r.b, bs);
     wave_lds_sync();
     GemmContrib nc = cb;
     int c2 = c, k2 = k0;
@@ -630,7 +670,7 @@ __global__ __launch_bounds__(kThreads) void gemm_wave_kernel(
 }
 
 // BF16x3 wave-tile kernel: per-wave hi/lo planes m-major with k contiguous,
-// as in gemm_bf16x3_kernel; each lane writes its 16 k as two 16-byte stores.
+// as in gemm_bf16x3_kernel.
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
@@ -666,25 +706,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
   WaveRegs r;
   if (live) wave_load(cb, in_base, m0, n0, M, N, 0, lane, r);
   while (live) {
-    {
-      const int am = (cb.flags & kTransA) ? (lane & 31) : (lane >> 1);
-      const int ak = (cb.flags & kTransA) ? (lane >> 5) * 16 : (lane & 1) * 16;
-      const int bn = (cb.flags & kTransB) ? (lane >> 1) : (lane & 31);
-      const int bk = (cb.flags & kTransB) ? (lane & 1) * 16 : (lane >> 5) * 16;
-      bf8v h, l;
-      split8(r.a, h, l);
-      *reinterpret_cast<bf8v*>(&ah_[am][ak]) = h;
-      *reinterpret_cast<bf8v*>(&al_[am][ak]) = l;
-      split8(r.a + 8, h, l);
-      *reinterpret_cast<bf8v*>(&ah_[am][ak + 8]) = h;
-      *reinterpret_cast<bf8v*>(&al_[am][ak + 8]) = l;
-      split8(r.b, h, l);
-      *reinterpret_cast<bf8v*>(&bh_[bn][bk]) = h;
-      *reinterpret_cast<bf8v*>(&bl_[bn][bk]) = l;
-      split8(r.b + 8, h, l);
-      *reinterpret_cast<bf8v*>(&bh_[bn][bk + 8]) = h;
-      *reinterpret_cast<bf8v*>(&bl_[bn][bk + 8]) = l;
-    }
+    wave_store_planes<LD>(!(cb.flags & kTransA), lane, r.a, ah_, al_);
+    wave_store_planes<LD>((cb.flags & kTransB) != 0, lane, r.b, bh_, bl_);
     wave_lds_sync();
     GemmContrib nc = cb;
     int c2 = c, k2 = k0;
