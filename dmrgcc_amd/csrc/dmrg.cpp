@@ -438,6 +438,7 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   // complex: 8 real flops per multiply-add and 16 bytes per entry
   fref_ = cplx_ ? 4.0 * fref : fref;
   bytes_min_ = 8 * cs * (2 * psi_dim_ + lbytes + rbytes);
+  ++gen_;
   items1_ = s1_.items;
   terms1_ = s1_.contribs;
   items2_ = s2_.items;
@@ -449,6 +450,7 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
 void HeffPlan::retile(int tile_mode, int split_k, hipStream_t stream) {
   // the previous launches may still read the descriptors being replaced
   DMRG_HIP(hipStreamSynchronize(stream));
+  ++gen_;
   GemmBatch* st[2] = {&s1_, &s2_};
   const std::vector<GemmItem>* it[2] = {&items1_, &items2_};
   const std::vector<GemmContrib>* tm[2] = {&terms1_, &terms2_};
@@ -2167,22 +2169,35 @@ void HeffBatch::build(const std::vector<HeffPlan*>& plans, const std::vector<con
   s1_.merge(b1, x, none_out, stream);
   s2_.merge(b2, none_in, y, stream);
   plans_ = plans;
+  gens_.clear();
+  for (HeffPlan* p : plans) gens_.push_back(p->generation());
   x_ = x;
   y_ = y;
   DMRG_HIP(hipStreamSynchronize(stream));
 }
 
+void HeffBatch::check_plans() const {
+  // a plan rebuilt or re-tiled since build() no longer owns the buffers and
+  // descriptors this batch points at
+  for (size_t i = 0; i < plans_.size(); ++i)
+    require(plans_[i]->generation() == gens_[i], "H_eff batch: a plan was rebuilt or re-tiled after the batch",
+            kErrArgument);
+}
+
 void HeffBatch::apply_each(int precision, hipStream_t stream) {
+  check_plans();
   for (size_t i = 0; i < plans_.size(); ++i) plans_[i]->apply(x_[i], y_[i], precision, stream);
 }
 
 void HeffBatch::apply(int precision, hipStream_t stream) {
+  check_plans();
   s1_.launch(stream, precision);
   s2_.launch(stream, precision);
 }
 
 void HeffBatch::apply_graph(int precision, hipStream_t stream) {
   require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+  check_plans();  // before any capture starts
   if (!exec_[precision]) {
     // kernels only (no memset nodes); captured on this thread's stream
     DMRG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

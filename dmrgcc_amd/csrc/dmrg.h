@@ -121,12 +121,15 @@ class HeffPlan {
   void retile(int tile_mode, int split_k, hipStream_t stream);
   int tile_mode() const { return s1_.tile_mode; }
   int split_k() const { return s1_.split_k; }
+  // bumped by every build and retile: batches built from the plan check it
+  int generation() const { return gen_; }
 
  private:
   GemmBatch s1_, s2_;
   // items and terms of both stages before tiling, kept for retile()
   std::vector<GemmItem> items1_, items2_;
   std::vector<GemmContrib> terms1_, terms2_;
+  int gen_ = 0;
   DevBuf<double> T_;
   bool cplx_ = false;
   int64_t psi_dim_ = 0;
@@ -154,6 +157,8 @@ class HeffBatch {
  private:
   GemmBatch s1_, s2_;
   std::vector<HeffPlan*> plans_;
+  std::vector<int> gens_;  // the plans' generations at build time
+  void check_plans() const;
   std::vector<const double*> x_;
   std::vector<double*> y_;
   double fref_ = 0;

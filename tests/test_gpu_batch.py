@@ -52,3 +52,23 @@ def test_batch_equals_plan_by_plan(prec):
         b.apply(prec, graph=True)
         for y, r in zip(ys, refs):
             assert np.array_equal(y.cpu().numpy(), r), rep
+
+
+def test_batch_refuses_retiled_plan():
+    # re-tiling a plan replaces the descriptors and buffers a batch built from
+    # it points at: the batch then refuses to launch, graph or not
+    runs, plans = _instances()
+    xs, ys = [], []
+    for p in plans:
+        n = p.info()["psi_dim"] * (2 if p.complex else 1)
+        xs.append(torch.zeros(n, dtype=torch.float64, device="cuda"))
+        ys.append(torch.zeros(n, dtype=torch.float64, device="cuda"))
+    b = dm.HeffBatch(plans, xs, ys)
+    b.apply(dm.FP64, graph=False)
+    plans[1].tune(dm.FP64, reps=1)
+    for graph in (False, True):
+        with pytest.raises(dm.DmrgError, match="re-tiled"):
+            b.apply(dm.FP64, graph=graph)
+    b2 = dm.HeffBatch(plans, xs, ys)  # a new batch is fine
+    b2.apply(dm.FP64, graph=True)
+    torch.cuda.synchronize()
