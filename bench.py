@@ -45,6 +45,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "H_eff matvec TFLOP/s + sweep time at m=1000, mixed vs FP64; dE vs FP64 oracle"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_GBS = 8000.0  # HBM3E
 
 
 def table(maxm, sweeps, niter=10, canpre=1e-10, maxb=400):
@@ -306,6 +307,23 @@ def main():
     log("matvec timing done: " + ", ".join("%s %.3f ms" % (k, v["matvec_ms"]) for k, v in tm.items()))
     if tuned:
         log("autotuned tiling (tile, split-K): %s" % {k: v["best"] for k, v in tuned.items()})
+    # HBM-side kernels of the same state: environment updates and the Lanczos
+    # vector ops, bytes per launch / GPU time against the 8 TB/s peak
+    envs = {}
+    for name, leigh in (("left", 1), ("right", 2)):
+        run64.env_time(site, leigh, 1)
+        e = run64.env_time(site, leigh, 5)
+        e["GB_per_s"] = e["bytes_min"] / (e["ms"] * 1e-3) / 1e9
+        e["frac_hbm"] = e["GB_per_s"] / PEAK_GBS
+        e["TFLOP_per_s"] = e["flops"] / (e["ms"] * 1e-3) / 1e12
+        envs[name] = e
+    vt = dm.vector_time(info["psi_dim"] * (2 if model.complex else 1), 50)
+    vec = {k: {"ms": vt[k + "_ms"], "GB_per_s": vt["bytes"][k] / (vt[k + "_ms"] * 1e-3) / 1e9,
+               "frac_hbm": vt["bytes"][k] / (vt[k + "_ms"] * 1e-3) / 1e9 / PEAK_GBS}
+           for k in ("dot", "axpy", "lanczos_step")}
+    log("env update GB/s: left %.0f, right %.0f; vector ops GB/s: %s" % (
+        envs["left"]["GB_per_s"], envs["right"]["GB_per_s"],
+        {k: round(v["GB_per_s"]) for k, v in vec.items()}))
     site_e = site_lanczos(run64, plan, site, tab[-1][2], dm)
     log("site Lanczos dE_rel: %s" % site_e["dE_rel"])
     two = None
@@ -400,6 +418,10 @@ def main():
                           "frac": achieved64 / PEAK_TFLOPS["fp64"],
                           "kernel_ms": tm["fp64"]["stage2_gemm_ms"],
                           "stage2_launch_ms": tm["fp64"]["stage2_ms"], "grid_workgroups": s2_grid("fp64")},
+        # environment updates of the centre site (two grouped FP64 GEMM launches
+        # each) and the Lanczos vector kernels on psi_dim vectors
+        "env_update": envs,
+        "vector_ops": vec,
         "cpu_baseline": cpu,
     }
     if rmx is not None:

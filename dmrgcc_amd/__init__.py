@@ -139,6 +139,8 @@ _SIGS = {
                                         _c_int_p, _c_dbl_p]),
     "dmrgcc_heff_config": (ctypes.c_int, [_vp, _c_int_p, _c_int_p]),
     "dmrgcc_gemm_config": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "dmrgcc_dmrg_env_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
+    "dmrgcc_vector_time": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_dbl_p]),
     "dmrgcc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                   _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
@@ -488,6 +490,16 @@ class DmrgRun:
         _check(lib().dmrgcc_dmrg_heff(self._h, site, ctypes.byref(h)))
         return HeffPlan._from_handle(h, self)
 
+    def env_time(self, site, leigh, reps=5):
+        """GPU time of the environment update of `site` on the current state
+        (leigh 1: left, from L_site; 2: right, from R_site): mean ms of the F
+        launch and of the output launch, the bytes with every used block read
+        or written once (and with the intermediate F), executed flops."""
+        v = np.zeros(5)
+        _check(lib().dmrgcc_dmrg_env_time(self._h, site, leigh, reps, _dp(v)))
+        return {"f_ms": v[0], "out_ms": v[1], "ms": v[0] + v[1], "bytes_min": v[2], "bytes_exec": v[3],
+                "flops": v[4]}
+
     def heff2(self, site):
         """Two-site H_eff of (site, site+1) on the current state (a HeffPlan)
         and its two-site state Theta = A_site B_site+1 as a device tensor."""
@@ -673,6 +685,15 @@ class HeffPlan:
         _check(lib().dmrgcc_lanczos(self._h, _ptr(ket), num_iter, precision, int(refine),
                                     ctypes.byref(e), ctypes.byref(it)))
         return e.value, it.value
+
+
+def vector_time(n, reps=50):
+    """Mean ms of the Lanczos vector kernels on n-double vectors: the
+    two-pass dot (16 n bytes), axpy (24 n) and the Lanczos step (32 n)."""
+    ms = np.zeros(3)
+    _check(lib().dmrgcc_vector_time(int(n), int(reps), _dp(ms)))
+    return {"dot_ms": ms[0], "axpy_ms": ms[1], "lanczos_step_ms": ms[2],
+            "bytes": {"dot": 16.0 * n, "axpy": 24.0 * n, "lanczos_step": 32.0 * n}}
 
 
 def gemm_config(tile_mode=-1, split_k=-1):

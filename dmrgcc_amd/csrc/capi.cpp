@@ -1036,6 +1036,56 @@ int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h) {
   return 0;
 }
 
+int dmrgcc_dmrg_env_time(dmrgcc_dmrg* h, int site, int leigh, int reps, double* out) {
+  return guarded([&] {
+    require(h && out && reps > 0 && (leigh == 1 || leigh == 2), "bad arguments", kErrArgument);
+    RealTensorRundmrg& r = *h->run;
+    require(site >= 0 && site < r.num_site(), "site out of range", kErrArgument);
+    double f = 0, o = 0;
+    EnvTiming tm;
+    for (int i = 0; i < reps; ++i) {
+      tm = EnvTiming{};
+      Env e = leigh == 1 ? left_env_update(r.lattice(site), r.op(site), r.left_env(site), r.context(), &tm)
+                         : right_env_update(r.lattice(site), r.op(site), r.right_env(site), r.context(), &tm);
+      f += tm.ms_f;
+      o += tm.ms_out;
+    }
+    out[0] = f / reps;
+    out[1] = o / reps;
+    out[2] = tm.bytes_min;
+    out[3] = tm.bytes_exec;
+    out[4] = tm.flops;
+  });
+}
+
+int dmrgcc_vector_time(int64_t n, int reps, double* ms) {
+  return guarded([&] {
+    require(ms && n > 0 && reps > 0, "bad arguments", kErrArgument);
+    Context cx(nullptr);
+    DevBuf<double> x((size_t)n), y((size_t)n), h((size_t)n), s(2);
+    fill(x.data(), n, 1e-3, cx.stream);
+    fill(y.data(), n, 2e-3, cx.stream);
+    fill(h.data(), n, 0.0, cx.stream);
+    fill(s.data(), 2, 0.5, cx.stream);
+    hipEvent_t e[2];
+    for (auto& ev : e) DMRG_HIP(hipEventCreate(&ev));
+    float t[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+      for (int rep = -1; rep < reps; ++rep) {  // rep -1: warm-up
+        if (rep == 0) DMRG_HIP(hipEventRecord(e[0], cx.stream));
+        if (k == 0) dot(x.data(), y.data(), n, s.data(), cx.partial, cx.stream);
+        else if (k == 1) axpy_dev(y.data(), x.data(), n, s.data() + 1, -1.0, cx.stream);
+        else lanczos_step(h.data(), y.data(), x.data(), n, 0.25, cx.stream);
+      }
+      DMRG_HIP(hipEventRecord(e[1], cx.stream));
+      DMRG_HIP(hipEventSynchronize(e[1]));
+      DMRG_HIP(hipEventElapsedTime(&t[k], e[0], e[1]));
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    for (int k = 0; k < 3; ++k) ms[k] = t[k] / reps;
+  });
+}
+
 int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms) {
   return guarded([&] {
     require(h && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);

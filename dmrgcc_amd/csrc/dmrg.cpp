@@ -14,6 +14,7 @@
 #include <fstream>
 #include <limits>
 #include <numeric>
+#include <set>
 #include <thread>
 
 #include "quicksort.h"
@@ -537,7 +538,8 @@ struct EnvTerm { int f; int kp; double w, wi; };
 
 void finish_env(Env& out, std::vector<std::vector<std::vector<EnvTerm>>>& terms,
                 const std::vector<Shape>& fshape, const std::vector<int64_t>& foff,
-                DevBuf<double>& F, const BlockTensor& ket, bool right, GemmBatch& s1, Context& cx) {
+                DevBuf<double>& F, const BlockTensor& ket, bool right, GemmBatch& s1, Context& cx,
+                EnvTiming* tm) {
   GemmBatch s2;
   for (size_t o = 0; o < out.size(); ++o) {
     BlockTensor& bt = out[o];
@@ -583,14 +585,52 @@ void finish_env(Env& out, std::vector<std::vector<std::vector<EnvTerm>>>& terms,
   }
   s1.finalize(cx.stream);
   s2.finalize(cx.stream);
+  if (!tm) {
+    s1.launch(cx.stream, kFP64);
+    s2.launch(cx.stream, kFP64);
+    cx.sync();
+    return;
+  }
+  hipEvent_t e[3];
+  for (auto& ev : e) DMRG_HIP(hipEventCreate(&ev));
+  DMRG_HIP(hipEventRecord(e[0], cx.stream));
   s1.launch(cx.stream, kFP64);
+  DMRG_HIP(hipEventRecord(e[1], cx.stream));
   s2.launch(cx.stream, kFP64);
+  DMRG_HIP(hipEventRecord(e[2], cx.stream));
+  DMRG_HIP(hipEventSynchronize(e[2]));
+  float a = 0, b = 0;
+  DMRG_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+  DMRG_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+  for (auto& ev : e) (void)hipEventDestroy(ev);
+  tm->ms_f = a;
+  tm->ms_out = b;
+  tm->flops = s1.flops + s2.flops;
+  // bytes: the caller adds the old environment and ket blocks
+  int64_t fsz = 0, osz = 0;
+  for (const Shape& sh : fshape) fsz += sh.size();
+  for (const BlockTensor& bt : out) osz += bt.size;
+  tm->bytes_min = 8.0 * ket.cs * (double)osz;
+  tm->bytes_exec = tm->bytes_min + 2 * 8.0 * ket.cs * (double)fsz;
   cx.sync();
+}
+
+// the used blocks of an old environment and of the ket, once each, in bytes
+void env_input_bytes(EnvTiming* tm, const Env& E, const std::vector<std::pair<int, int>>& used_e,
+                     const BlockTensor& ket, const std::vector<int>& used_k) {
+  if (!tm) return;
+  double b = 0;
+  std::set<std::pair<int, int>> se(used_e.begin(), used_e.end());
+  for (const auto& [o, a] : se) b += (double)E[o].blk[a].rows * E[o].blk[a].cols;
+  std::set<int> sk(used_k.begin(), used_k.end());
+  for (int k : sk) b += (double)ket.blk[k].rows * ket.blk[k].cols;
+  tm->bytes_min += 8.0 * ket.cs * b;
+  tm->bytes_exec += 8.0 * ket.cs * b;
 }
 
 }  // namespace
 
-Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx) {
+Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx, EnvTiming* tm) {
   // LeftComputeTensorContraction (real_tensor_contraction.cc:782-940)
   const BlockTensor& ket = lat.ket;
   const int nr = lat.nr();
@@ -676,11 +716,20 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
     s1.begin_item(Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
     s1.add(ket.ptr(ft[f].kp), kb.cols, Lb.ptr(ft[f].a), lbk.cols, kb.rows, kTransA, 1.0);
   }
-  finish_env(out, terms, fshape, foff, F, ket, false, s1, cx);
+  finish_env(out, terms, fshape, foff, F, ket, false, s1, cx, tm);
+  if (tm) {
+    std::vector<std::pair<int, int>> ue;
+    std::vector<int> uk;
+    for (const FTask& t : ft) { ue.push_back({t.o1, t.a}); uk.push_back(t.kp); }
+    for (const auto& to : terms)
+      for (const auto& tb : to)
+        for (const EnvTerm& t : tb) uk.push_back(t.kp);
+    env_input_bytes(tm, L, ue, ket, uk);
+  }
   return out;
 }
 
-Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx) {
+Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx, EnvTiming* tm) {
   // RightComputeTensorContraction (real_tensor_contraction.cc:959-1119)
   const BlockTensor& ket = lat.ket;
   const int nl = lat.nl();
@@ -765,7 +814,16 @@ Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Co
     s1.begin_item(Ff, fshape[f].cols, fshape[f].rows, fshape[f].cols, 0);
     s1.add(ket.ptr(ft[f].lp), kb.cols, Rb.ptr(ft[f].a), rbk.cols, kb.cols, 0, 1.0);
   }
-  finish_env(out, terms, fshape, foff, F, ket, true, s1, cx);
+  finish_env(out, terms, fshape, foff, F, ket, true, s1, cx, tm);
+  if (tm) {
+    std::vector<std::pair<int, int>> ue;
+    std::vector<int> uk;
+    for (const FTask& t : ft) { ue.push_back({t.o2, t.a}); uk.push_back(t.lp); }
+    for (const auto& to : terms)
+      for (const auto& tb : to)
+        for (const EnvTerm& t : tb) uk.push_back(t.kp);
+    env_input_bytes(tm, R, ue, ket, uk);
+  }
   return out;
 }
 

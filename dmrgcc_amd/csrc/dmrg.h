@@ -181,9 +181,19 @@ struct TwoSite {
 };
 
 // Environment updates (LeftComputeTensorContraction :782-940,
-// RightComputeTensorContraction :959-1119).
-Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx);
-Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx);
+// RightComputeTensorContraction :959-1119).  With `tm`, the two GEMM
+// launches (F = B^T L / F = W R B-part, then the new environment) are timed
+// with HIP events and their algorithmic bytes and flops are counted.
+struct EnvTiming {
+  double ms_f = 0, ms_out = 0;  // GPU time of the F launch and of the output launch
+  double bytes_min = 0;         // each used old-environment block, ket block and new block once
+  double bytes_exec = 0;        // bytes_min plus the intermediate F written and read back
+  double flops = 0;             // executed real flops of both launches
+};
+Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Context& cx,
+                    EnvTiming* tm = nullptr);
+Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx,
+                     EnvTiming* tm = nullptr);
 // Random matrices of the expansion in draw order (RandomMatrix,
 // real_matrix.cc:201-208; complex_matrix.cc:201-208 with cs = 2): matrix k
 // is rows[k] x cols[k], row-major (complex: interleaved), concatenated in

@@ -385,6 +385,21 @@ int dmrgcc_dmrg_destroy(dmrgcc_dmrg* h);
  * stage-2 GEMM kernel alone, back to back. */
 int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms);
 
+/* Time `reps` environment updates of `site` on the run's current state:
+ * leigh 1 = LeftComputeTensorContraction (real_tensor_contraction.cc:
+ * 782-940) from L_site, leigh 2 = RightComputeTensorContraction (:959-1119)
+ * from R_site; the results are discarded.  out (5 entries): [0] / [1] mean
+ * GPU ms of the F launch and of the output launch (HIP events), [2] bytes
+ * with every used old-environment block, ket block and new block read or
+ * written once, [3] the same plus the intermediate F written and read,
+ * [4] executed flops. */
+int dmrgcc_dmrg_env_time(dmrgcc_dmrg* h, int site, int leigh, int reps, double* out);
+/* Lanczos vector kernels on n-double device vectors, mean ms over `reps`
+ * back-to-back launches (HIP events): [0] the two-pass dot (reads 16 n B),
+ * [1] axpy with a device scalar (24 n B), [2] the Lanczos step
+ * psi_next = h / beta, h = -beta psi (32 n B).  No reference counterpart (its MKL BLAS-1 calls). */
+int dmrgcc_vector_time(int64_t n, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
