@@ -402,6 +402,7 @@ def main():
             "fp64_energies": [float(e) for e in r64["energies"]],
             "fp64_trunc_err": [float(e) for e in r64["trunc_err"]],
             "fp64_phase_seconds": r64["phase_seconds"],
+            "fp64_lanczos": {k: r64[k] for k in ("solves", "matvecs", "matvec_seconds")},
             "max_bond": r64["max_bond"],
             "rand_draws": r64["rand_draws"],
         },
@@ -430,6 +431,8 @@ def main():
         out["sweep"]["mixed_seconds_per_sweep"] = [float(rmx["seconds"][2 * i] + rmx["seconds"][2 * i + 1])
                                                    for i in range(len(rmx["seconds"]) // 2)]
         out["sweep"]["mixed_total_seconds"] = rmx["total_seconds"]
+        out["sweep"]["mixed_phase_seconds"] = rmx["phase_seconds"]
+        out["sweep"]["mixed_lanczos"] = {k: rmx[k] for k in ("solves", "matvecs", "matvec_seconds")}
         out["sweep"]["mixed_energies"] = [float(e) for e in emx]
         out["sweep"]["dE_rel"] = [float(v) for v in np.abs(emx - e64) / np.abs(e64)]
     print(json.dumps(out), flush=True)
