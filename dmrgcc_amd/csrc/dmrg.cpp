@@ -1367,18 +1367,18 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
   fill(h.data(), nd, 0.0, cx.stream);
   std::vector<double> alpha, beta;
   double old_energy = 10000.0, energy = 0.0;
+  // partial-sum buffers of the fused vector steps (two alternate in the
+  // Gram-Schmidt chain)
+  cx.partial.ensure(3 * 2 * kDotPartials);
+  double* part_a = cx.partial.data();
+  double* part_b = part_a + 2 * kDotPartials;
   for (int i = 0; i < num_iter; ++i) {
     double* pi = P + (int64_t)i * nd;
     const double t0 = now();
     plan.apply(pi, h.data(), precision, cx.stream);
-    if (cz) {
-      zdot(pi, h.data(), n, sc, cx.partial, cx.stream);            // alpha = <psi|H|psi>
-      zaxpy_dev(h.data(), pi, n, sc, -1.0, cx.stream);             // h -= alpha psi
-    } else {
-      dot(pi, h.data(), n, sc, cx.partial, cx.stream);             // alpha = <psi|H|psi>
-      axpy_dev(h.data(), pi, n, sc, -1.0, cx.stream);              // h -= alpha psi
-    }
-    dot(h.data(), h.data(), nd, sc + 2, cx.partial, cx.stream);    // beta^2 = Re <h|h>
+    lz_dot_partial(pi, h.data(), n, cz, part_a, cx.stream);     // alpha = <psi|H|psi>
+    lz_alpha(h.data(), pi, n, cz, part_a, part_b, cx.stream);    // h -= alpha psi; <h|h>
+    lz_finals(part_a, part_b, n, cz, sc, cx.stream);             // sc = (alpha, beta^2)
     double ab[3];
     DMRG_HIP(hipMemcpyAsync(ab, sc, 3 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
     cx.sync();
@@ -1395,7 +1395,16 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     }
     sym_eig(k, T, ev, vec);
     const double e_new = ev[0];
-    if (beta[i] < 1e-15 || std::fabs(e_new - old_energy) < 1e-15 || k == num_iter) {
+    // FP64 keeps the reference's stop (beta < 1e-15, |dE| < 1e-15, niter;
+    // real_tensor_lanczos.cc:110-112).  A mixed-precision run converges to
+    // the eigenvector of the rounded operator, whose FP64 Rayleigh quotient
+    // has an error floor of order (operand precision)^2 |H|; Ritz-value
+    // changes below eta |E| cannot lower the refined energy, so the loop
+    // stops there (eta: 1e-13 FP32, 1e-10 split BF16x3 with its ~16-bit
+    // operands) instead of running to niter against an unreachable 1e-15.
+    const double eta = precision == kFP32 ? 1e-13 : precision == kFP64 ? 0.0 : 1e-10;
+    const double de_tol = std::max(1e-15, eta * std::fabs(e_new));
+    if (beta[i] < 1e-15 || std::fabs(e_new - old_energy) < de_tol || k == num_iter) {
       energy = e_new;
       if (iters) *iters = k;
       std::vector<double> coef(k);
@@ -1421,20 +1430,21 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     old_energy = e_new;
     if (k < num_iter) {
       double* pn = P + (int64_t)(i + 1) * nd;
-      lanczos_step(pn, h.data(), pi, nd, beta[i], cx.stream);
-      // GramSchmidtMethod (real_tensor_lanczos.cc:188-201,
-      // complex_tensor_lanczos.cc:178-192)
+      // psi_{i+1} = h / beta, h = -beta psi_i, then GramSchmidtMethod
+      // (real_tensor_lanczos.cc:140-151, :188-201; complex_tensor_lanczos.cc:178-192):
+      // for j = 0..i, c = <psi_j|psi_{i+1}> on the updated vector, psi_{i+1} -= c psi_j;
+      // then psi_{i+1} /= |psi_{i+1}|.  One launch per projection: each
+      // forms c from the previous launch's partial sums and the partials
+      // of the next overlap.
+      double* pin = part_a;
+      double* pout = part_b;
+      lz_next(pn, h.data(), pi, P, n, cz, beta[i], pin, cx.stream);
       for (int j = 0; j <= i; ++j) {
-        if (cz) {
-          zdot(P + (int64_t)j * nd, pn, n, sc + 6, cx.partial, cx.stream);
-          zaxpy_dev(pn, P + (int64_t)j * nd, n, sc + 6, -1.0, cx.stream);
-        } else {
-          dot(P + (int64_t)j * nd, pn, n, sc + 6, cx.partial, cx.stream);
-          axpy_dev(pn, P + (int64_t)j * nd, n, sc + 6, -1.0, cx.stream);
-        }
+        const double* xnext = j < i ? P + (int64_t)(j + 1) * nd : nullptr;
+        lz_project(pn, P + (int64_t)j * nd, xnext, n, cz, pin, pout, cx.stream);
+        std::swap(pin, pout);
       }
-      dot(pn, pn, nd, sc + 8, cx.partial, cx.stream);
-      scale_inv_sqrt(pn, nd, sc + 8, cx.stream, 0);
+      lz_normalize(pn, n, cz, pin, cx.stream);
     }
   }
   return energy;

@@ -192,6 +192,33 @@ void combine(double* y, const double* V, int64_t stride, const double* coef, int
              int64_t n, hipStream_t stream);
 void fill(double* x, int64_t n, double v, hipStream_t stream);
 
+// ---- fused Lanczos vector steps ----
+// Each keeps the decomposition of dot(): per-block partial sums over the
+// grid of dot_grid(n) blocks (complex: (re, im) pairs of sum conj(x) y over
+// entries), and every block that needs a coefficient sums the partials of
+// the previous step itself, in dot()'s final order.  A projection step of
+// Gram-Schmidt is then one launch, with results equal to dot + axpy.
+// Partial buffers hold 2 * kDotPartials doubles; in and out must differ.
+constexpr int kDotPartials = 512;
+int dot_grid(int64_t n);
+// part <- partials of <x|y>
+void lz_dot_partial(const double* x, const double* y, int64_t n, bool cplx, double* part,
+                    hipStream_t stream);
+// alpha = final(part_in); h -= alpha psi; part_out <- partials of <h|h>
+void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* part_in,
+              double* part_out, hipStream_t stream);
+// out[0..1] = final(part_a) (alpha; real: out[1] = 0), out[2] = Re final(part_b) (beta^2)
+void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
+               hipStream_t stream);
+// nxt = h / beta; h = -beta psi; part_out <- partials of <x0|nxt>
+void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
+             double beta, double* part_out, hipStream_t stream);
+// c = final(part_in); y -= c xj; part_out <- partials of <xnext|y> (xnext null: <y|y>)
+void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
+                const double* part_in, double* part_out, hipStream_t stream);
+// y /= sqrt(Re final(part_in))
+void lz_normalize(double* y, int64_t n, bool cplx, const double* part_in, hipStream_t stream);
+
 // Column gather for the SVD scatter: for i < rows, jj < ncol:
 //   dst[i*dsr + jj*dsc] = scale[jj] * src[(r0 + i) + perm[jj]*ld]
 // (src column-major).  A NaN scale writes the unit vector e_jj instead

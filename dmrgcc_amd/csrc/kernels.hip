@@ -944,6 +944,190 @@ __global__ void fill_kernel(double* __restrict__ x, int64_t n, double v) {
     x[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Fused Lanczos vector steps.  Partial sums follow dot_partial_kernel's
+// decomposition (grid g = dot_grid(n) blocks of 256 threads, element i to
+// thread i mod (256 g), each thread summing its elements in increasing i, a
+// wave then a fixed four-wave combine); a coefficient is the sum of the
+// previous step's partials in dot_final_kernel's order, formed by every
+// block that needs it.  The element updates equal axpy_dev / zaxpy_dev /
+// lanczos_step, so a fused sequence gives the same doubles as the unfused
+// one (real vectors; complex norms sum |y_i|^2 over entries).
+// ---------------------------------------------------------------------------
+template <bool kC>
+__device__ inline void lz_block_store(double sr, double si, double* __restrict__ part) {
+  __shared__ double red[2][4];
+  sr = wave_sum(sr);
+  if (kC) si = wave_sum(si);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][w] = sr; red[1][w] = si; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (kC) {
+      part[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+      part[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    } else {
+      part[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    }
+  }
+}
+
+// sum of np partials (complex: pairs) in dot_final_kernel's order, returned
+// to every thread of the block
+template <bool kC>
+__device__ inline void lz_final(const double* __restrict__ part, int np, double& cr, double& ci) {
+  __shared__ double red[2][4];
+  __shared__ double res[2];
+  double sr = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    if (kC) { sr += part[2 * i]; si += part[2 * i + 1]; }
+    else sr += part[i];
+  }
+  sr = wave_sum(sr);
+  if (kC) si = wave_sum(si);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][w] = sr; red[1][w] = si; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    res[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    res[1] = kC ? (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]) : 0.0;
+  }
+  __syncthreads();
+  cr = res[0];
+  ci = res[1];
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_dot_partial_kernel(const double* __restrict__ x,
+                                                             const double* __restrict__ y, int64_t n,
+                                                             double* __restrict__ part) {
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (kC) {
+      const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+      sr += xr * yr + xi * yi;
+      si += xr * yi - xi * yr;
+    } else {
+      sr += x[i] * y[i];
+    }
+  }
+  lz_block_store<kC>(sr, si, part);
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_alpha_kernel(double* __restrict__ h,
+                                                       const double* __restrict__ psi, int64_t n,
+                                                       const double* __restrict__ part_in, int np,
+                                                       double* __restrict__ part_out) {
+  double cr, ci;
+  lz_final<kC>(part_in, np, cr, ci);
+  const double ar = -1.0 * cr, ai = -1.0 * ci;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (kC) {
+      const double xr = psi[2 * i], xi = psi[2 * i + 1];
+      const double yr = h[2 * i] + (ar * xr - ai * xi);
+      const double yi = h[2 * i + 1] + (ar * xi + ai * xr);
+      h[2 * i] = yr;
+      h[2 * i + 1] = yi;
+      s += yr * yr + yi * yi;
+    } else {
+      const double y = h[i] + ar * psi[i];
+      h[i] = y;
+      s += y * y;
+    }
+  }
+  lz_block_store<false>(s, 0.0, part_out);
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_finals_kernel(const double* __restrict__ part_a,
+                                                        const double* __restrict__ part_b, int np,
+                                                        double* __restrict__ out) {
+  double ar, ai, br, bi;
+  lz_final<kC>(part_a, np, ar, ai);
+  lz_final<false>(part_b, np, br, bi);
+  if (threadIdx.x == 0) {
+    out[0] = ar;
+    out[1] = ai;
+    out[2] = br;
+  }
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_next_kernel(double* __restrict__ nxt, double* __restrict__ h,
+                                                      const double* __restrict__ psi,
+                                                      const double* __restrict__ x0, int64_t n,
+                                                      double beta, double* __restrict__ part_out) {
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (kC) {
+      const double yr = h[2 * i] / beta, yi = h[2 * i + 1] / beta;
+      nxt[2 * i] = yr;
+      nxt[2 * i + 1] = yi;
+      h[2 * i] = -beta * psi[2 * i];
+      h[2 * i + 1] = -beta * psi[2 * i + 1];
+      const double xr = x0[2 * i], xi = x0[2 * i + 1];
+      sr += xr * yr + xi * yi;
+      si += xr * yi - xi * yr;
+    } else {
+      const double y = h[i] / beta;
+      nxt[i] = y;
+      h[i] = -beta * psi[i];
+      sr += x0[i] * y;
+    }
+  }
+  lz_block_store<kC>(sr, si, part_out);
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_project_kernel(double* __restrict__ y,
+                                                         const double* __restrict__ xj,
+                                                         const double* __restrict__ xnext, int64_t n,
+                                                         const double* __restrict__ part_in, int np,
+                                                         double* __restrict__ part_out) {
+  double cr, ci;
+  lz_final<kC>(part_in, np, cr, ci);
+  const double ar = -1.0 * cr, ai = -1.0 * ci;
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (kC) {
+      const double xr = xj[2 * i], xi = xj[2 * i + 1];
+      const double yr = y[2 * i] + (ar * xr - ai * xi);
+      const double yi = y[2 * i + 1] + (ar * xi + ai * xr);
+      y[2 * i] = yr;
+      y[2 * i + 1] = yi;
+      if (xnext) {
+        const double zr = xnext[2 * i], zi = xnext[2 * i + 1];
+        sr += zr * yr + zi * yi;
+        si += zr * yi - zi * yr;
+      } else {
+        sr += yr * yr + yi * yi;
+      }
+    } else {
+      const double v = y[i] + ar * xj[i];
+      y[i] = v;
+      sr += (xnext ? xnext[i] : v) * v;
+    }
+  }
+  lz_block_store<kC>(sr, si, part_out);
+}
+
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_normalize_kernel(double* __restrict__ y, int64_t nd,
+                                                           const double* __restrict__ part_in, int np) {
+  double cr, ci;
+  lz_final<kC>(part_in, np, cr, ci);
+  const double r = sqrt(cr);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] /= r;
+}
+
 inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
@@ -998,7 +1182,10 @@ static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
 static_assert(kJw >= 1 && kJw <= 16, "1 to 16 waves per visit");
 
 // Slabs of 16 rows handled together by one wave (their loads in flight at once)
-constexpr int kSlabs = 2;
+#ifndef DMRG_JACOBI_SLABS
+#define DMRG_JACOBI_SLABS 2
+#endif
+constexpr int kSlabs = DMRG_JACOBI_SLABS;
 
 // rows [0, rows) of the visit's columns: X[:, col] <- X[:, col] * Q
 __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int rows,
@@ -1050,27 +1237,20 @@ __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int ro
 }
 
 __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
-    const SvdProblem* __restrict__ probs, const int* __restrict__ active,
-    const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
+    const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
+    const int* __restrict__ vpair, int round, int* __restrict__ rotated) {
   __shared__ double Gp[kJw][kJc][kJc + 1];  // partial Grams, one per wave
   __shared__ double Q[kJc][kJc + 1];
   __shared__ int col[kJc];
   __shared__ int flag;
   double (*G)[kJc + 1] = Gp[0];  // the summed Gram overwrites wave 0's partial
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int q = blockIdx.x;
-  if (q >= prefix[nactive]) return;
-  int lo = 0, hi = nactive - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= q) lo = mid; else hi = mid - 1;
-  }
-  const int pi = active[lo];
+  // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
+  const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
   if (round >= nb2 - 1) return;
-  const int pr = q - prefix[lo];
   int bi, bj;
   if (pr == 0) { bi = round; bj = nb2 - 1; }
   else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
@@ -1315,8 +1495,8 @@ __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int r
 }
 
 __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
-    const SvdProblem* __restrict__ probs, const int* __restrict__ active,
-    const int* __restrict__ prefix, int nactive, int round, int* __restrict__ rotated) {
+    const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
+    const int* __restrict__ vpair, int round, int* __restrict__ rotated) {
   __shared__ double Pr[kZw][kZc][kZc + 1];
   __shared__ double Pi[kZw][kZc][kZc + 1];
   __shared__ double Qr[kZc][kZc + 1];
@@ -1327,19 +1507,12 @@ __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
   double (*Gi)[kZc + 1] = Pi[0];
   constexpr int kB = kZc / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int q = blockIdx.x;
-  if (q >= prefix[nactive]) return;
-  int lo = 0, hi = nactive - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= q) lo = mid; else hi = mid - 1;
-  }
-  const int pi = active[lo];
+  // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
+  const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kB - 1) / kB, nb2 = nb + (nb & 1);
   if (round >= nb2 - 1) return;
-  const int pr = q - prefix[lo];
   int bi, bj;
   if (pr == 0) { bi = round; bj = nb2 - 1; }
   else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
@@ -2144,6 +2317,69 @@ void scale_inv_sqrt(double* x, int64_t n, const double* nrm2, hipStream_t stream
   DMRG_HIP(hipGetLastError());
 }
 
+int dot_grid(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kDotBlocks, (n + 255) / 256));
+}
+
+void lz_dot_partial(const double* x, const double* y, int64_t n, bool cplx, double* part,
+                    hipStream_t stream) {
+  const int g = dot_grid(n);
+  if (cplx) hipLaunchKernelGGL(lz_dot_partial_kernel<true>, dim3(g), dim3(256), 0, stream, x, y, n, part);
+  else hipLaunchKernelGGL(lz_dot_partial_kernel<false>, dim3(g), dim3(256), 0, stream, x, y, n, part);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* part_in,
+              double* part_out, hipStream_t stream) {
+  const int g = dot_grid(n);
+  if (cplx)
+    hipLaunchKernelGGL(lz_alpha_kernel<true>, dim3(g), dim3(256), 0, stream, h, psi, n, part_in, g, part_out);
+  else
+    hipLaunchKernelGGL(lz_alpha_kernel<false>, dim3(g), dim3(256), 0, stream, h, psi, n, part_in, g, part_out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
+               hipStream_t stream) {
+  const int g = dot_grid(n);
+  if (cplx) hipLaunchKernelGGL(lz_finals_kernel<true>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out);
+  else hipLaunchKernelGGL(lz_finals_kernel<false>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
+             double beta, double* part_out, hipStream_t stream) {
+  const int g = dot_grid(n);
+  if (cplx)
+    hipLaunchKernelGGL(lz_next_kernel<true>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, part_out);
+  else
+    hipLaunchKernelGGL(lz_next_kernel<false>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, part_out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
+                const double* part_in, double* part_out, hipStream_t stream) {
+  const int g = dot_grid(n);
+  if (cplx)
+    hipLaunchKernelGGL(lz_project_kernel<true>, dim3(g), dim3(256), 0, stream, y, xj, xnext, n, part_in, g,
+                       part_out);
+  else
+    hipLaunchKernelGGL(lz_project_kernel<false>, dim3(g), dim3(256), 0, stream, y, xj, xnext, n, part_in, g,
+                       part_out);
+  DMRG_HIP(hipGetLastError());
+}
+
+void lz_normalize(double* y, int64_t n, bool cplx, const double* part_in, hipStream_t stream) {
+  const int64_t nd = cplx ? 2 * n : n;
+  if (nd <= 0) return;
+  const int g = dot_grid(n);
+  if (cplx)
+    hipLaunchKernelGGL(lz_normalize_kernel<true>, dim3(grid_for(nd)), dim3(256), 0, stream, y, nd, part_in, g);
+  else
+    hipLaunchKernelGGL(lz_normalize_kernel<false>, dim3(grid_for(nd)), dim3(256), 0, stream, y, nd, part_in, g);
+  DMRG_HIP(hipGetLastError());
+}
+
 void lanczos_step(double* psi_next, double* h, const double* psi, int64_t n, double beta,
                   hipStream_t stream) {
   hipLaunchKernelGGL(lanczos_step_kernel, dim3(grid_for(n)), dim3(256), 0, stream, psi_next, h, psi,
@@ -2372,35 +2608,40 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
-  // device ints: rotated[np] | active[np] | prefix[np + 1]
-  sweeps_out.ensure((size_t)3 * np + 1);
-  int* rotated = sweeps_out.data();
-  int* dactive = rotated + np;
-  int* dprefix = dactive + np;
   std::vector<int> active;
+  int max_visits = 0;
   for (int i = 0; i < np; ++i)
-    if (probs[i].N > 1) active.push_back(i);
+    if (probs[i].N > 1) {
+      active.push_back(i);
+      const int nb = (probs[i].N + kJb - 1) / kJb;
+      max_visits += (nb + (nb & 1)) / 2;
+    }
+  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits]
+  sweeps_out.ensure((size_t)np + 2 * (size_t)max_visits);
+  int* rotated = sweeps_out.data();
+  int* vprob = rotated + np;
   std::vector<int> hrot(np), last(np, 0);
   int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
     ++sweeps_done;
-    int visits = 0, rounds = 0;
-    std::vector<int> prefix(1, 0);
+    // visit q of a round = (problem, column-block pair) from this map
+    std::vector<int> h(np, 0), pairs;
+    int rounds = 0;
     for (int i : active) {
       const int nb = (probs[i].N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
-      visits += nb2 / 2;
-      prefix.push_back(visits);
+      for (int pr = 0; pr < nb2 / 2; ++pr) {
+        h.push_back(i);
+        pairs.push_back(pr);
+      }
       rounds = std::max(rounds, nb2 - 1);
     }
-    const int na = (int)active.size();
-    std::vector<int> h(np, 0);
-    h.insert(h.end(), active.begin(), active.end());
-    h.resize(2 * np, 0);
-    h.insert(h.end(), prefix.begin(), prefix.end());
+    const int visits = (int)pairs.size();
+    int* vpair = vprob + visits;
+    h.insert(h.end(), pairs.begin(), pairs.end());
     DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     for (int r = 0; r < rounds; ++r)
       hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(64 * kJw), 0, stream,
-                         scratch.data(), dactive, dprefix, na, r, rotated);
+                         scratch.data(), vprob, vpair, r, rotated);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
@@ -2448,33 +2689,38 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
   upload(scratch, probs, stream);
   hipLaunchKernelGGL(zjacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
   DMRG_HIP(hipGetLastError());
-  sweeps_out.ensure((size_t)3 * np + 1);
-  int* rotated = sweeps_out.data();
-  int* dactive = rotated + np;
-  int* dprefix = dactive + np;
-  std::vector<int> active;
-  for (int i = 0; i < np; ++i)
-    if (probs[i].N > 1) active.push_back(i);
-  std::vector<int> hrot(np);
   constexpr int kB = kZc / 2;
+  std::vector<int> active;
+  int max_visits = 0;
+  for (int i = 0; i < np; ++i)
+    if (probs[i].N > 1) {
+      active.push_back(i);
+      const int nb = (probs[i].N + kB - 1) / kB;
+      max_visits += (nb + (nb & 1)) / 2;
+    }
+  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits]
+  sweeps_out.ensure((size_t)np + 2 * (size_t)max_visits);
+  int* rotated = sweeps_out.data();
+  int* vprob = rotated + np;
+  std::vector<int> hrot(np);
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
-    int visits = 0, rounds = 0;
-    std::vector<int> prefix(1, 0);
+    std::vector<int> h(np, 0), pairs;
+    int rounds = 0;
     for (int i : active) {
       const int nb = (probs[i].N + kB - 1) / kB, nb2 = nb + (nb & 1);
-      visits += nb2 / 2;
-      prefix.push_back(visits);
+      for (int pr = 0; pr < nb2 / 2; ++pr) {
+        h.push_back(i);
+        pairs.push_back(pr);
+      }
       rounds = std::max(rounds, nb2 - 1);
     }
-    const int na = (int)active.size();
-    std::vector<int> h(np, 0);
-    h.insert(h.end(), active.begin(), active.end());
-    h.resize(2 * np, 0);
-    h.insert(h.end(), prefix.begin(), prefix.end());
+    const int visits = (int)pairs.size();
+    int* vpair = vprob + visits;
+    h.insert(h.end(), pairs.begin(), pairs.end());
     DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     for (int r = 0; r < rounds; ++r)
       hipLaunchKernelGGL(zjacobi_round_kernel, dim3(visits), dim3(64 * kZw), 0, stream, scratch.data(),
-                         dactive, dprefix, na, r, rotated);
+                         vprob, vpair, r, rotated);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
