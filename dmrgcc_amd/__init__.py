@@ -447,7 +447,7 @@ def _result_dict(e, t, s, st):
             "max_bond": int(st[7]),
             "phase_seconds": {"lanczos": st[8], "expansion": st[9], "expand": st[10],
                               "canonical": st[11], "svd": st[12], "env": st[13],
-                              "match_merge": st[14]},
+                              "match_merge": st[14], "plan_build": st[19]},
             "svd_max_m": int(st[15]), "svd_max_n": int(st[16]), "svd_calls": int(st[17]),
             "svd_work": st[18]}
 

@@ -227,6 +227,7 @@ void fill_result(const RunResult& r, int* nhalf_done, double* energies, double* 
     stats[12] = ph.svd; stats[13] = ph.env; stats[14] = ph.merge;
     stats[15] = (double)ph.svd_max_m; stats[16] = (double)ph.svd_max_n;
     stats[17] = (double)ph.svd_calls; stats[18] = ph.svd_work;
+    stats[19] = ph.plan;
   }
 }
 }  // namespace

@@ -7,6 +7,7 @@
 #include "dmrg.h"
 
 #include <algorithm>
+#include <unordered_map>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -337,6 +338,15 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   cplx_ = ket.cs == 2;
   std::vector<std::vector<int>> same_r(nr);
   for (int r = 0; r < nr; ++r) same_r[r] = ket.same(1, r);
+  // block (l, r) -> index per L channel, the first match as FindMatrixBlock
+  // returns it (real_matrix_block.cc:314-380), hashed once per plan instead
+  // of a scan per term
+  std::vector<std::unordered_map<int64_t, int>> lidx(L.size());
+  for (size_t o = 0; o < L.size(); ++o) {
+    lidx[o].reserve(2 * (size_t)L[o].n());
+    for (int i = 0; i < L[o].n(); ++i)
+      lidx[o].emplace(((int64_t)L[o].blk[i].l << 32) | (uint32_t)L[o].blk[i].r, i);
+  }
 
   struct Out { int o1, lp; double w, wi; };
   struct Task { int o2, ri, kp, jdx; int64_t toff; std::vector<Out> outs; std::vector<int> ops; };
@@ -360,7 +370,8 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
           const int p1 = bphys[op];
           for (int o1 = 0; o1 < W.lb; ++o1) {
             if (W.used(o1, o2, p1, p2)) {
-              const int lp = L[o1].find(idx, kdx);
+              const auto hit = lidx[o1].find(((int64_t)idx << 32) | (uint32_t)kdx);
+              const int lp = hit == lidx[o1].end() ? -1 : hit->second;
               if (lp != -1) {
                 t.outs.push_back(Out{o1, lp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
                 t.ops.push_back(op);
@@ -1294,7 +1305,9 @@ void RealTensorRundmrg::Initialize() {
 // --------------------------------------------------------------------------
 double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
   HeffPlan plan;
+  const double t0 = now();
   plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
+  ph_.plan += now() - t0;
   return lanczos_solve(plan, lat_[site].ket.data.data(), num_iter, p_.precision, p_.refine, cx_,
                        stats_);
 }
@@ -2030,6 +2043,7 @@ void RealTensorRundmrg::Step2(int h, int j) {
   TwoSite ts;
   HeffPlan plan;
   BuildTwoSite(a, ts, plan);
+  ph_.plan += now() - tp;
   const double energy = lanczos_solve(plan, ts.theta.data.data(), p_.niter[h], p_.precision, p_.refine, cx_,
                                       stats_);
   half_e_ = energy;

@@ -249,6 +249,7 @@ struct RunParams {
 // work) and SVD sizes seen.
 struct PhaseTimes {
   double lanczos = 0, expan = 0, expand = 0, canonical = 0, svd = 0, env = 0, merge = 0;
+  double plan = 0;  // H_eff plan construction, part of lanczos
   int64_t svd_max_m = 0, svd_max_n = 0, svd_calls = 0;
   double svd_work = 0;  // sum of M*N*N over sector SVDs
 };

@@ -280,7 +280,7 @@ int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N
  * solves, [7] max bond dim, [8..14] seconds in Lanczos, expansion
  * contraction, expansion fill, canonicalisation, of which SVD, environment
  * update, match+merge, [15..16] largest SVD M and N, [17] SVD count,
- * [18] sum of M*N*N over SVDs. */
+ * [18] sum of M*N*N over SVDs, [19] seconds building H_eff plans (part of [8]). */
 int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
