@@ -226,6 +226,11 @@ int BlockTensor::find(int l, int r) const {
   return -1;
 }
 
+BlockIndex::BlockIndex(const BlockTensor& t) {
+  m_.reserve(2 * (size_t)t.n());
+  for (int i = 0; i < t.n(); ++i) m_.emplace(key(t.blk[i].l, t.blk[i].r), i);
+}
+
 std::vector<int> BlockTensor::same(int leigh, int target) const {
   std::vector<int> out;
   for (int i = 0; i < n(); ++i)
@@ -341,12 +346,9 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   // block (l, r) -> index per L channel, the first match as FindMatrixBlock
   // returns it (real_matrix_block.cc:314-380), hashed once per plan instead
   // of a scan per term
-  std::vector<std::unordered_map<int64_t, int>> lidx(L.size());
-  for (size_t o = 0; o < L.size(); ++o) {
-    lidx[o].reserve(2 * (size_t)L[o].n());
-    for (int i = 0; i < L[o].n(); ++i)
-      lidx[o].emplace(((int64_t)L[o].blk[i].l << 32) | (uint32_t)L[o].blk[i].r, i);
-  }
+  std::vector<BlockIndex> lidx;
+  lidx.reserve(L.size());
+  for (const BlockTensor& b : L) lidx.emplace_back(b);
 
   struct Out { int o1, lp; double w, wi; };
   struct Task { int o2, ri, kp, jdx; int64_t toff; std::vector<Out> outs; std::vector<int> ops; };
@@ -370,8 +372,7 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
           const int p1 = bphys[op];
           for (int o1 = 0; o1 < W.lb; ++o1) {
             if (W.used(o1, o2, p1, p2)) {
-              const auto hit = lidx[o1].find(((int64_t)idx << 32) | (uint32_t)kdx);
-              const int lp = hit == lidx[o1].end() ? -1 : hit->second;
+              const int lp = lidx[o1].find(idx, kdx);
               if (lp != -1) {
                 t.outs.push_back(Out{o1, lp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
                 t.ops.push_back(op);
@@ -676,6 +677,9 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
     out[o2].define(li, ri);
     terms[o2].resize(li.size());
   }
+  std::vector<BlockIndex> oidx;
+  oidx.reserve(out.size());
+  for (const BlockTensor& b : out) oidx.emplace_back(b);
   // F = B(k,i)^T * L[o1](k,l) for each (o1, L block, ket block)
   std::vector<Shape> fshape;
   std::vector<int64_t> foff;
@@ -702,7 +706,7 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
           const int p2 = lat.phys[ldx][jdx];
           for (int o2 = 0; o2 < W.rb; ++o2) {
             if (W.used(o1, o2, p1, p2)) {
-              const int pos = out[o2].find(idx, jdx);
+              const int pos = oidx[o2].find(idx, jdx);
               terms[o2][pos].push_back(EnvTerm{f, lp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
             }
           }
@@ -775,6 +779,9 @@ Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Co
     out[o1].define(li, ri);
     terms[o1].resize(li.size());
   }
+  std::vector<BlockIndex> oidx;
+  oidx.reserve(out.size());
+  for (const BlockTensor& b : out) oidx.emplace_back(b);
   std::vector<Shape> fshape;
   std::vector<int64_t> foff;
   struct FTask { int o2, a, lp; };
@@ -800,7 +807,7 @@ Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Co
           const int p1 = lat.phys[idx][kdx];
           for (int o1 = 0; o1 < W.lb; ++o1) {
             if (W.used(o1, o2, p1, p2)) {
-              const int pos = out[o1].find(jdx, idx);
+              const int pos = oidx[o1].find(jdx, idx);
               terms[o1][pos].push_back(EnvTerm{f, kp, W.at(o1, o2, p1, p2), W.im(o1, o2, p1, p2)});
             }
           }
@@ -1531,6 +1538,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   const Env& env = left ? L_[site] : R_[site];
   const int nouter = left ? W.lb : W.rb;
   const int ninner = left ? W.rb : W.lb;
+  const BlockIndex eidx0(E.ket);
   auto wat = [&](int outer, int inner, int p1, int p2) {
     return left ? W.at(outer, inner, p1, p2) : W.at(inner, outer, p1, p2);
   };
@@ -1550,7 +1558,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
         if (left ? !W.mixed_nonzero(ob, p2) : !W.right_nonzero(ob, p2)) continue;
         for (int ib = 0; ib < ninner; ++ib) {
           const int ldx = mapping[ib][jdx];
-          const int pos = left ? E.ket.find(idx, ldx) : E.ket.find(ldx, idx);
+          const int pos = left ? eidx0.find(idx, ldx) : eidx0.find(ldx, idx);
           if (ldx != -1 && pos != -1) {
             const int p1 = left ? E.phys[idx][ldx] : E.phys[ldx][idx];
             if (wat(ob, ib, p1, p2) != 0.0 || wim(ob, ib, p1, p2) != 0.0) exist[ldx] = 1;
@@ -1597,6 +1605,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
         if (E.phys[i][j] != -1) { li.push_back(i); ri.push_back(j); }
     E.ket.define(li, ri);
   }
+  const BlockIndex eidx(E.ket);
   // ---- enumerate the expansion terms, drawing the random matrices ----
   struct First { int ob, a, kp; Shape sh; int64_t off; };
   struct Term { int f; int pos; int64_t roff; int e0, e1; int64_t at; };
@@ -1635,7 +1644,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
         const int r_rows = left ? Dj : wdt, r_cols = left ? wdt : Dj;
         for (int ib = 0; ib < ninner; ++ib) {
           const int ldx = etab[ib][jdx];
-          const int pos = left ? E.ket.find(idx, ldx) : E.ket.find(ldx, idx);
+          const int pos = left ? eidx.find(idx, ldx) : eidx.find(ldx, idx);
           if (ldx != -1 && pos != -1) {
             const int p1 = left ? E.phys[idx][ldx] : E.phys[ldx][idx];
             const double w = wat(ob, ib, p1, p2), wi = wim(ob, ib, p1, p2);
@@ -1684,11 +1693,11 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     for (int r = 0; r < E.nr(); ++r) {
       E.rd[r] = 0;
       for (int l = 0; l < E.nl(); ++l) {
-        const int pos = E.ket.find(l, r);
+        const int pos = eidx.find(l, r);
         if (pos != -1) E.rd[r] = std::max(E.rd[r], eshape[pos].cols);
       }
       for (int l = 0; l < E.nl(); ++l) {
-        const int pos = E.ket.find(l, r);
+        const int pos = eidx.find(l, r);
         if (pos != -1 && eshape[pos].rows == 0) {
           std::printf("row of expan_tensor is zero in LeftExpanTensorContraction\n");
           if (cs == 2) {  // complex_tensor_contraction.cc:446-451
@@ -1708,11 +1717,11 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     for (int l = 0; l < E.nl(); ++l) {
       E.ld[l] = 0;
       for (int r = 0; r < E.nr(); ++r) {
-        const int pos = E.ket.find(l, r);
+        const int pos = eidx.find(l, r);
         if (pos != -1) E.ld[l] = std::max(E.ld[l], eshape[pos].rows);
       }
       for (int r = 0; r < E.nr(); ++r) {
-        const int pos = E.ket.find(l, r);
+        const int pos = eidx.find(l, r);
         if (pos != -1 && eshape[pos].rows == 0) {
           std::printf("row of expan_tensor is zero in RightExpanTensorContraction\n");
           if (cs == 2) {  // complex_tensor_contraction.cc:762-767

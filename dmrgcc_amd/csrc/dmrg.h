@@ -9,6 +9,7 @@
 
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "common.h"
@@ -48,6 +49,22 @@ struct BlockTensor {
 };
 
 using Env = std::vector<BlockTensor>;  // one block-sparse matrix per MPO channel
+
+// Hashed (l, r) -> block index of a block list as it is when built; find
+// returns the first match, as BlockTensor::find (FindMatrixBlock,
+// real_matrix_block.cc:314-380) does, in O(1) for the contraction loops.
+class BlockIndex {
+ public:
+  explicit BlockIndex(const BlockTensor& t);
+  int find(int l, int r) const {
+    const auto it = m_.find(key(l, r));
+    return it == m_.end() ? -1 : it->second;
+  }
+
+ private:
+  static int64_t key(int l, int r) { return ((int64_t)l << 32) | (uint32_t)r; }
+  std::unordered_map<int64_t, int> m_;
+};
 
 // RealTensorLattice: one MPS site.
 struct Lattice {
