@@ -992,7 +992,15 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
     S.N = std::min(S.rows, S.cols);
     S.g = wsize; wsize += (int64_t)cs * S.M * S.N;  // doubles
     S.v = wsize; wsize += (int64_t)cs * S.N * S.N;
-    S.s = wsize; wsize += S.N + 1;  // sigma[N] and |A|_F^2
+  }
+  // sigma[N] and |A|_F^2 of every sector, contiguous after the matrices so
+  // that one copy brings all singular values to the host
+  const int64_t sig0 = wsize;
+  for (int s = 0; s < ns; ++s) {
+    Sec& S = sec[s];
+    if (!S.svd) continue;
+    S.s = wsize;
+    wsize += S.N + 1;
   }
   cx_.work.ensure((size_t)std::max<int64_t>(wsize, 1));
   double* w = cx_.work.data();
@@ -1044,11 +1052,14 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
   }
   ph_.svd_calls += (int64_t)probs.size();
   // singular values to the host, sorted descending per sector
-  std::vector<double> hw((size_t)std::max<int64_t>(wsize, 1));
   std::vector<std::vector<double>> sv(ns), strue(ns);
   std::vector<std::vector<int>> perm(ns);
   {
-    std::vector<double> sig;
+    std::vector<double> sig((size_t)std::max<int64_t>(wsize - sig0, 1));
+    if (wsize > sig0)
+      DMRG_HIP(hipMemcpyAsync(sig.data(), w + sig0, (wsize - sig0) * sizeof(double), hipMemcpyDeviceToHost,
+                              cx_.stream));
+    cx_.sync();
     for (int s = 0; s < ns; ++s) {
       Sec& S = sec[s];
       if (!S.svd) {
@@ -1057,11 +1068,8 @@ double RealTensorRundmrg::canonical(int leigh, int site) {
         perm[s] = {0};
         continue;
       }
-      std::vector<double> v(S.N);
-      DMRG_HIP(hipMemcpyAsync(v.data(), w + S.s, S.N * sizeof(double), hipMemcpyDeviceToHost, cx_.stream));
-      strue[s] = v;
+      strue[s].assign(sig.begin() + (S.s - sig0), sig.begin() + (S.s - sig0) + S.N);
     }
-    cx_.sync();
     for (int s = 0; s < ns; ++s) {
       if (!sec[s].svd) continue;
       std::vector<double>& v = strue[s];
