@@ -96,7 +96,7 @@ struct RandomDraw {
 // the last draw.  Large sets are split over host threads, each starting from
 // a jump-ahead copy of the stream, so the values are those of one sequential
 // pass.
-void fill_random(const std::vector<RandomDraw>& ds, std::vector<double>& rnd, int cs, GlibcRand& rng) {
+void fill_random(const std::vector<RandomDraw>& ds, double* rnd, int cs, GlibcRand& rng) {
   std::vector<int64_t> first(ds.size() + 1, 0);
   for (size_t k = 0; k < ds.size(); ++k) first[k + 1] = first[k] + (int64_t)cs * ds[k].rows * ds[k].cols;
   const int64_t total = first.back();
@@ -105,7 +105,7 @@ void fill_random(const std::vector<RandomDraw>& ds, std::vector<double>& rnd, in
     g.skip(first[k0]);
     for (size_t k = k0; k < k1; ++k) {
       const RandomDraw& d = ds[k];
-      double* R = rnd.data() + d.roff;
+      double* R = rnd + d.roff;
       if (cs == 2) {
         g.complex_matrix(d.rows, d.cols, R);
         for (int q = 0; q < (d.diag ? std::min(d.rows, d.cols) : 0); ++q) {
@@ -161,7 +161,7 @@ void random_matrices(GlibcRand& rng, int n, const int* rows, const int* cols, in
     off += (int64_t)cs * rows[k] * cols[k];
   }
   std::vector<double> buf((size_t)off);
-  fill_random(ds, buf, cs, rng);
+  fill_random(ds, buf.data(), cs, rng);
   std::copy(buf.begin(), buf.end(), out);
 }
 
@@ -260,7 +260,23 @@ Context::Context(hipStream_t s) : stream(s) {
 }
 
 Context::~Context() {
+  if (pinned_) (void)hipHostFree(pinned_);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+double* Context::pinned(int64_t n) {
+  if (n > pinned_n_) {
+    // the previous contents are not kept; callers finish their copies
+    // (synchronise the stream) before asking again
+    DMRG_HIP(hipStreamSynchronize(stream));
+    if (pinned_) DMRG_HIP(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    pinned_n_ = 0;
+    const int64_t cap = std::max<int64_t>(n, 1 << 20);
+    DMRG_HIP(hipHostMalloc((void**)&pinned_, (size_t)cap * sizeof(double), hipHostMallocDefault));
+    pinned_n_ = cap;
+  }
+  return pinned_;
 }
 
 // ===========================================================================
@@ -1619,7 +1635,7 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   struct Term { int f; int pos; int64_t roff; int e0, e1; int64_t at; };
   std::vector<First> firsts;
   std::vector<Term> terms;
-  std::vector<double> rnd;
+  int64_t rnd_size = 0;           // doubles of the random matrices
   std::vector<RandomDraw> draws;  // the random matrices, in draw order
   int64_t pending = 0;            // draws of an inactive expansion
   std::vector<Shape> eshape(E.ket.n());
@@ -1666,8 +1682,8 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
               t.pos = pos;
               t.e0 = r_rows;
               t.e1 = r_cols;
-              t.roff = (int64_t)rnd.size();  // doubles
-              rnd.resize(rnd.size() + (size_t)cs * r_rows * r_cols);
+              t.roff = rnd_size;  // doubles
+              rnd_size += (int64_t)cs * r_rows * r_cols;
               draws.push_back(RandomDraw{t.roff, r_rows, r_cols, w, wi});
               // product shape and its place in the (concatenated) block
               Shape ps = left ? Shape{f.sh.rows, r_cols} : Shape{r_rows, f.sh.cols};
@@ -1695,6 +1711,8 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     cx_.rng.skip(pending);
     return;
   }
+  // drawn straight into pinned host memory, so the upload is one DMA copy
+  double* rnd = cx_.pinned(rnd_size);
   fill_random(draws, rnd, cs, cx_.rng);
   // ---- dims of the expansion sectors; zero blocks for empty ones ----
   if (left) {
@@ -1753,9 +1771,9 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   E.ket.allocate();
   fill(E.ket.data.data(), E.ket.doubles(), 0.0, cx_.stream);
   // ---- device: first = L*K (or K*R), then noise * first * random ----
-  DevBuf<double> F((size_t)std::max<int64_t>(cs * fsize, 1)), Rd((size_t)std::max<size_t>(rnd.size(), 1));
-  if (!rnd.empty())
-    DMRG_HIP(hipMemcpyAsync(Rd.data(), rnd.data(), rnd.size() * sizeof(double), hipMemcpyHostToDevice,
+  DevBuf<double> F((size_t)std::max<int64_t>(cs * fsize, 1)), Rd((size_t)std::max<int64_t>(rnd_size, 1));
+  if (rnd_size > 0)
+    DMRG_HIP(hipMemcpyAsync(Rd.data(), rnd, rnd_size * sizeof(double), hipMemcpyHostToDevice,
                             cx_.stream));
   GemmBatch s1, s2;
   std::vector<char> fused(firsts.size(), 0);

@@ -96,7 +96,16 @@ struct Context {
   DevBuf<double> work;
   explicit Context(hipStream_t s = nullptr);
   ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
   void sync() { DMRG_HIP(hipStreamSynchronize(stream)); }
+  // pinned host staging of at least n doubles, reused between calls (grown
+  // when needed; growing discards the contents)
+  double* pinned(int64_t n);
+
+ private:
+  double* pinned_ = nullptr;
+  int64_t pinned_n_ = 0;
 };
 
 // H_eff matvec plan of one site (MultiplyEffectHamilton,
