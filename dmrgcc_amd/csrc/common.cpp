@@ -2,6 +2,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace dmrg {
 
@@ -11,6 +12,62 @@ thread_local std::string g_last_error;
 
 void set_last_error(const std::string& s) { g_last_error = s; }
 const char* last_error() { return g_last_error.c_str(); }
+
+namespace {
+// Pinned staging arena of upload_bytes: regions are handed out in order;
+// when the arena is exhausted every copy issued from it is waited for and
+// it starts over.
+struct Staging {
+  std::mutex mu;
+  char* buf = nullptr;
+  size_t cap = 0, head = 0;
+  std::vector<hipEvent_t> pending, spare;
+};
+constexpr size_t kStagingBytes = size_t(32) << 20;
+constexpr size_t kMaxPending = 4096;
+
+Staging& staging() {
+  static Staging* s = new Staging();  // leaked, as the device pool
+  return *s;
+}
+}  // namespace
+
+void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  if (bytes == 0) return;
+  Staging& st = staging();
+  std::lock_guard<std::mutex> g(st.mu);
+  if (!st.buf) {
+    DMRG_HIP(hipHostMalloc((void**)&st.buf, kStagingBytes, hipHostMallocDefault));
+    st.cap = kStagingBytes;
+  }
+  if (bytes > st.cap / 4) {
+    DMRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    DMRG_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (st.head + need > st.cap || st.pending.size() >= kMaxPending) {
+    for (hipEvent_t e : st.pending) {
+      DMRG_HIP(hipEventSynchronize(e));
+      st.spare.push_back(e);
+    }
+    st.pending.clear();
+    st.head = 0;
+  }
+  char* p = st.buf + st.head;
+  st.head += need;
+  std::memcpy(p, src, bytes);
+  DMRG_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, stream));
+  hipEvent_t e;
+  if (st.spare.empty()) {
+    DMRG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  } else {
+    e = st.spare.back();
+    st.spare.pop_back();
+  }
+  st.pending.push_back(e);
+  DMRG_HIP(hipEventRecord(e, stream));
+}
 
 DevicePool& DevicePool::get() {
   // Intentionally leaked: buffers may be released by static destructors

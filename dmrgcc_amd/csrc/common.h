@@ -103,16 +103,22 @@ class DevBuf {
   size_t n_ = 0;
 };
 
-// Upload a host vector into a (re)sized device buffer on `stream`.  The
-// copy is synchronous with respect to the host vector (the host memory may be
-// reused once this returns).
+// Asynchronous host-to-device copy of `bytes` from `src` (any host memory,
+// reusable as soon as this returns): the bytes are copied into a pinned
+// staging arena and a DMA copy is queued on `stream`, ordered with the
+// stream's other work; the host does not wait for the stream.  A staging
+// region is reused only after the copy that read it has completed (an event
+// per copy).  Copies larger than a quarter of the arena go through a
+// pageable copy and a stream synchronisation instead.  Not for use inside a
+// stream capture.
+void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
+// Upload a host vector into a (re)sized device buffer on `stream` (see
+// upload_bytes: the host vector may be reused once this returns).
 template <typename T>
 void upload(DevBuf<T>& dst, const std::vector<T>& src, hipStream_t stream) {
   dst.ensure(src.size());
-  if (!src.empty())
-    DMRG_HIP(hipMemcpyAsync(dst.data(), src.data(), src.size() * sizeof(T),
-                            hipMemcpyHostToDevice, stream));
-  DMRG_HIP(hipStreamSynchronize(stream));
+  if (!src.empty()) upload_bytes(dst.data(), src.data(), src.size() * sizeof(T), stream);
 }
 
 }  // namespace dmrg

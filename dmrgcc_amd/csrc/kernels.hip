@@ -2638,7 +2638,7 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     const int visits = (int)pairs.size();
     int* vpair = vprob + visits;
     h.insert(h.end(), pairs.begin(), pairs.end());
-    DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
     for (int r = 0; r < rounds; ++r)
       hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(64 * kJw), 0, stream,
                          scratch.data(), vprob, vpair, r, rotated);
@@ -2717,7 +2717,7 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
     const int visits = (int)pairs.size();
     int* vpair = vprob + visits;
     h.insert(h.end(), pairs.begin(), pairs.end());
-    DMRG_HIP(hipMemcpyAsync(rotated, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
     for (int r = 0; r < rounds; ++r)
       hipLaunchKernelGGL(zjacobi_round_kernel, dim3(visits), dim3(64 * kZw), 0, stream, scratch.data(),
                          vprob, vpair, r, rotated);
