@@ -334,6 +334,151 @@ void sym_eig(int n, std::vector<double> a, std::vector<double>& eval, std::vecto
   }
 }
 
+// Lowest eigenpair of a dense symmetric matrix (n x n, row-major): the part of
+// RealSymMatrixDiag (general.cc:47-64) the dense edge-site solve uses.
+//  1. Householder reduction to tridiagonal form T = Q^T A Q (Golub & Van Loan,
+//     Matrix Computations, Alg. 8.3.1, with the cancellation-free reflector of
+//     Alg. 5.1.1);
+//  2. the smallest eigenvalue of T by bisection on the Sturm count of
+//     T - x I (Golub & Van Loan, sec. 8.4.1), to the spacing of doubles;
+//  3. its eigenvector by inverse iteration on T - lambda I (tridiagonal
+//     elimination with partial pivoting, three steps);
+//  4. x = Q y, normalised, largest |component| (first on ties) positive.
+// For a degenerate lowest eigenvalue the vector lies in its eigenspace, as
+// with any eigensolver.
+void sym_eig_lowest(int n, std::vector<double> a, double& e0, std::vector<double>& v0) {
+  auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+  std::vector<std::vector<double>> hv(std::max(n - 2, 0));
+  std::vector<double> hb(std::max(n - 2, 0), 0.0);
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    std::vector<double> v(m);
+    for (int i = 0; i < m; ++i) v[i] = A(k + 1 + i, k);
+    double sigma = 0.0;
+    for (int i = 1; i < m; ++i) sigma += v[i] * v[i];
+    double beta = 0.0;
+    if (sigma > 0.0) {
+      const double x0 = v[0];
+      const double mu = std::sqrt(x0 * x0 + sigma);
+      const double w0 = x0 <= 0.0 ? x0 - mu : -sigma / (x0 + mu);
+      beta = 2.0 * w0 * w0 / (sigma + w0 * w0);
+      for (int i = 1; i < m; ++i) v[i] /= w0;
+      v[0] = 1.0;
+      // A22 <- P A22 P with P = I - beta v v^T: p = beta A22 v,
+      // w = p - (beta/2)(p^T v) v, A22 -= v w^T + w v^T
+      std::vector<double> pw(m, 0.0);
+      for (int i = 0; i < m; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < m; ++j) t += A(k + 1 + i, k + 1 + j) * v[j];
+        pw[i] = beta * t;
+      }
+      double pv = 0.0;
+      for (int i = 0; i < m; ++i) pv += pw[i] * v[i];
+      const double c = 0.5 * beta * pv;
+      for (int i = 0; i < m; ++i) pw[i] -= c * v[i];
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A(k + 1 + i, k + 1 + j) -= v[i] * pw[j] + pw[i] * v[j];
+      A(k + 1, k) = A(k, k + 1) = mu;  // P x = |x| e_1
+    }
+    hv[k] = std::move(v);
+    hb[k] = beta;
+  }
+  std::vector<double> d(n), e(std::max(n - 1, 0));
+  for (int i = 0; i < n; ++i) d[i] = A(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+  // smallest eigenvalue: Sturm count of eigenvalues below x
+  double tnorm = 0.0, lo = 0.0, hi = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = i == 0 ? d[i] - r : std::min(lo, d[i] - r);
+    hi = i == 0 ? d[i] + r : std::max(hi, d[i] + r);
+    tnorm = std::max(tnorm, std::fabs(d[i]) + r);
+  }
+  const double tiny = std::numeric_limits<double>::min() / std::numeric_limits<double>::epsilon();
+  auto below = [&](double x) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (q == 0.0) q = -tiny;
+    if (q < 0.0) ++cnt;
+    for (int i = 1; i < n; ++i) {
+      q = d[i] - x - e[i - 1] * e[i - 1] / q;
+      if (q == 0.0) q = -tiny;
+      if (q < 0.0) ++cnt;
+    }
+    return cnt;
+  };
+  lo -= std::numeric_limits<double>::epsilon() * tnorm + tiny;
+  hi += std::numeric_limits<double>::epsilon() * tnorm + tiny;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  const double lam = 0.5 * (lo + hi);
+  // inverse iteration: (T - lam I) y = b, elimination with row interchanges
+  std::vector<double> y(n, 1.0);
+  const double pivmin = std::numeric_limits<double>::epsilon() * std::max(tnorm, tiny);
+  for (int step = 0; step < 3 && n > 1; ++step) {
+    // rows of the banded system: r0 (diag), r1 (super), r2 (second super) after pivoting
+    std::vector<double> dd(n), du(n, 0.0), du2(n, 0.0), rhs = y;
+    std::vector<double> dl(n, 0.0);
+    for (int i = 0; i < n; ++i) dd[i] = d[i] - lam;
+    for (int i = 0; i + 1 < n; ++i) { du[i] = e[i]; dl[i] = e[i]; }
+    for (int i = 0; i + 1 < n; ++i) {
+      if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+        if (std::fabs(dd[i]) < pivmin) dd[i] = dd[i] < 0 ? -pivmin : pivmin;
+        const double f = dl[i] / dd[i];
+        dd[i + 1] -= f * du[i];
+        if (i + 2 < n) du2[i] = 0.0;
+        rhs[i + 1] -= f * rhs[i];
+      } else {  // swap rows i and i+1
+        const double f = dd[i] / dl[i];
+        dd[i] = dl[i];
+        const double t = dd[i + 1];
+        dd[i + 1] = du[i] - f * t;
+        du[i] = t;
+        if (i + 2 < n) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du2[i];
+        }
+        std::swap(rhs[i], rhs[i + 1]);
+        rhs[i + 1] -= f * rhs[i];
+      }
+    }
+    if (std::fabs(dd[n - 1]) < pivmin) dd[n - 1] = dd[n - 1] < 0 ? -pivmin : pivmin;
+    // back substitution
+    y[n - 1] = rhs[n - 1] / dd[n - 1];
+    if (n >= 2) y[n - 2] = (rhs[n - 2] - du[n - 2] * y[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; --i) y[i] = (rhs[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / dd[i];
+    double nrm = 0.0;
+    for (double x : y) nrm += x * x;
+    nrm = std::sqrt(nrm);
+    for (double& x : y) x /= nrm;
+  }
+  // x = Q y = P_0 P_1 ... P_{n-3} y
+  for (int k = n - 3; k >= 0; --k) {
+    if (hb[k] == 0.0) continue;
+    const std::vector<double>& v = hv[k];
+    double t = 0.0;
+    for (size_t i = 0; i < v.size(); ++i) t += v[i] * y[k + 1 + i];
+    t *= hb[k];
+    for (size_t i = 0; i < v.size(); ++i) y[k + 1 + i] -= t * v[i];
+  }
+  double nrm = 0.0;
+  for (double x : y) nrm += x * x;
+  nrm = std::sqrt(nrm);
+  int bi = 0;
+  for (int i = 0; i < n; ++i) {
+    y[i] /= nrm;
+    if (std::fabs(y[i]) > std::fabs(y[bi])) bi = i;
+  }
+  if (y[bi] < 0.0)
+    for (double& x : y) x = -x;
+  e0 = n == 1 ? d[0] : lam;
+  v0 = std::move(y);
+}
+
 // ===========================================================================
 // H_eff plan (MultiplyEffectHamilton, real_tensor_contraction.cc:1223-1291)
 // ===========================================================================
@@ -1387,20 +1532,21 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
           H[(size_t)i * m + c + n] = -im;
         }
       }
-    std::vector<double> ev, vec;
-    sym_eig((int)m, H, ev, vec);
+    double e0 = 0.0;
+    std::vector<double> v0;
+    sym_eig_lowest((int)m, H, e0, v0);
     std::vector<double> g(nd);
     for (int64_t i = 0; i < n; ++i) {
       if (!cz) {
-        g[i] = vec[(size_t)i * m];
+        g[i] = v0[i];
       } else {
-        g[2 * i] = vec[(size_t)i * m];
-        g[2 * i + 1] = vec[(size_t)(i + n) * m];
+        g[2 * i] = v0[i];
+        g[2 * i + 1] = v0[i + n];
       }
     }
     DMRG_HIP(hipMemcpyAsync(ket, g.data(), nd * sizeof(double), hipMemcpyHostToDevice, cx.stream));
     cx.sync();
-    return ev[0];
+    return e0;
   }
   require(num_iter >= 1 && num_iter <= 200, "num_iter must be in [1, 200]", kErrArgument);
   cx.scal.ensure(256);

@@ -237,6 +237,8 @@ Env boundary_env(int nbond, Context& cx, int cs = 1);
 // tridiagonal and the dense edge-site H_eff (RealSymMatrixDiag,
 // src/util/general.cc:47-64), both FP64 on the host.
 void sym_eig(int n, std::vector<double> a, std::vector<double>& eval, std::vector<double>& vec);
+// Lowest eigenvalue and its unit eigenvector (largest |component| positive).
+void sym_eig_lowest(int n, std::vector<double> a, double& e0, std::vector<double>& v0);
 
 // ComputeTruncateDim (real_tensor_lattice.cc:501-555): kept states per
 // sector from per-sector descending singular values (modified in place when a
