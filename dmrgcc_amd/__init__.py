@@ -37,6 +37,8 @@ _SIGS = {
     "dmrgcc_quicksort": (ctypes.c_int, [ctypes.c_int, _c_dbl_p, _c_int_p, ctypes.c_int]),
     "dmrgcc_truncate_dims": (ctypes.c_int, [ctypes.c_int, _c_int_p, _c_dbl_p, ctypes.c_int,
                                             ctypes.c_double, _c_int_p]),
+    "dmrgcc_sym_eig": (ctypes.c_int, [ctypes.c_int, _c_dbl_p, _c_dbl_p]),
+    "dmrgcc_sym_eig_lowest": (ctypes.c_int, [ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p]),
     "dmrgcc_model_spinless_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -253,6 +255,25 @@ def quicksort(vals, index=None, flag=0):
     idx = None if index is None else _i32(index).copy()
     _check(lib().dmrgcc_quicksort(len(v), _dp(v), None if idx is None else _ip(idx), flag))
     return v, idx
+
+
+def sym_eig(a):
+    """RealSymMatrixDiag: (ascending eigenvalues, eigenvectors as columns)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    n = a.shape[0]
+    w = np.zeros(n)
+    _check(lib().dmrgcc_sym_eig(n, _dp(a), _dp(w)))
+    return w, a
+
+
+def sym_eig_lowest(a):
+    """Lowest eigenvalue and unit eigenvector of a dense symmetric matrix."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    e0 = np.zeros(1)
+    v0 = np.zeros(n)
+    _check(lib().dmrgcc_sym_eig_lowest(n, _dp(a), _dp(e0), _dp(v0)))
+    return float(e0[0]), v0
 
 
 def truncate_dims(svals, max_dim, canpre):

@@ -268,6 +268,25 @@ int dmrgcc_quicksort(int n, double* vals, int* index, int flag) {
   });
 }
 
+int dmrgcc_sym_eig(int n, double* a, double* w) {
+  return guarded([&] {
+    require(n >= 1 && a && w, "bad arguments", kErrArgument);
+    std::vector<double> ev, vec;
+    sym_eig(n, std::vector<double>(a, a + (size_t)n * n), ev, vec);
+    std::copy(ev.begin(), ev.end(), w);
+    std::copy(vec.begin(), vec.end(), a);
+  });
+}
+
+int dmrgcc_sym_eig_lowest(int n, const double* a, double* e0, double* v0) {
+  return guarded([&] {
+    require(n >= 1 && a && e0 && v0, "bad arguments", kErrArgument);
+    std::vector<double> v;
+    sym_eig_lowest(n, std::vector<double>(a, a + (size_t)n * n), *e0, v);
+    std::copy(v.begin(), v.end(), v0);
+  });
+}
+
 int dmrgcc_truncate_dims(int nsec, const int* counts, double* svals, int max_dim, double canpre,
                          int* trunc) {
   return guarded([&] {

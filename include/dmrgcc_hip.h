@@ -66,6 +66,19 @@ int dmrgcc_quicksort(int n, double* vals, int* index, int flag);
 int dmrgcc_truncate_dims(int nsec, const int* counts, double* svals, int max_dim,
                          double canpre, int* trunc);
 
+/* RealSymMatrixDiag (src/util/general.cc:47-64): all eigenpairs of the dense
+ * symmetric n x n row-major matrix `a`, eigenvalues ascending in `w`, `a`
+ * overwritten by the eigenvectors as columns (a[k*n + j] = component k of
+ * vector j), as the reference's dsyevd call leaves them; each vector's
+ * largest |component| is positive.  Host cyclic Jacobi (the Lanczos
+ * tridiagonal eigensolve). */
+int dmrgcc_sym_eig(int n, double* a, double* w);
+/* The lowest eigenpair only (what RealSymMatrixDiag's callers read: w[0] and
+ * column 0), by Householder tridiagonalisation, Sturm bisection and inverse
+ * iteration (the dense edge-site solve of ComputeEffectHamilton,
+ * real_tensor_contraction.cc:1136-1205).  `a` is not modified. */
+int dmrgcc_sym_eig_lowest(int n, const double* a, double* e0, double* v0);
+
 /* Real spinless Bose-Hubbard on an nx x ny square lattice (site j + i*ny):
  * RealSpinlessBoseSquareSpace / RealSpinlessBoseSquareHamiltonian
  * (model/space/real_spinless_bose_square_space.cc:26-83,

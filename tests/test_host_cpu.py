@@ -1,6 +1,8 @@
 """The C-ABI library loads and exports every declared symbol, and its host
 restatements (rand stream, QuickSort, truncation rule, QN tables, MPO) agree
-bit-exactly with the oracle.  No GPU is touched."""
+bit-exactly with the oracle; the host eigensolvers (RealSymMatrixDiag and the
+lowest-eigenpair solve of the dense edge sites) agree with a 2 x 2 closed form
+and with numpy's eigh.  No GPU is touched."""
 import os
 import re
 
@@ -162,3 +164,66 @@ def test_rand_jump_ahead(skip):
     g = GlibcRand(1)
     g.skip(skip)
     assert list(dm.rand_draws(40, skip=skip)) == list(g.draws(40))
+
+
+def test_sym_eig_closed_form_2x2():
+    # eigenvalues of [[a, b], [b, c]]: (a + c)/2 -+ sqrt(((a - c)/2)^2 + b^2)
+    a, b, c = 1.5, -0.75, -2.0
+    r = np.hypot((a - c) / 2, b)
+    lo, hi = (a + c) / 2 - r, (a + c) / 2 + r
+    w, v = dm.sym_eig(np.array([[a, b], [b, c]]))
+    assert abs(w[0] - lo) < 1e-15 and abs(w[1] - hi) < 1e-15
+    # column 0 is proportional to (b, lo - a), largest |component| positive
+    u = np.array([b, lo - a]) / np.hypot(b, lo - a)
+    u *= np.sign(u[np.argmax(np.abs(u))])
+    assert np.abs(v[:, 0] - u).max() < 1e-15
+    e0, v0 = dm.sym_eig_lowest(np.array([[a, b], [b, c]]))
+    assert abs(e0 - lo) < 1e-15 and np.abs(v0 - u).max() < 1e-15
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 64, 99])
+def test_sym_eig_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    w, v = dm.sym_eig(a)
+    ref = np.linalg.eigvalsh(a)
+    scale = np.abs(a).max() * n
+    assert np.abs(w - ref).max() < 1e-14 * scale
+    assert np.abs(a @ v - v * w).max() < 1e-13 * scale
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-13
+    e0, v0 = dm.sym_eig_lowest(a)
+    assert abs(e0 - ref[0]) < 1e-14 * scale
+    assert abs(np.linalg.norm(v0) - 1.0) < 1e-14
+    assert np.abs(a @ v0 - e0 * v0).max() < 1e-13 * scale
+    # same vector as the Jacobi solver (non-degenerate lowest eigenvalue, same sign rule)
+    assert np.abs(v0 - v[:, 0]).max() < 1e-10
+    assert v0[np.argmax(np.abs(v0))] > 0
+
+
+@pytest.mark.parametrize("m", [5, 40, 99])
+def test_sym_eig_lowest_complex_embedding(m):
+    # the dense complex edge solve diagonalises [[A, -B], [B, A]] for H = A + iB;
+    # its lowest eigenvalue is H's (twice) and (x, y) gives the ground state x + iy
+    rng = np.random.default_rng(100 + m)
+    h = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    h = h + h.conj().T
+    emb = np.block([[h.real, -h.imag], [h.imag, h.real]])
+    e0, v0 = dm.sym_eig_lowest(emb)
+    ev, evec = np.linalg.eigh(h)
+    assert abs(e0 - ev[0]) < 1e-13 * np.abs(h).max() * m
+    g = v0[:m] + 1j * v0[m:]
+    assert abs(np.linalg.norm(g) - 1.0) < 1e-13
+    assert abs(abs(np.vdot(evec[:, 0], g)) - 1.0) < 1e-10
+
+
+def test_sym_eig_lowest_tridiagonal_and_split():
+    # a Lanczos-type tridiagonal matrix, and one that splits into two blocks
+    d = np.array([2.0, -1.0, 0.5, 3.0, -2.5, 1.0])
+    e = np.array([0.3, -0.7, 0.2, 1e-3, 0.9])
+    for ee in (e, np.where(np.arange(5) == 2, 0.0, e)):
+        t = np.diag(d) + np.diag(ee, 1) + np.diag(ee, -1)
+        e0, v0 = dm.sym_eig_lowest(t)
+        ref, rv = np.linalg.eigh(t)
+        assert abs(e0 - ref[0]) < 1e-14
+        assert abs(abs(v0 @ rv[:, 0]) - 1.0) < 1e-12
