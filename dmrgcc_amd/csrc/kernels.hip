@@ -899,12 +899,13 @@ __global__ void lanczos_step_kernel(double* __restrict__ nxt, double* __restrict
 }
 
 __global__ __launch_bounds__(256) void col_gather_kernel(const ColGatherTask* __restrict__ tasks) {
+  // workgroup (task, y) takes elements y*256 + tid + k*256*gridDim.y
   const ColGatherTask t = tasks[blockIdx.x];
-  const int64_t total = (int64_t)t.rows * t.ncol;
+  const int total = t.rows * t.ncol;  // < 2^31 (checked on the host)
   const bool cz = t.flags & kGatherComplex;
   const double cj = (t.flags & kGatherConj) ? -1.0 : 1.0;
-  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
-    const int jj = (int)(e / t.rows), i = (int)(e % t.rows);
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < total; e += gridDim.y * blockDim.x) {
+    const int jj = e / t.rows, i = e - jj * t.rows;
     const double s = t.scale[jj];
     const int64_t di = (int64_t)i * t.dsr + (int64_t)jj * t.dsc;
     if (!cz) {
@@ -1140,18 +1141,39 @@ inline int grid_for(int64_t n) {
 // kernel however many sectors there are.  The host runs the rounds of a
 // sweep and retires sectors whose sweep made no rotation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __restrict__ probs) {
+// Columns per workgroup of the per-column SVD epilogues (grid y splits a
+// problem's columns; every column is computed as by a single workgroup).
+constexpr int kFinCols = 32;
+
+__global__ __launch_bounds__(256) void jacobi_init_kernel(const SvdProblem* __restrict__ probs,
+                                                          double* __restrict__ part) {
+  // workgroup (problem, y): columns [c0, c1) of V set to the identity's, and
+  // the partial |G|^2 of the same columns of G into part[problem * gy + y]
   const SvdProblem p = probs[blockIdx.x];
-  const int64_t nn = (int64_t)p.N * p.N;
-  for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) p.V[e] = (e % (p.N + 1) == 0) ? 1.0 : 0.0;
+  const int c0 = blockIdx.y * kFinCols, c1 = min(p.N, c0 + kFinCols);
+  const int64_t nv = (int64_t)max(c1 - c0, 0) * p.N, ng = (int64_t)max(c1 - c0, 0) * p.M;
+  for (int64_t e = threadIdx.x; e < nv; e += blockDim.x) {
+    const int64_t idx = (int64_t)c0 * p.N + e;
+    p.V[idx] = (idx % (p.N + 1) == 0) ? 1.0 : 0.0;
+  }
   __shared__ double red[4];
   double s = 0.0;
-  const int64_t mn = (int64_t)p.M * p.N;
-  for (int64_t e = threadIdx.x; e < mn; e += blockDim.x) s += p.G[e] * p.G[e];
+  const double* g = p.G + (int64_t)c0 * p.M;
+  for (int64_t e = threadIdx.x; e < ng; e += blockDim.x) s += g[e] * g[e];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) *p.fro2 = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0)
+    part[(int64_t)blockIdx.x * gridDim.y + blockIdx.y] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// |A|_F^2 of every problem from its column-chunk partials, in chunk order
+__global__ __launch_bounds__(64) void svd_fro2_kernel(const SvdProblem* __restrict__ probs,
+                                                      const double* __restrict__ part, int gy) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < gy; i += 64) s += part[(int64_t)blockIdx.x * gy + i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) *probs[blockIdx.x].fro2 = s;
 }
 
 // Block one-sided Jacobi: the N columns are cut into blocks of kJb; one
@@ -1387,7 +1409,8 @@ __global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __
   if (p.precond) return;  // finished by qr_finish_kernel
   const int M = p.M, N = p.N;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int j = wave; j < N; j += nwaves) {
+  const int jend = min(N, (int)(blockIdx.y + 1) * kFinCols);
+  for (int j = blockIdx.y * kFinCols + wave; j < jend; j += nwaves) {
     double* g = p.G + (int64_t)j * M;
     double* v = p.V + (int64_t)j * N;
     double s = 0.0;
@@ -1429,21 +1452,26 @@ __global__ __launch_bounds__(256) void jacobi_finish_kernel(const SvdProblem* __
 constexpr int kZw = 8;
 constexpr int kZc = 16;
 
-__global__ __launch_bounds__(256) void zjacobi_init_kernel(const SvdProblem* __restrict__ probs) {
+__global__ __launch_bounds__(256) void zjacobi_init_kernel(const SvdProblem* __restrict__ probs,
+                                                           double* __restrict__ part) {
+  // as jacobi_init_kernel, on interleaved complex G and V
   const SvdProblem p = probs[blockIdx.x];
-  const int64_t nn = (int64_t)p.N * p.N;
-  for (int64_t e = threadIdx.x; e < nn; e += blockDim.x) {
-    p.V[2 * e] = (e % (p.N + 1) == 0) ? 1.0 : 0.0;
-    p.V[2 * e + 1] = 0.0;
+  const int c0 = blockIdx.y * kFinCols, c1 = min(p.N, c0 + kFinCols);
+  const int64_t nv = (int64_t)max(c1 - c0, 0) * p.N, ng = 2 * (int64_t)max(c1 - c0, 0) * p.M;
+  for (int64_t e = threadIdx.x; e < nv; e += blockDim.x) {
+    const int64_t idx = (int64_t)c0 * p.N + e;
+    p.V[2 * idx] = (idx % (p.N + 1) == 0) ? 1.0 : 0.0;
+    p.V[2 * idx + 1] = 0.0;
   }
   __shared__ double red[4];
   double s = 0.0;
-  const int64_t mn = 2 * (int64_t)p.M * p.N;
-  for (int64_t e = threadIdx.x; e < mn; e += blockDim.x) s += p.G[e] * p.G[e];
+  const double* g = p.G + 2 * (int64_t)c0 * p.M;
+  for (int64_t e = threadIdx.x; e < ng; e += blockDim.x) s += g[e] * g[e];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) *p.fro2 = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0)
+    part[(int64_t)blockIdx.x * gridDim.y + blockIdx.y] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // rows [0, rows) of the visit's complex columns: X[:, col] <- X[:, col] * Q
@@ -1644,7 +1672,8 @@ __global__ __launch_bounds__(256) void zjacobi_finish_kernel(const SvdProblem* _
   const SvdProblem p = probs[blockIdx.x];
   const int M = p.M, N = p.N;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int j = wave; j < N; j += nwaves) {
+  const int jend = min(N, (int)(blockIdx.y + 1) * kFinCols);
+  for (int j = blockIdx.y * kFinCols + wave; j < jend; j += nwaves) {
     double* g = p.G + 2 * (int64_t)j * M;
     double* v = p.V + 2 * (int64_t)j * N;
     double s = 0.0;
@@ -1899,7 +1928,8 @@ __global__ __launch_bounds__(256) void qr_finish_kernel(const QrJob* __restrict_
   if (!p.finish) return;
   const int M = p.M, N = p.N;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int j = wave; j < N; j += nwaves) {
+  const int jend = min(N, (int)(blockIdx.y + 1) * kFinCols);
+  for (int j = blockIdx.y * kFinCols + wave; j < jend; j += nwaves) {
     double* a = p.C + (int64_t)j * M;
     double* w = p.W + (int64_t)j * N;
     double* v = p.V + (int64_t)j * N;
@@ -2393,8 +2423,15 @@ void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGather
   for (const ColGatherTask& t : tasks)
     if (t.rows > 0 && t.ncol > 0) live.push_back(t);
   if (live.empty()) return;
+  int64_t most = 0;
+  for (const ColGatherTask& t : live) {
+    const int64_t tot = (int64_t)t.rows * t.ncol;
+    require(tot < (int64_t)1 << 31, "col_gather: block too large", kErrArgument);
+    most = std::max(most, tot);
+  }
   upload(scratch, live, stream);
-  hipLaunchKernelGGL(col_gather_kernel, dim3((unsigned)live.size()), dim3(256), 0, stream,
+  const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (most + 4095) / 4096));
+  hipLaunchKernelGGL(col_gather_kernel, dim3((unsigned)live.size(), gy), dim3(256), 0, stream,
                      scratch.data());
   DMRG_HIP(hipGetLastError());
   DMRG_HIP(hipStreamSynchronize(stream));
@@ -2535,7 +2572,8 @@ struct QrPhase {
       gemm.launch_group(stream, g, kFP64);
       gemm.launch_group(stream, g + 1, kFP64);
     }
-    hipLaunchKernelGGL(qr_finish_kernel, dim3(nj), dim3(256), 0, stream, d_jobs.data());
+    const unsigned gy = (unsigned)std::max(1, (maxN + kFinCols - 1) / kFinCols);
+    hipLaunchKernelGGL(qr_finish_kernel, dim3(nj, gy), dim3(256), 0, stream, d_jobs.data());
     DMRG_HIP(hipGetLastError());
   }
 };
@@ -2606,7 +2644,13 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     if (!j3.empty()) ph3.factor(stream);
   }
   upload(scratch, probs, stream);
-  hipLaunchKernelGGL(jacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  int init_n = 1;
+  for (const SvdProblem& q : probs) init_n = std::max(init_n, q.N);
+  const int init_gy = (init_n + kFinCols - 1) / kFinCols;
+  DevBuf<double> fro_part((size_t)np * init_gy);
+  hipLaunchKernelGGL(jacobi_init_kernel, dim3(np, init_gy), dim3(256), 0, stream, scratch.data(),
+                     fro_part.data());
+  hipLaunchKernelGGL(svd_fro2_kernel, dim3(np), dim3(64), 0, stream, scratch.data(), fro_part.data(), init_gy);
   DMRG_HIP(hipGetLastError());
   std::vector<int> active;
   int max_visits = 0;
@@ -2663,7 +2707,10 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     probs[i].precond = mode[i] == 1 ? 1 : 0;
   }
   upload(scratch, probs, stream);
-  hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  int fin_n = 1;
+  for (const SvdProblem& q : probs_in) fin_n = std::max(fin_n, q.N);
+  hipLaunchKernelGGL(jacobi_finish_kernel, dim3(np, (fin_n + kFinCols - 1) / kFinCols), dim3(256), 0, stream,
+                     scratch.data());
   DMRG_HIP(hipGetLastError());
   // the phases' buffers are released when this function returns
   if (!pq.empty()) DMRG_HIP(hipStreamSynchronize(stream));
@@ -2687,7 +2734,13 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
   for (const SvdProblem& p : probs)
     require(p.M >= p.N && p.N >= 1, "zjacobi_svd: needs M >= N >= 1", kErrInternal);
   upload(scratch, probs, stream);
-  hipLaunchKernelGGL(zjacobi_init_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  int init_n = 1;
+  for (const SvdProblem& q : probs) init_n = std::max(init_n, q.N);
+  const int init_gy = (init_n + kFinCols - 1) / kFinCols;
+  DevBuf<double> fro_part((size_t)np * init_gy);
+  hipLaunchKernelGGL(zjacobi_init_kernel, dim3(np, init_gy), dim3(256), 0, stream, scratch.data(),
+                     fro_part.data());
+  hipLaunchKernelGGL(svd_fro2_kernel, dim3(np), dim3(64), 0, stream, scratch.data(), fro_part.data(), init_gy);
   DMRG_HIP(hipGetLastError());
   constexpr int kB = kZc / 2;
   std::vector<int> active;
@@ -2729,7 +2782,10 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
       if (hrot[i]) still.push_back(i);
     active.swap(still);
   }
-  hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np), dim3(256), 0, stream, scratch.data());
+  int fin_n = 1;
+  for (const SvdProblem& q : probs) fin_n = std::max(fin_n, q.N);
+  hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np, (fin_n + kFinCols - 1) / kFinCols), dim3(256), 0, stream,
+                     scratch.data());
   DMRG_HIP(hipGetLastError());
 }
 
