@@ -765,9 +765,9 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __re
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void copy_kernel(const CopyTask* __restrict__ tasks) {
   const CopyTask t = tasks[blockIdx.x];
-  const int64_t total = (int64_t)t.rows * t.cols;
-  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
-    const int i = (int)(e / t.cols), j = (int)(e % t.cols);
+  const int total = t.rows * t.cols;  // one row chunk (launch_copies)
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i = e / t.cols, j = e - i * t.cols;
     const double v = t.src ? t.alpha * t.src[i * t.ssr + j * t.ssc] : 0.0;
     t.dst[i * t.dsr + j * t.dsc] = v;
   }
@@ -2283,12 +2283,14 @@ void zgemm_item(GemmBatch& g, double* C, int64_t ldc, int M, int N, int accumula
 void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch,
                    hipStream_t stream) {
   if (tasks.empty()) return;
-  // split large tasks into row chunks of <= 64K elements
+  // split tasks into row chunks of <= 8K elements (a single row may be
+  // longer), one workgroup each
   std::vector<CopyTask> split;
   split.reserve(tasks.size());
   for (const CopyTask& t : tasks) {
     if (t.rows <= 0 || t.cols <= 0) continue;
-    int rows_per = std::max(1, (int)(65536 / std::max(1, t.cols)));
+    require(t.cols < (1 << 30), "copy: row too long", kErrArgument);
+    int rows_per = std::max(1, (int)(8192 / std::max(1, t.cols)));
     for (int r0 = 0; r0 < t.rows; r0 += rows_per) {
       CopyTask c = t;
       c.rows = std::min(rows_per, t.rows - r0);
