@@ -34,6 +34,12 @@ Staging& staging() {
 
 void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream) {
   if (bytes == 0) return;
+  // a copy captured into a graph would re-read the staging arena at every
+  // replay, after the arena has been reused
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  DMRG_HIP(hipStreamIsCapturing(stream, &cap));
+  if (cap != hipStreamCaptureStatusNone)
+    throw Error(kErrInternal, "descriptor upload on a stream that is being captured");
   Staging& st = staging();
   std::lock_guard<std::mutex> g(st.mu);
   if (!st.buf) {

@@ -2464,7 +2464,15 @@ void HeffBatch::apply_graph(int precision, hipStream_t stream) {
   if (!exec_[precision]) {
     // kernels only (no memset nodes); captured on this thread's stream
     DMRG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    apply(precision, stream);
+    try {
+      apply(precision, stream);
+    } catch (...) {
+      // never leave the stream capturing
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(stream, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
     DMRG_HIP(hipStreamEndCapture(stream, &graph_[precision]));
     DMRG_HIP(hipGraphInstantiate(&exec_[precision], graph_[precision], nullptr, nullptr, 0));
   }

@@ -1862,12 +1862,19 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   }
   __syncthreads();
   // T upper triangular (dlarft): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
+  // (S[l][j] = y_l^T y_j sits at T[j][l]); row r of column j by 16 threads,
+  // shuffle-reduced within their 16 lanes
   for (int j = 0; j < bw; ++j) {
-    if (tid < j) {
+    const int r = tid >> 4, sub = tid & 15;
+    if (r < j) {
       double s = 0.0;
-      for (int l = tid; l < j; ++l) s += T[tid * b + l] * T[j * b + l];
-      T[tid * b + j] = -tau[j] * s;
-    } else if (tid == j) {
+      for (int l = r + sub; l < j; l += 16) s += T[r * b + l] * T[j * b + l];
+      s += __shfl_xor(s, 8, 64);
+      s += __shfl_xor(s, 4, 64);
+      s += __shfl_xor(s, 2, 64);
+      s += __shfl_xor(s, 1, 64);
+      if (sub == 0) T[r * b + j] = -tau[j] * s;
+    } else if (r == j && sub == 0) {
       T[j * b + j] = tau[j];
     }
     __syncthreads();
@@ -1877,14 +1884,34 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   double* Pk = p.P + (int64_t)k * b * M;
   for (int64_t e = tid; e < (int64_t)m * bw; e += blockDim.x) {
     const int j = (int)(e / m), i = (int)(e % m);
-    double s = 0.0;
-    const int lmax = min(j, i);
-    for (int l = 0; l <= lmax; ++l) {
-      const double yil = (i == l) ? 1.0 : pan[(int64_t)l * ld + i];
-      s += yil * T[l * b + j];
-    }
     Yk[e] = i < j ? 0.0 : (i == j ? 1.0 : pan[(int64_t)j * ld + i]);
-    Pk[e] = s;
+  }
+  {
+    // P = Y T with FP64 MFMA: 16 x 16 output tiles over the waves, K = the
+    // panel's columns in steps of 4 (Y[i][l] = 0 for i < l, 1 for i = l; T
+    // upper triangular, so K stops at the tile's last column; its strict lower
+    // triangle still holds Y^T Y and is read as zero)
+    const int i16 = lane & 15, kk = lane >> 4;
+    const int nib = (m + 15) >> 4, njb = (bw + 15) >> 4;
+    for (int tile = wave; tile < nib * njb; tile += nw) {
+      const int ib = tile / njb, jb = tile % njb;
+      const int ia = ib * 16 + i16;       // row of the A operand (Y)
+      const int jc = jb * 16 + i16;       // column of the B operand (T)
+      const int kend = min(bw, jb * 16 + 16);
+      d4v acc = {0.0, 0.0, 0.0, 0.0};
+      for (int l0 = 0; l0 < kend; l0 += 4) {
+        const int l = l0 + kk;
+        double ya = 0.0, tb = 0.0;
+        if (l < bw && ia < m) ya = ia < l ? 0.0 : (ia == l ? 1.0 : pan[(int64_t)l * ld + ia]);
+        if (l <= jc && jc < bw) tb = T[l * b + jc];  // upper triangle only (S below)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, tb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = ib * 16 + kk + 4 * v, col = jb * 16 + i16;
+        if (row < m && col < bw) Pk[(int64_t)col * m + row] = acc[v];
+      }
+    }
   }
   if (kLds) {  // R block of the panel back to A (upper triangle of its top bw rows)
     for (int64_t e = tid; e < (int64_t)bw * bw; e += blockDim.x) {
