@@ -1778,20 +1778,30 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   double* tau = T + b * b;
   __syncthreads();
   // Column j: every thread forms the reflector H = I - tau v v^T with
-  // v = (1, x[1:] * scal) from the block-reduced norm (same arithmetic, same
-  // order, so identical values), the trailing panel columns are updated in
-  // parallel (16 threads per column, shuffle-reduced dot), and x is scaled
-  // into v while the next column's norm is formed.  Two barriers per column.
-  for (int j = 0; j < bw; ++j) {
-    double* x = pan + (int64_t)j * ld + j;  // x[0 .. len-1], len = m - j
-    const int len = m - j;
+  // v = (1, x[1:] * scal) from |x[1:]|^2 (the same value for every thread),
+  // the trailing panel columns are updated in parallel (16 threads per
+  // column, shuffle-reduced dot), and x is scaled into v after the barrier.
+  // The 16 threads that update column j+1 also form its |x[1:]|^2 for the
+  // next step (column 0's comes from a block reduction), so a column costs
+  // one barrier.
+  __shared__ double colss[2];
+  {
     double ss = 0.0;
-    for (int i = 1 + tid; i < len; i += blockDim.x) ss += x[i] * x[i];
+    for (int i = 1 + tid; i < m; i += blockDim.x) ss += pan[i] * pan[i];
     ss = wave_sum(ss);
     if (lane == 0) red[wave] = ss;
     __syncthreads();
-    double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += red[w];
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += red[w];
+      colss[0] = t;
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < bw; ++j) {
+    double* x = pan + (int64_t)j * ld + j;  // x[0 .. len-1], len = m - j
+    const int len = m - j;
+    const double t = colss[j & 1];
     const double alpha = x[0];
     double tj = 0.0, beta = alpha, scal = 0.0;
     if (t > 0.0) {
@@ -1800,10 +1810,12 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       scal = 1.0 / (alpha - beta);
     }
     if (tid == 0) tau[j] = tj;
-    if (tj != 0.0) {
-      const int c = j + 1 + (tid >> 4), sub = tid & 15;
-      if (c < bw) {
-        double* y = pan + (int64_t)c * ld + j;
+    const int c = j + 1 + (tid >> 4), sub = tid & 15;
+    if (c < bw) {
+      double* y = pan + (int64_t)c * ld + j;
+      const bool next = c == j + 1;  // this group also forms column j+1's norm
+      double acc = 0.0;
+      if (tj != 0.0) {
         double d = 0.0;
         // unrolled so that the LDS reads of four iterations are in flight
         // together (same accumulation order)
@@ -1816,8 +1828,21 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
         d = (y[0] + scal * d) * tj;  // tau v^T y
         const double ds = d * scal;
 #pragma unroll 4
-        for (int i = 1 + sub; i < len; i += 16) y[i] -= ds * x[i];
+        for (int i = 1 + sub; i < len; i += 16) {
+          const double yn = y[i] - ds * x[i];
+          y[i] = yn;
+          if (next && i >= 2) acc += yn * yn;
+        }
         if (sub == 0) y[0] -= d;
+      } else if (next) {
+        for (int i = 2 + sub; i < len; i += 16) acc += y[i] * y[i];
+      }
+      if (next) {
+        acc += __shfl_xor(acc, 8, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 1, 64);
+        if (sub == 0) colss[(j + 1) & 1] = acc;
       }
     }
     __syncthreads();
