@@ -21,7 +21,8 @@ struct Staging {
   std::mutex mu;
   char* buf = nullptr;
   size_t cap = 0, head = 0;
-  std::vector<hipEvent_t> pending, spare;
+  std::vector<std::pair<hipEvent_t, hipStream_t>> pending;  // copy done event, its stream
+  std::vector<hipEvent_t> spare;
 };
 constexpr size_t kStagingBytes = size_t(32) << 20;
 constexpr size_t kMaxPending = 4096;
@@ -53,9 +54,16 @@ void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream) 
   }
   const size_t need = (bytes + 255) & ~size_t(255);
   if (st.head + need > st.cap || st.pending.size() >= kMaxPending) {
-    for (hipEvent_t e : st.pending) {
-      DMRG_HIP(hipEventSynchronize(e));
-      st.spare.push_back(e);
+    bool all = true;
+    for (const auto& pe : st.pending) {
+      if (hipEventSynchronize(pe.first) != hipSuccess) all = false;
+      st.spare.push_back(pe.first);
+    }
+    if (!all) {
+      // an event whose stream was destroyed by its owner cannot be waited
+      // on; wait for the whole device instead before the arena is reused
+      (void)hipGetLastError();
+      DMRG_HIP(hipDeviceSynchronize());
     }
     st.pending.clear();
     st.head = 0;
@@ -71,8 +79,23 @@ void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream) 
     e = st.spare.back();
     st.spare.pop_back();
   }
-  st.pending.push_back(e);
+  st.pending.push_back({e, stream});
   DMRG_HIP(hipEventRecord(e, stream));
+}
+
+void upload_forget_stream(hipStream_t stream) {
+  Staging& st = staging();
+  std::lock_guard<std::mutex> g(st.mu);
+  std::vector<std::pair<hipEvent_t, hipStream_t>> keep;
+  for (const auto& pe : st.pending) {
+    if (pe.second == stream) {
+      (void)hipEventSynchronize(pe.first);
+      st.spare.push_back(pe.first);
+    } else {
+      keep.push_back(pe);
+    }
+  }
+  st.pending.swap(keep);
 }
 
 DevicePool& DevicePool::get() {

@@ -112,6 +112,10 @@ class DevBuf {
 // pageable copy and a stream synchronisation instead.  Not for use inside a
 // stream capture.
 void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream);
+// Wait for the staging copies issued on `stream` and stop tracking them; call
+// before `stream` is destroyed (an event is not waited on after its stream
+// is gone).
+void upload_forget_stream(hipStream_t stream);
 
 // Upload a host vector into a (re)sized device buffer on `stream` (see
 // upload_bytes: the host vector may be reused once this returns).

@@ -260,6 +260,7 @@ Context::Context(hipStream_t s) : stream(s) {
 }
 
 Context::~Context() {
+  upload_forget_stream(stream);
   if (pinned_) (void)hipHostFree(pinned_);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
