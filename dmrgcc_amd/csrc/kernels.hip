@@ -1866,15 +1866,23 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       const int i = lane & 15, kk = lane >> 4;
       const int cl = ba * 16 + i, cj = bb * 16 + i;
       d4v acc = {0.0, 0.0, 0.0, 0.0};
+      // loads from clamped (in-bounds) addresses first, the triangle and edge
+      // selects after them, so a slab's eight loads are in flight together
+      const double* pa = pan + (int64_t)min(cl, bw - 1) * ld;
+      const double* pb = pan + (int64_t)min(cj, bw - 1) * ld;
       for (int base = 0; base < m; base += 16) {
+        double va[4], vb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int rc = min(base + 4 * kk + t, m - 1);
+          va[t] = pa[rc];
+          vb[t] = pb[rc];
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int r = base + 4 * kk + t;
-          double ya = 0.0, yb = 0.0;
-          if (r < m) {
-            if (cl < bw) ya = r < cl ? 0.0 : (r == cl ? 1.0 : pan[(int64_t)cl * ld + r]);
-            if (cj < bw) yb = r < cj ? 0.0 : (r == cj ? 1.0 : pan[(int64_t)cj * ld + r]);
-          }
+          const double ya = (r < m && cl < bw) ? (r < cl ? 0.0 : (r == cl ? 1.0 : va[t])) : 0.0;
+          const double yb = (r < m && cj < bw) ? (r < cj ? 0.0 : (r == cj ? 1.0 : vb[t])) : 0.0;
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, yb, acc, 0, 0, 0);
         }
       }
@@ -1926,9 +1934,10 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       d4v acc = {0.0, 0.0, 0.0, 0.0};
       for (int l0 = 0; l0 < kend; l0 += 4) {
         const int l = l0 + kk;
-        double ya = 0.0, tb = 0.0;
-        if (l < bw && ia < m) ya = ia < l ? 0.0 : (ia == l ? 1.0 : pan[(int64_t)l * ld + ia]);
-        if (l <= jc && jc < bw) tb = T[l * b + jc];  // upper triangle only (S below)
+        const double pv = pan[(int64_t)min(l, bw - 1) * ld + min(ia, m - 1)];  // clamped, in bounds
+        const double tv = T[min(l, bw - 1) * b + min(jc, bw - 1)];
+        const double ya = (l < bw && ia < m) ? (ia < l ? 0.0 : (ia == l ? 1.0 : pv)) : 0.0;
+        const double tb = (l <= jc && jc < bw) ? tv : 0.0;  // upper triangle only (S below)
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, tb, acc, 0, 0, 0);
       }
 #pragma unroll
