@@ -580,6 +580,27 @@ def _ptr(t):
     return _vp(t.data_ptr()) if t is not None else _vp()
 
 
+def _vec_ptr(t, doubles, cplx, what):
+    """Device pointer of a vector the library reads or writes `doubles`
+    float64 words of.  The C ABI takes no length, so a tensor that is not a
+    contiguous CUDA float64 (complex128 for a complex plan) vector of at least
+    that many words is refused here, before any library call."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise DmrgError("%s: expected a torch tensor, got %s" % (what, type(t).__name__))
+    ok = (torch.float64, torch.complex128) if cplx else (torch.float64,)
+    if t.dtype not in ok:
+        raise DmrgError("%s: dtype %s, expected %s" % (what, t.dtype, " or ".join(str(d) for d in ok)))
+    if not t.is_cuda:
+        raise DmrgError("%s: tensor is on %s, expected a CUDA device" % (what, t.device))
+    if not t.is_contiguous():
+        raise DmrgError("%s: tensor is not contiguous" % what)
+    words = t.numel() * (2 if t.dtype == torch.complex128 else 1)
+    if words < doubles:
+        raise DmrgError("%s: %d float64 words, the plan needs %d" % (what, words, doubles))
+    return _vp(t.data_ptr())
+
+
 def _sync_torch():
     """Order the caller's pending torch work before a library call."""
     import sys
@@ -644,6 +665,12 @@ class HeffPlan:
             _ptr(L_data), len(Rb), _ip(Rb.o), _ip(Rb.l), _ip(Rb.r), _ip(Rb.rows), _ip(Rb.cols),
             _lp(Rb.off), _ptr(R_data), _vp(stream) if stream else _vp(), ctypes.byref(h)))
         self._h = h
+        self._set_words()
+
+    def _set_words(self):
+        v = np.zeros(10)
+        _check(lib().dmrgcc_heff_info(self._h, _dp(v)))
+        self.words = int(v[0]) * (2 if self.complex else 1)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -651,9 +678,12 @@ class HeffPlan:
             self._h = None
 
     def apply(self, x, y, precision=FP64):
-        """y += H_eff x (device float64 tensors)."""
+        """y += H_eff x (contiguous CUDA float64 tensors of psi_dim entries,
+        or complex128 / 2 psi_dim float64 for a complex plan)."""
+        px = _vec_ptr(x, self.words, self.complex, "x")
+        py = _vec_ptr(y, self.words, self.complex, "y")
         _sync_torch()
-        _check(lib().dmrgcc_heff_apply(self._h, _ptr(x), _ptr(y), precision))
+        _check(lib().dmrgcc_heff_apply(self._h, px, py, precision))
 
     @classmethod
     def _from_handle(cls, h, owner):
@@ -661,6 +691,7 @@ class HeffPlan:
         self._h = h
         self._keep = owner
         self.complex = bool(getattr(getattr(owner, "model", None), "complex", False))
+        self._set_words()
         return self
 
     def info(self):
@@ -702,8 +733,9 @@ class HeffPlan:
     def lanczos(self, ket, num_iter, precision=FP64, refine=True):
         e = ctypes.c_double()
         it = ctypes.c_int()
+        pk = _vec_ptr(ket, self.words, self.complex, "ket")
         _sync_torch()
-        _check(lib().dmrgcc_lanczos(self._h, _ptr(ket), num_iter, precision, int(refine),
+        _check(lib().dmrgcc_lanczos(self._h, pk, num_iter, precision, int(refine),
                                     ctypes.byref(e), ctypes.byref(it)))
         return e.value, it.value
 
@@ -730,12 +762,16 @@ class HeffBatch:
 
     def __init__(self, plans, xs, ys):
         n = len(plans)
+        if len(xs) != n or len(ys) != n:
+            raise DmrgError("HeffBatch: one x and one y per plan")
         self._keep = (list(plans), list(xs), list(ys))
         arr = lambda v: (_vp * n)(*v)
+        px = [_vec_ptr(x, p.words, p.complex, "x[%d]" % i) for i, (p, x) in enumerate(zip(plans, xs))]
+        py = [_vec_ptr(y, p.words, p.complex, "y[%d]" % i) for i, (p, y) in enumerate(zip(plans, ys))]
         h = _vp()
         _sync_torch()
-        _check(lib().dmrgcc_heff_batch_create(n, arr([p._h for p in plans]), arr([_ptr(x) for x in xs]),
-                                              arr([_ptr(y) for y in ys]), ctypes.byref(h)))
+        _check(lib().dmrgcc_heff_batch_create(n, arr([p._h for p in plans]), arr(px), arr(py),
+                                              ctypes.byref(h)))
         self._h = h
 
     def __del__(self):

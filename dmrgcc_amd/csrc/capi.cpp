@@ -30,11 +30,19 @@ struct dmrgcc_heff {
   HeffPlan plan;
   LanczosStats st;
   TwoSite two;  // the two-site state of a dmrgcc_dmrg_heff2 plan
+  // A plan built from a run (dmrgcc_dmrg_heff / _heff2) holds device
+  // pointers into that run's environments and site tensors.  It is valid
+  // while the run's epoch equals the value recorded here; advancing or
+  // destroying the run bumps the epoch (the buffers go back to the pool).
+  std::shared_ptr<const int64_t> run_epoch;
+  int64_t epoch = 0;
 };
 
 struct dmrgcc_dmrg {
   dmrgcc_model* model;
   std::unique_ptr<RealTensorRundmrg> run;
+  std::shared_ptr<int64_t> epoch = std::make_shared<int64_t>(0);
+  ~dmrgcc_dmrg() { ++*epoch; }
 };
 
 struct dmrgcc_heff_batch {
@@ -51,6 +59,11 @@ struct dmrgcc_blocks {
 };
 
 namespace {
+
+void require_live(const dmrgcc_heff* h) {
+  require(!h->run_epoch || *h->run_epoch == h->epoch,
+          "stale H_eff plan: the run it was built from has advanced or been destroyed", kErrArgument);
+}
 
 template <typename F>
 int guarded(F&& f) {
@@ -182,6 +195,18 @@ int env_update(int cs, int leigh, int d, int lb, int rb, const double* W, int nl
 }  // namespace
 
 namespace {
+// The items of RealMatrix::MultiplyToMatrix calls: C_i = A_i B_i, row-major
+void block_gemm_items(GemmBatch& gb, int n, const double* const* A, const double* const* B,
+                      double* const* C, const int* M, const int* N, const int* K, int precision) {
+  require(n >= 0 && (n == 0 || (A && B && C && M && N && K)), "bad arguments", kErrArgument);
+  require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+  for (int i = 0; i < n; ++i) {
+    require(M[i] >= 0 && N[i] >= 0 && K[i] >= 0, "negative GEMM size");
+    gb.begin_item(C[i], N[i], M[i], N[i], 0);
+    gb.add(A[i], K[i], B[i], N[i], K[i], 0, 1.0);
+  }
+}
+
 RunParams make_params(int first_block, int first_dim, int nhalf, const int* maxm, const int* maxb,
                       const int* niter, const double* canpre, const double* noise, int precision,
                       int refine, int verbose) {
@@ -474,15 +499,9 @@ int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn)
 int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, double* const* C,
                       const int* M, const int* N, const int* K, int precision, void* stream) {
   return guarded([&] {
-    require(n >= 0 && (n == 0 || (A && B && C && M && N && K)), "bad arguments", kErrArgument);
-    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
     hipStream_t s = static_cast<hipStream_t>(stream);
     GemmBatch gb;
-    for (int i = 0; i < n; ++i) {
-      require(M[i] >= 0 && N[i] >= 0 && K[i] >= 0, "negative GEMM size");
-      gb.begin_item(C[i], N[i], M[i], N[i], 0);
-      gb.add(A[i], K[i], B[i], N[i], K[i], 0, 1.0);
-    }
+    block_gemm_items(gb, n, A, B, C, M, N, K, precision);
     gb.finalize(s);
     gb.launch(s, precision);
     DMRG_HIP(hipStreamSynchronize(s));
@@ -537,6 +556,7 @@ int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision)
   return guarded([&] {
     require(h && x && y, "null argument", kErrArgument);
     require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    require_live(h);
     h->plan.apply(x, y, precision, h->cx->stream);
     h->cx->sync();
   });
@@ -568,6 +588,7 @@ int dmrgcc_heff_tune(dmrgcc_heff* h, int precision, int reps, int cap, int* ncan
   return guarded([&] {
     require(h && ncand && tile_mode && split_k && ms, "null argument", kErrArgument);
     require(precision >= 0 && precision <= 2 && reps > 0, "bad arguments", kErrArgument);
+    require_live(h);
     const std::vector<HeffPlan::Tuning> t = h->plan.tune(precision, reps, h->cx->stream);
     require(cap >= (int)t.size(), "tune: output arrays too short", kErrArgument);
     *ncand = (int)t.size();
@@ -600,6 +621,7 @@ int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int
   return guarded([&] {
     require(h && ket && energy, "null argument", kErrArgument);
     require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    require_live(h);
     int it = 0;
     *energy = lanczos_solve(h->plan, ket, num_iter, precision, refine, *h->cx, h->st, &it);
     if (iters_out) *iters_out = it;
@@ -879,6 +901,7 @@ int dmrgcc_dmrg_write_space(dmrgcc_dmrg* h, const char* path) {
 int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached) {
   return guarded([&] {
     require(h != nullptr, "null handle", kErrArgument);
+    ++*h->epoch;  // plans built from the run point at buffers this may replace
     const bool r = h->run->Advance(half, site);
     if (reached) *reached = r ? 1 : 0;
   });
@@ -897,6 +920,8 @@ int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     auto e = std::make_unique<dmrgcc_heff>();
     e->cx = std::make_unique<Context>(nullptr);
     h->run->BuildHeff(site, e->plan);
+    e->run_epoch = h->epoch;
+    e->epoch = *h->epoch;
     *out = e.release();
   });
 }
@@ -905,6 +930,20 @@ int dmrgcc_heff_batch_create(int n, dmrgcc_heff* const* plans, const double* con
                              dmrgcc_heff_batch** out) {
   return guarded([&] {
     require(n > 0 && plans && x && y && out, "bad arguments", kErrArgument);
+    // Each plan owns one stage buffer; the same plan twice in one launch
+    // would have both instances write it at once.  Checked before anything
+    // is dereferenced or allocated.
+    {
+      std::vector<const dmrgcc_heff*> seen(plans, plans + n);
+      std::sort(seen.begin(), seen.end());
+      require(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
+              "H_eff batch: the same plan appears more than once (one stage buffer per plan)",
+              kErrArgument);
+    }
+    for (int i = 0; i < n; ++i) {
+      require(plans[i] != nullptr, "null plan in batch", kErrArgument);
+      require_live(plans[i]);
+    }
     auto b = std::make_unique<dmrgcc_heff_batch>();
     b->cx = std::make_unique<Context>(nullptr);
     std::vector<HeffPlan*> ps;
@@ -925,6 +964,7 @@ int dmrgcc_heff_batch_create(int n, dmrgcc_heff* const* plans, const double* con
 int dmrgcc_heff_batch_apply(dmrgcc_heff_batch* b, int precision, int use_graph) {
   return guarded([&] {
     require(b && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    for (const dmrgcc_heff* p : b->plans) require_live(p);
     if (use_graph) b->batch.apply_graph(precision, b->cx->stream);
     else b->batch.apply(precision, b->cx->stream);
     DMRG_HIP(hipStreamSynchronize(b->cx->stream));
@@ -934,6 +974,7 @@ int dmrgcc_heff_batch_apply(dmrgcc_heff_batch* b, int precision, int use_graph) 
 int dmrgcc_heff_batch_time(dmrgcc_heff_batch* b, int precision, int reps, double* ms) {
   return guarded([&] {
     require(b && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    for (const dmrgcc_heff* p : b->plans) require_live(p);
     hipStream_t s = b->cx->stream;
     hipEvent_t e0, e1;
     DMRG_HIP(hipEventCreate(&e0));
@@ -978,6 +1019,8 @@ int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     auto e = std::make_unique<dmrgcc_heff>();
     e->cx = std::make_unique<Context>(nullptr);
     h->run->BuildTwoSite(site, e->two, e->plan);
+    e->run_epoch = h->epoch;
+    e->epoch = *h->epoch;
     *out = e.release();
   });
 }
@@ -986,6 +1029,7 @@ int dmrgcc_heff_state(const dmrgcc_heff* h, double* dst) {
   return guarded([&] {
     require(h && dst, "null argument", kErrArgument);
     require(h->two.theta.size > 0, "no two-site state on this plan", kErrArgument);
+    require_live(h);
     DMRG_HIP(hipMemcpyAsync(dst, h->two.theta.base(), h->two.theta.doubles() * sizeof(double),
                             hipMemcpyDeviceToDevice, h->cx->stream));
     DMRG_HIP(hipStreamSynchronize(h->cx->stream));
@@ -1109,6 +1153,7 @@ int dmrgcc_vector_time(int64_t n, int reps, double* ms) {
 int dmrgcc_heff_time(dmrgcc_heff* h, int precision, int reps, double* ms) {
   return guarded([&] {
     require(h && ms && reps > 0 && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
+    require_live(h);
     const int64_t n = std::max<int64_t>(h->plan.doubles(), 1);
     DevBuf<double> x((size_t)n), y((size_t)n);
     fill(x.data(), n, 1e-3, h->cx->stream);

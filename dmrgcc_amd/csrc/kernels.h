@@ -93,6 +93,13 @@ struct GemmBatch {
   DevBuf<GemmTile> d_tiles;
   DevBuf<SplitTask> d_splits;
   DevBuf<double> part;
+  // where launch() reads the descriptors: the buffers above (finalize) or a
+  // caller's arena (place)
+  GemmItem* dev_items = nullptr;
+  GemmContrib* dev_contribs = nullptr;
+  GemmTile* dev_tiles = nullptr;
+  SplitTask* dev_splits = nullptr;
+  std::vector<int> part_items;  // split-K chunk items awaiting bind_parts()
   int split_k = default_split_k();
   // 32: 32x32 wave tiles (four independent tiles per workgroup); otherwise
   // 64x64 workgroup tiles (environment DMRGCC_TILE)
@@ -107,7 +114,7 @@ struct GemmBatch {
 
   void clear() {
     items.clear(); contribs.clear(); tiles.clear(); splits.clear();
-    group_items.clear(); group_tiles.clear(); flops = 0;
+    group_items.clear(); group_tiles.clear(); part_items.clear(); flops = 0;
   }
   // Start an output item; subsequent add() calls attach terms to it.
   void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0);
@@ -116,6 +123,18 @@ struct GemmBatch {
   int end_group() { group_items.push_back((int)items.size()); return (int)group_items.size() - 1; }
   // Finish construction: tile every item and upload the descriptors.
   void finalize(hipStream_t stream, int tile_m = 64, int tile_n = 64);
+  // The host half of finalize(): split-K and tiling only.  Returns the
+  // number of split-K partial doubles the batch needs (their items hold
+  // offsets until bind_parts()).
+  int64_t layout(int tile_m = 64, int tile_n = 64);
+  void bind_parts(double* base);
+  // Device bytes place() needs for the descriptors and `psize` partials.
+  size_t placed_bytes(int64_t psize) const;
+  // Instead of finalize()'s allocation and upload: put the descriptors and
+  // split-K partials of a laid-out batch into `arena` (device memory of
+  // `cap` bytes owned by the caller) with plain copies on `stream`.
+  // Allocates nothing; launch() then reads the arena.
+  void place(void* arena, size_t cap, int64_t psize, hipStream_t stream);
   // reduce = false launches the GEMM kernel only (timing of that kernel)
   void launch(hipStream_t stream, int precision, const double* in = nullptr,
               double* out = nullptr, bool reduce = true) const;

@@ -2,8 +2,10 @@
 //
 // * grouped block GEMM: every sector product of the H_eff matvec, the
 //   environment updates, the merges and the expansion runs through one
-//   persistent-descriptor launch.  A workgroup owns one 64x64 output tile of
-//   one output block and accumulates all of that block's product terms
+//   grouped launch described by uploaded descriptors (one workgroup per
+//   output tile, in a cost-sorted, XCD-aware order).  A workgroup owns one
+//   64x64 output tile of one output block and accumulates all of that
+//   block's product terms
 //   (K-concatenation), so there are no atomics and results are deterministic.
 //   Four waves per workgroup, each a 32x32 sub-tile = 2x2 MFMA 16x16 blocks.
 //   FP64 uses v_mfma_f64_16x16x4f64, FP32 v_mfma_f32_16x16x4f32, BF16x3
@@ -2072,14 +2074,75 @@ void set_gemm_config(int tile_mode, int split_k) {
 }
 
 void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
+  const int64_t psize = layout(tile_m, tile_n);
+  if (psize > 0) {
+    part.ensure((size_t)psize);
+    bind_parts(part.data());
+  }
+  upload(d_items, items, stream);
+  upload(d_contribs, contribs, stream);
+  upload(d_tiles, tiles, stream);
+  if (!splits.empty()) upload(d_splits, splits, stream);
+  dev_items = d_items.data();
+  dev_contribs = d_contribs.data();
+  dev_tiles = d_tiles.data();
+  dev_splits = d_splits.data();
+}
+
+void GemmBatch::bind_parts(double* base) {
+  for (int i : part_items) items[i].C = base + (intptr_t)items[i].C;
+  for (SplitTask& t : splits) t.part = base + (intptr_t)t.part;
+  part_items.clear();
+}
+
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+size_t GemmBatch::placed_bytes(int64_t psize) const {
+  return align256(items.size() * sizeof(GemmItem)) + align256(contribs.size() * sizeof(GemmContrib)) +
+         align256(tiles.size() * sizeof(GemmTile)) + align256(splits.size() * sizeof(SplitTask)) +
+         align256((size_t)psize * sizeof(double));
+}
+
+void GemmBatch::place(void* arena, size_t cap, int64_t psize, hipStream_t stream) {
+  const size_t need = placed_bytes(psize);
+  require(need <= cap,
+          "GEMM descriptors and split-K partials need " + std::to_string(need) + " bytes, the arena holds " +
+              std::to_string(cap),
+          kErrArgument);
+  char* p = static_cast<char*>(arena);
+  auto take = [&](size_t bytes) { char* q = p; p += align256(bytes); return q; };
+  GemmItem* di = reinterpret_cast<GemmItem*>(take(items.size() * sizeof(GemmItem)));
+  GemmContrib* dc = reinterpret_cast<GemmContrib*>(take(contribs.size() * sizeof(GemmContrib)));
+  GemmTile* dt = reinterpret_cast<GemmTile*>(take(tiles.size() * sizeof(GemmTile)));
+  SplitTask* ds = reinterpret_cast<SplitTask*>(take(splits.size() * sizeof(SplitTask)));
+  double* dp = reinterpret_cast<double*>(take((size_t)psize * sizeof(double)));
+  if (psize > 0) bind_parts(dp);
+  // plain copies from the host vectors: no staging buffer, nothing allocated
+  auto put = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) DMRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+  };
+  put(di, items.data(), items.size() * sizeof(GemmItem));
+  put(dc, contribs.data(), contribs.size() * sizeof(GemmContrib));
+  put(dt, tiles.data(), tiles.size() * sizeof(GemmTile));
+  put(ds, splits.data(), splits.size() * sizeof(SplitTask));
+  dev_items = di;
+  dev_contribs = dc;
+  dev_tiles = dt;
+  dev_splits = ds;
+}
+
+int64_t GemmBatch::layout(int tile_m, int tile_n) {
   tiles.clear();
   splits.clear();
   group_tiles.clear();
+  part_items.clear();
+  int64_t psize = 0;
   if (!group_items.empty() && group_items.back() != (int)items.size()) end_group();
   // split-K: chunk the contribution lists of heavy items
   if (split_k > 0 && group_items.empty()) {
     std::vector<GemmItem> extra;
-    int64_t psize = 0;
     struct Pending { size_t item; int first_extra, n; };
     std::vector<Pending> pend;
     const size_t nitems = items.size();
@@ -2114,10 +2177,11 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
       pend.push_back(pd);
     }
     if (!extra.empty()) {
-      part.ensure((size_t)psize);
+      // chunk outputs hold offsets into the partial buffer until
+      // bind_parts() gives it an address
       const size_t base = items.size();
       for (GemmItem& e : extra) {
-        e.C = part.data() + (intptr_t)e.C;
+        part_items.push_back((int)items.size());
         items.push_back(e);
       }
       for (const Pending& pd : pend) {
@@ -2208,10 +2272,7 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
     group_tiles.push_back((int)tiles.size());
     first = end;
   }
-  upload(d_items, items, stream);
-  upload(d_contribs, contribs, stream);
-  upload(d_tiles, tiles, stream);
-  if (!splits.empty()) upload(d_splits, splits, stream);
+  return psize;
 }
 
 void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out,
@@ -2222,25 +2283,25 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   if (wave_tiles) {
     const dim3 grid((unsigned)(tiles.size() / 4));
     if (precision == kFP64)
-      hipLaunchKernelGGL(gemm_wave_kernel<double>, grid, block, 0, stream, d_tiles.data(), d_items.data(),
-                         d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_wave_kernel<double>, grid, block, 0, stream, dev_tiles, dev_items,
+                         dev_contribs, in, out);
     else if (precision == kFP32)
-      hipLaunchKernelGGL(gemm_wave_kernel<float>, grid, block, 0, stream, d_tiles.data(), d_items.data(),
-                         d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_wave_kernel<float>, grid, block, 0, stream, dev_tiles, dev_items,
+                         dev_contribs, in, out);
     else
-      hipLaunchKernelGGL(gemm_bf16x3_wave_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
-                         d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_bf16x3_wave_kernel, grid, block, 0, stream, dev_tiles, dev_items,
+                         dev_contribs, in, out);
   } else {
     const dim3 grid((unsigned)tiles.size());
     if (precision == kFP64)
-      hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, d_tiles.data(),
-                         d_items.data(), d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, dev_tiles,
+                         dev_items, dev_contribs, in, out);
     else if (precision == kFP32)
-      hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, d_tiles.data(),
-                         d_items.data(), d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, dev_tiles,
+                         dev_items, dev_contribs, in, out);
     else
-      hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, d_tiles.data(), d_items.data(),
-                         d_contribs.data(), in, out);
+      hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, dev_tiles, dev_items,
+                         dev_contribs, in, out);
   }
   DMRG_HIP(hipGetLastError());
   if (reduce && !splits.empty()) {
@@ -2248,7 +2309,7 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
     for (const SplitTask& t : splits) mx = std::max(mx, t.M);
     const unsigned gx = (unsigned)std::max(1, std::min(mx, 64));
     hipLaunchKernelGGL(split_reduce_kernel, dim3(gx, (unsigned)splits.size()), dim3(256), 0, stream,
-                       d_splits.data(), out);
+                       dev_splits, out);
     DMRG_HIP(hipGetLastError());
   }
 }
@@ -2286,6 +2347,7 @@ void GemmBatch::merge(const std::vector<const GemmBatch*>& bs, const std::vector
   split_k = sk;
   splits = sp;
   if (!splits.empty()) upload(d_splits, splits, stream);
+  dev_splits = d_splits.data();
 }
 
 void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const {
@@ -2293,16 +2355,16 @@ void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const
   const int t0 = group == 0 ? 0 : group_tiles[group - 1], t1 = group_tiles[group];
   if (t1 <= t0) return;
   const dim3 grid((unsigned)(t1 - t0)), block(kThreads);
-  const GemmTile* tl = d_tiles.data() + t0;
+  const GemmTile* tl = dev_tiles + t0;
   if (precision == kFP64)
-    hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, tl, d_items.data(),
-                       d_contribs.data(), nullptr, nullptr);
+    hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, tl, dev_items,
+                       dev_contribs, nullptr, nullptr);
   else if (precision == kFP32)
-    hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, tl, d_items.data(),
-                       d_contribs.data(), nullptr, nullptr);
+    hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, tl, dev_items,
+                       dev_contribs, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, tl, d_items.data(),
-                       d_contribs.data(), nullptr, nullptr);
+    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, tl, dev_items,
+                       dev_contribs, nullptr, nullptr);
   DMRG_HIP(hipGetLastError());
 }
 

@@ -328,7 +328,12 @@ int dmrgcc_run_files(dmrgcc_model* m, int first_block, int first_dim, int nhalf,
  * (*reached = 0; pass half = -1 to run to the end).  heff: the H_eff plan of
  * `site` on the current state (valid until the next advance); ket: device
  * pointer and length of that site's ket vector.  result: as dmrgcc_run.
- * set_precision: arithmetic of the following Lanczos solves. */
+ * set_precision: arithmetic of the following Lanczos solves.
+ * Plans from dmrgcc_dmrg_heff / dmrgcc_dmrg_heff2 point into the run's
+ * environment and site buffers: every dmrgcc_dmrg_advance and
+ * dmrgcc_dmrg_destroy makes them stale, and apply / lanczos / tune / time /
+ * state / batch calls on a stale plan fail with status 3 (info still
+ * works).  Destroy the plan, then build a new one. */
 typedef struct dmrgcc_dmrg dmrgcc_dmrg;
 int dmrgcc_dmrg_create(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                        const int* maxb, const int* niter, const double* canpre, const double* noise,
@@ -368,7 +373,10 @@ int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
  * use_graph != 0 captures the launches once per arithmetic into a HIP
  * graph and replays it.  time: ms[0] mean graph replay of the whole batch,
  * ms[1] mean of the same products plan by plan.  info: [0] sum of F_ref,
- * [1] tiles, [2] plans. */
+ * [1] tiles, [2] plans.  Each plan appears at most once (its stage buffer
+ * is written by the launch): a repeated plan pointer, or a stale run plan,
+ * fails with status 3.  One operator applied to several vectors takes one
+ * plan per vector. */
 typedef struct dmrgcc_heff_batch dmrgcc_heff_batch;
 int dmrgcc_heff_batch_create(int n, dmrgcc_heff* const* plans, const double* const* x, double* const* y,
                              dmrgcc_heff_batch** out);

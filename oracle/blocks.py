@@ -125,3 +125,36 @@ class Lattice:
             self.phys[li[k]][ri[k]] = pi[k]
             mats.append(rng.complex_matrix(rows, cols) if cplx else rng.random_matrix(rows, cols))
         self.ket = BlockMat(li, ri, mats)
+
+
+def lattice_from_export(d, phys, blocks, data):
+    """A Lattice from a site exported as flat block lists: the physical-index
+    table phys[nl][nr] and the ket blocks (o, l, r, rows, cols, off) over the
+    flat `data` (the VectorizeMatrixBlock layout).  Sector positions stand
+    for the QN-table indices; sector dimensions are read off the blocks."""
+    phys = np.asarray(phys)
+    lat = Lattice(d)
+    nl, nr = phys.shape
+    lat.left_block = list(range(nl))
+    lat.right_block = list(range(nr))
+    lat.left_dim = [0] * nl
+    lat.right_dim = [0] * nr
+    lat.phys = phys.tolist()
+    mats = []
+    for (_, l, r, rows, cols, off) in blocks:
+        mats.append(np.array(data[off:off + rows * cols]).reshape(rows, cols))
+        lat.left_dim[l] = rows
+        lat.right_dim[r] = cols
+    lat.ket = BlockMat([b[1] for b in blocks], [b[2] for b in blocks], mats)
+    return lat
+
+
+def env_from_export(blocks, data, nch):
+    """An environment (one BlockMat per MPO channel) from flat block lists
+    (o, l, r, rows, cols, off) over `data`."""
+    out = [BlockMat() for _ in range(nch)]
+    for (o, l, r, rows, cols, off) in blocks:
+        out[o].left.append(l)
+        out[o].right.append(r)
+        out[o].mats.append(np.array(data[off:off + rows * cols]).reshape(rows, cols))
+    return out

@@ -72,3 +72,27 @@ def test_batch_refuses_retiled_plan():
     b2 = dm.HeffBatch(plans, xs, ys)  # a new batch is fine
     b2.apply(dm.FP64, graph=True)
     torch.cuda.synchronize()
+
+
+def test_batch_refuses_repeated_plan_and_stale_plans():
+    runs, plans = _instances()
+    n = plans[0].info()["psi_dim"]
+    x = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(2)]
+    y = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(2)]
+    with pytest.raises(dm.DmrgError, match="more than once"):
+        dm.HeffBatch([plans[0], plans[0]], x, y)
+    # advancing the run that built a plan makes the plan stale: every call
+    # that would read its buffers is refused, info() still answers
+    b = dm.HeffBatch([plans[0]], x[:1], y[:1])
+    b.apply(dm.FP64, graph=False)
+    assert runs[0].advance(3, 2)
+    with pytest.raises(dm.DmrgError, match="stale"):
+        plans[0].apply(x[0], y[0], dm.FP64)
+    with pytest.raises(dm.DmrgError, match="stale"):
+        plans[0].lanczos(x[0], 4)
+    with pytest.raises(dm.DmrgError, match="stale"):
+        b.apply(dm.FP64, graph=True)
+    assert plans[0].info()["psi_dim"] == n
+    fresh = runs[0].heff(2)
+    assert fresh.info()["psi_dim"] > 0
+    torch.cuda.synchronize()

@@ -1,0 +1,368 @@
+"""Parity at the sizes of the five measured workloads.
+
+The library sweeps each workload on the device; the state it reaches (ket,
+left and right environments of one site) is exported through the C ABI and
+the oracle recomputes from it:
+
+  8x4 cylinder, m = 1000 (the headline workload): the centre-site H_eff in
+      FP64 / FP32 / BF16x3, both environment updates and the Lanczos energy;
+  4x4 square, m = 200, 6 sweeps: the whole trajectory (energy and discarded
+      weight per half sweep, rand() draws, final bond dimensions) against the
+      oracle's run stored in tests/golden/c1_trajectory.json;
+  64-chain scan: every feasible chain's centre-site H_eff in one graph
+      launch, bit-identical to plan by plan, and a sample against the oracle;
+  4x4 two-component bosons with flux, m = 2000: the complex centre-site H_eff
+      and environment updates;
+  10x6 cylinder, m = 4000: the centre-site H_eff against the oracle, its
+      symmetry, and the SVD of the largest sectors.
+
+Tolerances: FP64 1e-12, FP32 1e-5, BF16x3 2e-4 relative to max|y| (the
+operands' rounding); energies 1e-10 relative (FP64) and 1e-9 (BF16x3 with
+the FP64 refinement)."""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+import dmrgcc_amd as dm
+from dmrgcc_amd import workloads
+from oracle.blocks import env_from_export, lattice_from_export
+from oracle.dmrg import heff_apply, heff_flops, heff_tasks, lanczos, left_env, right_env
+from oracle.model import SpinfulBoseSquare, SpinlessBoseSquare
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = {dm.FP64: 1e-12, dm.FP32: 1e-5, dm.BF16X3: 2e-4}
+
+
+def log(msg):
+    print("[configs] " + msg, flush=True)
+
+
+def export(run, model, site, W):
+    """(oracle lattice, L, R, raw L export, raw R export) of `site`."""
+    kb, kd = run.site_blocks(site, 0)
+    lat = lattice_from_export(model.d, run.phys(site), kb, kd)
+    Lx = run.site_blocks(site, 1)
+    Rx = run.site_blocks(site, 2)
+    L = env_from_export(Lx[0], Lx[1], W.shape[0])
+    R = env_from_export(Rx[0], Rx[1], W.shape[1])
+    return lat, L, R, Lx, Rx
+
+
+def dev(a):
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a.view(np.float64) if np.iscomplexobj(a) else a).cuda()
+
+
+def check_heff(plan, lat, W, L, R, seed, precisions=(dm.FP64, dm.FP32, dm.BF16X3)):
+    """y0 + H x of the plan in each arithmetic against the oracle."""
+    n = lat.ket.dim()
+    cplx = np.iscomplexobj(W)
+    assert plan.info()["psi_dim"] == n
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
+    y0 = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
+    tasks = heff_tasks(lat, W, L, R)
+    t0 = time.time()
+    ref = heff_apply(lat, W, L, R, x, y0.copy(), tasks)
+    log("oracle matvec psi_dim %d: %.2f s" % (n, time.time() - t0))
+    assert plan.info()["flops_ref"] == heff_flops(lat, W, L, R, tasks)
+    errs = {}
+    for prec in precisions:
+        yd = dev(y0.copy())
+        plan.apply(dev(x), yd, prec)
+        got = yd.cpu().numpy()
+        if cplx:
+            got = got.view(np.complex128)
+        errs[prec] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    log("H_eff rel. errors %s" % {dm.PRECISION_NAMES[k]: "%.1e" % v for k, v in errs.items()})
+    for prec, e in errs.items():
+        assert e < TOL[prec], (dm.PRECISION_NAMES[prec], e)
+    return x, ref
+
+
+def check_env(lat, W, E, Ex, leigh):
+    """The library's environment update (C ABI, from the exported state)
+    against the oracle's, block by block."""
+    kb = lat.ket
+    phys = np.array(lat.phys, dtype=np.int32)
+    rows = [m.shape[0] for m in kb.mats]
+    cols = [m.shape[1] for m in kb.mats]
+    blocks, data = Ex
+    bl = dm.BlockList(*[[b[i] for b in blocks] for i in range(6)])
+    res = dm.env_update(leigh, W, phys, kb.left, kb.right, rows, cols, dev(kb.to_vector()), bl, dev(data))
+    t0 = time.time()
+    ref = (left_env if leigh == 0 else right_env)(lat, W, E)
+    log("oracle env update (%s): %.2f s" % ("left" if leigh == 0 else "right", time.time() - t0))
+    host = res.to_host()
+    got = {(ch, l, r): host[of:of + nr_ * nc].reshape(nr_, nc) for (ch, l, r, nr_, nc, of) in res.blocks}
+    assert len(res.blocks) == sum(b.n for b in ref)
+    scale = max(max((np.abs(m).max() for b in ref for m in b.mats if m.size), default=1.0), 1.0)
+    worst = 0.0
+    for ch, bm in enumerate(ref):
+        for k in range(bm.n):
+            m = bm.mats[k]
+            g = got[(ch, bm.left[k], bm.right[k])]
+            assert g.shape == m.shape
+            if m.size:
+                worst = max(worst, float(np.abs(g - m).max()))
+    log("env update max abs diff %.1e (scale %.1e)" % (worst, scale))
+    assert worst <= 1e-12 * scale
+
+
+# --------------------------------------------------------------------------
+# 8x4 cylinder, m = 1000
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def cylinder():
+    model, tab = workloads.spinless("c2")
+    nx, ny, N, d, U, py, maxm, _ = workloads.SPINLESS["c2"]
+    W = SpinlessBoseSquare(nx, ny, N, d, jx=-1.0, jy=-1.0, U=U / 2, period_y=py).mpo()[0]
+    site = model.L // 2
+    run = model.start(tab)
+    t0 = time.time()
+    assert run.advance(6, site)  # first half sweep without noise, centre site
+    log("8x4 cylinder: half sweep 6, site %d reached in %.1f s" % (site, time.time() - t0))
+    assert np.array_equal(model.mpo_site(site), W[site])
+    lat, L, R, Lx, Rx = export(run, model, site, W[site])
+    log("8x4 cylinder: psi_dim %d, %d ket blocks, largest %s" % (
+        lat.ket.dim(), lat.ket.n, max((m.shape for m in lat.ket.mats), key=lambda s: s[0] * s[1])))
+    return run, model, site, W[site], lat, L, R, Lx, Rx
+
+
+def test_cylinder_heff_matches_oracle(cylinder):
+    run, model, site, W, lat, L, R, _, _ = cylinder
+    assert lat.ket.dim() > 100000  # the m = 1000 regime, many multi-tile blocks
+    plan = run.heff(site)
+    x, _ = check_heff(plan, lat, W, L, R, seed=1)
+    # symmetry of the operator at full size: <x|H y> = <y|H x>
+    y = np.random.default_rng(9).standard_normal(x.size)
+    hx, hy = torch.zeros(x.size, dtype=torch.float64, device="cuda"), \
+        torch.zeros(x.size, dtype=torch.float64, device="cuda")
+    plan.apply(dev(x), hx, dm.FP64)
+    plan.apply(dev(y), hy, dm.FP64)
+    a, b = float(np.dot(x, hy.cpu().numpy())), float(np.dot(y, hx.cpu().numpy()))
+    assert abs(a - b) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(hy.cpu().numpy()) * 10
+
+
+@pytest.mark.parametrize("leigh", [0, 1])
+def test_cylinder_env_updates_match_oracle(cylinder, leigh):
+    run, model, site, W, lat, L, R, Lx, Rx = cylinder
+    check_env(lat, W, L if leigh == 0 else R, Lx if leigh == 0 else Rx, leigh)
+
+
+def test_cylinder_lanczos_matches_oracle(cylinder):
+    run, model, site, W, lat, L, R, _, _ = cylinder
+    plan = run.heff(site)
+    ket0 = lat.ket.to_vector()
+    lat2 = lat.copy()
+    t0 = time.time()
+    e_ref = lanczos(lat2, W, L, R, 10)
+    log("oracle Lanczos: %.2f s, E = %.12f" % (time.time() - t0, e_ref))
+    v = dev(ket0.copy())
+    e, it = plan.lanczos(v, 10, dm.FP64)
+    log("library Lanczos FP64: E = %.12f (%d iterations)" % (e, it))
+    assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+    assert abs(abs(np.dot(v.cpu().numpy(), lat2.ket.to_vector())) - 1.0) < 1e-8
+    v2 = dev(ket0.copy())
+    e2, _ = plan.lanczos(v2, 10, dm.BF16X3, refine=True)
+    log("library Lanczos BF16x3 + FP64 refinement: dE/E = %.1e" % (abs(e2 - e_ref) / abs(e_ref)))
+    assert abs(e2 - e_ref) <= 1e-9 * abs(e_ref)
+
+
+# --------------------------------------------------------------------------
+# 4x4 square, m = 200, 6 sweeps: the whole trajectory
+# --------------------------------------------------------------------------
+def bond_dims(run, L):
+    out = []
+    for s in range(L):
+        used = {}
+        for (_, l, r, rows, cols, off) in run.site_blocks(s, 0)[0]:
+            used[r] = cols
+        out.append(int(sum(used.values())))
+    return out
+
+
+def check_trajectory(model, tab, name):
+    ref = json.load(open(os.path.join(HERE, "golden", "%s_trajectory.json" % name)))
+    assert [list(r) for r in tab] == ref["table"]
+    run = model.start(tab)
+    t0 = time.time()
+    assert not run.advance()
+    res = run.result()
+    log("%s: %.1f s, E = %.12f (oracle %.12f)" % (name, time.time() - t0, res["energies"][-1],
+                                                  ref["energies"][-1]))
+    assert res["rand_draws"] == ref["rand_draws"]
+    e, er = np.array(res["energies"]), np.array(ref["energies"])
+    log("per half sweep |dE|/|E|: %s" % ["%.1e" % v for v in np.abs(e - er) / np.abs(er)])
+    assert np.all(np.abs(e - er) <= 1e-10 * np.abs(er))
+    t, tr = np.array(res["trunc_err"]), np.array(ref["trunc_err"])
+    assert np.all(np.abs(t - tr) <= 1e-9 * np.maximum(np.abs(tr), 1e-12))
+    assert bond_dims(run, model.L) == ref["bond_dims"]
+
+
+def test_square_m200_trajectory_matches_oracle():
+    model, tab = workloads.spinless("c1")
+    check_trajectory(model, tab, "c1")
+
+
+def test_spinful_m64_trajectory_matches_oracle():
+    # the two-component flux model of the m = 2000 workload at m = 64, 2
+    # sweeps (the oracle's complex run takes minutes at this size)
+    model, tab = workloads.spinful("c4", maxm=64, sweeps=2)
+    check_trajectory(model, tab, "c4_m64")
+
+
+# --------------------------------------------------------------------------
+# 64-chain scan
+# --------------------------------------------------------------------------
+def test_chain_scan_batch_matches_plans_and_oracle():
+    draws = workloads.chain_draws(64)
+    runs, plans, feasible, infeasible = [], [], [], []
+    t0 = time.time()
+    for k, (L, U, m) in enumerate(draws):
+        if L >= 48:
+            # the real stack sizes a zero expansion block by left_dim_[r]
+            # (real_tensor_contraction.cc:448, :754), which fails a later
+            # shape check on these chains, as the reference's exit(-1) would
+            infeasible.append(k)
+            continue
+        model = dm.SpinlessModel(L, 1, L, 4, hop_x=-1.0, hop_y=0.0, U=U / 2)
+        run = model.start(workloads.table(m, 2))
+        assert run.advance(2, L // 2)
+        runs.append((run, model, L, U, m))
+        plans.append(run.heff(L // 2))
+        feasible.append(k)
+    log("chain scan: %d feasible, %d with L >= 48 recorded infeasible (%.1f s)" % (
+        len(feasible), len(infeasible), time.time() - t0))
+    assert len(feasible) + len(infeasible) == 64 and len(feasible) >= 30
+    # one L >= 48 chain does fail its shape check on the real stack
+    L, U, m = draws[infeasible[0]]
+    bad = dm.SpinlessModel(L, 1, L, 4, hop_x=-1.0, hop_y=0.0, U=U / 2).start(workloads.table(m, 2))
+    with pytest.raises(dm.DmrgError, match="do not match"):
+        bad.advance(2, L // 2)
+    del bad
+    rng = np.random.default_rng(3)
+    xs, ys, y0s, each = [], [], [], []
+    for p in plans:
+        n = p.info()["psi_dim"]
+        x = torch.tensor(rng.standard_normal(n), device="cuda")
+        y0 = torch.tensor(rng.standard_normal(n), device="cuda")
+        xs.append(x)
+        ys.append(y0.clone())
+        y0s.append(y0)
+    for prec in (dm.FP64, dm.BF16X3):
+        each = []
+        for p, x, y0 in zip(plans, xs, y0s):
+            y = y0.clone()
+            p.apply(x, y, prec)
+            each.append(y.cpu().numpy())
+        b = dm.HeffBatch(plans, xs, ys)
+        for rep in range(3):
+            for y, y0 in zip(ys, y0s):
+                y.copy_(y0)
+            torch.cuda.synchronize()
+            b.apply(prec, graph=True)
+            torch.cuda.synchronize()
+            for y, r in zip(ys, each):
+                assert np.array_equal(y.cpu().numpy(), r)
+        del b
+        # a sample against the oracle (every fifth feasible chain)
+        for i in range(0, len(plans), 5):
+            run, model, L, U, m = runs[i]
+            W = SpinlessBoseSquare(L, 1, L, 4, jx=-1.0, jy=0.0, U=U / 2).mpo()[0][L // 2]
+            lat, Le, Re, _, _ = export(run, model, L // 2, W)
+            x = xs[i].cpu().numpy()
+            ref = heff_apply(lat, W, Le, Re, x, y0s[i].cpu().numpy().copy())
+            err = np.abs(each[i] - ref).max() / np.abs(ref).max()
+            assert err < TOL[prec], (i, L, m, dm.PRECISION_NAMES[prec], err)
+    torch.cuda.synchronize()
+
+
+# --------------------------------------------------------------------------
+# 4x4 two-component bosons with flux, m = 2000 (complex)
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def spinful():
+    model, tab = workloads.spinful("c4")
+    nx, ny, nu, nd, d, flux, uu, udd, ud, _, _ = workloads.SPINFUL["c4"]
+    W = SpinfulBoseSquare(nx, ny, nu, nd, d, flux=flux, jx=-1.0, jy=-1.0, U_uu=uu, U_dd=udd,
+                          U_ud=ud).mpo()[0]
+    site = model.L // 2
+    run = model.start(tab)
+    t0 = time.time()
+    assert run.advance(4, site)
+    log("two-component 4x4: half sweep 4, site %d reached in %.1f s" % (site, time.time() - t0))
+    assert np.array_equal(model.mpo_site(site), W[site])
+    lat, L, R, Lx, Rx = export(run, model, site, W[site])
+    log("two-component 4x4: psi_dim %d (complex), %d ket blocks" % (lat.ket.dim(), lat.ket.n))
+    return run, model, site, W[site], lat, L, R, Lx, Rx
+
+
+def test_spinful_heff_matches_oracle(spinful):
+    run, model, site, W, lat, L, R, _, _ = spinful
+    assert lat.ket.dim() > 100000
+    plan = run.heff(site)
+    x, _ = check_heff(plan, lat, W, L, R, seed=4)
+    # Hermitian at full size: <x|H y> = conj(<y|H x>)
+    y = np.random.default_rng(10).standard_normal(x.size) * (1 + 0.5j)
+    hx = torch.zeros(2 * x.size, dtype=torch.float64, device="cuda")
+    hy = torch.zeros(2 * x.size, dtype=torch.float64, device="cuda")
+    plan.apply(dev(x), hx, dm.FP64)
+    plan.apply(dev(y), hy, dm.FP64)
+    hxn, hyn = hx.cpu().numpy().view(np.complex128), hy.cpu().numpy().view(np.complex128)
+    a, b = np.vdot(x, hyn), np.vdot(y, hxn)
+    assert abs(a - np.conj(b)) <= 1e-11 * np.linalg.norm(x) * np.linalg.norm(hyn)
+
+
+@pytest.mark.parametrize("leigh", [0, 1])
+def test_spinful_env_updates_match_oracle(spinful, leigh):
+    run, model, site, W, lat, L, R, Lx, Rx = spinful
+    check_env(lat, W, L if leigh == 0 else R, Lx if leigh == 0 else Rx, leigh)
+
+
+# --------------------------------------------------------------------------
+# 10x6 cylinder, m = 4000
+# --------------------------------------------------------------------------
+def test_large_cylinder_heff_and_svd():
+    model, tab = workloads.spinless("c5")
+    nx, ny, N, d, U, py, maxm, _ = workloads.SPINLESS["c5"]
+    site = model.L // 2
+    run = model.start(tab)
+    t0 = time.time()
+    assert run.advance(1, site)
+    log("10x6 cylinder: half sweep 1, site %d reached in %.1f s" % (site, time.time() - t0))
+    W = SpinlessBoseSquare(nx, ny, N, d, jx=-1.0, jy=-1.0, U=U / 2, period_y=py).mpo()[0][site]
+    assert np.array_equal(model.mpo_site(site), W)
+    lat, L, R, _, _ = export(run, model, site, W)
+    big = max(max(m.shape) for m in lat.ket.mats)
+    log("10x6 cylinder: psi_dim %d, largest block dimension %d" % (lat.ket.dim(), big))
+    assert lat.ket.dim() > 500000 and big >= 300
+    plan = run.heff(site)
+    x, ref = check_heff(plan, lat, W, L, R, seed=6)
+    # the sector SVD of the largest right sectors of this ket, as the
+    # left-canonical step stacks them (real_tensor_lattice.cc:596-610)
+    sectors = {}
+    for l, r, m in zip(lat.ket.left, lat.ket.right, lat.ket.mats):
+        sectors.setdefault(r, []).append(m)
+    mats = sorted((np.vstack(v) for v in sectors.values()), key=lambda a: -a.shape[0] * a.shape[1])[:3]
+    log("SVD shapes %s" % [a.shape for a in mats])
+    for pre in (0, 1, 2):
+        U, S, Vt = dm.sector_svd([dev(a) for a in mats], precondition=pre)
+        for a, u, s, vt in zip(mats, U, S, Vt):
+            u, s, vt = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+            sref = np.linalg.svd(a, compute_uv=False)
+            assert np.abs(s - sref).max() <= 1e-14 * max(a.shape) * sref[0]
+            rec = (u * s[None, :]) @ vt
+            assert np.abs(rec - a).max() <= 1e-12 * sref[0] * max(a.shape) ** 0.5
+            # the factor each form keeps exactly orthonormal: both (plain
+            # Jacobi), the long side (one QR), the short side (two QRs)
+            tall = a.shape[0] >= a.shape[1]
+            exact = {0: [u, vt.T], 1: [u if tall else vt.T], 2: [vt.T if tall else u]}[pre]
+            for f in exact:
+                assert np.abs(f.T @ f - np.eye(f.shape[1])).max() < 1e-12 * max(a.shape)
