@@ -262,12 +262,16 @@ int dmrgcc_zenv_update(int leigh, int d, int lb, int rb, const double* W, int nl
  * Jacobi on the device.  Outputs (device): U_i (M_i x k_i), S_i (k_i,
  * descending), Vt_i (k_i x N_i), k_i = min(M_i, N_i).  precondition selects
  * the QR preconditioning (matrices with k_i >= 48; about a third of the
- * Jacobi sweeps): 0 = none, plain one-sided Jacobi, both factors orthonormal;
+ * Jacobi sweeps): 0 = none, plain one-sided Jacobi: the short-side factor
+ * is orthonormal to working precision, the long-side one (the rotated
+ * columns divided by sigma) only on singular values well above eps * |A|;
  * 1 = A = Q R and Jacobi on R^T: the long-side factor (U when M_i >= N_i,
  * else Vt) is orthonormal to working precision, the short-side one only for
  * singular values well above eps * |A| (sigma = 0 columns come back zero);
  * 2 = a second QR of R^T and Jacobi on its R^T: the short-side factor is
- * orthonormal to working precision, the long-side one as in 0. */
+ * orthonormal to working precision, the long-side one as in 0.  The sweep's
+ * canonical step uses the form whose exact factor becomes the new site
+ * tensor. */
 int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
                       double* const* U, double* const* S, double* const* Vt, int precondition,
                       void* stream);

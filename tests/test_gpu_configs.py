@@ -202,7 +202,9 @@ def check_trajectory(model, tab, name):
     log("per half sweep |dE|/|E|: %s" % ["%.1e" % v for v in np.abs(e - er) / np.abs(er)])
     assert np.all(np.abs(e - er) <= 1e-10 * np.abs(er))
     t, tr = np.array(res["trunc_err"]), np.array(ref["trunc_err"])
-    assert np.all(np.abs(t - tr) <= 1e-9 * np.maximum(np.abs(tr), 1e-12))
+    # discarded weights are sums of sigma^2 of a unit-norm state: their
+    # absolute accuracy is a few eps, whatever their size
+    assert np.all(np.abs(t - tr) <= 1e-14 + 1e-9 * np.abs(tr))
     assert bond_dims(run, model.L) == ref["bond_dims"]
 
 
@@ -360,9 +362,20 @@ def test_large_cylinder_heff_and_svd():
             assert np.abs(s - sref).max() <= 1e-14 * max(a.shape) * sref[0]
             rec = (u * s[None, :]) @ vt
             assert np.abs(rec - a).max() <= 1e-12 * sref[0] * max(a.shape) ** 0.5
-            # the factor each form keeps exactly orthonormal: both (plain
-            # Jacobi), the long side (one QR), the short side (two QRs)
+            # The factor each form keeps orthonormal to working precision for
+            # every singular value: the short side (plain Jacobi, whose other
+            # factor is G / sigma), the long side (one QR), the short side (two
+            # QRs).  The other factor is orthonormal on the columns whose
+            # sigma is well above eps |A| (these ket sectors are rank
+            # deficient: sigma reaches the round-off floor).
             tall = a.shape[0] >= a.shape[1]
-            exact = {0: [u, vt.T], 1: [u if tall else vt.T], 2: [vt.T if tall else u]}[pre]
-            for f in exact:
-                assert np.abs(f.T @ f - np.eye(f.shape[1])).max() < 1e-12 * max(a.shape)
+            short, long_ = (vt.T, u) if tall else (u, vt.T)
+            exact, other = {0: (short, long_), 1: (long_, short), 2: (short, long_)}[pre]
+            ortho = lambda f: float(np.abs(f.T @ f - np.eye(f.shape[1])).max()) if f.shape[1] else 0.0
+            keep = int(np.sum(s >= 1e-3 * s[0]))
+            e_exact, e_other = ortho(exact), ortho(other[:, :keep])
+            log("SVD form %d %s: rank(1e-10) %d, exact-factor error %.1e, other factor on %d columns "
+                "%.1e (all columns %.1e)" % (pre, a.shape, int(np.sum(s > 1e-10 * s[0])), e_exact, keep,
+                                             e_other, ortho(other)))
+            assert e_exact < 1e-12 * max(a.shape)
+            assert e_other < 1e-10
