@@ -66,6 +66,9 @@ _SIGS = {
     "dmrgcc_block_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(_vp), _c_int_p, _c_int_p, _c_int_p,
                                          ctypes.c_int, _vp]),
+    "dmrgcc_block_gemm_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                 ctypes.POINTER(_vp), _c_int_p, _c_int_p, _c_int_p,
+                                                 ctypes.c_int, _vp]),
     "dmrgcc_heff_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p,
                                           ctypes.c_int, ctypes.c_int, _c_int_p,
                                           ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,

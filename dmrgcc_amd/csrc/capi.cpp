@@ -508,6 +508,22 @@ int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, dou
   });
 }
 
+int dmrgcc_block_gemm_noalloc(int n, const double* const* A, const double* const* B, double* const* C,
+                              const int* M, const int* N, const int* K, int precision, void* stream) {
+  return guarded([&] {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GemmBatch gb;
+    block_gemm_items(gb, n, A, B, C, M, N, K, precision);
+    // the same tiling as dmrgcc_block_gemm, placed in the fixed arena
+    const int64_t psize = gb.layout();
+    size_t cap = 0;
+    void* arena = noalloc_arena(&cap);
+    gb.place(arena, cap, psize, s);
+    gb.launch(s, precision);
+    DMRG_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int dmrgcc_zblock_gemm(int n, const double* const* A, const double* const* B, double* const* C,
                        const int* M, const int* N, const int* K, int precision, void* stream) {
   return guarded([&] {

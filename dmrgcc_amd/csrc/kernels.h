@@ -11,6 +11,12 @@
 
 namespace dmrg {
 
+// Size of the fixed per-GPU device arena of the allocation-free entry point.
+constexpr size_t kNoAllocArenaBytes = size_t(8) << 20;
+// Device address of the current GPU's arena (looked up with
+// hipGetSymbolAddress on the first call per device); *bytes = its size.
+void* noalloc_arena(size_t* bytes);
+
 // Arithmetic form of a contraction.
 enum Precision : int {
   kFP64 = 0,    // FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulate

@@ -21,9 +21,16 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <numeric>
 
 #include "kernels.h"
+
+// Fixed device arena of the allocation-free entry point
+// (dmrgcc_block_gemm_noalloc): GEMM descriptors and split-K partials.  Every
+// GPU the library runs on holds its own copy until the process ends; calls
+// on one GPU share it.
+static __device__ __align__(256) unsigned char scratch[dmrg::kNoAllocArenaBytes];
 
 namespace dmrg {
 
@@ -2912,6 +2919,32 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
   hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np, (fin_n + kFinCols - 1) / kFinCols), dim3(256), 0, stream,
                      scratch.data());
   DMRG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free arena: its device address, looked up once per device
+// with that device current (the array's name in host code is only a
+// host-side stand-in, never a device address).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_arena_mu;
+void* g_arena[kMaxDevices] = {};
+}  // namespace
+
+void* noalloc_arena(size_t* bytes) {
+  int dev = 0;
+  DMRG_HIP(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxDevices, "device index out of range", kErrArgument);
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  if (!g_arena[dev]) {
+    void* p = nullptr;
+    DMRG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the arena", kErrHip);
+    g_arena[dev] = p;
+  }
+  if (bytes) *bytes = kNoAllocArenaBytes;
+  return g_arena[dev];
 }
 
 }  // namespace dmrg

@@ -146,6 +146,19 @@ int dmrgcc_model_mpo_table(const dmrgcc_model* m, int bond, int* count, int* qn)
 int dmrgcc_block_gemm(int n, const double* const* A, const double* const* B, double* const* C,
                       const int* M, const int* N, const int* K, int precision, void* stream);
 
+/* No counterpart in the reference: the allocation-free form of
+ * dmrgcc_block_gemm, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same products, bit for bit; the GEMM
+ * descriptors (items, terms, the tile queue) and any split-K partials go
+ * into a fixed 8 MiB device arena that each GPU holds until the process
+ * ends, so no hipMalloc / hipFree is called.  A call whose descriptors do
+ * not fit (about 500,000 64x64 output tiles) fails with status 3.  Every
+ * call on one GPU shares that GPU's arena: calls that overlap on one GPU
+ * (two host threads, or two streams before the first call returns) race. */
+int dmrgcc_block_gemm_noalloc(int n, const double* const* A, const double* const* B, double* const* C,
+                              const int* M, const int* N, const int* K, int precision, void* stream);
+
 /* Complex (ComplexMatrix::MultiplyToMatrix, complex_matrix.cc:294-318):
  * C_i = A_i * B_i with interleaved complex entries (re, im), so every
  * matrix of M x N entries is 2*M*N doubles. */
