@@ -1,13 +1,17 @@
-// Input-file parsing of the reference's drivers (src/util/input.cc:1-331,
-// real_tensor_rundmrg.cc:3-113, model/*_model.cc), restated on a
-// std::istringstream over the file's text so that reads, put-backs and the
-// stream extractors behave as on the reference's ifstream.
+// Input files of the reference's drivers: a one-pass index of the text
+// (line heads, bracket depths) and lookups as walks over that index.  The
+// grammar and the reference behaviours kept are listed with each function;
+// values go through the C++ stream extractors, so "2.5" read as an int is 2
+// and an empty value makes the extractor read on into the next line, as in
+// the reference (src/util/input.cc, real_tensor_rundmrg.cc:3-113).
 #include "input.h"
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <functional>
 
 #include "common.h"
 
@@ -15,348 +19,410 @@ namespace dmrg {
 
 bool InputGroup::echo = true;
 
-void InputFile::open() {
-  // InputFile::open (input.cc:3-10): every lookup starts at the top
-  if (!loaded_) {
-    std::ifstream f(name_, std::ios::binary);
-    require(f.good(), "can't open input file", kErrArgument);
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    text_ = ss.str();
-    loaded_ = true;
-  }
-  is_.clear();
-  is_.str(text_);
-  is_.seekg(0);
-}
-
 namespace {
-
-// eatwhite / eatblank (input.cc:24-50)
-void eatwhite(std::istream& is) {
-  char c;
-  while (is.get(c))
-    if (!std::isspace((unsigned char)c)) {
-      is.putback(c);
-      break;
-    }
-}
-void eatblank(std::istream& is) {
-  char c;
-  while (is.get(c))
-    if (!std::isblank((unsigned char)c)) {
-      is.putback(c);
-      break;
-    }
-}
-
-// matchbracket (input.cc:52-65): skip to the bracket closing an open one
-void matchbracket(std::istream& is) {
-  int nest = 1;
-  char c;
-  while (is >> c) {
-    if (c == '{') nest++;
-    else if (c == '}') nest--;
-    if (nest == 0) break;
-  }
-  require(nest == 0, "unterminated bracket", kErrArgument);
-}
-
-// gettoken (input.cc:67-93): an identifier, or one non-identifier character
-int gettoken(std::istream& is, std::string& s) {
-  eatwhite(is);
-  std::string t;
-  for (int i = 0; i < 511; ++i) {
-    char ch;
-    if (!is.get(ch)) return 0;
-    if (std::isalpha((unsigned char)ch) || ch == '_' || (i != 0 && std::isdigit((unsigned char)ch))) {
-      t.push_back(ch);
-    } else {
-      if (i == 0) {
-        s = std::string(1, ch);
-        return 1;
-      }
-      is.putback(ch);
-      break;
-    }
-  }
-  s = t;
-  return 1;
-}
-
-// the rest of a line that did not match, with any bracketed group on it
-// (input.cc:138-151, :179-191); false at the end of the file
-bool skip_rest(std::istream& is) {
-  char c;
-  while (is.get(c)) {
-    if (c == '{') matchbracket(is);
-    if (c == '\n') {
-      eatwhite(is);
-      return true;
-    }
-  }
-  return false;
-}
-
+constexpr size_t kNone = std::string::npos;
+bool is_ws(char c) { return std::isspace((unsigned char)c) != 0; }
+bool ident_start(char c) { return std::isalpha((unsigned char)c) || c == '_'; }
+bool ident_char(char c) { return ident_start(c) || std::isdigit((unsigned char)c); }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The index
+// ---------------------------------------------------------------------------
+void InputFile::load() {
+  std::ifstream f(name_, std::ios::binary);
+  require(f.good(), "can't open input file", kErrArgument);
+  std::ostringstream ss;
+  ss << f.rdbuf();
+  text_ = ss.str();
+  const size_t n = text_.size();
+  // depth_[i]: '{' minus '}' strictly before character i
+  depth_.assign(n + 1, 0);
+  for (size_t i = 0; i < n; ++i)
+    depth_[i + 1] = depth_[i] + (text_[i] == '{') - (text_[i] == '}');
+  // a line head is the first non-blank character of the file and the first
+  // non-whitespace character after each newline (blank lines add none)
+  heads_.clear();
+  bool line_start = true;
+  for (size_t i = 0; i < n; ++i) {
+    if (text_[i] == '\n') { line_start = true; continue; }
+    if (is_ws(text_[i])) continue;
+    if (line_start) heads_.push_back(i);
+    line_start = false;
+  }
+  is_.str(text_);
+  loaded_ = true;
+}
+
+const std::string& InputFile::text() {
+  if (!loaded_) load();
+  return text_;
+}
+
+std::istream& InputFile::at(size_t pos) {
+  text();
+  is_.clear();
+  is_.seekg((std::streamoff)std::min(pos, text_.size()));
+  return is_;
+}
+
+size_t InputFile::next_char(size_t pos) const {
+  while (pos < text_.size() && is_ws(text_[pos])) ++pos;
+  return pos < text_.size() ? pos : kNone;
+}
+
+size_t InputFile::token_start(size_t from) const { return next_char(from); }
+
+std::string InputFile::token(size_t pos) const {
+  if (pos >= text_.size()) return std::string();
+  if (!ident_start(text_[pos])) return std::string(1, text_[pos]);
+  size_t e = pos;
+  while (e < text_.size() && ident_char(text_[e])) ++e;
+  return text_.substr(pos, e - pos);
+}
+
+std::vector<size_t> InputFile::visible_heads(size_t from) const {
+  // A scan reads the token at a line head; when it is not the one sought
+  // it skips to the end of the line, jumping over every bracketed region
+  // that opens on the way (a region's own lines are never heads of the
+  // scan).  So a later line head is met exactly when its bracket depth is
+  // the lowest depth seen since `from`: a higher one lies inside a region
+  // opened after the scan started.  Regions the scan starts inside do not
+  // stop it: past their closing bracket it goes on in the enclosing text.
+  std::vector<size_t> out;
+  if (from == kNone || from >= text_.size()) return out;
+  out.push_back(from);
+  int floor = depth_[from];
+  size_t prev = from;
+  auto it = std::upper_bound(heads_.begin(), heads_.end(), from);
+  for (; it != heads_.end(); ++it) {
+    const size_t h = *it;
+    for (size_t i = prev + 1; i <= h; ++i) floor = std::min(floor, depth_[i]);
+    if (depth_[h] == floor) out.push_back(h);
+    prev = h;
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Group and key lookup
+// ---------------------------------------------------------------------------
 InputGroup::InputGroup(const std::string& file_name, const std::string& group_name)
     : file_(std::make_shared<InputFile>(file_name)), name_(group_name) {}
 
 InputGroup::InputGroup(InputGroup& parent, const std::string& name)
     : file_(parent.file_), parent_(&parent), name_(name) {}
 
-int InputGroup::GotoGroup() {
-  // input.cc:112-154
+size_t InputGroup::body() {
+  // Grammar: a group is its name as a line head, then (after any
+  // whitespace, newlines included) '{'; the body starts at the first token
+  // after the bracket.  The top-level group is sought from the start of
+  // the file, a nested one from its parent's body start: the first line
+  // head equal to the name wins.  A name head not followed by '{' is an
+  // error ("bracket does not follow group name").  The search is not bounded
+  // by the parent's closing bracket (the reference's scan is line-based and
+  // runs on).  End of file: not found (the reference's scan does not end
+  // at an end of file that has no final newline; here it does).
+  const std::string& t = file_->text();
+  size_t from = 0;
   if (parent_) {
-    if (!parent_->GotoGroup()) return 0;
-  } else {
-    file_->open();
+    from = parent_->body();
+    if (from == kNone) return kNone;
   }
-  std::istream& is = file();
-  eatwhite(is);
-  while (true) {
-    std::string s;
-    if (!gettoken(is, s)) return 0;
-    if (s == name_) {
-      eatwhite(is);
-      char c = 0;
-      is.get(c);
-      require(c == '{', "bracket does not follow group name", kErrArgument);
-      eatwhite(is);
-      return 1;
-    }
-    if (s[0] == '{') matchbracket(is);
-    if (!skip_rest(is)) return 0;
+  for (size_t h : file_->visible_heads(file_->token_start(from))) {
+    const std::string tok = file_->token(h);
+    if (tok != name_) continue;
+    const size_t b = file_->next_char(h + tok.size());
+    require(b != kNone && t[b] == '{', "bracket does not follow group name", kErrArgument);
+    const size_t first = file_->next_char(b + 1);
+    return first == kNone ? t.size() : first;
   }
+  return kNone;
+}
+
+size_t InputGroup::value(const std::string& key) {
+  // Grammar: an entry is `key = value` with the key as a line head of the
+  // group body.  The first head equal to the key decides: when the next
+  // non-whitespace character is not '=', the key counts as absent.  A head
+  // '}' (the group's closing bracket at the start of a line) ends the
+  // search; nested groups are jumped over, so their keys are not seen.  The
+  // value starts after '=' and the blanks (spaces, tabs) that follow it.
+  const size_t b = body();
+  if (b == kNone) return kNone;
+  const std::string& t = file_->text();
+  for (size_t h : file_->visible_heads(b)) {
+    const std::string tok = file_->token(h);
+    if (tok == "}") return kNone;
+    if (tok != key) continue;
+    const size_t eq = file_->next_char(h + tok.size());
+    if (eq == kNone || t[eq] != '=') return kNone;
+    size_t v = eq + 1;
+    while (v < t.size() && (t[v] == ' ' || t[v] == '\t')) ++v;
+    return v;
+  }
+  return kNone;
+}
+
+int InputGroup::GotoGroup() {
+  const size_t b = body();
+  if (b == kNone) return 0;
+  file_->at(b);
+  return 1;
 }
 
 int InputGroup::GotoToken(const std::string& s) {
-  // input.cc:157-193
-  if (!GotoGroup()) return 0;
-  std::istream& is = file();
-  while (true) {
-    std::string t;
-    if (!gettoken(is, t)) return 0;
-    if (t[0] == '}') return 0;
-    if (t == s) {
-      eatwhite(is);
-      char c;
-      if (!is.get(c)) return 0;
-      if (c != '=') return 0;
-      eatblank(is);
-      return 1;
-    }
-    if (t[0] == '{') matchbracket(is);
-    if (!skip_rest(is)) return 0;
-  }
+  const size_t v = value(s);
+  if (v == kNone) return 0;
+  file_->at(v);
+  return 1;
 }
 
 void InputGroup::SkipLine() {
-  // input.cc:195-201
-  char c = '\0';
-  while (c != '\n')
-    if (!file().get(c)) break;
-  eatwhite(file());
+  std::istream& is = file();
+  std::string rest;
+  std::getline(is, rest);
+  is >> std::ws;
 }
 
+// ---------------------------------------------------------------------------
+// Typed values and the reference's echo lines
+// ---------------------------------------------------------------------------
+namespace {
+// Extract a T at the value of `key`: 1 when the key is there and the
+// extractor succeeds.
+template <typename T>
+int extract(InputGroup& g, const std::string& key, T& out) {
+  if (!g.GotoToken(key)) return 0;
+  T v{};
+  if (!(g.file() >> v)) return 0;
+  out = v;
+  return 1;
+}
+
+void say(const char* tag, const std::string& group, const std::string& key, const std::string& val) {
+  if (InputGroup::echo) std::printf("%s %s.%s = %s\n", tag, group.c_str(), key.c_str(), val.c_str());
+}
+std::string fmt_int(long v) { return std::to_string(v); }
+std::string fmt_dbl(double v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%.2f", v);
+  return b;
+}
+}  // namespace
+
 int InputGroup::GetInt(const std::string& s, int& res, bool hasdf, int df) {
-  if (!GotoToken(s) || !(file() >> res)) {
-    if (hasdf && echo) std::printf("Def %s.%s = %d\n", name_.c_str(), s.c_str(), df);
+  if (!extract(*this, s, res)) {
+    if (hasdf) say("Def", name_, s, fmt_int(df));
     return 0;
   }
-  if (echo) std::printf("Got %s.%s = %d\n", name_.c_str(), s.c_str(), res);
+  say("Got", name_, s, fmt_int(res));
   return 1;
 }
 
 int InputGroup::GetDouble(const std::string& s, double& res, bool hasdf, double df) {
-  if (!GotoToken(s) || !(file() >> res)) {
-    if (hasdf && echo) std::printf("Def %s.%s = %.2f\n", name_.c_str(), s.c_str(), df);
+  if (!extract(*this, s, res)) {
+    if (hasdf) say("Def", name_, s, fmt_dbl(df));
     return 0;
   }
-  if (echo) std::printf("Got %s.%s = %.2f\n", name_.c_str(), s.c_str(), res);
+  say("Got", name_, s, fmt_dbl(res));
   return 1;
 }
 
 int InputGroup::GetString(const std::string& s, std::string& res, bool hasdf, const std::string& df) {
-  // input.cc:223-244: a key with nothing after '=' counts as found, empty
-  if (GotoToken(s)) {
-    char c;
-    if (file().get(c)) {
-      if (std::isspace((unsigned char)c)) {
-        if (echo) {
-          if (hasdf) std::printf("Def %s.%s = %s\n", name_.c_str(), s.c_str(), df.c_str());
-          else std::printf("No  %s.%s = %s\n", name_.c_str(), s.c_str(), "None");
-        }
-        return 1;
-      }
-      file().putback(c);
-    }
+  // a key whose value is empty (whitespace right after the blanks that
+  // follow '=') is found, with res left as it is
+  const size_t v = value(s);
+  const std::string& t = file_->text();
+  if (v != kNone && v < t.size() && is_ws(t[v])) {
+    if (hasdf) say("Def", name_, s, df);
+    else say("No ", name_, s, "None");
+    return 1;
   }
-  if (!GotoToken(s) || !(file() >> res)) {
-    if (hasdf && echo) std::printf("Def %s.%s = %s\n", name_.c_str(), s.c_str(), df.c_str());
+  if (!extract(*this, s, res)) {
+    if (hasdf) say("Def", name_, s, df);
     return 0;
   }
-  if (echo) std::printf("Got %s.%s = %s\n", name_.c_str(), s.c_str(), res.c_str());
+  say("Got", name_, s, res);
   return 1;
 }
 
 int InputGroup::GetBool(const std::string& s, bool& res, bool hasdf, bool df) {
-  // input.cc:246-262: "true"/"1" and "false"/"0"; other words leave res
-  std::string t;
-  if (!GotoToken(s) || !(file() >> t)) {
-    if (hasdf && echo) std::printf("Def %s.%s = %d\n", name_.c_str(), s.c_str(), (int)df);
+  // "true" / "1" and "false" / "0"; any other word is found and leaves res
+  std::string w;
+  if (!extract(*this, s, w)) {
+    if (hasdf) say("Def", name_, s, fmt_int(df));
     return 0;
   }
-  if (t == "true" || t == "1") res = true;
-  else if (t == "false" || t == "0") res = false;
-  if (echo) std::printf("Got %s.%s = %d\n", name_.c_str(), s.c_str(), (int)res);
+  if (w == "true" || w == "1") res = true;
+  if (w == "false" || w == "0") res = false;
+  say("Got", name_, s, fmt_int(res));
   return 1;
 }
 
 int InputGroup::getInt(const std::string& s, int def) {
-  int res = 0;
-  return GetInt(s, res, true, def) ? res : def;
+  int r = 0;
+  return GetInt(s, r, true, def) ? r : def;
 }
 double InputGroup::getDouble(const std::string& s, double def) {
-  double res = 0.0;
-  return GetDouble(s, res, true, def) ? res : def;
+  double r = 0.0;
+  return GetDouble(s, r, true, def) ? r : def;
 }
 std::string InputGroup::getString(const std::string& s, const std::string& def) {
-  std::string res;
-  return GetString(s, res, true, def) ? res : def;
+  std::string r;
+  return GetString(s, r, true, def) ? r : def;
 }
 bool InputGroup::getBool(const std::string& s, bool def) {
-  bool res = false;
-  return GetBool(s, res, true, def) ? res : def;
-}
-int InputGroup::getInt(const std::string& s) {
-  int res = 0;
-  require(GetInt(s, res), "there is no item: " + s, kErrArgument);
-  return res;
-}
-double InputGroup::getDouble(const std::string& s) {
-  double res = 0.0;
-  require(GetDouble(s, res), "there is no item: " + s, kErrArgument);
-  return res;
-}
-std::string InputGroup::getString(const std::string& s) {
-  std::string res;
-  require(GetString(s, res), "there is no item: " + s, kErrArgument);
-  return res;
-}
-bool InputGroup::getBool(const std::string& s) {
-  bool res = false;
-  require(GetBool(s, res), "there is no item: " + s, kErrArgument);
-  return res;
+  bool r = false;
+  return GetBool(s, r, true, def) ? r : def;
 }
 
-RunConfig read_run_config(InputGroup& input) {
-  // RealTensorRundmrg::RealTensorRundmrg (real_tensor_rundmrg.cc:3-42), same
-  // keys in the complex twin (complex_tensor_rundmrg.cc:3-42)
-  RunConfig c;
-  c.disk_cache = input.getBool("disk_cache");
-  c.cache_name = input.getString("cache_name");
-  c.cache_record = input.getBool("cache_record");
-  c.record_name = input.getString("record_name");
-  c.cache_resume = input.getBool("cache_resume");
-  c.resume_name = input.getString("resume_name");
-  c.record_process = input.getBool("record_process");
-  c.process_name = input.getString("process_name");
-  c.first_block = input.getInt("first_block");
-  c.first_dim = input.getInt("first_dim");
-  c.num_sweep = input.getInt("num_sweep");
-  const int n = 2 * c.num_sweep;
-  require(n > 0, "num_sweep must be positive", kErrArgument);
-  // Sweep (real_tensor_rundmrg.cc:70-113): rows after the header line; a
-  // row with maxm 0, or the first one the extractor cannot read (which then
-  // stores 0), ends the table, and the rest repeats the last row with maxm
-  // taken from its maxb column, as the reference does
-  InputGroup table(input, "sweeps");
-  require(table.GotoGroup(), "Couldn't find table " + table.name(), kErrArgument);
-  table.SkipLine();
-  std::vector<int> maxm(n, 0), maxb(n, 0), niter(n, 0);
-  std::vector<double> canpre(n, 0.0), noise(n, 0.0);
-  int nlast = n;
+namespace {
+template <typename T, typename F>
+T required(const std::string& key, F&& get) {
+  T r{};
+  require(get(r) != 0, "there is no item: " + key, kErrArgument);
+  return r;
+}
+}  // namespace
+
+int InputGroup::getInt(const std::string& s) {
+  return required<int>(s, [&](int& r) { return GetInt(s, r); });
+}
+double InputGroup::getDouble(const std::string& s) {
+  return required<double>(s, [&](double& r) { return GetDouble(s, r); });
+}
+std::string InputGroup::getString(const std::string& s) {
+  return required<std::string>(s, [&](std::string& r) { return GetString(s, r); });
+}
+bool InputGroup::getBool(const std::string& s) {
+  return required<bool>(s, [&](bool& r) { return GetBool(s, r); });
+}
+
+// ---------------------------------------------------------------------------
+// The run section and the sweep table
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SweepRow {
+  int maxm = 0, maxb = 0, niter = 0;
+  double canpre = 0.0, noise = 0.0;
+};
+
+// The table rows after its header line (RealTensorRundmrg::Sweep,
+// real_tensor_rundmrg.cc:70-113), five stream-extracted fields per row.  A
+// row whose maxm is 0, or is where the extractor fails (it stores 0 there,
+// and every later field of a failed stream stays 0 here, where the
+// reference leaves it uninitialised), ends the table.  Rows past the last
+// one read repeat it, with maxm taken from its maxb column (the reference's
+// fill-in).  An empty table is an error here (the reference reads index -1).
+std::vector<SweepRow> read_sweeps(InputGroup& table, int nhalf) {
+  require(table.GotoGroup() != 0, "Couldn't find table " + table.name(), kErrArgument);
+  table.SkipLine();  // the column names
   std::istream& is = table.file();
-  for (int i = 0; i < n; ++i) {
-    is >> maxm[i] >> maxb[i] >> niter[i] >> canpre[i] >> noise[i];
-    if (maxm[i] == 0) {
-      nlast = i - 1;
-      break;
-    }
+  std::vector<SweepRow> rows;
+  while ((int)rows.size() < nhalf) {
+    SweepRow r;
+    is >> r.maxm >> r.maxb >> r.niter >> r.canpre >> r.noise;
+    if (r.maxm == 0) break;
+    rows.push_back(r);
   }
-  require(nlast >= 0, "empty sweep table", kErrArgument);
-  for (int i = nlast + 1; i < n; ++i) {
-    maxm[i] = maxb[nlast];
-    maxb[i] = maxb[nlast];
-    niter[i] = niter[nlast];
-    canpre[i] = canpre[nlast];
-    noise[i] = noise[nlast];
+  require(!rows.empty(), "empty sweep table", kErrArgument);
+  const SweepRow last = rows.back();
+  while ((int)rows.size() < nhalf) {
+    SweepRow f = last;
+    f.maxm = last.maxb;
+    rows.push_back(f);
+  }
+  return rows;
+}
+
+}  // namespace
+
+RunConfig read_run_config(InputGroup& input) {
+  // The keys of the RealTensorRundmrg / ComplexTensorRundmrg constructors
+  // (real_tensor_rundmrg.cc:3-42, complex_tensor_rundmrg.cc:3-42), in the
+  // order they read (and echo) them.
+  RunConfig c;
+  struct Key { const char* name; std::function<void()> read; };
+  const Key keys[] = {
+      {"disk_cache", [&] { c.disk_cache = input.getBool("disk_cache"); }},
+      {"cache_name", [&] { c.cache_name = input.getString("cache_name"); }},
+      {"cache_record", [&] { c.cache_record = input.getBool("cache_record"); }},
+      {"record_name", [&] { c.record_name = input.getString("record_name"); }},
+      {"cache_resume", [&] { c.cache_resume = input.getBool("cache_resume"); }},
+      {"resume_name", [&] { c.resume_name = input.getString("resume_name"); }},
+      {"record_process", [&] { c.record_process = input.getBool("record_process"); }},
+      {"process_name", [&] { c.process_name = input.getString("process_name"); }},
+      {"first_block", [&] { c.first_block = input.getInt("first_block"); }},
+      {"first_dim", [&] { c.first_dim = input.getInt("first_dim"); }},
+      {"num_sweep", [&] { c.num_sweep = input.getInt("num_sweep"); }},
+  };
+  for (const Key& k : keys) k.read();
+  const int nhalf = 2 * c.num_sweep;
+  require(nhalf > 0, "num_sweep must be positive", kErrArgument);
+  InputGroup table(input, "sweeps");
+  for (const SweepRow& r : read_sweeps(table, nhalf)) {
+    c.maxm.push_back(r.maxm);
+    c.maxb.push_back(r.maxb);
+    c.niter.push_back(r.niter);
+    c.canpre.push_back(r.canpre);
+    c.noise.push_back(r.noise);
   }
   if (InputGroup::echo) {
     std::printf("Sweeps:\n");
-    for (int i = 0; i < n; ++i)
-      std::printf("%d Maxm=%d, Maxb=%d, Niter=%d, Canon=%g, Noise=%g\n", i + 1, maxm[i], maxb[i], niter[i],
-                  canpre[i], noise[i]);
+    for (int i = 0; i < nhalf; ++i)
+      std::printf("%d Maxm=%d, Maxb=%d, Niter=%d, Canon=%g, Noise=%g\n", i + 1, c.maxm[i], c.maxb[i],
+                  c.niter[i], c.canpre[i], c.noise[i]);
   }
-  c.maxm = maxm;
-  c.maxb = maxb;
-  c.niter = niter;
-  c.canpre = canpre;
-  c.noise = noise;
   return c;
 }
 
+// ---------------------------------------------------------------------------
+// The model drivers
+// ---------------------------------------------------------------------------
 void read_model(InputGroup& input, const std::string& driver, std::unique_ptr<RealTensorSpace>& space,
                 std::unique_ptr<RealTensorHamiltonian>& ham) {
-  if (driver == "real_spinless_bose_square" || driver == "complex_spinless_bose_square") {
-    // model/real_spinless_bose_square_model.cc:11-22 (the complex driver
-    // reads the same keys); flux is read and passed on, and the Hamiltonian
-    // does not use it
-    const int nx = input.getInt("num_site_x");
-    const int ny = input.getInt("num_site_y");
+  const bool spinless = driver == "real_spinless_bose_square" || driver == "complex_spinless_bose_square";
+  const bool spinful = driver == "complex_spinful_bose_square";
+  require(spinless || spinful, "unknown model driver: " + driver, kErrArgument);
+  // keys in the order each driver's main() reads them
+  const int nx = input.getInt("num_site_x");
+  const int ny = input.getInt("num_site_y");
+  if (spinless) {
+    // model/{real,complex}_spinless_bose_square_model.cc:11-22; flux (in
+    // units of pi) is passed on and the Hamiltonian does not use it
     const int nb = input.getInt("num_boson");
     const int d = input.getInt("physics_dim", 2);
-    const double flux = input.getDouble("flux") * M_PI;
-    const double jx = input.getDouble("Jx");
-    const double jy = input.getDouble("Jy");
-    const double U = input.getDouble("U");
-    const double mu = input.getDouble("mu");
-    const bool px = input.getBool("period_x");
-    const bool py = input.getBool("period_y");
+    const double flux = M_PI * input.getDouble("flux");
+    const double jx = input.getDouble("Jx"), jy = input.getDouble("Jy");
+    const double U = input.getDouble("U"), mu = input.getDouble("mu");
+    const bool px = input.getBool("period_x"), py = input.getBool("period_y");
     require(nx > 0 && ny > 0 && d > 1 && nb >= 0, "bad model parameters in input file", kErrArgument);
     space = std::make_unique<RealSpinlessBoseSquareSpace>(nx, ny, nb, d);
-    if (driver == "real_spinless_bose_square")
+    if (driver[0] == 'r')
       ham = std::make_unique<RealSpinlessBoseSquareHamiltonian>(nx, ny, d, flux, jx, jy, U, mu, px, py);
     else
       ham = std::make_unique<ComplexSpinlessBoseSquareHamiltonian>(nx, ny, d, flux, jx, jy, U, mu, px, py);
-  } else if (driver == "complex_spinful_bose_square") {
-    // model/complex_spinful_bose_square_model.cc:11-36
-    const int nx = input.getInt("num_site_x");
-    const int ny = input.getInt("num_site_y");
-    const int d = input.getInt("physics_dim", 2);
-    const bool px = input.getBool("period_x");
-    const bool py = input.getBool("period_y");
-    const int nup = input.getInt("Nup");
-    const int ndn = input.getInt("Ndown");
-    double flux[2], jx[2], jy[2], mu[2], U[3];
-    for (int t = 0; t < 2; ++t) {
-      flux[t] = input.getDouble("flux");
-      jx[t] = input.getDouble("Jx");
-      jy[t] = input.getDouble("Jy");
-      mu[t] = input.getDouble("mu");
-    }
-    U[0] = input.getDouble("U_uu");
-    U[1] = input.getDouble("U_dd");
-    U[2] = input.getDouble("U_ud");
-    require(nx > 0 && ny > 0 && d > 1 && nup >= 0 && ndn >= 0, "bad model parameters in input file",
-            kErrArgument);
-    space = std::make_unique<ComplexSpinfulBoseSquareSpace>(nx, ny, nup, ndn, d);
-    ham = std::make_unique<ComplexSpinfulBoseSquareHamiltonian>(nx, ny, d, flux, jx, jy, U, mu, px, py);
-  } else {
-    throw Error(kErrArgument, "unknown model driver: " + driver);
+    return;
   }
+  // model/complex_spinful_bose_square_model.cc:11-36: per-species values
+  // are read twice from the same keys (both species alike)
+  const int d = input.getInt("physics_dim", 2);
+  const bool px = input.getBool("period_x"), py = input.getBool("period_y");
+  const int nup = input.getInt("Nup"), ndn = input.getInt("Ndown");
+  double flux[2], jx[2], jy[2], mu[2], U[3];
+  for (int sp = 0; sp < 2; ++sp) {
+    flux[sp] = input.getDouble("flux");
+    jx[sp] = input.getDouble("Jx");
+    jy[sp] = input.getDouble("Jy");
+    mu[sp] = input.getDouble("mu");
+  }
+  const char* inter[3] = {"U_uu", "U_dd", "U_ud"};
+  for (int k = 0; k < 3; ++k) U[k] = input.getDouble(inter[k]);
+  require(nx > 0 && ny > 0 && d > 1 && nup >= 0 && ndn >= 0, "bad model parameters in input file",
+          kErrArgument);
+  space = std::make_unique<ComplexSpinfulBoseSquareSpace>(nx, ny, nup, ndn, d);
+  ham = std::make_unique<ComplexSpinfulBoseSquareHamiltonian>(nx, ny, d, flux, jx, jy, U, mu, px, py);
 }
 
 }  // namespace dmrg

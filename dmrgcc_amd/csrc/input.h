@@ -1,13 +1,15 @@
-// Input files of the reference's model drivers: InputFile / InputGroup
-// (src/util/input.h:7-89, src/util/input.cc:1-331), the sweep table of
-// RealTensorRundmrg (real_tensor_rundmrg.cc:3-113) and the model drivers'
-// keys (model/*_model.cc).
+// Input files of the reference's model drivers.  The public classes keep the
+// reference's interface (InputFile / InputGroup with GotoGroup, GotoToken,
+// SkipLine, Get* / get*, src/util/input.h:7-89), its key names and its
+// "Got / Def / No" echo lines; the sweep table and the drivers' keys are
+// those of RealTensorRundmrg (real_tensor_rundmrg.cc:3-113) and
+// model/*_model.cc.
 //
-// Format: named groups `name { ... }` holding `key = value` lines; a nested
-// group (the sweep table) is a header line followed by whitespace-separated
-// rows.  Every lookup rescans from the top of the file, skipping lines and
-// bracketed sub-groups that do not match, as the reference does; values are
-// read with the C++ stream extractors, so their parsing is the same.
+// How a file is read here: the text is loaded once and indexed (the
+// position of every line head, the bracket depth before every character).
+// A lookup is then a walk over the line heads a scan from the group body
+// can see, and the value is extracted with the C++ stream extractors from
+// the position the walk returns.
 #pragma once
 
 #include <memory>
@@ -22,13 +24,30 @@ namespace dmrg {
 class InputFile {
  public:
   explicit InputFile(std::string name) : name_(std::move(name)) {}
-  void open();  // (re)start at the top of the file
-  std::istream& file() { return is_; }
   const std::string& filename() const { return name_; }
+  // the text and its index, loaded on first use ("can't open input file")
+  const std::string& text();
+  // the stream the Get* extractors read, positioned at `pos`
+  std::istream& at(size_t pos);
+  std::istream& file() { return is_; }
+  // First token position at or after `from` (whitespace skipped), or npos.
+  size_t token_start(size_t from) const;
+  // The token at `pos`: an identifier [A-Za-z_][A-Za-z0-9_]*, or one other
+  // character.
+  std::string token(size_t pos) const;
+  // The line heads a line-by-line scan starting at token position `from`
+  // meets, in order: `from` itself, then the first token of every later
+  // line that is not inside a bracket opened after `from`.
+  std::vector<size_t> visible_heads(size_t from) const;
+  // first non-whitespace position at or after `pos` (npos at the end)
+  size_t next_char(size_t pos) const;
 
  private:
+  void load();
   std::string name_, text_;
   bool loaded_ = false;
+  std::vector<int> depth_;     // bracket depth before each character
+  std::vector<size_t> heads_;  // first token of each line, in order
   std::istringstream is_;
 };
 
@@ -39,8 +58,12 @@ class InputGroup {
   const std::string& name() const { return name_; }
   std::istream& file() { return file_->file(); }
 
+  // Position the stream at the first token of the group body (1), or 0
+  // when the group is not found.
   int GotoGroup();
+  // Position the stream at the value of `key = value` in the group (1), or 0.
   int GotoToken(const std::string& s);
+  // Skip the rest of the current line and the whitespace after it.
   void SkipLine();
   int GetInt(const std::string& s, int& res, bool hasdf = false, int df = 0);
   int GetDouble(const std::string& s, double& res, bool hasdf = false, double df = 0.0);
@@ -61,6 +84,10 @@ class InputGroup {
   static bool echo;
 
  private:
+  // body start of this group (npos when absent)
+  size_t body();
+  // value position of `key` in this group (npos when absent)
+  size_t value(const std::string& key);
   std::shared_ptr<InputFile> file_;
   InputGroup* parent_ = nullptr;
   std::string name_;

@@ -97,3 +97,80 @@ def test_echo_lines(capfd, tmp_path):
     assert "Got input.period_y = 1" in out
     assert "No  input.resume_name = None" in out
     assert "Got input.num_sweep = 3" in out
+
+
+RUN = """    disk_cache = false
+    cache_name = c
+    cache_record = false
+    record_name = r
+    cache_resume = false
+    resume_name = q
+    record_process = false
+    process_name = p
+    first_block = 20
+    first_dim = 4.9
+    num_sweep = 2
+"""
+
+
+def test_grammar_of_keys(tmp_path):
+    # keys are line heads of the group body; the first head equal to the key
+    # decides; nested groups and other top-level groups are not looked into
+    p = tmp_path / "input_grammar"
+    p.write_text("other { a = wrong\n}\n"
+                 "input { a = hello\n"
+                 "    b =   \n"
+                 "    c hello\n"
+                 "    c = later\n"
+                 "    inner\n    {\n        d = 1\n    }\n"
+                 "    e =\n    next\n"
+                 "    f=g h\n"
+                 "}\n"
+                 "z = 9\n")
+    s = lambda k: dm.input_string(str(p), k)
+    assert s("a") == "hello"         # on the line of the group's bracket
+    assert s("b") == ""              # empty value: found, empty
+    assert s("c") is None            # no '=' after the first "c": absent
+    assert s("d") is None            # inside a nested group
+    assert s("e") == ""              # nothing after '=' on its line
+    assert s("f") == "g"             # one whitespace-delimited word
+    assert s("z") is None            # past the group's closing bracket
+
+
+def test_file_without_final_newline_and_missing_keys(tmp_path):
+    p = tmp_path / "input_nonl"
+    p.write_text("input\n{\n    x = 1\n    y = two")
+    assert dm.input_string(str(p), "y") == "two"
+    assert dm.input_string(str(p), "nope") is None
+
+
+def test_int_values_and_partial_table_rows(tmp_path):
+    # ints go through the stream extractor ("4.9" reads 4); a row the
+    # extractor fails in keeps zeros for the fields it did not reach, the
+    # next row ends the table, and the fill-in repeats the last row
+    p = tmp_path / "input_partial"
+    p.write_text("input\n{\n" + RUN + "    sweeps\n    {\n        maxm maxb niter canpre noise\n"
+                 "        60 400 10 1E-8 2\n        70 300\n    }\n}\n")
+    c = dm.input_run_config(str(p))
+    assert c["first_dim"] == 4
+    assert c["table"] == [(60, 400, 10, 1e-8, 2.0), (70, 300, 0, 0.0, 0.0), (300, 300, 0, 0.0, 0.0),
+                          (300, 300, 0, 0.0, 0.0)]
+
+
+def test_nested_group_search_runs_past_the_parent(tmp_path):
+    # a nested group is sought line by line from the parent's body start,
+    # and the scan does not stop at the parent's closing bracket: a table
+    # after the group is found
+    p = tmp_path / "input_after"
+    p.write_text("input\n{\n" + RUN + "}\nsweeps\n{\n    maxm maxb niter canpre noise\n"
+                 "    50 100 5 1E-7 1\n}\n")
+    c = dm.input_run_config(str(p))
+    assert c["table"] == [(50, 100, 5, 1e-7, 1.0)] + [(100, 100, 5, 1e-7, 1.0)] * 3
+    q = tmp_path / "input_badgroup"
+    q.write_text("input = 3\n")
+    with pytest.raises(dm.DmrgError, match="bracket does not follow group name"):
+        dm.input_string(str(q), "x")
+    r = tmp_path / "input_empty_table"
+    r.write_text("input\n{\n" + RUN + "    sweeps\n    {\n        maxm\n        0 0 0 0 0\n    }\n}\n")
+    with pytest.raises(dm.DmrgError, match="empty sweep table"):
+        dm.input_run_config(str(r))
