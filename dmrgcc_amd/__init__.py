@@ -95,6 +95,19 @@ _SIGS = {
                                           ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                           ctypes.POINTER(_vp), _vp]),
     "dmrgcc_heff_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "dmrgcc_heff_apply_async": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               _vp]),
+    "dmrgcc_alloc": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "dmrgcc_input_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dbl_p, _c_int_p]),
+    "dmrgcc_free": (ctypes.c_int, [_vp]),
+    "dmrgcc_upload": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp]),
+    "dmrgcc_download": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp]),
+    "dmrgcc_lattice_canonical": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                _c_int_p, _c_int_p, ctypes.c_int, _c_int_p, _c_int_p,
+                                                _c_int_p, _c_int_p, _vp, ctypes.c_int, ctypes.c_double, _vp,
+                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_int_p, _c_int_p,
+                                                _c_dbl_p]),
     "dmrgcc_heff_info": (ctypes.c_int, [_vp, _c_dbl_p]),
     "dmrgcc_heff_destroy": (ctypes.c_int, [_vp]),
     "dmrgcc_lanczos": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p,
@@ -431,6 +444,19 @@ def input_run_config(path, echo=False):
     return out
 
 
+def input_value(path, key, kind="int", group="input", default=None, echo=False):
+    """An int / double / bool value as the reference's InputGroup reads it
+    (None when absent and no default is given)."""
+    k = {"int": 0, "double": 1, "bool": 2}[kind]
+    v, f = ctypes.c_double(), ctypes.c_int()
+    _check(lib().dmrgcc_input_get(os.fsencode(path), group.encode(), key.encode(), k, int(echo),
+                                  int(default is not None), float(default or 0), ctypes.byref(v),
+                                  ctypes.byref(f)))
+    if not f.value and default is None:
+        return None
+    return {"int": int, "double": float, "bool": bool}[kind](v.value)
+
+
 def input_string(path, key):
     """String value of `key` in the file's "input" group (None if absent)."""
     buf = ctypes.create_string_buffer(4096)
@@ -688,6 +714,19 @@ class HeffPlan:
         _sync_torch()
         _check(lib().dmrgcc_heff_apply(self._h, px, py, precision))
 
+    def apply_async(self, x, y, alpha=1.0, beta=1.0, precision=FP64, stream=None):
+        """y = alpha H_eff x + beta y, launched on `stream` (a torch.cuda.Stream
+        or raw handle; default: torch's current stream) without waiting, so
+        it can be captured in a CUDA graph."""
+        import torch
+        px = _vec_ptr(x, self.words, self.complex, "x")
+        py = _vec_ptr(y, self.words, self.complex, "y")
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        raw = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        _check(lib().dmrgcc_heff_apply_async(self._h, px, py, float(alpha), float(beta), precision,
+                                             _vp(raw)))
+
     @classmethod
     def _from_handle(cls, h, owner):
         self = cls.__new__(cls)
@@ -843,6 +882,26 @@ class EnvResult:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.dmrgcc_blocks_destroy(self._h)
             self._h = None
+
+
+def lattice_canonical(leigh, ldim, rdim, ket_l, ket_r, ket_rows, ket_cols, ket_data, max_dim, canpre):
+    """Left (leigh 0) / right (leigh 1) canonical form of one site
+    (Left/RightCanonicalTensorLattice).  Returns (ket EnvResult, canonical
+    EnvResult, new sector dims, match dims, discarded weight)."""
+    cplx = ket_data.is_complex() if hasattr(ket_data, "is_complex") else False
+    ldim, rdim = _i32(ldim), _i32(rdim)
+    kl, kr, krows, kcols = _i32(ket_l), _i32(ket_r), _i32(ket_rows), _i32(ket_cols)
+    k, c = _vp(), _vp()
+    ns = len(rdim) if leigh == 0 else len(ldim)
+    dims = np.zeros(ns, dtype=np.int32)
+    match = np.zeros(ns, dtype=np.int32)
+    disc = ctypes.c_double()
+    _sync_torch()
+    _check(lib().dmrgcc_lattice_canonical(leigh, int(cplx), len(ldim), len(rdim), _ip(ldim), _ip(rdim), len(kl),
+                                          _ip(kl), _ip(kr), _ip(krows), _ip(kcols), _ptr(ket_data), int(max_dim),
+                                          float(canpre), _vp(), ctypes.byref(k), ctypes.byref(c), _ip(dims),
+                                          _ip(match), ctypes.byref(disc)))
+    return EnvResult(k, cplx), EnvResult(c, cplx), dims, match, disc.value
 
 
 def sector_svd(mats, stream=None, precondition=0):

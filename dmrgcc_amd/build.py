@@ -39,6 +39,26 @@ def _compile(src):
     return obj, msg
 
 
+ROOT = os.path.dirname(HERE)
+DROPIN_SRC = os.path.join(ROOT, "tests", "cxx", "dropin_test.cpp")
+DROPIN_LIB = os.path.join(ROOT, "tests", "cxx", "libdropin_test.so")
+
+
+def build_dropin_test():
+    """The test harness of the drop-in class headers (include/dmrgcc/):
+    host C++ only, linked against the library built above."""
+    if not os.path.exists(DROPIN_SRC):
+        return None
+    tmp = DROPIN_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-fPIC", "-shared", "-I" + ROOT, DROPIN_SRC, "-o", tmp,
+           "-L" + HERE, "-ldmrgcc_amd", "-Wl,-rpath,$ORIGIN/../../dmrgcc_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in test harness failed to build:\n%s" % (r.stdout + r.stderr))
+    os.replace(tmp, DROPIN_LIB)
+    return DROPIN_LIB
+
+
 def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -53,6 +73,7 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % (r.stdout + r.stderr))
     os.replace(tmp, LIB)
+    build_dropin_test()
     return LIB
 
 

@@ -150,6 +150,7 @@ int heff_create(int cs, int d, int lb, int rb, const double* W, int nl, int nr, 
     h->L = make_env(lb, nL, L_o, L_l, L_r, L_rows, L_cols, L_off, L_data, cs);
     h->R = make_env(rb, nR, R_o, R_l, R_r, R_rows, R_cols, R_off, R_data, cs);
     h->plan.build(h->lat, h->W, h->L, h->R, h->cx->stream);
+    h->cx->sync();  // descriptors in place: the plan may run on any stream
     *out = h.release();
   });
 }
@@ -423,6 +424,32 @@ int dmrgcc_input_string(const char* path, const char* key, char* buf, int buflen
   });
 }
 
+int dmrgcc_input_get(const char* path, const char* group, const char* key, int kind, int echo,
+                     int has_default, double default_value, double* value, int* found) {
+  return guarded([&] {
+    require(path && group && key && value && found && kind >= 0 && kind <= 2, "bad arguments", kErrArgument);
+    const bool saved = InputGroup::echo;
+    InputGroup::echo = echo != 0;
+    InputGroup g(path, group);
+    int got = 0;
+    if (kind == 0) {
+      int v = 0;
+      got = g.GetInt(key, v, has_default != 0, (int)default_value);
+      *value = got ? v : default_value;
+    } else if (kind == 1) {
+      double v = 0.0;
+      got = g.GetDouble(key, v, has_default != 0, default_value);
+      *value = got ? v : default_value;
+    } else {
+      bool v = false;
+      got = g.GetBool(key, v, has_default != 0, default_value != 0.0);
+      *value = got ? (v ? 1.0 : 0.0) : default_value;
+    }
+    InputGroup::echo = saved;
+    *found = got;
+  });
+}
+
 int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
                             int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags) {
   return guarded([&] {
@@ -578,6 +605,29 @@ int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision)
   });
 }
 
+int dmrgcc_heff_apply_async(dmrgcc_heff* h, const double* x, double* y, double alpha, double beta,
+                            int precision, void* stream) {
+  return guarded([&] {
+    require(h && x && y, "null argument", kErrArgument);
+    require(precision >= 0 && precision <= 2, "unknown precision", kErrArgument);
+    require(x != y, "x and y must not overlap", kErrArgument);
+    require_live(h);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->cx->stream;
+    const int64_t n = h->plan.doubles();
+    // y = alpha H x + beta y as y <- (beta / alpha) y, y += H x, y <- alpha y:
+    // kernels only (no memset, no host wait), so the call can be captured
+    if (alpha == 0.0) {
+      if (beta == 0.0) fill(y, n, 0.0, s);
+      else if (beta != 1.0) axpby(y, y, n, beta, 0.0, s);
+      return;
+    }
+    if (beta == 0.0) fill(y, n, 0.0, s);
+    else if (beta != alpha) axpby(y, y, n, beta / alpha, 0.0, s);
+    h->plan.apply(x, y, precision, s);
+    if (alpha != 1.0) axpby(y, y, n, alpha, 0.0, s);
+  });
+}
+
 int dmrgcc_heff_info(const dmrgcc_heff* h, double* info) {
   return guarded([&] {
     require(h && info, "null argument", kErrArgument);
@@ -644,6 +694,39 @@ int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int
   });
 }
 
+int dmrgcc_alloc(int64_t bytes, void** dev) {
+  return guarded([&] {
+    require(dev && bytes >= 0, "bad arguments", kErrArgument);
+    *dev = bytes > 0 ? DevicePool::get().alloc((size_t)bytes) : nullptr;
+  });
+}
+
+int dmrgcc_free(void* dev) {
+  return guarded([&] {
+    if (dev) DevicePool::get().release(dev);
+  });
+}
+
+int dmrgcc_upload(void* dev, const void* host, int64_t bytes, void* stream) {
+  return guarded([&] {
+    require(bytes >= 0 && (bytes == 0 || (dev && host)), "bad arguments", kErrArgument);
+    if (bytes == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DMRG_HIP(hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, s));
+    DMRG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dmrgcc_download(void* host, const void* dev, int64_t bytes, void* stream) {
+  return guarded([&] {
+    require(bytes >= 0 && (bytes == 0 || (dev && host)), "bad arguments", kErrArgument);
+    if (bytes == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DMRG_HIP(hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, s));
+    DMRG_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int dmrgcc_blocks_count(const dmrgcc_blocks* b) { return b ? (int)b->o.size() : -1; }
 
 int dmrgcc_blocks_get(const dmrgcc_blocks* b, int i, int* o, int* l, int* r, int* rows, int* cols,
@@ -674,6 +757,63 @@ int dmrgcc_blocks_download(const dmrgcc_blocks* b, double* host) {
 int dmrgcc_blocks_destroy(dmrgcc_blocks* b) {
   delete b;
   return 0;
+}
+
+int dmrgcc_lattice_canonical(int leigh, int is_complex, int nl, int nr, const int* ldim, const int* rdim,
+                             int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                             const int* ket_cols, const double* ket_data, int max_dim, double canpre,
+                             void* stream, dmrgcc_blocks** ket_out, dmrgcc_blocks** can_out, int* dims_out,
+                             int* match_dim, double* discarded) {
+  return guarded([&] {
+    require(leigh == 0 || leigh == 1, "leigh must be 0 or 1", kErrArgument);
+    require(nl > 0 && nr > 0 && ldim && rdim && nket >= 0 && ket_out && can_out && dims_out && match_dim,
+            "bad arguments", kErrArgument);
+    require(nket == 0 || (ket_l && ket_r && ket_rows && ket_cols && ket_data), "bad block list", kErrArgument);
+    const int cs = is_complex ? 2 : 1;
+    Context cx(static_cast<hipStream_t>(stream));
+    Lattice lat;
+    std::vector<int> li(nl), ri(nr);
+    std::iota(li.begin(), li.end(), 0);
+    std::iota(ri.begin(), ri.end(), 0);
+    lat.define_sectors(li, ri, std::vector<int>(ldim, ldim + nl), std::vector<int>(rdim, rdim + nr));
+    lat.ket.cs = cs;
+    int64_t acc = 0;
+    for (int b = 0; b < nket; ++b) {
+      require(ket_l[b] >= 0 && ket_l[b] < nl && ket_r[b] >= 0 && ket_r[b] < nr, "ket block sector out of range");
+      require(ket_rows[b] >= 0 && ket_cols[b] >= 0, "negative block shape");
+      lat.ket.blk.push_back(Block{ket_l[b], ket_r[b], ket_rows[b], ket_cols[b], acc});
+      acc += (int64_t)ket_rows[b] * ket_cols[b];
+    }
+    lat.ket.size = acc;
+    lat.ket.ext = ket_data;
+    const double d = canonicalize(lat, leigh, max_dim, canpre, cx);
+    auto export_blocks = [&](const BlockTensor& t) {
+      auto b = std::make_unique<dmrgcc_blocks>();
+      b->cs = cs;
+      for (const Block& k : t.blk) {
+        b->o.push_back(0);
+        b->l.push_back(k.l);
+        b->r.push_back(k.r);
+        b->rows.push_back(k.rows);
+        b->cols.push_back(k.cols);
+        b->off.push_back(k.off);
+      }
+      b->data.resize((size_t)std::max<int64_t>(t.doubles(), 1));
+      if (t.size > 0)
+        DMRG_HIP(hipMemcpyAsync(b->data.data(), t.base(), t.doubles() * sizeof(double),
+                                hipMemcpyDeviceToDevice, cx.stream));
+      return b;
+    };
+    auto k = export_blocks(lat.ket);
+    auto c = export_blocks(lat.can);
+    cx.sync();
+    const std::vector<int>& dims = leigh == 0 ? lat.rd : lat.ld;
+    std::copy(dims.begin(), dims.end(), dims_out);
+    std::copy(lat.match_dim.begin(), lat.match_dim.end(), match_dim);
+    if (discarded) *discarded = d;
+    *ket_out = k.release();
+    *can_out = c.release();
+  });
 }
 
 int dmrgcc_env_update(int leigh, int d, int lb, int rb, const double* W, int nl, int nr,
@@ -936,6 +1076,7 @@ int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     auto e = std::make_unique<dmrgcc_heff>();
     e->cx = std::make_unique<Context>(nullptr);
     h->run->BuildHeff(site, e->plan);
+    h->run->context().sync();
     e->run_epoch = h->epoch;
     e->epoch = *h->epoch;
     *out = e.release();
@@ -1035,6 +1176,7 @@ int dmrgcc_dmrg_heff2(dmrgcc_dmrg* h, int site, dmrgcc_heff** out) {
     auto e = std::make_unique<dmrgcc_heff>();
     e->cx = std::make_unique<Context>(nullptr);
     h->run->BuildTwoSite(site, e->two, e->plan);
+    h->run->context().sync();
     e->run_epoch = h->epoch;
     e->epoch = *h->epoch;
     *out = e.release();

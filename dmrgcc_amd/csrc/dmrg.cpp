@@ -1114,10 +1114,18 @@ void RealTensorRundmrg::DefineTensorSpace(int first_block, int first_dim) {
 // device (one-sided Jacobi) and the truncation rule on the host.
 // --------------------------------------------------------------------------
 double RealTensorRundmrg::canonical(int leigh, int site) {
-  Lattice& lat = lat_[site];
+  return canonicalize(lat_[site], leigh, max_dim_, canpre_, cx_, &ph_);
+}
+
+double canonicalize(Lattice& lat, int leigh, int max_dim, double canpre, Context& cx, PhaseTimes* ph) {
+  PhaseTimes scratch_times;
+  PhaseTimes& ph_ = ph ? *ph : scratch_times;
+  Context& cx_ = cx;
+  const int max_dim_ = max_dim;
+  const double canpre_ = canpre;
   BlockTensor& ket = lat.ket;
   const bool left = leigh == 0;
-  const int cs = cs_;
+  const int cs = ket.cs;
   const int ns = left ? lat.nr() : lat.nl();
   struct Sec {
     std::vector<int> pos;

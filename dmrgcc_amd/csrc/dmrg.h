@@ -282,6 +282,16 @@ struct PhaseTimes {
   double svd_work = 0;  // sum of M*N*N over sector SVDs
 };
 
+// Left (leigh 0) / right (leigh 1) canonical form of one site
+// (Left/RightCanonicalTensorLattice, real_tensor_lattice.cc:558-676,
+// :794-911): per sector the stacked blocks' SVD on the device,
+// ComputeTruncateDim with (max_dim, canpre), the orthonormal factor into the
+// ket blocks and Sigma V^T (U Sigma) into lat.can; lat.match_dim and the
+// truncated sector dimensions updated.  Returns the discarded weight; with
+// `ph` the SVD time and sizes are added there.
+double canonicalize(Lattice& lat, int leigh, int max_dim, double canpre, Context& cx,
+                    PhaseTimes* ph = nullptr);
+
 struct RunResult {
   PhaseTimes phase;
   std::vector<double> energies;   // per half sweep (right, left, right, ...)

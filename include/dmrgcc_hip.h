@@ -21,8 +21,8 @@
  *               2 = BF16x3 (split bf16 MFMA, FP32 accumulate).
  *  - stream: a hipStream_t, or NULL for the library's own stream (a
  *    non-blocking stream: the caller finishes the work that produces the
- *    inputs before the call).  Every entry point has completed its device
- *    work when it returns.
+ *    inputs before the call).  Every entry point except
+ *    dmrgcc_heff_apply_async has completed its device work when it returns.
  *  - Not thread-safe per handle.
  */
 #ifndef DMRGCC_HIP_H_
@@ -123,6 +123,13 @@ int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dm
  * cache_resume, record_process. */
 int dmrgcc_input_run_config(const char* path, int echo, int* first_block, int* first_dim, int* nhalf,
                             int* maxm, int* maxb, int* niter, double* canpre, double* noise, int* flags);
+/* One int (kind 0), double (1) or bool (2) value of `group` as
+ * InputGroup::GetInt / GetDouble / GetBool read it (src/util/input.cc:
+ * 203-262): *found = 0/1, *value = the value (bools as 0 / 1), or
+ * default_value when absent; echo prints the reference's Got / Def line
+ * (Def only with has_default). */
+int dmrgcc_input_get(const char* path, const char* group, const char* key, int kind, int echo,
+                     int has_default, double default_value, double* value, int* found);
 /* One string value of the "input" group (InputGroup::GetString,
  * input.cc:223-244; a key with nothing after '=' is found and empty):
  * the value into buf (NUL-terminated, buflen bytes), *found = 0/1. */
@@ -203,6 +210,15 @@ int dmrgcc_zheff_create(int d, int lb, int rb, const double* W, int nl, int nr, 
 /* y += H_eff x  (the reference's accumulate-into-state semantics).
  * x, y: device vectors of psi_dim doubles. */
 int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision);
+/* y = alpha * H_eff x + beta * y on `stream` (NULL: the plan's own stream),
+ * returning without waiting: only kernel launches (no copies, memsets or
+ * host synchronisation), so the call can be captured into a HIP graph.
+ * alpha = beta = 1 is the reference's accumulate (state += H psi); beta = 0
+ * does not read y.  The caller orders the work that produces x and y before
+ * it on `stream`; x and y must not overlap.  No counterpart in the
+ * reference (MultiplyEffectHamilton is synchronous, on host memory). */
+int dmrgcc_heff_apply_async(dmrgcc_heff* h, const double* x, double* y, double alpha, double beta,
+                            int precision, void* stream);
 /* info[0] = psi_dim, [1] = F_ref (reference-order flops), [2] = F_exec,
  * [3] = minimum HBM bytes per matvec, [4] = stage-1 tasks, [5] = tiles,
  * [6] = stage-1 flops, [7] = stage-2 flops, [8] = stage-1 tiles,
@@ -233,6 +249,15 @@ int dmrgcc_gemm_config(int tile_mode, int split_k);
  * Rayleigh quotient of the Ritz vector.  iters_out may be NULL. */
 int dmrgcc_lanczos(dmrgcc_heff* h, double* ket, int num_iter, int precision, int refine,
                    double* energy, int* iters_out);
+
+/* ---- device buffers (the boundary's upload / download of blocks) ------- */
+/* Device memory from the library's caching pool; upload / download copy
+ * `bytes` between host and device memory on `stream` (NULL: the null
+ * stream) and return when the copy is done. */
+int dmrgcc_alloc(int64_t bytes, void** dev);
+int dmrgcc_free(void* dev);
+int dmrgcc_upload(void* dev, const void* host, int64_t bytes, void* stream);
+int dmrgcc_download(void* host, const void* dev, int64_t bytes, void* stream);
 
 /* ---- block-sparse result handle (environment updates) ------------------ */
 typedef struct dmrgcc_blocks dmrgcc_blocks;
@@ -269,6 +294,26 @@ int dmrgcc_zenv_update(int leigh, int d, int lb, int rb, const double* W, int nl
                        const int* E_o, const int* E_l, const int* E_r, const int* E_rows,
                        const int* E_cols, const int64_t* E_off, const double* E_data, void* stream,
                        dmrgcc_blocks** out);
+
+/* ---- canonical form of one site ----------------------------------------
+ * RealTensorLattice::LeftCanonicalTensorLattice (leigh 0,
+ * real_tensor_lattice.cc:558-676) / RightCanonicalTensorLattice (leigh 1,
+ * :794-911) with (max_dim, canonical_precision) on a site given as its
+ * sector dimensions (ldim: nl, rdim: nr), ket block list and device ket
+ * data (interleaved complex when is_complex).  Per sector the stacked
+ * blocks are decomposed by the sector SVD, ComputeTruncateDim
+ * (:501-555) picks the kept states, the orthonormal factor replaces the ket
+ * blocks and Sigma V^T (left) / U Sigma (right) forms the canonical tensor,
+ * one block per sector (l = r = sector).  Outputs: the new ket and
+ * canonical tensor (channel 0 of the handles), the new sector dimensions
+ * (right dims for leigh 0: nr entries; left dims for leigh 1: nl), the match
+ * dimensions and the discarded weight (may be NULL).  Signs: the largest
+ * |entry| of every kept orthonormal vector is positive (real positive). */
+int dmrgcc_lattice_canonical(int leigh, int is_complex, int nl, int nr, const int* ldim, const int* rdim,
+                             int nket, const int* ket_l, const int* ket_r, const int* ket_rows,
+                             const int* ket_cols, const double* ket_data, int max_dim, double canpre,
+                             void* stream, dmrgcc_blocks** ket_out, dmrgcc_blocks** can_out, int* dims_out,
+                             int* match_dim, double* discarded);
 
 /* ---- sector SVD (RealMatrix::SVDMatrix, real_matrix.cc:588-620) -------- */
 /* Batched thin SVD of n row-major device matrices A_i (M_i x N_i), one-sided

@@ -70,3 +70,15 @@ def test_batch_refuses_a_repeated_plan():
     assert rc == 3
     assert "more than once" in lib.dmrgcc_last_error().decode()
     assert not out.value
+
+
+def test_dropin_harness_builds_and_exports():
+    # the C++ harness of the drop-in class headers is built with the library
+    # and loads next to it (its calls need a GPU: tests/test_gpu_dropin.py)
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    dm.lib()
+    h = ctypes.CDLL(os.path.join(here, "cxx", "libdropin_test.so"))
+    for f in ("dropin_site_create", "dropin_site_env", "dropin_site_heff", "dropin_site_lanczos",
+              "dropin_site_env_update", "dropin_site_canonical", "dropin_site_destroy", "dropin_run"):
+        assert hasattr(h, f), f

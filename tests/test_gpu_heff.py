@@ -190,3 +190,36 @@ def test_lanczos_matches_oracle(name):
     ket2 = torch.tensor(lat.ket.to_vector(), device="cuda")
     e2, _ = plan.lanczos(ket2, 10, dm.BF16X3, refine=True)
     assert abs(e2 - e_ref) < 1e-8 * abs(e_ref)
+
+
+def test_apply_async_alpha_beta_and_graph_capture():
+    # y = alpha H x + beta y on a given stream without host waits; captured
+    # into a CUDA graph and replayed with eager work between the replays
+    dmo, site = state("4x3_d4_left")
+    plan, lat = make_plan(dmo, site)
+    n = lat.ket.dim()
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    hx = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, np.zeros(n))
+    xd = torch.tensor(x, device="cuda")
+    for alpha, beta in ((1.0, 1.0), (0.5, -2.0), (1.0, 0.0), (0.0, 3.0), (-1.5, 1.0)):
+        yd = torch.tensor(y0, device="cuda")
+        plan.apply_async(xd, yd, alpha, beta, dm.FP64)
+        torch.cuda.synchronize()
+        ref = alpha * hx + beta * y0
+        assert np.abs(yd.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (alpha, beta)
+    # graph: y = 2 H x (beta = 0 never reads y)
+    yg = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        plan.apply_async(xd, yg, 2.0, 0.0, dm.FP64, stream=s)
+    for rep in range(3):
+        xd.mul_(-0.5).add_(0.25)   # eager work between the replays
+        ye = torch.zeros(n, dtype=torch.float64, device="cuda")
+        plan.apply(xd, ye, dm.FP64)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(yg, 2.0 * ye), rep
