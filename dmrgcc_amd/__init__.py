@@ -98,6 +98,10 @@ _SIGS = {
     "dmrgcc_heff_apply_async": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                _vp]),
     "dmrgcc_alloc": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "dmrgcc_model_write_mpo": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "dmrgcc_model_read_mpo": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "dmrgcc_dmrg_write_env": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+    "dmrgcc_env_file_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "dmrgcc_input_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, _c_dbl_p, _c_int_p]),
     "dmrgcc_free": (ctypes.c_int, [_vp]),
@@ -338,6 +342,15 @@ class _Model:
                                            _dp(W.view(np.float64))))
         return W
 
+    def write_mpo(self, path):
+        """RealTensorHamiltonian::WriteTensorHamiltonian of this model's MPO."""
+        _check(lib().dmrgcc_model_write_mpo(self._h, os.fsencode(path)))
+
+    def read_mpo(self, path):
+        """Replace this model's MPO by the one in an MPO file
+        (ReadTensorHamiltonian)."""
+        _check(lib().dmrgcc_model_read_mpo(self._h, os.fsencode(path)))
+
     def mpo_table(self, bond):
         c = ctypes.c_int()
         _check(lib().dmrgcc_model_mpo_table(self._h, bond, ctypes.byref(c), None))
@@ -549,6 +562,11 @@ class DmrgRun:
         _check(lib().dmrgcc_dmrg_env_time(self._h, site, leigh, reps, _dp(v)))
         return {"f_ms": v[0], "out_ms": v[1], "ms": v[0] + v[1], "bytes_min": v[2], "bytes_exec": v[3],
                 "flops": v[4]}
+
+    def write_env(self, site, leigh, path):
+        """WriteTensorContraction of the left (leigh 0) or right (1)
+        environment of `site` on the current state."""
+        _check(lib().dmrgcc_dmrg_write_env(self._h, site, leigh, os.fsencode(path)))
 
     def heff2(self, site):
         """Two-site H_eff of (site, site+1) on the current state (a HeffPlan)
@@ -882,6 +900,14 @@ class EnvResult:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.dmrgcc_blocks_destroy(self._h)
             self._h = None
+
+
+def env_file_read(path, nbond, complex=False):
+    """ReadTensorContraction of an environment file with nbond channels:
+    an EnvResult (blocks with their channel, device data)."""
+    h = _vp()
+    _check(lib().dmrgcc_env_file_read(os.fsencode(path), int(nbond), int(complex), ctypes.byref(h)))
+    return EnvResult(h, complex)
 
 
 def lattice_canonical(leigh, ldim, rdim, ket_l, ket_r, ket_rows, ket_cols, ket_data, max_dim, canpre):

@@ -345,6 +345,8 @@ int dmrgcc_model_spinless_create(int nx, int ny, int num_boson, int physics_dim,
     m->ham = std::make_unique<RealSpinlessBoseSquareHamiltonian>(nx, ny, physics_dim, 0.0, hop_x, hop_y,
                                                                  U, mu, period_x != 0, period_y != 0);
     m->ham->DefineTensorHamiltonian();
+    for (const SiteOperator& w : m->ham->sites())
+      require(w.lb > 0 && w.rb > 0, "the Hamiltonian has no term on some site of this lattice", kErrArgument);
     m->tables = m->space->DefineQuantumTable();
     *out = m.release();
   });
@@ -361,6 +363,8 @@ int dmrgcc_model_zspinless_create(int nx, int ny, int num_boson, int physics_dim
     m->ham = std::make_unique<ComplexSpinlessBoseSquareHamiltonian>(nx, ny, physics_dim, 0.0, hop_x, hop_y,
                                                                     U, mu, period_x != 0, period_y != 0);
     m->ham->DefineTensorHamiltonian();
+    for (const SiteOperator& w : m->ham->sites())
+      require(w.lb > 0 && w.rb > 0, "the Hamiltonian has no term on some site of this lattice", kErrArgument);
     m->tables = m->space->DefineQuantumTable();
     *out = m.release();
   });
@@ -379,6 +383,8 @@ int dmrgcc_model_spinful_create(int nx, int ny, int n_up, int n_down, int physic
     m->ham = std::make_unique<ComplexSpinfulBoseSquareHamiltonian>(nx, ny, physics_dim, flux, hop_x, hop_y,
                                                                    inter, mu, period_x != 0, period_y != 0);
     m->ham->DefineTensorHamiltonian();
+    for (const SiteOperator& w : m->ham->sites())
+      require(w.lb > 0 && w.rb > 0, "the Hamiltonian has no term on some site of this lattice", kErrArgument);
     m->tables = m->space->DefineQuantumTable();
     *out = m.release();
   });
@@ -403,6 +409,8 @@ int dmrgcc_input_model_create(const char* path, const char* driver, int echo, dm
     auto m = std::make_unique<dmrgcc_model>();
     read_model(input, driver, m->space, m->ham);
     m->ham->DefineTensorHamiltonian();
+    for (const SiteOperator& w : m->ham->sites())
+      require(w.lb > 0 && w.rb > 0, "the Hamiltonian has no term on some site of this lattice", kErrArgument);
     m->tables = m->space->DefineQuantumTable();
     *out = m.release();
     std::fflush(stdout);
@@ -1051,6 +1059,62 @@ int dmrgcc_dmrg_write_space(dmrgcc_dmrg* h, const char* path) {
   return guarded([&] {
     require(h && path, "null argument", kErrArgument);
     h->run->WriteTensorSpace(path);
+  });
+}
+
+int dmrgcc_dmrg_write_env(dmrgcc_dmrg* h, int site, int leigh, const char* path) {
+  return guarded([&] {
+    require(h && path && site >= 0 && site < h->run->num_site() && (leigh == 0 || leigh == 1), "bad arguments",
+            kErrArgument);
+    const Env& e = leigh == 0 ? h->run->left_env(site) : h->run->right_env(site);
+    require(!e.empty(), "no environment at this site on the current state", kErrArgument);
+    h->run->context().sync();
+    WriteTensorContraction(e, path, h->run->context().stream);
+  });
+}
+
+int dmrgcc_env_file_read(const char* path, int nbond, int is_complex, dmrgcc_blocks** out) {
+  return guarded([&] {
+    require(path && out && nbond > 0, "bad arguments", kErrArgument);
+    const int cs = is_complex ? 2 : 1;
+    Context cx(nullptr);
+    Env e = ReadTensorContraction(path, nbond, cs, cx.stream);
+    auto b = std::make_unique<dmrgcc_blocks>();
+    b->cs = cs;
+    int64_t total = 0;
+    for (const auto& t : e) total += t.size;
+    b->data.resize((size_t)std::max<int64_t>(cs * total, 1));
+    int64_t acc = 0;
+    for (size_t o = 0; o < e.size(); ++o) {
+      for (const Block& k : e[o].blk) {
+        b->o.push_back((int)o);
+        b->l.push_back(k.l);
+        b->r.push_back(k.r);
+        b->rows.push_back(k.rows);
+        b->cols.push_back(k.cols);
+        b->off.push_back(acc + k.off);
+      }
+      if (e[o].size > 0)
+        DMRG_HIP(hipMemcpyAsync(b->data.data() + cs * acc, e[o].data.data(), cs * e[o].size * sizeof(double),
+                                hipMemcpyDeviceToDevice, cx.stream));
+      acc += e[o].size;
+    }
+    cx.sync();
+    *out = b.release();
+  });
+}
+
+int dmrgcc_model_write_mpo(const dmrgcc_model* m, const char* path) {
+  return guarded([&] {
+    require(m && path, "null argument", kErrArgument);
+    m->ham->WriteTensorHamiltonian(path);
+  });
+}
+
+int dmrgcc_model_read_mpo(dmrgcc_model* m, const char* path) {
+  return guarded([&] {
+    require(m && path, "null argument", kErrArgument);
+    m->ham->ReadTensorHamiltonian(path);
   });
 }
 

@@ -2693,6 +2693,29 @@ void read_lattice(std::istream& is, Lattice& lat, int cs, hipStream_t stream) {
 
 }  // namespace
 
+void WriteTensorContraction(const Env& env, const std::string& path, hipStream_t stream) {
+  // RealTensorContraction::WriteTensorContraction
+  // (real_tensor_contraction.cc:73-90): one WriteMatrixBlock per channel
+  std::ofstream os(path, std::ios::binary | std::ios::out);
+  require(os.good(), "can't open environment file for writing: " + path, kErrArgument);
+  for (const BlockTensor& t : env) write_blocks(os, t, stream);
+  os.flush();
+  require(os.good(), "write error on environment file: " + path, kErrInternal);
+}
+
+Env ReadTensorContraction(const std::string& path, int nbond, int cs, hipStream_t stream) {
+  // ReadTensorContraction (real_tensor_contraction.cc:92-109): nbond
+  // ReadMatrixBlock in channel order
+  std::ifstream is(path, std::ios::binary | std::ios::in);
+  require(is.good(), "can't open environment file: " + path, kErrArgument);
+  require(nbond > 0, "an environment has at least one channel", kErrArgument);
+  Env env(nbond);
+  for (BlockTensor& t : env) read_blocks(is, t, cs, stream);
+  is.peek();
+  require(is.eof(), "trailing bytes in environment file: " + path, kErrArgument);
+  return env;
+}
+
 void RealTensorRundmrg::WriteTensorSpace(const std::string& path) {
   // RealTensorSpace::WriteTensorSpace (real_tensor_space.cc:182-208)
   cx_.sync();

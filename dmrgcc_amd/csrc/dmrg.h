@@ -220,6 +220,15 @@ Env left_env_update(const Lattice& lat, const SiteOperator& W, const Env& L, Con
                     EnvTiming* tm = nullptr);
 Env right_env_update(const Lattice& lat, const SiteOperator& W, const Env& R, Context& cx,
                      EnvTiming* tm = nullptr);
+// Environment files (RealTensorContraction::Write/ReadTensorContraction,
+// real_tensor_contraction.cc:73-109): per channel RealMatrixBlock::
+// WriteMatrixBlock (block count, then per block left, right and
+// RealMatrix::WriteMatrix: rows, cols, rows*cols, entries; complex entries
+// as the 16-byte Complex).  The reader takes the channel count, as the
+// reference's object does.
+void WriteTensorContraction(const Env& env, const std::string& path, hipStream_t stream);
+Env ReadTensorContraction(const std::string& path, int nbond, int cs, hipStream_t stream);
+
 // Random matrices of the expansion in draw order (RandomMatrix,
 // real_matrix.cc:201-208; complex_matrix.cc:201-208 with cs = 2): matrix k
 // is rows[k] x cols[k], row-major (complex: interleaved), concatenated in

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <fstream>
 
 #include "common.h"
 #include "quicksort.h"
@@ -231,6 +232,122 @@ void RealTensorHamiltonian::Init(int num_site, int physics_dim, int num_quantum,
   qt_[num_site] = QNTable{QN(nq_, 0)};
 }
 
+namespace {
+template <typename T>
+void put(std::ostream& os, const T& v) {
+  os.write(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+template <typename T>
+T get(std::istream& is) {
+  T v{};
+  is.read(reinterpret_cast<char*>(&v), sizeof(T));
+  require(bool(is), "MPO file ends early", kErrArgument);
+  return v;
+}
+// RealMatrix::WriteMatrix (real_matrix.cc:139-151): rows, cols, rows*cols,
+// the entries (ComplexMatrix: re, im pairs, complex_matrix.cc:139-175)
+void put_matrix(std::ostream& os, int rows, int cols, const double* re, const double* im, bool cplx) {
+  put<int>(os, rows);
+  put<int>(os, cols);
+  put<int>(os, rows * cols);
+  for (int e = 0; e < rows * cols; ++e) {
+    put<double>(os, re[e]);
+    if (cplx) put<double>(os, im ? im[e] : 0.0);
+  }
+}
+void get_matrix(std::istream& is, int rows, int cols, double* re, double* im, bool cplx) {
+  const int r = get<int>(is), c = get<int>(is), n = get<int>(is);
+  require(r == rows && c == cols && n == rows * cols, "bad matrix header in MPO file", kErrArgument);
+  for (int e = 0; e < n; ++e) {
+    re[e] = get<double>(is);
+    const double i = cplx ? get<double>(is) : 0.0;
+    if (im) im[e] = i;
+  }
+}
+}  // namespace
+
+void RealTensorHamiltonian::WriteTensorHamiltonian(const std::string& path) const {
+  std::ofstream os(path, std::ios::binary | std::ios::out);
+  require(os.good(), "can't open MPO file for writing: " + path, kErrArgument);
+  put<int>(os, num_site_);
+  put<int>(os, num_site_ + 1);
+  put<int>(os, num_site_ - 1);
+  put<int>(os, (int)basic_.size());
+  for (size_t k = 0; k < basic_.size(); ++k)
+    put_matrix(os, d_, d_, basic_[k].data(), k < basic_im_.size() ? basic_im_[k].data() : nullptr, cplx_);
+  put<int>(os, nq_);
+  for (int i = 0; i <= num_site_; ++i) put<int>(os, (int)qt_[i].size());
+  for (int i = 0; i <= num_site_; ++i)
+    for (const QN& q : qt_[i])
+      for (int k = 0; k < nq_; ++k) put<int>(os, q[k]);
+  const int dd = d_ * d_;
+  for (const SiteOperator& W : W_) {
+    put<int>(os, W.d);
+    put<int>(os, W.lb);
+    put<int>(os, W.rb);
+    for (int a = 0; a < W.lb; ++a)
+      for (int b = 0; b < W.rb; ++b) {
+        const size_t o = ((size_t)a * W.rb + b) * dd;
+        put_matrix(os, W.d, W.d, W.w.data() + o, W.wi.empty() ? nullptr : W.wi.data() + o, cplx_);
+      }
+  }
+  os.flush();
+  require(os.good(), "write error on MPO file: " + path, kErrInternal);
+}
+
+void RealTensorHamiltonian::ReadTensorHamiltonian(const std::string& path) {
+  // The reference's reader reads the basic operators into matrices it never
+  // allocated (real_tensor_hamiltonian.cc:132: basic_operator_[i] is an
+  // uninitialised pointer) and never reads the site operators back (it
+  // deletes tensor_hamiltonian_ and stops after the quantum tables).  Fixed
+  // here: both are read, in the order the writer puts them.
+  std::ifstream is(path, std::ios::binary | std::ios::in);
+  require(is.good(), "can't open MPO file: " + path, kErrArgument);
+  const int L = get<int>(is), Lpp = get<int>(is), Lmm = get<int>(is);
+  require(L == num_site_ && Lpp == L + 1 && Lmm == L - 1, "MPO file has another number of sites", kErrArgument);
+  const int nop = get<int>(is);
+  require(nop >= 0, "bad operator count in MPO file", kErrArgument);
+  std::vector<std::vector<double>> basic(nop, std::vector<double>((size_t)d_ * d_));
+  std::vector<std::vector<double>> basic_im(cplx_ ? nop : 0, std::vector<double>((size_t)d_ * d_));
+  for (int k = 0; k < nop; ++k) get_matrix(is, d_, d_, basic[k].data(), cplx_ ? basic_im[k].data() : nullptr, cplx_);
+  const int nq = get<int>(is);
+  require(nq == nq_, "MPO file has another number of quantum numbers", kErrArgument);
+  std::vector<int> nt(Lpp);
+  for (int& n : nt) {
+    n = get<int>(is);
+    require(n >= 0, "bad quantum table size in MPO file", kErrArgument);
+  }
+  std::vector<QNTable> qt(Lpp);
+  for (int i = 0; i < Lpp; ++i)
+    for (int j = 0; j < nt[i]; ++j) {
+      QN q(nq);
+      for (int& v : q) v = get<int>(is);
+      qt[i].push_back(q);
+    }
+  std::vector<SiteOperator> W(L);
+  for (SiteOperator& w : W) {
+    w.d = get<int>(is);
+    w.lb = get<int>(is);
+    w.rb = get<int>(is);
+    require(w.d == d_ && w.lb > 0 && w.rb > 0, "bad operator header in MPO file", kErrArgument);
+    const size_t n = (size_t)w.lb * w.rb * d_ * d_;
+    w.w.assign(n, 0.0);
+    if (cplx_) w.wi.assign(n, 0.0);
+    for (int a = 0; a < w.lb; ++a)
+      for (int b = 0; b < w.rb; ++b) {
+        const size_t o = ((size_t)a * w.rb + b) * d_ * d_;
+        get_matrix(is, d_, d_, w.w.data() + o, cplx_ ? w.wi.data() + o : nullptr, cplx_);
+      }
+  }
+  for (int i = 0; i < L; ++i)
+    require((i == 0 || W[i].lb == W[i - 1].rb) && (int)qt[i].size() == W[i].lb && (int)qt[i + 1].size() == W[i].rb,
+            "MPO file: bond dimensions and quantum tables do not match", kErrArgument);
+  basic_ = std::move(basic);
+  basic_im_ = std::move(basic_im);
+  qt_ = std::move(qt);
+  W_ = std::move(W);
+}
+
 void RealTensorHamiltonian::ExpanTensorHamiltonian(int site, int op_index, double cr, double ci,
                                                    const QN& table) {
   // RealTensorOperator::ExpanTensorOperator (real_tensor_operator.cc:141-197;
@@ -272,6 +389,11 @@ void RealTensorHamiltonian::ExpanTensorHamiltonian(int site, int op_index, doubl
 }
 
 void RealTensorHamiltonian::ParallelTensorHamiltonian() {
+  // nothing to deparallelise while a site still has no term (a lattice
+  // without any bond or on-site term leaves the MPO empty; the model
+  // constructors refuse that)
+  for (const SiteOperator& W : W_)
+    if (W.lb == 0 || W.rb == 0) return;
   if (cplx_) parallel_sweep<zd>(W_, qt_, num_site_, d_);
   else parallel_sweep<double>(W_, qt_, num_site_, d_);
 }

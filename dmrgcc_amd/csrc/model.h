@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstddef>
+#include <string>
 #include <vector>
 
 namespace dmrg {
@@ -78,6 +79,20 @@ class RealTensorHamiltonian {
   const std::vector<SiteOperator>& sites() const { return W_; }
   const std::vector<QNTable>& bond_tables() const { return qt_; }
 
+  // MPO files: RealTensorHamiltonian::Write/ReadTensorHamiltonian
+  // (real_tensor_hamiltonian.cc:74-150) with RealTensorOperator::Write/
+  // ReadTensorOperator (real_tensor_operator.cc:96-139).  Layout: int
+  // num_site, num_site+1, num_site-1; int num_operator and the basic
+  // operators (RealMatrix::WriteMatrix: rows, cols, rows*cols, entries);
+  // int num_quantum, the bond table sizes and entries; per site int
+  // physics_dim, left_bond, right_bond and the left_bond*right_bond matrices
+  // W[o1][o2] (d x d).  Entries are 8-byte doubles, or 16-byte Complex on a
+  // complex Hamiltonian; native byte order.  Reading replaces this
+  // Hamiltonian's MPO (same number of sites, quantum numbers and arithmetic
+  // kind required).
+  void WriteTensorHamiltonian(const std::string& path) const;
+  void ReadTensorHamiltonian(const std::string& path);
+
  protected:
   void Init(int num_site, int physics_dim, int num_quantum, std::vector<std::vector<double>> basic);
   // ExpanTensorHamiltonian (real_tensor_hamiltonian.cc:151-187; complex
@@ -93,6 +108,9 @@ class RealTensorHamiltonian {
   bool cplx_ = false;  // set by complex models before DefineTensorHamiltonian
   int num_site_ = 0, d_ = 0, nq_ = 0;
   std::vector<std::vector<double>> basic_;  // d*d row-major operators
+  // imaginary parts of the basic operators of a complex MPO read from a
+  // file (the models' own basic operators are real); empty otherwise
+  std::vector<std::vector<double>> basic_im_;
   std::vector<SiteOperator> W_;
   std::vector<QNTable> qt_;
 };

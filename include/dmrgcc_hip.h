@@ -135,6 +135,20 @@ int dmrgcc_input_get(const char* path, const char* group, const char* key, int k
  * the value into buf (NUL-terminated, buflen bytes), *found = 0/1. */
 int dmrgcc_input_string(const char* path, const char* key, char* buf, int buflen, int* found);
 int dmrgcc_model_destroy(dmrgcc_model* m);
+/* MPO files: RealTensorHamiltonian::WriteTensorHamiltonian /
+ * ReadTensorHamiltonian (real_tensor_hamiltonian.cc:74-150, the site
+ * operators as RealTensorOperator::WriteTensorOperator,
+ * real_tensor_operator.cc:96-139): int num_site, num_site+1, num_site-1;
+ * int num_operator, the basic operators (rows, cols, rows*cols, entries);
+ * int num_quantum, the MPO bond table sizes and entries; per site int
+ * physics_dim, left_bond, right_bond and the W[o1][o2] d x d matrices.
+ * Entries are doubles, 16-byte Complex for a complex model; native order.
+ * read replaces the model's MPO (same sites, quantum numbers and kind).
+ * The reference's reader never allocates the basic operators it reads
+ * into (real_tensor_hamiltonian.cc:132) and does not read the site
+ * operators back; this one reads both. */
+int dmrgcc_model_write_mpo(const dmrgcc_model* m, const char* path);
+int dmrgcc_model_read_mpo(dmrgcc_model* m, const char* path);
 int dmrgcc_model_num_site(const dmrgcc_model* m);
 /* Number of sites, local dimension, QNs per sector and whether the MPO is
  * complex (any pointer may be NULL). */
@@ -414,6 +428,14 @@ int dmrgcc_dmrg_create_files(dmrgcc_model* m, int first_block, int first_dim, in
  * Complex, native byte order. */
 int dmrgcc_dmrg_write_space(dmrgcc_dmrg* h, const char* path);
 int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached);
+/* Environment files: RealTensorContraction::WriteTensorContraction(leigh,
+ * name) (real_tensor_contraction.cc:73-90) of `site` on the current state
+ * (leigh 0: its left environment L_site, 1: its right environment R_site):
+ * per channel the block count and per block left, right, rows, cols,
+ * rows*cols and the entries.  env_file_read: ReadTensorContraction
+ * (:92-109) of nbond channels into a block handle (channel = o). */
+int dmrgcc_dmrg_write_env(dmrgcc_dmrg* h, int site, int leigh, const char* path);
+int dmrgcc_env_file_read(const char* path, int nbond, int is_complex, dmrgcc_blocks** out);
 int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine);
 int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
