@@ -30,8 +30,25 @@ or index, or a transposed operand):
                     sigma([[1,2,3],[4,5,6]]) closed form.
 * ``ed``         -- textbook Fock-space exact diagonalisation; pinned by
                     closed forms (two sites, U=0 tight-binding spectrum).
+* ``dmrg``, the sweep-level path (tests/test_oracle_sweep.py) -- the
+                    expansion keeps every original block unchanged as the
+                    leading part of the enlarged one, appends the expansion
+                    block and pads the rest with zeros; its new sectors are
+                    QN merges of the MPO channels with the old sectors, in
+                    table order, at most max_block; the noise factor is a pure
+                    scale (same draws); every expansion column lies in the span
+                    of the L K products that may feed it; with expansion on or
+                    off and no truncation, the state reaching the next site has
+                    the Rayleigh quotient the Lanczos solve found (expansion,
+                    canonical form, match and merge preserve the physical
+                    state).  Lanczos: the Krylov basis is orthonormal to
+                    1e-12 and alpha / beta are H's matrix elements in it.
+* ``io``         -- hand-written byte fixtures in the reference's field order
+                    (tensor-space, environment and MPO files) read back
+                    value by value and rewritten byte for byte.
 
-Parity unpinned: exact per-step trajectories at finite m (truncation at
-near-ties, expansion sector order), since the reference cannot be built here
-(it needs ``mkl.h``, which this image lacks).
+Parity unpinned: the order in which the reference pairs two rand() draws in
+one complex entry (C++ leaves it unspecified; g++ is followed) and
+truncation decisions at near-ties of singular values, since the reference
+cannot be built here (it needs ``mkl.h``, which this image lacks).
 """

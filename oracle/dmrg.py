@@ -299,11 +299,12 @@ def sym_eig(A):
     return w, v * fix_signs(v)[None, :]
 
 
-def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
+def lanczos(lat, W, Lenv, Renv, num_iter, stats=None, trace=None):
     """LanczosMethod (real_tensor_lanczos.cc:14-169; complex
     complex_tensor_lanczos.cc:14-160: complex alpha subtracted, beta real,
     the tridiagonal's real diagonal diagonalised).  Writes the Ritz vector
-    into lat.ket and returns the energy."""
+    into lat.ket and returns the energy.  With a dict `trace`, the Krylov
+    vectors and the alpha / beta of every iteration are stored there."""
     psi_dim = lat.ket.dim()
     if psi_dim < 100:
         H = heff_dense(lat, W, Lenv, Renv)
@@ -335,6 +336,8 @@ def lanczos(lat, W, Lenv, Renv, num_iter, stats=None):
         T = np.diag(alpha) + np.diag(beta[:k - 1], 1) + np.diag(beta[:k - 1], -1)
         ev, vec = sym_eig(T)
         new = float(ev[0])
+        if trace is not None:
+            trace["psi"], trace["alpha"], trace["beta"] = [v.copy() for v in psi], list(alpha), list(beta)
         if b < 1e-15 or abs(new - old) < 1e-15 or k == num_iter:
             energy = new
             res = np.zeros(psi_dim, dtype=h.dtype)

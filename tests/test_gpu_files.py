@@ -78,3 +78,19 @@ def test_oracle_and_fixture_environment_files_read_by_library(tmp_path):
         assert np.array_equal(got[0][0][2], a) and np.array_equal(got[0][1][2], b)
     with pytest.raises(dm.DmrgError, match="trailing"):
         dm.env_file_read(str(q), 1, True)
+
+
+def test_library_resumes_the_hand_written_space_file(tmp_path):
+    # cache_resume from the byte fixture: the library and the oracle read the
+    # same file and run the same sweeps (no noise) to the same energies
+    from oracle.dmrg import DMRG
+    from tests.test_oracle_sweep import space_fixture
+    _, raw, _ = space_fixture()
+    p = tmp_path / "fixture_space.dat"
+    p.write_bytes(raw)
+    rows = sweep_rows(20, 4, niter=6, canpre=1e-10, noise=[-1] * 4)
+    r = dm.SpinlessModel(2, 1, 2, 3, hop_x=-1.0, U=1.0).run(rows, first_block=20, first_dim=4, resume=str(p))
+    from oracle.model import SpinlessBoseSquare
+    o = DMRG(SpinlessBoseSquare(2, 1, 2, 3, jx=-1.0, U=1.0), rows, 2, resume=str(p))
+    e = np.array(o.run())
+    assert np.abs(r["energies"] - e).max() < 1e-12 * np.abs(e).max()
