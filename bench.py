@@ -270,6 +270,8 @@ def main():
 
     # ---- FP64 run, stopped at the centre site of the last half sweep ----
     run64 = model.start(tab, precision=dm.FP64)
+    if not args.no_tune:
+        run64.set_autotune(True)   # the tile autotuner inside the sweep
     t0 = time.time()
     reached = drive(run64, L, nhalf, "FP64", (nhalf - 1, site))
     t_to_site = time.time() - t0
@@ -325,6 +327,8 @@ def main():
     rmx = None
     if not args.no_mixed_sweep:
         runmx = model.start(tab, precision=dm.BF16X3, refine=True)
+        if not args.no_tune:
+            runmx.set_autotune(True)
         drive(runmx, L, nhalf, "mixed")
         rmx = runmx.result()
         log("mixed run done: %.1f s" % rmx["total_seconds"])
@@ -386,6 +390,7 @@ def main():
             "fp64_phase_seconds": r64["phase_seconds"],
             "fp64_lanczos": {k: r64[k] for k in ("solves", "matvecs", "matvec_seconds")},
             "max_bond": r64["max_bond"],
+            "fp64_sweep_autotune": r64["autotune"],
             "rand_draws": r64["rand_draws"],
         },
         # dominant kernel: the stage-2 GEMM launch (out += w L T), timed alone
@@ -418,6 +423,7 @@ def main():
         out["sweep"]["mixed_total_seconds"] = rmx["total_seconds"]
         out["sweep"]["mixed_phase_seconds"] = rmx["phase_seconds"]
         out["sweep"]["mixed_lanczos"] = {k: rmx[k] for k in ("solves", "matvecs", "matvec_seconds")}
+        out["sweep"]["mixed_sweep_autotune"] = rmx["autotune"]
         out["sweep"]["mixed_energies"] = [float(e) for e in emx]
         out["sweep"]["dE_rel"] = [float(v) for v in np.abs(emx - e64) / np.abs(e64)]
     print(json.dumps(out), flush=True)

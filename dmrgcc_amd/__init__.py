@@ -98,6 +98,7 @@ _SIGS = {
     "dmrgcc_heff_apply_async": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                _vp]),
     "dmrgcc_alloc": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "dmrgcc_dmrg_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64]),
     "dmrgcc_model_write_mpo": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "dmrgcc_model_read_mpo": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "dmrgcc_dmrg_write_env": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
@@ -512,7 +513,7 @@ def _result_dict(e, t, s, st):
                               "canonical": st[11], "svd": st[12], "env": st[13],
                               "match_merge": st[14], "plan_build": st[19]},
             "svd_max_m": int(st[15]), "svd_max_n": int(st[16]), "svd_calls": int(st[17]),
-            "svd_work": st[18]}
+            "svd_work": st[18], "autotune": {"tunes": int(st[20]), "tiling": (int(st[21]), int(st[22]))}}
 
 
 class DmrgRun:
@@ -547,6 +548,11 @@ class DmrgRun:
 
     def set_precision(self, precision, refine=True):
         _check(lib().dmrgcc_dmrg_set_precision(self._h, precision, int(refine)))
+
+    def set_autotune(self, on=True, min_dim=0):
+        """Tile autotuner inside the sweep (see dmrgcc_dmrg_set_autotune);
+        plans with psi_dim below ``min_dim`` (20000 when 0) keep their tiling."""
+        _check(lib().dmrgcc_dmrg_set_autotune(self._h, int(on), int(min_dim)))
 
     def heff(self, site):
         h = _vp()

@@ -254,6 +254,9 @@ void fill_result(const RunResult& r, int* nhalf_done, double* energies, double* 
     stats[15] = (double)ph.svd_max_m; stats[16] = (double)ph.svd_max_n;
     stats[17] = (double)ph.svd_calls; stats[18] = ph.svd_work;
     stats[19] = ph.plan;
+    stats[20] = r.stats.tunes;
+    stats[21] = r.stats.tile_mode;
+    stats[22] = r.stats.split_k;
   }
 }
 }  // namespace
@@ -1131,6 +1134,13 @@ int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine) {
   return guarded([&] {
     require(h != nullptr && precision >= 0 && precision <= 2, "bad arguments", kErrArgument);
     h->run->set_precision(precision, refine);
+  });
+}
+
+int dmrgcc_dmrg_set_autotune(dmrgcc_dmrg* h, int on, int64_t min_dim) {
+  return guarded([&] {
+    require(h != nullptr, "null handle", kErrArgument);
+    h->run->set_autotune(on != 0, min_dim);
   });
 }
 

@@ -261,8 +261,25 @@ Context::Context(hipStream_t s) : stream(s) {
 
 Context::~Context() {
   upload_forget_stream(stream);
+  if (!events_.empty()) (void)hipStreamSynchronize(stream);
+  for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+  if (lz_host_) (void)hipHostFree(lz_host_);
   if (pinned_) (void)hipHostFree(pinned_);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+double* Context::lanczos_host() {
+  if (!lz_host_) DMRG_HIP(hipHostMalloc((void**)&lz_host_, sizeof(double) * (4 * 200 + 200 + 8)));
+  return lz_host_;
+}
+
+std::vector<hipEvent_t> Context::events(int n) {
+  while ((int)events_.size() < n) {
+    hipEvent_t e;
+    DMRG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    events_.push_back(e);
+  }
+  return std::vector<hipEvent_t>(events_.begin(), events_.begin() + n);
 }
 
 double* Context::pinned(int64_t n) {
@@ -496,8 +513,8 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   // may share a sector pair (the two-site state, one block per middle sector).
   s1_.clear();
   s2_.clear();
-  s1_.split_k = s2_.split_k = default_split_k();
-  s1_.tile_mode = s2_.tile_mode = default_tile_mode();
+  s1_.split_k = s2_.split_k = want_split_ >= 0 ? want_split_ : default_split_k();
+  s1_.tile_mode = s2_.tile_mode = want_tile_ >= 0 ? want_tile_ : default_tile_mode();
   psi_dim_ = ket.size;
   // complex plans (ComplexTensorContraction::MultiplyEffectHamilton,
   // complex_tensor_contraction.cc:1244-1312): same task structure, every
@@ -1491,7 +1508,21 @@ void RealTensorRundmrg::Initialize() {
 double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
   HeffPlan plan;
   const double t0 = now();
+  const int64_t n = lat_[site].ket.size;
+  const bool tune = p_.autotune && n >= p_.tune_min_dim && n >= 4 * tuned_psi_;
+  if (!tune) plan.set_tiling(tune_tile_, tune_split_);
   plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
+  if (tune) {
+    // the plan ends up tiled with the fastest candidate; the trial matvecs
+    // run on scratch vectors, the state is untouched
+    plan.tune(p_.precision, 3, cx_.stream);
+    tune_tile_ = plan.tile_mode() == 32 ? 32 : 64;
+    tune_split_ = plan.split_k();
+    tuned_psi_ = n;
+    ++stats_.tunes;
+    stats_.tile_mode = tune_tile_;
+    stats_.split_k = tune_split_;
+  }
   ph_.plan += now() - t0;
   return lanczos_solve(plan, lat_[site].ket.data.data(), num_iter, p_.precision, p_.refine, cx_,
                        stats_);
@@ -1558,10 +1589,17 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     return e0;
   }
   require(num_iter >= 1 && num_iter <= 200, "num_iter must be in [1, 200]", kErrArgument);
-  cx.scal.ensure(256);
+  // Device scalars: per iteration i, slot sc[4 i ..] = (alpha re, im, beta^2);
+  // then the Ritz coefficients and the refinement's two dots.  The host reads
+  // each slot through the context's pinned Lanczos buffer.
+  constexpr int kSlot = 4;
+  cx.scal.ensure((size_t)kSlot * num_iter + num_iter + 8);
+  double* sc = cx.scal.data();
+  double* coef_dev = sc + kSlot * num_iter;
+  double* rq_dev = coef_dev + num_iter;
+  double* host = cx.lanczos_host();
   DevBuf<double> psi((size_t)num_iter * nd), h((size_t)nd);
   double* P = psi.data();
-  double* sc = cx.scal.data();
   DMRG_HIP(hipMemcpyAsync(P, ket, nd * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
   fill(h.data(), nd, 0.0, cx.stream);
   std::vector<double> alpha, beta;
@@ -1571,22 +1609,58 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
   cx.partial.ensure(3 * 2 * kDotPartials);
   double* part_a = cx.partial.data();
   double* part_b = part_a + 2 * kDotPartials;
-  for (int i = 0; i < num_iter; ++i) {
+  std::vector<hipEvent_t> done = cx.events(num_iter);
+  // psi_{i+1} = h / beta_i, h = -beta_i psi_i, then GramSchmidtMethod
+  // (real_tensor_lanczos.cc:140-151, :188-201; complex_tensor_lanczos.cc:
+  // 178-192): for j = 0..i, c = <psi_j|psi_{i+1}> on the updated vector,
+  // psi_{i+1} -= c psi_j; then psi_{i+1} /= |psi_{i+1}|.  One launch per
+  // projection: each forms c from the previous launch's partial sums and the
+  // partials of the next overlap.  beta_i is read on the device.
+  auto next_vector = [&](int i) {
+    double* pn = P + (int64_t)(i + 1) * nd;
+    double* pin = part_a;
+    double* pout = part_b;
+    lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, sc + kSlot * i + 2);
+    for (int j = 0; j <= i; ++j) {
+      const double* xnext = j < i ? P + (int64_t)(j + 1) * nd : nullptr;
+      lz_project(pn, P + (int64_t)j * nd, xnext, n, cz, pin, pout, cx.stream);
+      std::swap(pin, pout);
+    }
+    lz_normalize(pn, n, cz, pin, cx.stream);
+  };
+  // iteration i on the stream: psi_i (when i > 0), h += H psi_i,
+  // alpha = <psi_i|h>, h -= alpha psi_i, beta^2 = <h|h>, then the three
+  // scalars to pinned host memory behind event done[i]
+  auto launch_iteration = [&](int i) {
+    if (i > 0) next_vector(i - 1);
     double* pi = P + (int64_t)i * nd;
-    const double t0 = now();
     plan.apply(pi, h.data(), precision, cx.stream);
-    lz_dot_partial(pi, h.data(), n, cz, part_a, cx.stream);     // alpha = <psi|H|psi>
-    lz_alpha(h.data(), pi, n, cz, part_a, part_b, cx.stream);    // h -= alpha psi; <h|h>
-    lz_finals(part_a, part_b, n, cz, sc, cx.stream);             // sc = (alpha, beta^2)
-    double ab[3];
-    DMRG_HIP(hipMemcpyAsync(ab, sc, 3 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
-    cx.sync();
+    lz_dot_partial(pi, h.data(), n, cz, part_a, cx.stream);   // alpha = <psi|H|psi>
+    lz_alpha(h.data(), pi, n, cz, part_a, part_b, cx.stream);  // h -= alpha psi; <h|h>
+    lz_finals(part_a, part_b, n, cz, sc + kSlot * i, cx.stream);
+    DMRG_HIP(hipMemcpyAsync(host + kSlot * i, sc + kSlot * i, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            cx.stream));
+    DMRG_HIP(hipEventRecord(done[i], cx.stream));
+  };
+  // The host decides the reference's stop for iteration i while the device
+  // already runs iteration i + 1 (launched before the host waits).  When i
+  // stops, that iteration is discarded: it writes only psi_{i+1}, h and its
+  // own slot, which the Ritz vector and the refinement do not read (the
+  // refinement clears h first).  DMRGCC_LANCZOS_BLOCKING=1 launches each
+  // iteration only after the previous stop test (same results, no overlap).
+  const bool blocking = std::getenv("DMRGCC_LANCZOS_BLOCKING") != nullptr;
+  launch_iteration(0);
+  for (int i = 0; i < num_iter; ++i) {
+    const int k = i + 1;
+    const double t0 = now();
+    if (!blocking && k < num_iter) launch_iteration(k);
+    DMRG_HIP(hipEventSynchronize(done[i]));
     st.matvec_seconds += now() - t0;
     st.flops += plan.fref();
     ++st.matvecs;
+    const double* ab = host + kSlot * i;
     alpha.push_back(ab[0]);
     beta.push_back(std::sqrt(ab[2]));
-    const int k = i + 1;
     std::vector<double> T((size_t)k * k, 0.0), ev, vec;
     for (int j = 0; j < k; ++j) {
       T[(size_t)j * k + j] = alpha[j];
@@ -1606,45 +1680,27 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     if (beta[i] < 1e-15 || std::fabs(e_new - old_energy) < de_tol || k == num_iter) {
       energy = e_new;
       if (iters) *iters = k;
-      std::vector<double> coef(k);
+      double* coef = host + kSlot * num_iter;
       for (int j = 0; j < k; ++j) coef[j] = vec[(size_t)j * k];
-      DMRG_HIP(hipMemcpyAsync(sc + 16, coef.data(), k * sizeof(double), hipMemcpyHostToDevice, cx.stream));
-      combine(ket, P, nd, sc + 16, k, nd, cx.stream);
+      DMRG_HIP(hipMemcpyAsync(coef_dev, coef, k * sizeof(double), hipMemcpyHostToDevice, cx.stream));
+      combine(ket, P, nd, coef_dev, k, nd, cx.stream);
       if (precision != kFP64 && refine) {
         // FP64 Rayleigh-quotient refinement of the mixed-precision Ritz pair
         fill(h.data(), nd, 0.0, cx.stream);
         plan.apply(ket, h.data(), kFP64, cx.stream);
-        dot(ket, h.data(), nd, sc + 3, cx.partial, cx.stream);   // Re <ket|H ket>
-        dot(ket, ket, nd, sc + 4, cx.partial, cx.stream);
-        double rq[2];
-        DMRG_HIP(hipMemcpyAsync(rq, sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
+        dot(ket, h.data(), nd, rq_dev, cx.partial, cx.stream);   // Re <ket|H ket>
+        dot(ket, ket, nd, rq_dev + 1, cx.partial, cx.stream);
+        DMRG_HIP(hipMemcpyAsync(host, rq_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, cx.stream));
         cx.sync();
         st.flops += plan.fref();
         ++st.matvecs;
-        energy = rq[0] / rq[1];
+        energy = host[0] / host[1];
       }
       cx.sync();
       break;
     }
     old_energy = e_new;
-    if (k < num_iter) {
-      double* pn = P + (int64_t)(i + 1) * nd;
-      // psi_{i+1} = h / beta, h = -beta psi_i, then GramSchmidtMethod
-      // (real_tensor_lanczos.cc:140-151, :188-201; complex_tensor_lanczos.cc:178-192):
-      // for j = 0..i, c = <psi_j|psi_{i+1}> on the updated vector, psi_{i+1} -= c psi_j;
-      // then psi_{i+1} /= |psi_{i+1}|.  One launch per projection: each
-      // forms c from the previous launch's partial sums and the partials
-      // of the next overlap.
-      double* pin = part_a;
-      double* pout = part_b;
-      lz_next(pn, h.data(), pi, P, n, cz, beta[i], pin, cx.stream);
-      for (int j = 0; j <= i; ++j) {
-        const double* xnext = j < i ? P + (int64_t)(j + 1) * nd : nullptr;
-        lz_project(pn, P + (int64_t)j * nd, xnext, n, cz, pin, pout, cx.stream);
-        std::swap(pin, pout);
-      }
-      lz_normalize(pn, n, cz, pin, cx.stream);
-    }
+    if (blocking && k < num_iter) launch_iteration(k);
   }
   return energy;
 }

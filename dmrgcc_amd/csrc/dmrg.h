@@ -102,10 +102,18 @@ struct Context {
   // pinned host staging of at least n doubles, reused between calls (grown
   // when needed; growing discards the contents)
   double* pinned(int64_t n);
+  // pinned host memory of the Lanczos loop (4 * 200 + 200 doubles: per
+  // iteration alpha and beta^2, then the Ritz coefficients), separate from
+  // the staging above so that it never waits for another copy
+  double* lanczos_host();
+  // at least n events (timing disabled), reused between calls
+  std::vector<hipEvent_t> events(int n);
 
  private:
   double* pinned_ = nullptr;
   int64_t pinned_n_ = 0;
+  double* lz_host_ = nullptr;
+  std::vector<hipEvent_t> events_;
 };
 
 // H_eff matvec plan of one site (MultiplyEffectHamilton,
@@ -149,9 +157,12 @@ class HeffPlan {
   int split_k() const { return s1_.split_k; }
   // bumped by every build and retile: batches built from the plan check it
   int generation() const { return gen_; }
+  // tiling of the next build (-1: the process-wide default of that field)
+  void set_tiling(int tile_mode, int split_k) { want_tile_ = tile_mode; want_split_ = split_k; }
 
  private:
   GemmBatch s1_, s2_;
+  int want_tile_ = -1, want_split_ = -1;
   // items and terms of both stages before tiling, kept for retile()
   std::vector<GemmItem> items1_, items2_;
   std::vector<GemmContrib> terms1_, terms2_;
@@ -256,6 +267,7 @@ std::vector<int> truncate_dims(int max_dim, double canpre, std::vector<std::vect
 
 struct LanczosStats {
   int matvecs = 0, dense = 0, solves = 0;
+  int tunes = 0, tile_mode = 0, split_k = 0;  // sweep autotuner: runs and the last choice
   double matvec_seconds = 0, flops = 0;
 };
 
@@ -271,6 +283,12 @@ struct RunParams {
   std::vector<int> maxm, maxb, niter;          // per half sweep
   std::vector<double> canpre, noise;           // noise already 0.1^x
   int precision = kFP64;  // H_eff arithmetic inside Lanczos
+  // tile autotuner inside the sweep: the H_eff plan of the first site whose
+  // psi_dim reaches tune_min_dim, and of every later site whose psi_dim is 4x
+  // the last tuned one, is re-tiled with each candidate and timed; later
+  // plans are built with the fastest tiling
+  bool autotune = false;
+  int64_t tune_min_dim = 20000;
   int refine = 1;         // mixed precision: final FP64 Rayleigh quotient
   int verbose = 0;
   // cache_resume: start from this tensor-space file instead of
@@ -332,6 +350,10 @@ class RealTensorRundmrg {
   int num_site() const { return L_sites_; }
   bool complex() const { return cs_ == 2; }
   void set_precision(int precision, int refine) { p_.precision = precision; p_.refine = refine; }
+  void set_autotune(bool on, int64_t min_dim) {
+    p_.autotune = on;
+    p_.tune_min_dim = min_dim > 0 ? min_dim : 20000;
+  }
   // H_eff plan on the current state of `site` (valid when the site is next).
   void BuildHeff(int site, HeffPlan& plan) {
     plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
@@ -403,6 +425,8 @@ class RealTensorRundmrg {
   double canpre_ = 1e-30, noise_ = 10;
   LanczosStats stats_;
   PhaseTimes ph_;
+  int64_t tuned_psi_ = 0;                  // psi_dim of the last autotuned plan
+  int tune_tile_ = -1, tune_split_ = -1;   // its choice, used by later plans
 };
 
 }  // namespace dmrg

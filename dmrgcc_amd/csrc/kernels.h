@@ -68,6 +68,10 @@ int default_split_k();
 // Tile shape of finalize(): 32 or 64 (0 means 64), from set_gemm_config, else
 // the environment variable DMRGCC_TILE.
 int default_tile_mode();
+// Workgroup cap of a persistent launch (0 = one workgroup per tile): from
+// set_gemm_grid_cap, else the environment variable DMRGCC_GRID_CAP, else 0.
+int default_grid_cap();
+void set_gemm_grid_cap(int cap);
 // Process-wide GEMM configuration for batches built from now on (the tile
 // autotuner's choice); a negative value restores the environment / built-in
 // default of that field.
@@ -111,6 +115,10 @@ struct GemmBatch {
   // 64x64 workgroup tiles (environment DMRGCC_TILE)
   int tile_mode = default_tile_mode();
   bool wave_tiles = false;  // set by finalize: four 32x32 tiles per workgroup
+  // persistent grid: at most grid_cap workgroups walk the tiles with a
+  // static stride (0: one workgroup per tile, or per four wave tiles)
+  int grid_cap = default_grid_cap();
+  unsigned launch_grid(int units) const;
   double flops = 0;  // executed real flops (2*M*N*K summed)
   // Groups: end_group() closes the items added since the previous call into
   // one group that launch_group() runs on its own (a chain of dependent
@@ -235,9 +243,10 @@ void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* 
 // out[0..1] = final(part_a) (alpha; real: out[1] = 0), out[2] = Re final(part_b) (beta^2)
 void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
                hipStream_t stream);
-// nxt = h / beta; h = -beta psi; part_out <- partials of <x0|nxt>
+// nxt = h / beta; h = -beta psi; part_out <- partials of <x0|nxt>.  With
+// beta2 (device), beta = sqrt(*beta2) is formed on the device instead.
 void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream);
+             double beta, double* part_out, hipStream_t stream, const double* beta2 = nullptr);
 // c = final(part_in); y -= c xj; part_out <- partials of <xnext|y> (xnext null: <y|y>)
 void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
                 const double* part_in, double* part_out, hipStream_t stream);

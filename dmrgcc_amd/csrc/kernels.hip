@@ -212,18 +212,23 @@ __device__ inline void store_step(const GemmContrib& cb, int tid, const StepRegs
   }
 }
 
-template <typename T>
+template <typename T, bool kLoop = false>
 __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
-    double* __restrict__ out_base) {
+    double* __restrict__ out_base, int ntiles) {
   constexpr int LD = kTile + 2;  // padded LDS row
   __shared__ T As[kKT][LD];
   __shared__ T Bs[kKT][LD];
   typedef typename MfmaAcc<T>::V V;
 
-  const GemmTile tl = tiles[blockIdx.x];
-  if (tl.item < 0) return;  // padding of the XCD-interleaved order
+  // one tile per workgroup, or (persistent grid) tiles blockIdx.x, +gridDim.x, ...
+  for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+  const GemmTile tl = tiles[tix];
+  if (tl.item < 0) {  // padding of the XCD-interleaved order
+    if constexpr (kLoop) continue;
+    else return;
+  }
   const GemmItem it = items[tl.item];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -287,6 +292,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(
           *p = it.accumulate ? (*p + r2) : r2;
         }
       }
+  if constexpr (!kLoop) break;  // one tile per workgroup: no loop at all
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -390,10 +397,11 @@ __device__ inline void store_planes(bool kcont, int tid, int kofs, const double 
   }
 }
 
+template <bool kLoop = false>
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
-    double* __restrict__ out_base) {
+    double* __restrict__ out_base, int ntiles) {
   constexpr int KT = kBKT, NR = KT / 32;
   constexpr int LD = KT + 8;  // rows of 80 or 144 bytes keep 16-byte alignment
   __shared__ __attribute__((aligned(16))) __bf16 Ah[kTile][LD];
@@ -401,8 +409,12 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
   __shared__ __attribute__((aligned(16))) __bf16 Bh[kTile][LD];
   __shared__ __attribute__((aligned(16))) __bf16 Bl[kTile][LD];
 
-  const GemmTile tl = tiles[blockIdx.x];
-  if (tl.item < 0) return;  // padding of the XCD-interleaved order
+  for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+  const GemmTile tl = tiles[tix];
+  if (tl.item < 0) {  // padding of the XCD-interleaved order
+    if constexpr (kLoop) continue;
+    else return;
+  }
   const GemmItem it = items[tl.item];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -481,6 +493,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
           *p = it.accumulate ? (*p + r2) : r2;
         }
       }
+  if constexpr (!kLoop) break;  // one tile per workgroup: no loop at all
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -619,19 +633,24 @@ __device__ inline void wave_store_out(const GemmItem& it, double* __restrict__ o
       }
 }
 
-template <typename T>
+template <typename T, bool kLoop = false>
 __global__ __launch_bounds__(kThreads) void gemm_wave_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
-    double* __restrict__ out_base) {
+    double* __restrict__ out_base, int ntiles) {
   constexpr int LD = kWT + 2;
   __shared__ T As[4][kKT][LD];  // k-major, per wave
   __shared__ T Bs[4][kKT][LD];
   typedef typename MfmaAcc<T>::V V;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const GemmTile tl = tiles[blockIdx.x * 4 + wave];
-  if (tl.item < 0) return;  // no block barriers: a wave without a tile leaves
+  // ntiles counts groups of four tiles (one per wave)
+  for (int gix = blockIdx.x; gix < ntiles; gix += gridDim.x) {
+  const GemmTile tl = tiles[gix * 4 + wave];
+  if (tl.item < 0) {  // no block barriers: a wave without a tile skips it
+    if constexpr (kLoop) continue;
+    else return;
+  }
   const GemmItem it = items[tl.item];
   const int M = it.M, N = it.N, m0 = tl.m0, n0 = tl.n0;
   T (*as)[LD] = As[wave];
@@ -676,14 +695,17 @@ __global__ __launch_bounds__(kThreads) void gemm_wave_kernel(
     c = c2; k0 = k2; cb = nc;
   }
   wave_store_out<T>(it, out_base, m0, n0, lane, acc);
+  if constexpr (!kLoop) break;
+  }
 }
 
 // BF16x3 wave-tile kernel: per-wave hi/lo planes m-major with k contiguous,
 // as in gemm_bf16x3_kernel.
+template <bool kLoop = false>
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
     const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
-    double* __restrict__ out_base) {
+    double* __restrict__ out_base, int ntiles) {
   constexpr int LD = kKT + 8;
   __shared__ __attribute__((aligned(16))) __bf16 Ah[4][kWT][LD];
   __shared__ __attribute__((aligned(16))) __bf16 Al[4][kWT][LD];
@@ -691,8 +713,12 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
   __shared__ __attribute__((aligned(16))) __bf16 Bl[4][kWT][LD];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const GemmTile tl = tiles[blockIdx.x * 4 + wave];
-  if (tl.item < 0) return;
+  for (int gix = blockIdx.x; gix < ntiles; gix += gridDim.x) {
+  const GemmTile tl = tiles[gix * 4 + wave];
+  if (tl.item < 0) {
+    if constexpr (kLoop) continue;
+    else return;
+  }
   const GemmItem it = items[tl.item];
   const int M = it.M, N = it.N, m0 = tl.m0, n0 = tl.n0;
   __bf16 (*ah_)[LD] = Ah[wave];
@@ -748,6 +774,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
     c = c2; k0 = k2; cb = nc;
   }
   wave_store_out<float>(it, out_base, m0, n0, lane, acc);
+  if constexpr (!kLoop) break;
+  }
 }
 
 // split-K reduction: C (+)= sum of the partial buffers, in chunk order
@@ -1070,7 +1098,11 @@ template <bool kC>
 __global__ __launch_bounds__(256) void lz_next_kernel(double* __restrict__ nxt, double* __restrict__ h,
                                                       const double* __restrict__ psi,
                                                       const double* __restrict__ x0, int64_t n,
-                                                      double beta, double* __restrict__ part_out) {
+                                                      double beta, const double* __restrict__ beta2,
+                                                      double* __restrict__ part_out) {
+  // beta from the device (sqrt of beta^2, correctly rounded like the
+  // host's), when the host has not read it yet
+  if (beta2) beta = sqrt(beta2[0]);
   double sr = 0.0, si = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -2066,6 +2098,19 @@ int default_split_k() {
   return e ? std::max(0, std::atoi(e)) : 256;
 }
 
+namespace {
+std::atomic<int> g_grid_cap{-1};
+}  // namespace
+
+int default_grid_cap() {
+  const int g = g_grid_cap.load();
+  if (g >= 0) return g;
+  const char* e = std::getenv("DMRGCC_GRID_CAP");
+  return e ? std::max(0, std::atoi(e)) : 0;
+}
+
+void set_gemm_grid_cap(int cap) { g_grid_cap.store(cap < 0 ? -1 : cap); }
+
 int default_tile_mode() {
   const int g = g_tile_mode.load();
   if (g >= 0) return g;
@@ -2282,33 +2327,64 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   return psize;
 }
 
+namespace {
+// one launch of the 64x64 (launch_tile) or wave-tile (launch_wave) kernels;
+// kLoop: the persistent form (each workgroup walks tiles with a stride of the
+// grid)
+template <bool kLoop>
+void launch_tile(int precision, dim3 grid, dim3 block, hipStream_t stream, const GemmTile* t, const GemmItem* it,
+                 const GemmContrib* c, const double* in, double* out, int n) {
+  if (precision == kFP64)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_mfma_kernel<double, kLoop>), grid, block, 0, stream, t, it, c, in, out,
+                       n);
+  else if (precision == kFP32)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_mfma_kernel<float, kLoop>), grid, block, 0, stream, t, it, c, in, out,
+                       n);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_bf16x3_kernel<kLoop>), grid, block, 0, stream, t, it, c, in, out, n);
+}
+template <bool kLoop>
+void launch_wave(int precision, dim3 grid, dim3 block, hipStream_t stream, const GemmTile* t, const GemmItem* it,
+                 const GemmContrib* c, const double* in, double* out, int n) {
+  if (precision == kFP64)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_wave_kernel<double, kLoop>), grid, block, 0, stream, t, it, c, in, out,
+                       n);
+  else if (precision == kFP32)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_wave_kernel<float, kLoop>), grid, block, 0, stream, t, it, c, in, out,
+                       n);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(gemm_bf16x3_wave_kernel<kLoop>), grid, block, 0, stream, t, it, c, in, out,
+                       n);
+}
+}  // namespace
+
+unsigned GemmBatch::launch_grid(int units) const {
+  // a persistent grid of at most grid_cap workgroups (a multiple of the XCD
+  // count, so that tile t stays on XCD t % 8) walks the tile list with a
+  // stride of the grid; grid_cap 0 launches one workgroup per tile
+  if (grid_cap <= 0 || units <= grid_cap) return (unsigned)std::max(units, 1);
+  return (unsigned)std::max(kXcds, grid_cap - grid_cap % kXcds);
+}
+
 void GemmBatch::launch(hipStream_t stream, int precision, const double* in, double* out,
                        bool reduce) const {
   if (tiles.empty()) return;
   require(precision >= kFP64 && precision <= kBF16x3, "unknown precision", kErrArgument);
   const dim3 block(kThreads);
   if (wave_tiles) {
-    const dim3 grid((unsigned)(tiles.size() / 4));
-    if (precision == kFP64)
-      hipLaunchKernelGGL(gemm_wave_kernel<double>, grid, block, 0, stream, dev_tiles, dev_items,
-                         dev_contribs, in, out);
-    else if (precision == kFP32)
-      hipLaunchKernelGGL(gemm_wave_kernel<float>, grid, block, 0, stream, dev_tiles, dev_items,
-                         dev_contribs, in, out);
+    const int ng = (int)(tiles.size() / 4);
+    const dim3 grid(launch_grid(ng));
+    if ((int)grid.x < ng)
+      launch_wave<true>(precision, grid, block, stream, dev_tiles, dev_items, dev_contribs, in, out, ng);
     else
-      hipLaunchKernelGGL(gemm_bf16x3_wave_kernel, grid, block, 0, stream, dev_tiles, dev_items,
-                         dev_contribs, in, out);
+      launch_wave<false>(precision, grid, block, stream, dev_tiles, dev_items, dev_contribs, in, out, ng);
   } else {
-    const dim3 grid((unsigned)tiles.size());
-    if (precision == kFP64)
-      hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, dev_tiles,
-                         dev_items, dev_contribs, in, out);
-    else if (precision == kFP32)
-      hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, dev_tiles,
-                         dev_items, dev_contribs, in, out);
+    const int nt = (int)tiles.size();
+    const dim3 grid(launch_grid(nt));
+    if ((int)grid.x < nt)
+      launch_tile<true>(precision, grid, block, stream, dev_tiles, dev_items, dev_contribs, in, out, nt);
     else
-      hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, dev_tiles, dev_items,
-                         dev_contribs, in, out);
+      launch_tile<false>(precision, grid, block, stream, dev_tiles, dev_items, dev_contribs, in, out, nt);
   }
   DMRG_HIP(hipGetLastError());
   if (reduce && !splits.empty()) {
@@ -2362,16 +2438,17 @@ void GemmBatch::launch_group(hipStream_t stream, int group, int precision) const
   const int t0 = group == 0 ? 0 : group_tiles[group - 1], t1 = group_tiles[group];
   if (t1 <= t0) return;
   const dim3 grid((unsigned)(t1 - t0)), block(kThreads);
+  const int nt = t1 - t0;
   const GemmTile* tl = dev_tiles + t0;
   if (precision == kFP64)
     hipLaunchKernelGGL(gemm_mfma_kernel<double>, grid, block, 0, stream, tl, dev_items,
-                       dev_contribs, nullptr, nullptr);
+                       dev_contribs, nullptr, nullptr, nt);
   else if (precision == kFP32)
     hipLaunchKernelGGL(gemm_mfma_kernel<float>, grid, block, 0, stream, tl, dev_items,
-                       dev_contribs, nullptr, nullptr);
+                       dev_contribs, nullptr, nullptr, nt);
   else
-    hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, block, 0, stream, tl, dev_items,
-                       dev_contribs, nullptr, nullptr);
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, grid, block, 0, stream, tl, dev_items,
+                       dev_contribs, nullptr, nullptr, nt);
   DMRG_HIP(hipGetLastError());
 }
 
@@ -2510,12 +2587,14 @@ void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx,
 }
 
 void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream) {
+             double beta, double* part_out, hipStream_t stream, const double* beta2) {
   const int g = dot_grid(n);
   if (cplx)
-    hipLaunchKernelGGL(lz_next_kernel<true>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, part_out);
+    hipLaunchKernelGGL(lz_next_kernel<true>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
+                       part_out);
   else
-    hipLaunchKernelGGL(lz_next_kernel<false>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, part_out);
+    hipLaunchKernelGGL(lz_next_kernel<false>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
+                       part_out);
   DMRG_HIP(hipGetLastError());
 }
 

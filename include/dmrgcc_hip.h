@@ -369,7 +369,8 @@ int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N
  * solves, [7] max bond dim, [8..14] seconds in Lanczos, expansion
  * contraction, expansion fill, canonicalisation, of which SVD, environment
  * update, match+merge, [15..16] largest SVD M and N, [17] SVD count,
- * [18] sum of M*N*N over SVDs, [19] seconds building H_eff plans (part of [8]). */
+ * [18] sum of M*N*N over SVDs, [19] seconds building H_eff plans (part of [8]),
+ * [20..22] sweep autotuner runs, last tile mode, last split-K. */
 int dmrgcc_run(dmrgcc_model* m, int first_block, int first_dim, int nhalf, const int* maxm,
                const int* maxb, const int* niter, const double* canpre, const double* noise,
                int precision, int refine, int verbose, double* energies, double* trunc,
@@ -437,6 +438,14 @@ int dmrgcc_dmrg_advance(dmrgcc_dmrg* h, int half, int site, int* reached);
 int dmrgcc_dmrg_write_env(dmrgcc_dmrg* h, int site, int leigh, const char* path);
 int dmrgcc_env_file_read(const char* path, int nbond, int is_complex, dmrgcc_blocks** out);
 int dmrgcc_dmrg_set_precision(dmrgcc_dmrg* h, int precision, int refine);
+/* The tile autotuner inside the sweep (no reference counterpart; off by
+ * default): the H_eff plan of the first site with psi_dim >= min_dim
+ * (20000 when min_dim <= 0), and of
+ * every later site whose psi_dim is 4x the last tuned one, is re-tiled with
+ * every candidate (as dmrgcc_heff_tune, 3 timed matvecs each) and the fastest
+ * tiling is used for the plans that follow.  stats[20..22] of the result:
+ * tunes run, the last tile mode and split-K threshold chosen. */
+int dmrgcc_dmrg_set_autotune(dmrgcc_dmrg* h, int on, int64_t min_dim);
 int dmrgcc_dmrg_heff(dmrgcc_dmrg* h, int site, dmrgcc_heff** out);
 int dmrgcc_dmrg_ket(dmrgcc_dmrg* h, int site, double** ket, int64_t* n);
 /* Two-site effective Hamiltonian of (site, site+1) on the current state:

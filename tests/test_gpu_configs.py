@@ -188,10 +188,11 @@ def bond_dims(run, L):
     return out
 
 
-def check_trajectory(model, tab, name):
+def check_trajectory(model, tab, name, autotune=False, tune_min_dim=0):
     ref = json.load(open(os.path.join(HERE, "golden", "%s_trajectory.json" % name)))
     assert [list(r) for r in tab] == ref["table"]
     run = model.start(tab)
+    run.set_autotune(autotune, tune_min_dim)
     t0 = time.time()
     assert not run.advance()
     res = run.result()
@@ -206,11 +207,23 @@ def check_trajectory(model, tab, name):
     # absolute accuracy is a few eps, whatever their size
     assert np.all(np.abs(t - tr) <= 1e-14 + 1e-9 * np.abs(tr))
     assert bond_dims(run, model.L) == ref["bond_dims"]
+    return res
 
 
 def test_square_m200_trajectory_matches_oracle():
     model, tab = workloads.spinless("c1")
     check_trajectory(model, tab, "c1")
+
+
+def test_square_m200_trajectory_with_sweep_autotuner():
+    # the tile autotuner inside the sweep re-tiles the larger plans (psi_dim
+    # >= 2000 here, the workload's states stay below the 20000 default); the
+    # trajectory stays the oracle's
+    model, tab = workloads.spinless("c1")
+    res = check_trajectory(model, tab, "c1", autotune=True, tune_min_dim=2000)
+    log("sweep autotuner: %s" % res["autotune"])
+    assert res["autotune"]["tunes"] >= 1
+    assert res["autotune"]["tiling"][0] in (32, 64)
 
 
 def test_spinful_m64_trajectory_matches_oracle():

@@ -61,3 +61,18 @@ def test_run_mixed_precision_energy(prec):
     e0 = ed.ground_energy(om.L, d, N, om.bonds(), U)
     rmx = pm.run(rows, first_dim=fd, precision=prec, refine=True)
     assert abs(rmx["energies"][-1] - e0) < 1e-9 * abs(e0)
+
+
+@pytest.mark.parametrize("prec", [dm.FP64, dm.BF16X3])
+def test_pipelined_lanczos_is_bit_identical(monkeypatch, prec):
+    # the Lanczos loop decides each iteration's stop while the device runs the
+    # next one; the arithmetic is that of the blocking loop, bit for bit
+    nx, ny, N, d, U, py, maxm, nhalf, fd = CASES[2]
+    rows = sweep_rows(maxm, nhalf, niter=10, canpre=1e-8)
+    pm = dm.SpinlessModel(nx, ny, N, d, hop_x=-1.0, hop_y=-1.0, U=U, period_y=py)
+    fast = pm.run(rows, first_block=20, first_dim=fd, precision=prec)
+    monkeypatch.setenv("DMRGCC_LANCZOS_BLOCKING", "1")
+    slow = pm.run(rows, first_block=20, first_dim=fd, precision=prec)
+    assert np.array_equal(fast["energies"], slow["energies"])
+    assert np.array_equal(fast["trunc_err"], slow["trunc_err"])
+    assert (fast["matvecs"], fast["rand_draws"]) == (slow["matvecs"], slow["rand_draws"])
