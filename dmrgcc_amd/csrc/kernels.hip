@@ -1242,7 +1242,7 @@ constexpr int kJc = 2 * kJb;
 constexpr int kJw = DMRG_JACOBI_WAVES;
 constexpr int kJn = kJc / 16;  // 16 x 16 MFMA blocks per visit edge
 static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
-static_assert(kJw >= 1 && kJw <= 16, "1 to 16 waves per visit");
+static_assert(kJw >= 2 && kJw <= 16, "2 to 16 waves per visit (wave 0 sweeps while the others update V)");
 
 // Slabs of 16 rows handled together by one wave (their loads in flight at once)
 #ifndef DMRG_JACOBI_SLABS
@@ -1250,10 +1250,11 @@ static_assert(kJw >= 1 && kJw <= 16, "1 to 16 waves per visit");
 #endif
 constexpr int kSlabs = DMRG_JACOBI_SLABS;
 
-// rows [0, rows) of the visit's columns: X[:, col] <- X[:, col] * Q
+// rows [0, rows) of the visit's columns: X[:, col] <- X[:, col] * Q, by
+// waves wave, wave + nwaves, ... of the workgroup (16-row slabs)
 __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int rows,
                                       const int* col, const double (*Q)[kJc + 1], int lane,
-                                      int wave) {
+                                      int wave, int nwaves) {
   const int i = lane & 15, kk = lane >> 4;
   constexpr int kT = kJc / 4;  // MFMA k-steps
   int cs[kT];
@@ -1267,13 +1268,13 @@ __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int ro
   for (int h = 0; h < kJn; ++h)
 #pragma unroll
     for (int t = 0; t < kT; ++t) qb[h][t] = Q[4 * t + kk][16 * h + i];
-  for (int base = wave * 16; base < rows; base += 16 * kJw * kSlabs) {
+  for (int base = wave * 16; base < rows; base += 16 * nwaves * kSlabs) {
     double a[kSlabs][kT];
 #pragma unroll
     for (int u = 0; u < kSlabs; ++u)
 #pragma unroll
       for (int t = 0; t < kT; ++t) {
-        const int r = base + 16 * kJw * u + i;
+        const int r = base + 16 * nwaves * u + i;
         a[u][t] = (cs[t] >= 0 && r < rows) ? X[(int64_t)cs[t] * ld + r] : 0.0;
       }
 #pragma unroll
@@ -1288,7 +1289,7 @@ __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int ro
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = base + 16 * kJw * u + kk + 4 * v;
+        const int r = base + 16 * nwaves * u + kk + 4 * v;
         if (r < rows) {
 #pragma unroll
           for (int h = 0; h < kJn; ++h)
@@ -1299,12 +1300,32 @@ __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int ro
   }
 }
 
+// column c of visit slot `slot` (< kJc) of block pair pr in round `round`
+// of the block round-robin over nb2 (even) blocks; -1 past the last column
+__device__ inline int visit_column(int round, int pr, int nb, int nb2, int N, int slot) {
+  int bi, bj;
+  if (pr == 0) { bi = round; bj = nb2 - 1; }
+  else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+  const int blk = slot < kJb ? bi : bj;
+  const int c = blk * kJb + (slot & (kJb - 1));
+  return (blk < nb && c < N) ? c : -1;
+}
+
+// One round.  The rotation of a visit is applied to G at once and to V one
+// round later: launch `round` also applies the previous round's rotation of
+// the same (problem, block pair) to V, by waves 1 .. kJw-1 while wave 0 runs
+// the inner sweep (V is read by nothing else during the sweep, and every V
+// column still receives its rotations in round order).  The rotation and a
+// flag go through qbuf / qflag, double-buffered by round parity; a launch
+// with round == the sweep's round count only flushes the last rotations.
 __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
-    const int* __restrict__ vpair, int round, int* __restrict__ rotated) {
+    const int* __restrict__ vpair, int round, int* __restrict__ rotated,
+    double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
   __shared__ double Gp[kJw][kJc][kJc + 1];  // partial Grams, one per wave
   __shared__ double Q[kJc][kJc + 1];
-  __shared__ int col[kJc];
+  __shared__ double Qp[kJc][kJc + 1];       // the previous round's rotation
+  __shared__ int col[kJc], colp[kJc];
   __shared__ int flag;
   double (*G)[kJc + 1] = Gp[0];  // the summed Gram overwrites wave 0's partial
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1313,16 +1334,23 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
-  if (round >= nb2 - 1) return;
-  int bi, bj;
-  if (pr == 0) { bi = round; bj = nb2 - 1; }
-  else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+  const bool cur = round < nb2 - 1;
+  const bool prev = round >= 1 && round - 1 < nb2 - 1 &&
+                    qflag[((round - 1) & 1) * nvisits + blockIdx.x] != 0;
+  if (!cur && !prev) return;
   if (tid < kJc) {
-    const int blk = tid < kJb ? bi : bj;
-    const int c = blk * kJb + (tid & (kJb - 1));
-    col[tid] = (blk < nb && c < N) ? c : -1;
+    col[tid] = cur ? visit_column(round, pr, nb, nb2, N, tid) : -1;
+    colp[tid] = prev ? visit_column(round - 1, pr, nb, nb2, N, tid) : -1;
+  }
+  if (prev) {
+    const double* qs = qbuf + ((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (kJc * kJc);
+    for (int e = tid; e < kJc * kJc; e += blockDim.x) Qp[e / kJc][e % kJc] = qs[e];
   }
   __syncthreads();
+  if (!cur) {
+    apply_rotation(p.V, N, N, colp, Qp, lane, wave, kJw);
+    return;
+  }
   // partial Gram of this wave's slabs: lane (i, kk) feeds X[row][col_{16h+i}]
   {
     const int i = lane & 15, kk = lane >> 4;
@@ -1436,12 +1464,22 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     }
     any = __any(any);
     if (lane == 0) flag = any;
+  } else if (prev) {
+    apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
   }
   __syncthreads();
-  if (!flag) return;
-  apply_rotation(p.G, M, M, col, Q, lane, wave);
-  apply_rotation(p.V, N, N, col, Q, lane, wave);
-  if (tid == 0) rotated[pi] = 1;
+  int* qf = qflag + (round & 1) * nvisits + blockIdx.x;
+  if (!flag) {
+    if (tid == 0) *qf = 0;
+    return;
+  }
+  apply_rotation(p.G, M, M, col, Q, lane, wave, kJw);
+  double* qd = qbuf + ((int64_t)(round & 1) * nvisits + blockIdx.x) * (kJc * kJc);
+  for (int e = tid; e < kJc * kJc; e += blockDim.x) qd[e] = Q[e / kJc][e % kJc];
+  if (tid == 0) {
+    *qf = 1;
+    rotated[pi] = 1;
+  }
 }
 
 // singular values and the sign convention, one workgroup per problem
@@ -2058,6 +2096,84 @@ __global__ __launch_bounds__(256) void qr_finish_kernel(const QrJob* __restrict_
     const double sa = sg * s, sv = s > 0.0 ? sg / s : 0.0;
     for (int i = lane; i < M; i += 64) a[i] *= sa;
     for (int i = lane; i < N; i += 64) v[i] = w[i] * sv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column presort of the QR-preconditioned problems.  The first QR factors
+// A P, P ordering the columns by decreasing norm (ties: lower index first);
+// the Jacobi on the resulting triangle then needs fewer sweeps.  A P = U S W^T
+// gives A = U S (P W)^T, so the right factor comes back with its rows
+// scattered through P (row perm[j] of the result is row j of the computed
+// factor).
+// ---------------------------------------------------------------------------
+struct SortJob {
+  const double* A;  // M x N (ld M)
+  double* dst;      // M x N: A P; later the N x N scratch of the row scatter
+  double* key;      // N column norms^2
+  int* perm;        // column j of A P is column perm[j] of A
+  double* C;        // N x N (ld N) whose rows are scattered back through P
+  int M, N;
+};
+constexpr int kSortMaxN = 8192;  // keys staged in LDS
+
+// key[j] = |A e_j|^2, one wave per column
+__global__ __launch_bounds__(256) void sort_colnorm_kernel(const SortJob* __restrict__ jobs) {
+  const SortJob p = jobs[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int jend = min(p.N, (int)(blockIdx.y + 1) * kFinCols);
+  for (int j = blockIdx.y * kFinCols + wave; j < jend; j += 4) {
+    const double* a = p.A + (int64_t)j * p.M;
+    double s = 0.0;
+    for (int i = lane; i < p.M; i += 64) s += a[i] * a[i];
+    s = wave_sum(s);
+    if (lane == 0) p.key[j] = s;
+  }
+}
+
+// perm from ranks: column i goes to position #{k : key_k > key_i, or equal
+// with k < i} (a total order, so the ranks are a permutation)
+__global__ __launch_bounds__(1024) void sort_rank_kernel(const SortJob* __restrict__ jobs) {
+  extern __shared__ double keys[];
+  const SortJob p = jobs[blockIdx.x];
+  for (int i = threadIdx.x; i < p.N; i += blockDim.x) keys[i] = p.key[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < p.N; i += blockDim.x) {
+    const double ki = keys[i];
+    int r = 0;
+    for (int k = 0; k < p.N; ++k) {
+      const double kk = keys[k];
+      r += (kk > ki || (kk == ki && k < i)) ? 1 : 0;
+    }
+    p.perm[r] = i;
+  }
+}
+
+// dst[:, j] = A[:, perm[j]]
+__global__ __launch_bounds__(256) void sort_gather_kernel(const SortJob* __restrict__ jobs) {
+  const SortJob p = jobs[blockIdx.x];
+  const int c0 = blockIdx.y * kFinCols, c1 = min(p.N, c0 + kFinCols);
+  const int64_t n = (int64_t)max(c1 - c0, 0) * p.M;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const int j = c0 + (int)(e / p.M), i = (int)(e % p.M);
+    p.dst[e + (int64_t)c0 * p.M] = p.A[i + (int64_t)p.perm[j] * p.M];
+  }
+}
+
+// dst[perm[i], j] = C[i, j]  (mode 0), then C = dst  (mode 1)
+__global__ __launch_bounds__(256) void sort_scatter_rows_kernel(const SortJob* __restrict__ jobs, int mode) {
+  const SortJob p = jobs[blockIdx.x];
+  const int N = p.N;
+  const int c0 = blockIdx.y * kFinCols, c1 = min(N, c0 + kFinCols);
+  const int64_t n = (int64_t)max(c1 - c0, 0) * N;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const int64_t at = e + (int64_t)c0 * N;
+    if (mode == 0) {
+      const int j = (int)(at / N), i = (int)(at % N);
+      p.dst[p.perm[i] + (int64_t)j * N] = p.C[at];
+    } else {
+      p.C[at] = p.dst[at];
+    }
   }
 }
 
@@ -2799,7 +2915,11 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   //   precond 1 (K = 3): A = (Q_1 Q_3 J) (Q_2 W)^T, G <- Q_1 Q_3 J exact;
   //   precond 2 (K = 2): A = (Q_1 W) (Q_2 J)^T,    V <- Q_2 J exact.
   // Each extra QR of the transposed triangle makes X more nearly diagonal
-  // (fewer sweeps).
+  // (fewer sweeps).  B_1 is A P with the columns in decreasing norm order
+  // (up to kSortMaxN columns); the right factor is then P (Q_2 W) or P (Q_2 J)
+  // (rows scattered back before the finish).  On the sectors of an m = 1000
+  // sweep with expansion noise this takes the K = 2 form from 13 - 14 sweeps
+  // to 8 - 9; the K = 3 form stays at 8.
   std::vector<SvdProblem> probs = probs_in;
   std::vector<int> mode(np, 0), pq;
   for (int i = 0; i < np; ++i) {
@@ -2809,15 +2929,29 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     if (mode[i]) pq.push_back(i);
   }
   QrPhase ph1, ph2, ph3;
-  DevBuf<double> xbuf;
-  int nq1 = 0;
+  DevBuf<double> xbuf, sbuf;
+  DevBuf<int> sperm;
+  DevBuf<SortJob> d_sort;
+  std::vector<SortJob> sorts;
+  int nq1 = 0, sort_n = 0;
   if (!pq.empty()) {
     // X_k = R_k^T, N x N each: three per precond-1 problem, two per precond 2
-    int64_t total = 0;
-    for (int i : pq) total += (int64_t)(mode[i] == 1 ? 3 : 2) * probs_in[i].N * probs_in[i].N;
+    int64_t total = 0, stotal = 0, ptotal = 0;
+    for (int i : pq) {
+      const SvdProblem& p = probs_in[i];
+      total += (int64_t)(mode[i] == 1 ? 3 : 2) * p.N * p.N;
+      if (p.N <= kSortMaxN) {
+        stotal += (int64_t)p.M * p.N + p.N;
+        ptotal += p.N;
+      }
+    }
     xbuf.ensure((size_t)total);
+    sbuf.ensure((size_t)std::max<int64_t>(stotal, 1));
+    sperm.ensure((size_t)std::max<int64_t>(ptotal, 1));
     std::vector<QrJob> j1, j2, j3;
     double* x = xbuf.data();
+    double* sb = sbuf.data();
+    int* sp = sperm.data();
     for (int i : pq) {
       const SvdProblem& p = probs_in[i];
       const int64_t nn = (int64_t)p.N * p.N;
@@ -2825,8 +2959,18 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
       double* X2 = x + nn;
       double* X3 = mode[i] == 1 ? x + 2 * nn : nullptr;
       x += (mode[i] == 1 ? 3 : 2) * nn;
+      // the first QR factors A P (a presorted copy) when the keys fit in LDS
+      const double* A0 = p.G;
+      if (p.N <= kSortMaxN) {
+        SortJob sj{p.G, sb, sb + (int64_t)p.M * p.N, sp, mode[i] == 1 ? X3 : p.V, p.M, p.N};
+        sb += (int64_t)p.M * p.N + p.N;
+        sp += p.N;
+        sorts.push_back(sj);
+        sort_n = std::max(sort_n, p.N);
+        A0 = sj.dst;
+      }
       QrJob a{};
-      a.A = p.G; a.X = X1; a.C = p.G; a.V = p.V; a.sigma = p.sigma;
+      a.A = const_cast<double*>(A0); a.X = X1; a.C = p.G; a.V = p.V; a.sigma = p.sigma;
       a.M = p.M; a.N = p.N; a.u_in_v = p.u_in_v;
       if (mode[i] == 1) { a.S = p.V; a.W = X3; a.finish = 1; }  // [Q_3 J; 0], Q_2 W
       else a.S = X2;                                           // [W; 0]
@@ -2850,6 +2994,15 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     ph1.plan(j1, stream);
     ph2.plan(j2, stream);
     if (!j3.empty()) ph3.plan(j3, stream);
+    if (!sorts.empty()) {
+      upload(d_sort, sorts, stream);
+      const unsigned ns = (unsigned)sorts.size(), gy = (unsigned)((sort_n + kFinCols - 1) / kFinCols);
+      hipLaunchKernelGGL(sort_colnorm_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data());
+      hipLaunchKernelGGL(sort_rank_kernel, dim3(ns), dim3(1024), sort_n * sizeof(double), stream,
+                         d_sort.data());
+      hipLaunchKernelGGL(sort_gather_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data());
+      DMRG_HIP(hipGetLastError());
+    }
     ph1.factor(stream);
     ph2.factor(stream);
     if (!j3.empty()) ph3.factor(stream);
@@ -2871,10 +3024,13 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
       const int nb = (probs[i].N + kJb - 1) / kJb;
       max_visits += (nb + (nb & 1)) / 2;
     }
-  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits]
-  sweeps_out.ensure((size_t)np + 2 * (size_t)max_visits);
+  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits] |
+  // qflag[2 max_visits]; the rotations of two rounds in qbuf
+  sweeps_out.ensure((size_t)np + 4 * (size_t)max_visits);
   int* rotated = sweeps_out.data();
   int* vprob = rotated + np;
+  int* qflag = vprob + 2 * max_visits;
+  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * kJc * kJc);
   std::vector<int> hrot(np), last(np, 0);
   int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
@@ -2894,9 +3050,10 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     int* vpair = vprob + visits;
     h.insert(h.end(), pairs.begin(), pairs.end());
     upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
-    for (int r = 0; r < rounds; ++r)
+    // rounds 0 .. rounds-1, then the flush of the last rotations into V
+    for (int r = 0; r <= rounds; ++r)
       hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(64 * kJw), 0, stream,
-                         scratch.data(), vprob, vpair, r, rotated);
+                         scratch.data(), vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
@@ -2910,6 +3067,12 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   if (nq1 > 0) ph3.back(stream);        // V = Q_3 J             (precond 1)
   if (!pq.empty()) {
     ph2.back(stream);                   // X3 = Q_2 W or V = Q_2 J
+    if (!sorts.empty()) {               // rows of X3 or V back through P
+      const unsigned ns = (unsigned)sorts.size(), gy = (unsigned)((sort_n + kFinCols - 1) / kFinCols);
+      hipLaunchKernelGGL(sort_scatter_rows_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data(), 0);
+      hipLaunchKernelGGL(sort_scatter_rows_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data(), 1);
+      DMRG_HIP(hipGetLastError());
+    }
     ph1.back(stream);                   // G = Q_1 [Q_3 J; 0] (+ finish) or Q_1 [W; 0]
   }
   // sigma and sign convention from G = U Sigma and V (plain and precond 2)
@@ -2962,10 +3125,13 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
       const int nb = (probs[i].N + kB - 1) / kB;
       max_visits += (nb + (nb & 1)) / 2;
     }
-  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits]
-  sweeps_out.ensure((size_t)np + 2 * (size_t)max_visits);
+  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits] |
+  // qflag[2 max_visits]; the rotations of two rounds in qbuf
+  sweeps_out.ensure((size_t)np + 4 * (size_t)max_visits);
   int* rotated = sweeps_out.data();
   int* vprob = rotated + np;
+  int* qflag = vprob + 2 * max_visits;
+  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * kJc * kJc);
   std::vector<int> hrot(np);
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
     std::vector<int> h(np, 0), pairs;
