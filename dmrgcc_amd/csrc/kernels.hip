@@ -2034,6 +2034,113 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   }
 }
 
+// Block reflector of panel k applied to a matrix, X <- X - L (R^T X), with
+// X the trailing columns of A and (L, R) = (Y_k, P_k) while factorising
+// (Q_k^T A), or X = C and (L, R) = (P_k, Y_k) in the back-transformation
+// (Q_k C).  Workgroup (column block, job) takes kQrUpCols columns of X
+// (m rows, from row c0): Z = R^T X_blk (b x kQrUpCols, K = m) on FP64 MFMA
+// with K split over the waves and the partial Z summed in a fixed order in
+// LDS, then X_blk -= L Z in 16-row slabs.  One launch per panel replaces the
+// two grouped GEMMs of the same product and spreads it over every column
+// block of every job.
+constexpr int kQrUpCols = 16;
+template <bool kBack>
+__global__ __launch_bounds__(512) void qr_update_kernel(const QrJob* __restrict__ probs, int k, int b) {
+  constexpr int kW = 8;
+  __shared__ double Zp[kW][32][kQrUpCols + 1];
+  __shared__ double Z[32][kQrUpCols + 1];
+  const QrJob p = probs[blockIdx.y];
+  const int M = p.M, N = p.N;
+  const int c0 = k * b;
+  if (c0 >= N) return;
+  const int bw = min(b, N - c0), m = M - c0;
+  const int x0 = kBack ? 0 : c0 + bw;  // first column of X
+  const int ncol = N - x0;
+  const int j0 = blockIdx.x * kQrUpCols;
+  if (j0 >= ncol) return;
+  double* X = (kBack ? p.C : p.A) + c0 + (int64_t)(x0 + j0) * M;  // ld M, m rows
+  const double* Yk = p.Y + (int64_t)k * b * M;                   // ld m
+  const double* Pk = p.P + (int64_t)k * b * M;
+  const double* L = kBack ? Pk : Yk;
+  const double* R = kBack ? Yk : Pk;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  const bool colok = j0 + i < ncol;
+  const double* xc = X + (int64_t)(colok ? i : 0) * M;
+  // 1. partial Z over this wave's row chunks of 16: lane (i, kk) feeds rows
+  //    r0 + 4 kk .. r0 + 4 kk + 3 (one MFMA each) of Z rows 16 h + i
+  {
+    d4v acc[2] = {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}};
+    const double* rc[2];
+    bool rok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      rok[h] = 16 * h + i < bw;
+      rc[h] = R + (int64_t)(rok[h] ? 16 * h + i : 0) * m;
+    }
+    for (int r0 = 16 * wave; r0 < m; r0 += 16 * kW) {
+      double xa[4], ra[2][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = r0 + 4 * kk + t;
+        const int rr = min(r, m - 1);
+        xa[t] = (colok && r < m) ? xc[rr] : 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) ra[h][t] = (rok[h] && r < m) ? rc[h][rr] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[h][t], xa[t], acc[h], 0, 0, 0);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Zp[wave][16 * h + kk + 4 * v][i] = acc[h][v];
+  }
+  __syncthreads();
+  for (int e = tid; e < 32 * kQrUpCols; e += blockDim.x) {
+    const int r = e / kQrUpCols, c = e % kQrUpCols;
+    double s = Zp[0][r][c];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) s += Zp[w][r][c];
+    Z[r][c] = s;
+  }
+  __syncthreads();
+  // 2. X_blk -= L Z, 16-row slabs over the waves (K = the panel's columns)
+  double zb[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) zb[t] = Z[4 * t + kk][i];  // rows >= bw are zero
+  for (int r0 = 16 * wave; r0 < m; r0 += 16 * kW) {
+    const int ra = min(r0 + i, m - 1);
+    double la[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int l = 4 * t + kk;
+      const double lv = L[(int64_t)min(l, bw - 1) * m + ra];  // clamped, in bounds
+      la[t] = (l < bw && r0 + i < m) ? -lv : 0.0;
+    }
+    d4v acc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + kk + 4 * v;
+      const double xv = xc[min(r, m - 1)];
+      acc[v] = (colok && r < m) ? xv : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(la[t], zb[t], acc, 0, 0, 0);
+    if (colok) {
+      double* xw = X + (int64_t)i * M;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + kk + 4 * v;
+        if (r < m) xw[r] = acc[v];
+      }
+    }
+  }
+}
+
 // X = R^T (N x N column-major) from the upper triangle of the factorised A
 __global__ __launch_bounds__(256) void qr_rt_kernel(const QrJob* __restrict__ probs) {
   const QrJob p = probs[blockIdx.y];
@@ -2779,20 +2886,19 @@ void fill(double* x, int64_t n, double v, hipStream_t stream) {
 namespace {
 constexpr int kQrMinN = 48;  // smaller problems are not worth the QR launches
 
-// One batched blocked-Householder QR over a set of jobs.  The GEMM batch
-// holds, per panel k, the trailing update (groups 2k, 2k+1) and, in reverse
-// panel order, the back-transformation C <- Q_k C (groups 2*npan + ...).
+// One batched blocked-Householder QR over a set of jobs: per panel k the
+// panel kernel, then the trailing update Q_k^T A (qr_update_kernel); the
+// back-transformation C <- Q_1 ... Q_npan C applies the panels in reverse.
 struct QrPhase {
   std::vector<QrJob> jobs;
   DevBuf<QrJob> d_jobs;
   DevBuf<double> work;
-  GemmBatch gemm;
   int b = 32, npan = 0, maxN = 0, maxM = 0;
   bool lds = true;
   size_t lds_bytes = 0;
 
   // jobs: every field but Y and P set by the caller (X included); the plan
-  // adds the Y, P and Z buffers.
+  // adds the Y and P buffers.
   void plan(std::vector<QrJob> in, hipStream_t stream) {
     jobs = std::move(in);
     maxM = maxN = 0;
@@ -2811,62 +2917,15 @@ struct QrPhase {
     for (const QrJob& j : jobs) {
       const int np = (j.N + b - 1) / b;
       off.push_back(total);
-      total += 2 * (int64_t)np * b * j.M + (int64_t)b * j.N;
+      total += 2 * (int64_t)np * b * j.M;
     }
     work.ensure((size_t)std::max<int64_t>(total, 1));
-    std::vector<double*> Z;
     for (size_t t = 0; t < jobs.size(); ++t) {
       QrJob& r = jobs[t];
       const int np = (r.N + b - 1) / b;
       r.Y = work.data() + off[t];
       r.P = r.Y + (int64_t)np * b * r.M;
-      Z.push_back(r.P + (int64_t)np * b * r.M);
     }
-    GemmBatch& g = gemm;
-    g.clear();
-    g.split_k = 0;
-    // factorisation: Z^T = A_trail^T P_k ; A_trail^T -= Z^T Y_k^T  (row-major
-    // views of the column-major blocks)
-    for (int k = 0; k < npan; ++k) {
-      for (int half = 0; half < 2; ++half) {
-        for (size_t t = 0; t < jobs.size(); ++t) {
-          const QrJob& r = jobs[t];
-          const int c0 = k * b;
-          if (c0 >= r.N) continue;
-          const int bw = std::min(b, r.N - c0), m = r.M - c0, nt = r.N - c0 - bw;
-          if (nt <= 0) continue;
-          double* At = r.A + c0 + (int64_t)(c0 + bw) * r.M;
-          if (half == 0) {
-            g.begin_item(Z[t], bw, nt, bw, 0);
-            g.add(At, r.M, r.P + (int64_t)k * b * r.M, m, m, kTransB, 1.0);
-          } else {
-            g.begin_item(At, r.M, nt, m, 1);
-            g.add(Z[t], bw, r.Y + (int64_t)k * b * r.M, m, bw, 0, -1.0);
-          }
-        }
-        g.end_group();
-      }
-    }
-    // back-transformation, last panel first: Z^T = C^T Y_k ; C^T -= Z^T P_k^T
-    for (int k = npan - 1; k >= 0; --k) {
-      for (int half = 0; half < 2; ++half) {
-        for (size_t t = 0; t < jobs.size(); ++t) {
-          const QrJob& r = jobs[t];
-          const int c0 = k * b;
-          if (c0 >= r.N) continue;
-          const int bw = std::min(b, r.N - c0), m = r.M - c0;
-          if (half == 0) {
-            g.begin_item(Z[t], bw, r.N, bw, 0);
-            g.add(r.C + c0, r.M, r.Y + (int64_t)k * b * r.M, m, m, kTransB, 1.0);
-          } else {
-            g.begin_item(r.C + c0, r.M, r.N, m, 1);
-            g.add(Z[t], bw, r.P + (int64_t)k * b * r.M, m, bw, 0, -1.0);
-          }
-        }
-        g.end_group();
-      }
-    }
-    g.finalize(stream);
     upload(d_jobs, jobs, stream);
   }
 
@@ -2881,8 +2940,9 @@ struct QrPhase {
         hipLaunchKernelGGL(qr_panel_kernel<false>, dim3(nj), dim3(512), lds_bytes, stream,
                            d_jobs.data(), k, b);
       DMRG_HIP(hipGetLastError());
-      gemm.launch_group(stream, 2 * k, kFP64);
-      gemm.launch_group(stream, 2 * k + 1, kFP64);
+      const unsigned gx = (unsigned)((maxN - std::min(maxN, k * b) + kQrUpCols - 1) / kQrUpCols);
+      if (gx > 0) hipLaunchKernelGGL(qr_update_kernel<false>, dim3(gx, nj), dim3(512), 0, stream, d_jobs.data(), k, b);
+      DMRG_HIP(hipGetLastError());
     }
     const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxN * maxN + 255) / 256);
     hipLaunchKernelGGL(qr_rt_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
@@ -2895,10 +2955,10 @@ struct QrPhase {
     const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxM * maxN + 255) / 256);
     hipLaunchKernelGGL(qr_seed_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
     DMRG_HIP(hipGetLastError());
-    for (int k = npan - 1, g = 2 * npan; k >= 0; --k, g += 2) {
-      gemm.launch_group(stream, g, kFP64);
-      gemm.launch_group(stream, g + 1, kFP64);
-    }
+    const unsigned gu = (unsigned)((maxN + kQrUpCols - 1) / kQrUpCols);
+    for (int k = npan - 1; k >= 0; --k)
+      hipLaunchKernelGGL(qr_update_kernel<true>, dim3(gu, nj), dim3(512), 0, stream, d_jobs.data(), k, b);
+    DMRG_HIP(hipGetLastError());
     const unsigned gy = (unsigned)std::max(1, (maxN + kFinCols - 1) / kFinCols);
     hipLaunchKernelGGL(qr_finish_kernel, dim3(nj, gy), dim3(256), 0, stream, d_jobs.data());
     DMRG_HIP(hipGetLastError());
