@@ -821,6 +821,40 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// DPP lane move of a double within rows of 16 lanes (kCtrl: a DPP control
+// code: 0xB1 / 0x4E quad permutations (lane ^ 1, lane ^ 2), 0x141 half-row
+// mirror, 0x140 row mirror)
+template <int kCtrl>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each row of 16 lanes, the same value in all 16 (a + b and b + a
+// round alike), on DPP moves: no LDS round trips
+__device__ inline double row16_sum(double v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return v;
+}
+
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the wave, wave-uniform: row sums, then rows (0 + 1) + (2 + 3)
+__device__ inline double wave_sum_dpp(double v) {
+  v = row16_sum(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
 __global__ __launch_bounds__(256) void dot_partial_kernel(const double* __restrict__ x,
                                                           const double* __restrict__ y,
                                                           int64_t n, double* __restrict__ part) {
@@ -1816,7 +1850,12 @@ struct QrJob {
 // Factorise panel k of every problem (one workgroup each).  kLds: the panel is
 // staged in LDS (dynamic, m*b + b*b + b doubles); otherwise it is worked on in
 // place in global memory and LDS holds only T and tau (b*b + b doubles).
-template <bool kLds>
+// kU > 0 (b = 32, m <= 64 kU, LDS sized for 64 kU rows): the column loop keeps the panel in
+// registers instead, wave w holding columns w, w + 8, w + 16, w + 24 and lane
+// l rows l, l + 64, ...; a column is written to LDS once, by its wave, when
+// its last update is done, and each reflector value read from there serves
+// the four columns of a lane.
+template <bool kLds, int kU = 0>
 __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__ probs, int k,
                                                        int b) {
   extern __shared__ double smem[];
@@ -1832,7 +1871,11 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   double* pan;
   int64_t ld;
   double* T;
-  if (kLds) {
+  if (kLds && kU > 0) {
+    pan = smem;
+    ld = 64 * kU;
+    T = smem + (int64_t)ld * b;
+  } else if (kLds) {
     pan = smem;
     ld = m;
     T = smem + (int64_t)m * bw;
@@ -1864,6 +1907,125 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   // next step (column 0's comes from a block reduction), so a column costs
   // one barrier.
   __shared__ double colss[2];
+  if constexpr (kU > 0) {
+    // wave w holds columns w, w + 8, w + 16, w + 24 of the panel, lane l rows
+    // l + 64 u (u < kU): a reflector value read from LDS serves four columns.
+    // The LDS columns have 64 kU rows (ld), zero past m, so only register 0
+    // (rows 0 .. 63, the ones that can be <= j < 32) needs a row test.
+    const int nwv = (int)(blockDim.x >> 6);  // 8
+    double y[4][kU];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = wave + nwv * q;
+      const double* src = Ag + (int64_t)min(c, bw - 1) * M;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = lane + 64 * u;
+        const double v = src[min(i, m - 1)];  // clamped, in bounds
+        y[q][u] = (c < bw && i < m) ? v : 0.0;
+      }
+    }
+    // column 0 to LDS with |x[1:]|^2
+    if (wave == 0) {
+      double ss = 0.0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        pan[lane + 64 * u] = y[0][u];
+        ss += (u > 0 || lane >= 1) ? y[0][u] * y[0][u] : 0.0;
+      }
+      ss = wave_sum_dpp(ss);
+      if (lane == 0) colss[0] = ss;
+    }
+    __syncthreads();
+    double scal_prev = 0.0, tj_prev = 0.0, beta_prev = 0.0;
+    for (int j = 0; j < bw; ++j) {
+      // column j - 1 (read by nobody from here on) as v = (beta, scal x[j:]),
+      // from its owner's registers, which hold its final values
+      if (j > 0 && (j - 1) % nwv == wave) {
+        const int q = (j - 1) / nwv;
+        double* xp = pan + (int64_t)(j - 1) * ld;
+        const double sc = tj_prev != 0.0 ? scal_prev : 1.0;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+          if (qq == q) {
+            if (lane >= j - 1) xp[lane] = lane == j - 1 ? beta_prev : y[qq][0] * sc;
+#pragma unroll
+            for (int u = 1; u < kU; ++u) xp[lane + 64 * u] = y[qq][u] * sc;
+          }
+      }
+      const double* x = pan + (int64_t)j * ld;  // column j
+      // reflector values first (they do not wait for the scalars)
+      double xv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) xv[u] = x[lane + 64 * u];
+      xv[0] = lane > j ? xv[0] : 0.0;
+      const double t = colss[j & 1];
+      const double alpha = x[j];
+      // x^T y over rows > j for the wave's four columns, reduced together
+      // (a column that is done or absent gives an unused value)
+      double dot[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double dp[2] = {0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kU; ++u) dp[u & 1] += xv[u] * y[q][u];
+        dot[q] = row16_sum(dp[0] + dp[1]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        dot[q] = (readlane_d(dot[q], 0) + readlane_d(dot[q], 16)) +
+                 (readlane_d(dot[q], 32) + readlane_d(dot[q], 48));
+      double tj = 0.0, beta = alpha, scal = 0.0;
+      if (t > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + t), alpha);
+        tj = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      if (tid == 0) tau[j] = tj;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = wave + nwv * q;
+        if (c <= j || c >= bw) continue;  // uniform over the wave
+        // d = tau (y_j + scal x^T y); y -= d v, v = (1, scal x[j+1:]); row
+        // j < 32 sits in lane j, register 0
+        if (tj != 0.0) {
+          const double yj = readlane_d(y[q][0], j);
+          const double d = (yj + scal * dot[q]) * tj;
+          const double ds = d * scal;
+          y[q][0] -= lane == j ? d : ds * xv[0];
+#pragma unroll
+          for (int u = 1; u < kU; ++u) y[q][u] -= ds * xv[u];
+        }
+        if (c == j + 1) {  // final: to LDS with |x[1:]|^2 for the next step
+          double* xo = pan + (int64_t)c * ld;
+          double ss = 0.0;
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            xo[lane + 64 * u] = y[q][u];
+            ss += (u > 0 || lane > c) ? y[q][u] * y[q][u] : 0.0;
+          }
+          ss = wave_sum_dpp(ss);
+          if (lane == 0) colss[(j + 1) & 1] = ss;
+        }
+      }
+      scal_prev = scal;
+      tj_prev = tj;
+      beta_prev = beta;
+      __syncthreads();
+    }
+    if ((bw - 1) % nwv == wave) {
+      const int q = (bw - 1) / nwv;
+      double* xp = pan + (int64_t)(bw - 1) * ld;
+      const double sc = tj_prev != 0.0 ? scal_prev : 1.0;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+        if (qq == q) {
+          if (lane >= bw - 1) xp[lane] = lane == bw - 1 ? beta_prev : y[qq][0] * sc;
+#pragma unroll
+          for (int u = 1; u < kU; ++u) xp[lane + 64 * u] = y[qq][u] * sc;
+        }
+    }
+  } else {
   {
     double ss = 0.0;
     for (int i = 1 + tid; i < m; i += blockDim.x) ss += pan[i] * pan[i];
@@ -1928,6 +2090,7 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
     if (tj != 0.0)
       for (int i = 1 + tid; i < len; i += blockDim.x) x[i] *= scal;
     if (tid == 0) x[0] = beta;
+  }
   }
   __syncthreads();
   // S = Y^T Y with FP64 MFMA, one wave per upper 16 x 16 block (a <= b);
@@ -2893,7 +3056,7 @@ struct QrPhase {
   std::vector<QrJob> jobs;
   DevBuf<QrJob> d_jobs;
   DevBuf<double> work;
-  int b = 32, npan = 0, maxN = 0, maxM = 0;
+  int b = 32, npan = 0, maxN = 0, maxM = 0, reg_u = 0;
   bool lds = true;
   size_t lds_bytes = 0;
 
@@ -2911,6 +3074,12 @@ struct QrPhase {
       if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
     }
     if (!lds) { b = 32; lds_bytes = ((size_t)b * b + b) * sizeof(double); }
+    // register column loop (b = 32, panel columns padded to 64 kU rows)
+    reg_u = 0;
+    if (lds && b == 32 && maxM <= 64 * 9) {
+      reg_u = maxM <= 64 * 4 ? 4 : 9;
+      lds_bytes = ((size_t)64 * reg_u * b + (size_t)b * b + b) * sizeof(double);
+    }
     npan = (maxN + b - 1) / b;
     int64_t total = 0;
     std::vector<int64_t> off;
@@ -2933,7 +3102,13 @@ struct QrPhase {
   void factor(hipStream_t stream) const {
     const unsigned nj = (unsigned)jobs.size();
     for (int k = 0; k < npan; ++k) {
-      if (lds)
+      if (reg_u == 4)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 4>), dim3(nj), dim3(512), lds_bytes, stream,
+                           d_jobs.data(), k, b);
+      else if (reg_u == 9)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 9>), dim3(nj), dim3(512), lds_bytes, stream,
+                           d_jobs.data(), k, b);
+      else if (lds)
         hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(nj), dim3(512), lds_bytes, stream,
                            d_jobs.data(), k, b);
       else
