@@ -124,12 +124,25 @@ def site_lanczos(run, plan, site, niter, dm):
     import torch
     _, kd = run.site_blocks(site, 0)
     x0 = torch.from_numpy(np.ascontiguousarray(kd).view(np.float64).copy()).cuda()
-    e, it = {}, {}
+    e, it, ms, mv = {}, {}, {}, {}
     for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3)):
-        v = x0.clone()
-        e[name], it[name] = plan.lanczos(v, niter, prec, refine=True)
+        mv[name] = plan.time(prec, 10)["matvec_ms"]  # the plan's tiling as the solve uses it
+        best = None
+        for _ in range(4):  # the first solve also warms the buffers
+            v = x0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e[name], it[name] = plan.lanczos(v, niter, prec, refine=True)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        ms[name] = best * 1e3
     ref = e["fp64"]
-    return {"niter": niter, "energy": e, "iterations": it,
+    # wall time of a whole solve (start vector in HBM to Ritz vector, the
+    # mixed forms' FP64 refinement matvec included) per Lanczos iteration
+    per_it = {k: ms[k] / it[k] for k in ms}
+    return {"niter": niter, "energy": e, "iterations": it, "solve_ms": ms, "ms_per_iteration": per_it,
+            "matvec_ms": mv, "iteration_over_matvec": {k: per_it[k] / mv[k] for k in ms},
             "dE_rel": {k: abs(e[k] - ref) / abs(ref) for k in ("fp32", "bf16x3")}}
 
 
@@ -309,7 +322,8 @@ def main():
         envs["left"]["GB_per_s"], envs["right"]["GB_per_s"],
         {k: round(v["GB_per_s"]) for k, v in vec.items()}))
     site_e = site_lanczos(run64, plan, site, tab[-1][2], dm)
-    log("site Lanczos dE_rel: %s" % site_e["dE_rel"])
+    log("site Lanczos dE_rel: %s; ms per iteration / matvec: %s" % (
+        site_e["dE_rel"], {k: round(v, 2) for k, v in site_e["iteration_over_matvec"].items()}))
     two = None
     if not args.no_two_site:
         two = two_site(run64, site, args.steps, dm)

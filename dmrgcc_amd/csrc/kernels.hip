@@ -849,6 +849,14 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// v of lane src (ds_bpermute, both halves)
+__device__ inline double bperm_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)b);
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Sum over the wave, wave-uniform: row sums, then rows (0 + 1) + (2 + 3)
 __device__ inline double wave_sum_dpp(double v) {
   v = row16_sum(v);
@@ -1437,6 +1445,88 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     Q[r][c] = r == c ? 1.0 : 0.0;
   }
   __syncthreads();
+  if constexpr (kJc == 16) {
+  if (wave == 0) {
+    // The inner sweep in registers, XOR order: step d rotates the pairs
+    // (x, x ^ d), d = 1 .. 15 (every pair once).  Lane (c, q) = c + 16 q
+    // holds G[c ^ (4 q + s)][c] and Q[c ^ (4 q + s)][c] in slot s < 4 and the
+    // diagonal D = G[c][c]; every lane forms its column's rotation from the
+    // same three values as its partner (bit-identical angles).  Per step:
+    // column update G J, Q J (the partner column sits at lane ^ (d | (d >> 2)
+    // << 4), slot s ^ (d & 3)), then the row update through the symmetry of
+    // J^T G J: G'[r][c] = G'[c][r] = (G J)[c][r] cs + (G J)[c ^ d][r] s_c.
+    const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
+    const double tol2 = tol * tol;
+    const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
+    const int c = lane & 15, q = lane >> 4;
+    double g[4], qv[4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      g[s2] = G[c ^ (4 * q + s2)][c];
+      qv[s2] = (4 * q + s2) == 0 ? 1.0 : 0.0;
+    }
+    double D = G[c][c];
+    int any = 0;
+#pragma unroll
+    for (int d = 1; d < 16; ++d) {
+      const int hb = d >= 8 ? 8 : d >= 4 ? 4 : d >= 2 ? 2 : 1;  // top bit of d
+      const bool lo = (c & hb) == 0;
+      const int a = lo ? c : (c ^ d);
+      const double Dp = bperm_d(D, lane ^ d);
+      // G[a ^ d][a] sits in column a at offset d: quarter d >> 2, slot d & 3
+      const double gm = bperm_d(g[d & 3], a + 16 * (d >> 2));
+      const double al = lo ? D : Dp, be = lo ? Dp : D;
+      double cs = 1.0, sn = 0.0;
+      const bool doit = al > zero2 && be > zero2 && gm * gm > tol2 * al * be;
+      if (doit) {
+        // tan(theta) = sgn(dd) gg / (|dd| + hypot(dd, gg)), dd = be - al,
+        // gg = 2 gm (the one-sided angle); rsq/rcp with one Newton step each
+        const double dd = be - al, gg = 2.0 * gm;
+        const double w = dd * dd + gg * gg;
+        double ri = __builtin_amdgcn_rsq(w);
+        ri = ri * (1.5 - 0.5 * w * ri * ri);
+        const double den = fabs(dd) + w * ri;
+        double rc = __builtin_amdgcn_rcp(den);
+        rc = rc * (2.0 - den * rc);
+        const double t = (dd >= 0.0 ? gg : -gg) * rc;
+        const double u = 1.0 + t * t;
+        double ci = __builtin_amdgcn_rsq(u);
+        ci = ci * (1.5 - 0.5 * u * ci * ci);
+        cs = ci;
+        sn = ci * t;
+        any = 1;
+      }
+      // new column a = cs a - sn b, new column b = sn a + cs b
+      const double sg = lo ? -sn : sn;
+      const int mask = d | ((d >> 2) << 4);
+      double h[4], pq[4];
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const double pg = bperm_d(g[s2 ^ (d & 3)], lane ^ mask);
+        pq[s2] = bperm_d(qv[s2 ^ (d & 3)], lane ^ mask);  // all read before any update
+        h[s2] = cs * g[s2] + sg * pg;
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) qv[s2] = cs * qv[s2] + sg * pq[s2];
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const int r = c ^ (4 * q + s2);
+        const double hc = bperm_d(h[s2], r + 16 * q);                      // (G J)[c][r]
+        const double hd = bperm_d(h[s2 ^ (d & 3)], r + 16 * (q ^ (d >> 2)));  // (G J)[c ^ d][r]
+        g[s2] = cs * hc + sg * hd;
+      }
+      D = lo ? cs * cs * al - 2.0 * cs * sn * gm + sn * sn * be
+             : sn * sn * al + 2.0 * cs * sn * gm + cs * cs * be;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) Q[c ^ (4 * q + s2)][c] = qv[s2];
+    any = __any(any);
+    if (lane == 0) flag = any;
+  } else if (prev) {
+    apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
+  }
+  } else {
+  // 32-column visits: one sweep of two-sided Jacobi in LDS on wave 0
   if (wave == 0) {
     const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
     const double tol2 = tol * tol;
@@ -1500,6 +1590,7 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     if (lane == 0) flag = any;
   } else if (prev) {
     apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
+  }
   }
   __syncthreads();
   int* qf = qflag + (round & 1) * nvisits + blockIdx.x;
