@@ -1371,22 +1371,37 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
   __shared__ int flag;
   double (*G)[kJc + 1] = Gp[0];  // the summed Gram overwrites wave 0's partial
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // The previous round's flag and rotation of this visit slot depend only on
+  // the launch index, so their loads are issued with the descriptor's (the
+  // rotation is read whether or not it is used: the slot is always in bounds)
+  const int pflag = round >= 1 ? qflag[((round - 1) & 1) * nvisits + blockIdx.x] : 0;
+  double qpre[(kJc * kJc + 64 * kJw - 1) / (64 * kJw)];
+  if (round >= 1) {
+    const double* qs = qbuf + ((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (kJc * kJc);
+#pragma unroll
+    for (int u = 0; u * 64 * kJw < kJc * kJc; ++u) {
+      const int e = tid + u * 64 * kJw;
+      qpre[u] = e < kJc * kJc ? qs[e] : 0.0;
+    }
+  }
   // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
   const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
   const bool cur = round < nb2 - 1;
-  const bool prev = round >= 1 && round - 1 < nb2 - 1 &&
-                    qflag[((round - 1) & 1) * nvisits + blockIdx.x] != 0;
+  const bool prev = round >= 1 && round - 1 < nb2 - 1 && pflag != 0;
   if (!cur && !prev) return;
   if (tid < kJc) {
     col[tid] = cur ? visit_column(round, pr, nb, nb2, N, tid) : -1;
     colp[tid] = prev ? visit_column(round - 1, pr, nb, nb2, N, tid) : -1;
   }
   if (prev) {
-    const double* qs = qbuf + ((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (kJc * kJc);
-    for (int e = tid; e < kJc * kJc; e += blockDim.x) Qp[e / kJc][e % kJc] = qs[e];
+#pragma unroll
+    for (int u = 0; u * 64 * kJw < kJc * kJc; ++u) {
+      const int e = tid + u * 64 * kJw;
+      if (e < kJc * kJc) Qp[e / kJc][e % kJc] = qpre[u];
+    }
   }
   __syncthreads();
   if (!cur) {
