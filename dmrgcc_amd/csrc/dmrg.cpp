@@ -610,6 +610,121 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
     }
     if (cplx_) zgemm_item(s2_, orel, ob.cols, ob.rows, ob.cols, 1, kRelC, zt);
   }
+  // Pre-split BF16x3 stages (real plans): the same items and terms on bf16
+  // pair groups.  Offsets first (every used L and R block packed once, x per
+  // ket block, T^T per task), then the buffers, then the descriptors.
+  pre_ = false;
+  p1_.clear();
+  p2_.clear();
+  {
+    const char* ev = std::getenv("DMRGCC_PRESPLIT");
+    const bool on = want_pre_ && !cplx_ && !(ev && ev[0] == '0');
+    if (on) {
+      auto groups = [](int K) { return (int64_t)((K + 7) / 8); };
+      std::vector<std::vector<int64_t>> loff(L.size()), roff(R.size());
+      for (size_t i = 0; i < L.size(); ++i) loff[i].assign(L[i].n(), -1);
+      for (size_t i = 0; i < R.size(); ++i) roff[i].assign(R[i].n(), -1);
+      int64_t lr = 0, xs = 0, ts = 0, lrmax = 0;
+      std::vector<PackTask> lrpack, xpack;
+      std::vector<int64_t> xoff(ket.n(), -1), toff_s(tasks.size(), -1);
+      for (int kp = 0; kp < ket.n(); ++kp) {
+        const Block& b = ket.blk[kp];
+        if ((int64_t)b.rows * b.cols == 0) continue;
+        xoff[kp] = xs;
+        xs += (int64_t)b.rows * groups(b.cols) * 4;
+      }
+      for (size_t ti = 0; ti < tasks.size(); ++ti) {
+        const Task& t = tasks[ti];
+        if (t.outs.empty()) continue;
+        const Block& kb = ket.blk[t.kp];
+        const Block& rbk = R[t.o2].blk[t.ri];
+        toff_s[ti] = ts;
+        ts += (int64_t)rbk.cols * groups(kb.rows) * 4;
+        if (roff[t.o2][t.ri] < 0) {
+          roff[t.o2][t.ri] = lr;
+          lr += (int64_t)rbk.cols * groups(rbk.rows) * 4;
+        }
+        for (const Out& o : t.outs)
+          if (loff[o.o1][o.lp] < 0) {
+            const Block& lbk = L[o.o1].blk[o.lp];
+            loff[o.o1][o.lp] = lr;
+            lr += (int64_t)lbk.rows * groups(lbk.cols) * 4;
+          }
+      }
+      Xs_.ensure((size_t)std::max<int64_t>(xs, 1));
+      Ts_.ensure((size_t)std::max<int64_t>(ts, 1));
+      LRs_.ensure((size_t)std::max<int64_t>(lr, 1));
+      fill(Ts_.data(), std::max<int64_t>(ts, 1), 0.0, stream);  // the padding k stay zero
+      // pack tasks: the used R blocks as B^T (rows n, k = R rows), L as A
+      for (size_t i = 0; i < R.size(); ++i)
+        for (int b = 0; b < R[i].n(); ++b)
+          if (roff[i][b] >= 0) {
+            const Block& bk = R[i].blk[b];
+            lrpack.push_back(PackTask{R[i].ptr(b), 1, bk.cols, bk.cols, bk.rows, LRs_.data() + roff[i][b]});
+            lrmax = std::max<int64_t>(lrmax, (int64_t)bk.cols * groups(bk.rows));
+          }
+      for (size_t i = 0; i < L.size(); ++i)
+        for (int b = 0; b < L[i].n(); ++b)
+          if (loff[i][b] >= 0) {
+            const Block& bk = L[i].blk[b];
+            lrpack.push_back(PackTask{L[i].ptr(b), bk.cols, 1, bk.rows, bk.cols, LRs_.data() + loff[i][b]});
+            lrmax = std::max<int64_t>(lrmax, (int64_t)bk.rows * groups(bk.cols));
+          }
+      int64_t xmax = 0;
+      for (int kp = 0; kp < ket.n(); ++kp) {
+        if (xoff[kp] < 0) continue;
+        const Block& b = ket.blk[kp];
+        xpack.push_back(PackTask{reinterpret_cast<const double*>((intptr_t)b.off), b.cols, 1, b.rows, b.cols,
+                                 Xs_.data() + xoff[kp]});
+        xmax = std::max<int64_t>(xmax, (int64_t)b.rows * groups(b.cols));
+      }
+      if (!lrpack.empty()) {
+        DevBuf<PackTask> dl;
+        upload(dl, lrpack, stream);
+        launch_pack(dl.data(), (int)lrpack.size(), lrmax, nullptr, stream);
+        DMRG_HIP(hipStreamSynchronize(stream));  // dl is released here
+      }
+      upload(d_xpack_, xpack, stream);
+      nxpack_ = (int)xpack.size();
+      xpack_groups_ = xmax;
+      // stage 1: T^T (pairs) = K * R; stage 2: y += w L T, the same term order
+      for (size_t ti = 0; ti < tasks.size(); ++ti) {
+        const Task& t = tasks[ti];
+        if (t.outs.empty()) continue;
+        const Block& kb = ket.blk[t.kp];
+        const Block& rbk = R[t.o2].blk[t.ri];
+        if (kb.cols <= 0) continue;  // (no K: the FP64 T of such a task is zero as well)
+        p1_.begin_item(Ts_.data() + toff_s[ti], groups(kb.rows), kb.rows, rbk.cols, 0, kPairC);
+        p1_.add(Xs_.data() + xoff[t.kp], groups(kb.cols), LRs_.data() + roff[t.o2][t.ri], groups(rbk.rows),
+                kb.cols, 0, 1.0);
+      }
+      for (int op = 0; op < ket.n(); ++op) {
+        if (by_out[op].empty()) continue;
+        const Block& ob = ket.blk[op];
+        p2_.begin_item(reinterpret_cast<double*>((intptr_t)ob.off), ob.cols, ob.rows, ob.cols, 1, kRelC);
+        for (auto [ti, k] : by_out[op]) {
+          const Task& t = tasks[ti];
+          const Out& o = t.outs[k];
+          const Block& lbk = L[o.o1].blk[o.lp];
+          const Block& kb = ket.blk[t.kp];
+          if (lbk.cols <= 0) continue;  // empty terms are dropped (the kernel assumes K > 0)
+          p2_.add(LRs_.data() + loff[o.o1][o.lp], groups(lbk.cols), Ts_.data() + toff_s[ti], groups(kb.rows),
+                  lbk.cols, 0, o.w);
+        }
+      }
+      pitems1_ = p1_.items;
+      pterms1_ = p1_.contribs;
+      pitems2_ = p2_.items;
+      pterms2_ = p2_.contribs;
+      for (GemmBatch* g : {&p1_, &p2_}) {
+        g->presplit = true;
+        g->tile_mode = 64;
+        g->split_k = s2_.split_k;
+        g->finalize(stream);
+      }
+      pre_ = true;
+    }
+  }
   // F_ref counts the reference's K*R products even when no L term uses them.
   double fref = 0.0;
   int64_t lbytes = 0, rbytes = 0;
@@ -643,14 +758,14 @@ void HeffPlan::retile(int tile_mode, int split_k, hipStream_t stream) {
   // the previous launches may still read the descriptors being replaced
   DMRG_HIP(hipStreamSynchronize(stream));
   ++gen_;
-  GemmBatch* st[2] = {&s1_, &s2_};
-  const std::vector<GemmItem>* it[2] = {&items1_, &items2_};
-  const std::vector<GemmContrib>* tm[2] = {&terms1_, &terms2_};
-  for (int k = 0; k < 2; ++k) {
+  GemmBatch* st[4] = {&s1_, &s2_, &p1_, &p2_};
+  const std::vector<GemmItem>* it[4] = {&items1_, &items2_, &pitems1_, &pitems2_};
+  const std::vector<GemmContrib>* tm[4] = {&terms1_, &terms2_, &pterms1_, &pterms2_};
+  for (int k = 0; k < (pre_ ? 4 : 2); ++k) {
     GemmBatch& g = *st[k];
     g.items = *it[k];
     g.contribs = *tm[k];
-    g.tile_mode = tile_mode == 32 ? 32 : 64;
+    g.tile_mode = k >= 2 || tile_mode != 32 ? 64 : 32;  // the pre-split stages tile 64x64
     g.split_k = std::max(0, split_k);
     g.finalize(stream);
   }
@@ -692,12 +807,27 @@ std::vector<HeffPlan::Tuning> HeffPlan::tune(int precision, int reps, hipStream_
 }
 
 void HeffPlan::apply(const double* x, double* y, int precision, hipStream_t stream) {
+  if (use_pre(precision)) {
+    launch_pack(d_xpack_.data(), nxpack_, xpack_groups_, x, stream);
+    p1_.launch(stream, precision, nullptr, nullptr);
+    p2_.launch(stream, precision, nullptr, y);
+    return;
+  }
   s1_.launch(stream, precision, x, nullptr);
   s2_.launch(stream, precision, nullptr, y);
 }
 
 void HeffPlan::apply_stage(int stage, const double* x, double* y, int precision,
                            hipStream_t stream, bool gemm_only) {
+  if (use_pre(precision)) {
+    if (stage == 1) {
+      launch_pack(d_xpack_.data(), nxpack_, xpack_groups_, x, stream);
+      p1_.launch(stream, precision, nullptr, nullptr, !gemm_only);
+    } else {
+      p2_.launch(stream, precision, nullptr, y, !gemm_only);
+    }
+    return;
+  }
   if (stage == 1) s1_.launch(stream, precision, x, nullptr, !gemm_only);
   else s2_.launch(stream, precision, nullptr, y, !gemm_only);
 }
@@ -1511,6 +1641,7 @@ double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
   const int64_t n = lat_[site].ket.size;
   const bool tune = p_.autotune && n >= p_.tune_min_dim && n >= 4 * tuned_psi_;
   if (!tune) plan.set_tiling(tune_tile_, tune_split_);
+  plan.set_presplit(p_.precision == kBF16x3);  // the packed operands serve BF16x3 only
   plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
   if (tune) {
     // the plan ends up tiled with the fastest candidate; the trial matvecs
@@ -2628,6 +2759,7 @@ void RealTensorRundmrg::BuildTwoSite(int site, TwoSite& ts, HeffPlan& plan) {
   g.finalize(cx_.stream);
   g.launch(cx_.stream, kFP64);
   ts.W2 = fuse_sites(ham_.sites()[site], ham_.sites()[site + 1]);
+  plan.set_presplit(p_.precision == kBF16x3);
   plan.build_blocks(th, ts.bphys, B.nr(), ts.W2, L_[site], R_[site + 1], cx_.stream);
   cx_.sync();
 }

@@ -159,10 +159,28 @@ class HeffPlan {
   int generation() const { return gen_; }
   // tiling of the next build (-1: the process-wide default of that field)
   void set_tiling(int tile_mode, int split_k) { want_tile_ = tile_mode; want_split_ = split_k; }
+  // Pre-split BF16x3 form of the next build (default on; the environment
+  // variable DMRGCC_PRESPLIT=0 turns it off): the used L and R blocks are
+  // packed once into bf16 pair groups, x is packed per apply, stage 1 writes
+  // T^T as pair groups, and the BF16x3 apply on 64x64 tiles runs both
+  // stages on the pre-split kernel.  FP64 and FP32 keep the FP64 T.
+  void set_presplit(bool on) { want_pre_ = on; }
+  bool presplit() const { return pre_; }
 
  private:
+  bool use_pre(int precision) const { return pre_ && precision == kBF16x3 && !s1_.wave_tiles; }
   GemmBatch s1_, s2_;
   int want_tile_ = -1, want_split_ = -1;
+  // pre-split BF16x3 stages: their batches, items and terms (for retile), and
+  // the pair-group buffers of x, T^T and the used L and R blocks
+  bool want_pre_ = true, pre_ = false;
+  GemmBatch p1_, p2_;
+  std::vector<GemmItem> pitems1_, pitems2_;
+  std::vector<GemmContrib> pterms1_, pterms2_;
+  DevBuf<double> Xs_, Ts_, LRs_;
+  DevBuf<PackTask> d_xpack_;
+  int nxpack_ = 0;
+  int64_t xpack_groups_ = 0;
   // items and terms of both stages before tiling, kept for retile()
   std::vector<GemmItem> items1_, items2_;
   std::vector<GemmContrib> terms1_, terms2_;

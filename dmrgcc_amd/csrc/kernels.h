@@ -35,7 +35,16 @@ enum Precision : int {
 // contiguous dimension sit two doubles apart (one part of an interleaved
 // complex matrix); lda / ldb / ldc and offsets still count doubles.
 enum : int { kTransA = 1, kTransB = 2, kRelA = 4, kRelB = 8, kStrided2 = 16 };
-enum : int { kRelC = 1, kStrided2C = 2 };
+enum : int { kRelC = 1, kStrided2C = 2, kPairC = 4 };
+// Pre-split operands (GemmBatch::presplit, BF16x3 only): every operand is
+// stored k-contiguous as bf16 pair groups, 32 bytes per 8 consecutive k of
+// one row: the 8 hi halves, then the 8 lo halves (x = hi + lo, hi = bf16(x),
+// lo = bf16(x - hi), from x rounded to FP32), rows padded with zero k to a
+// whole group.  A is M x K by rows (m), B is stored N x K by rows (n); lda /
+// ldb count groups per row and K is the real k extent.  kPairC on an item:
+// C is written the same way, transposed (row n, groups over m; ldc in
+// groups), so that it can be the B operand of a later pre-split batch.
+// Pointers and offsets of pair data count doubles (4 per group).
 struct GemmContrib {
   const double* A;
   const double* B;
@@ -115,6 +124,8 @@ struct GemmBatch {
   // 64x64 workgroup tiles (environment DMRGCC_TILE)
   int tile_mode = default_tile_mode();
   bool wave_tiles = false;  // set by finalize: four 32x32 tiles per workgroup
+  bool presplit = false;    // operands in pair groups (64x64 tiles, BF16x3 only)
+  static constexpr int kPreTerms = 128;  // terms per pre-split item (layout cuts longer lists)
   // persistent grid: at most grid_cap workgroups walk the tiles with a
   // static stride (0: one workgroup per tile, or per four wave tiles)
   int grid_cap = default_grid_cap();
@@ -181,6 +192,22 @@ struct ZTerm {
 };
 void zgemm_item(GemmBatch& g, double* C, int64_t ldc, int M, int N, int accumulate, int cflags,
                 const std::vector<ZTerm>& terms);
+
+// Pair-group packing of one matrix view: element (r, k) = src[r*srs + k*sks]
+// for r < rows, k < K goes to row r, group k/8 of dst (G = ceil(K/8) groups
+// per row, 4 doubles each), the padding k of the last group as zeros.
+struct PackTask {
+  const double* src;
+  int64_t srs, sks;
+  int rows, K;
+  double* dst;
+};
+// One launch for all tasks (grid y over a task's rows x groups, max_groups
+// the largest of those counts).  A non-null `base`
+// makes every src an element offset from it (the input vector of an H_eff
+// apply).
+void launch_pack(const PackTask* dev_tasks, int ntasks, int64_t max_groups, const double* base,
+                 hipStream_t stream);
 
 // Strided 2-D copy task: dst[i*dsr + j*dsc] = alpha * src[i*ssr + j*ssc]
 // for i < rows, j < cols.  src == nullptr writes zeros.  The tasks of one

@@ -99,6 +99,16 @@ typedef double __attribute__((address_space(1)))* gdptr;
 __device__ inline gcdptr gload_ptr(const double* p) { return (gcdptr)p; }
 __device__ inline gdptr gstore_ptr(double* p) { return (gdptr)p; }
 
+// Workgroup barrier that orders LDS traffic only: __syncthreads() also waits
+// for every global load in flight (its fence covers all address spaces),
+// which would drain the operand loads issued for later K-steps.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+
 // v[q] = sc * p[q * step] for q < nv, 0 for q >= nv (nv may be <= 0).  A
 // partial run reads clamped addresses (its last valid element) without
 // per-element branches; a run with no valid element issues no loads.  p must
@@ -397,6 +407,24 @@ __device__ inline void store_planes(bool kcont, int tid, int kofs, const double 
   }
 }
 
+// acc[a][b] += lo.hi, hi.lo, hi.hi of the 2 x 2 blocks of a wave, issued as
+// three passes over the four independent accumulators
+__device__ inline void bf16x3_products(const bf8v (&ah)[2], const bf8v (&al)[2], const bf8v (&bh)[2],
+                                       const bf8v (&bl)[2], f4v (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+}
+
 template <bool kLoop = false>
 __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
     const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
@@ -466,14 +494,9 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
         bh[b] = *reinterpret_cast<const bf8v*>(&Bh[rr][kc]);
         bl[b] = *reinterpret_cast<const bf8v*>(&Bl[rr][kc]);
       }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        }
+      // each product over all four blocks before the next, so no MFMA waits
+      // for the one before it (the per-block order lo.hi, hi.lo, hi.hi stays)
+      bf16x3_products(ah, al, bh, bl, acc);
     }
     __syncthreads();
     c = c2; k0 = k2; cb = nc;
@@ -494,6 +517,220 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(
         }
       }
   if constexpr (!kLoop) break;  // one tile per workgroup: no loop at all
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pre-split BF16x3 GEMM (GemmBatch::presplit).  The operands arrive as bf16
+// pair groups, so a lane's MFMA operand (8 consecutive k of one row, hi and
+// lo) is one 32-byte group: a K-step of the 64x64 tile is one 16-byte load
+// pair per thread and operand, copied to LDS as it is (no FP64 -> bf16
+// conversion, half the bytes of FP64 operands).  Four waves of 32x32 (2 x 2
+// blocks of 16x16); three K-steps (in the same or the following terms) are
+// in flight in registers while the current one is multiplied.
+// Each contribution accumulates in FP32 and is added, times its scale, into
+// an FP64 tile total.
+// ---------------------------------------------------------------------------
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef const u4v __attribute__((address_space(1)))* gcu4ptr;
+
+// one thread's share of a K-step: group 4 s + tid % 4 of row tid / 4 of the
+// A tile and of the B tile (zeros past the item or the term's k)
+struct PairRegs {
+  u4v ah, al, bh, bl;
+  bool oka, okb;  // in range; applied when the registers are stored, so that
+                  // nothing waits for the loads before then
+};
+
+__device__ inline void load_pair_regs(const GemmContrib& cb, const double* __restrict__ in_base, int s,
+                                      int m0, int M, int n0, int N, int tid, PairRegs& r) {
+  const double* A = (cb.flags & kRelA) ? in_base + (intptr_t)cb.A : cb.A;
+  const double* B = (cb.flags & kRelB) ? in_base + (intptr_t)cb.B : cb.B;
+  const int G = (cb.K + 7) >> 3;
+  const int g = 4 * s + (tid & 3), row = tid >> 2;
+  const bool gok = g < G;
+  const gcu4ptr pa = (gcu4ptr)A, pb = (gcu4ptr)B;
+  const int64_t ia = ((int64_t)min(m0 + row, M - 1) * cb.lda + min(g, G - 1)) * 2;
+  const int64_t ib = ((int64_t)min(n0 + row, N - 1) * cb.ldb + min(g, G - 1)) * 2;
+  r.ah = pa[ia];
+  r.al = pa[ia + 1];
+  r.bh = pb[ib];
+  r.bl = pb[ib + 1];
+  r.oka = gok && m0 + row < M;
+  r.okb = gok && n0 + row < N;
+}
+
+__global__ __launch_bounds__(kThreads) void gemm_bf16x3_pre_kernel(
+    const GemmTile* __restrict__ tiles, const GemmItem* __restrict__ items,
+    const GemmContrib* __restrict__ cons, const double* __restrict__ in_base,
+    double* __restrict__ out_base) {
+  constexpr int LD = 40;  // bf16 per LDS row: 32 k + 8 (16-byte aligned rows, as the split kernel)
+  __shared__ GemmContrib scon[GemmBatch::kPreTerms];  // the item's terms
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Al[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[kTile][LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bl[kTile][LD];
+  const GemmTile tl = tiles[blockIdx.x];
+  if (tl.item < 0) return;  // padding of the XCD-interleaved order
+  const GemmItem it = items[tl.item];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int M = it.M, N = it.N;
+  const int m0 = tl.m0, n0 = tl.n0;
+  const int am0 = m0 + wm * 32, bn0 = n0 + wn * 32;
+  for (int e = tid; e < it.c1 - it.c0; e += blockDim.x) scon[e] = cons[it.c0 + e];
+  __syncthreads();
+  // Terms are read from LDS only (the layout caps a pre-split item at
+  // GemmBatch::kPreTerms terms): a read that may be LDS or global would be a FLAT load,
+  // whose wait also drains the operand loads in flight.
+  auto con = [&](int c) -> GemmContrib { return scon[c - it.c0]; };
+  f4v acc[2][2];
+  double tot[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = f4v{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) tot[a][b][v] = 0.0;
+    }
+  // Steps in order (term, 32-k step).  The load cursor runs three steps
+  // ahead of the compute cursor: a thread's share of a step is 64 bytes, so
+  // a ring of three register slots keeps three steps in flight per workgroup
+  // while one LDS buffer serves the step being multiplied.  Every term has
+  // K > 0 (the plan drops empty ones), and the loads are issued without
+  // branches (past the last step they re-read a valid step, marked out of
+  // range), so the compiler's waits stay per slot.
+  int pc = it.c0, ps = 0;
+  int cc = it.c0, cs = 0;
+  GemmContrib ccb = con(cc);
+  auto load_next = [&](PairRegs& r) -> bool {
+    const bool valid = pc < it.c1;
+    const GemmContrib cb = con(valid ? pc : it.c1 - 1);
+    load_pair_regs(cb, in_base, valid ? ps : 0, m0, M, n0, N, tid, r);
+    r.oka = r.oka && valid;
+    r.okb = r.okb && valid;
+    if (valid && 32 * (++ps) >= cb.K) {
+      ps = 0;
+      ++pc;
+    }
+    return valid;
+  };
+  auto step = [&](PairRegs& r) {
+    lds_barrier();  // the previous step's LDS reads are done (the loads in flight stay so)
+    {
+      const int row = tid >> 2, k = 8 * (tid & 3);
+      const u4v z{0, 0, 0, 0};
+      *reinterpret_cast<u4v*>(&Ah[row][k]) = r.oka ? r.ah : z;
+      *reinterpret_cast<u4v*>(&Al[row][k]) = r.oka ? r.al : z;
+      *reinterpret_cast<u4v*>(&Bh[row][k]) = r.okb ? r.bh : z;
+      *reinterpret_cast<u4v*>(&Bl[row][k]) = r.okb ? r.bl : z;
+    }
+    lds_barrier();
+    const int kc = 8 * (lane >> 4);
+    bf8v ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int rr = wm * 32 + a * 16 + (lane & 15);
+      ah[a] = *reinterpret_cast<const bf8v*>(&Ah[rr][kc]);
+      al[a] = *reinterpret_cast<const bf8v*>(&Al[rr][kc]);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int rr = wn * 32 + b * 16 + (lane & 15);
+      bh[b] = *reinterpret_cast<const bf8v*>(&Bh[rr][kc]);
+      bl[b] = *reinterpret_cast<const bf8v*>(&Bl[rr][kc]);
+    }
+    bf16x3_products(ah, al, bh, bl, acc);
+    if (32 * (++cs) >= ccb.K) {  // term done: scale into the FP64 total
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tot[a][b][v] += ccb.scale * (double)acc[a][b][v];
+          acc[a][b] = f4v{0, 0, 0, 0};
+        }
+      cs = 0;
+      if (++cc < it.c1) ccb = con(cc);
+    }
+  };
+  PairRegs r0, r1, r2;
+  if (it.c1 <= it.c0) return;  // (not tiled by the layout)
+  bool v0 = load_next(r0);
+  bool v1 = load_next(r1);
+  bool v2 = load_next(r2);
+  while (v0) {
+    step(r0);
+    v0 = load_next(r0);
+    if (!v1) break;
+    step(r1);
+    v1 = load_next(r1);
+    if (!v2) break;
+    step(r2);
+    v2 = load_next(r2);
+  }
+  double* Cb = (it.flags & kRelC) ? out_base + (intptr_t)it.C : it.C;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int gm = am0 + a * 16 + 4 * (lane >> 4);  // rows gm .. gm + 3
+      const int gn = bn0 + b * 16 + (lane & 15);
+      if (gn >= N) continue;
+      if (it.flags & kPairC) {
+        // row gn of the transposed pair matrix, group gm / 8, slots gm % 8 ..
+        __bf16 hv[4], lv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float x = (float)tot[a][b][v];
+          hv[v] = (__bf16)x;
+          lv[v] = (__bf16)(x - (float)hv[v]);
+        }
+        __bf16* g = reinterpret_cast<__bf16*>(Cb + ((int64_t)gn * it.ldc + (gm >> 3)) * 4) + (gm & 7);
+        if (gm + 3 < M) {
+          typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<bf4v*>(g) = bf4v{hv[0], hv[1], hv[2], hv[3]};
+          *reinterpret_cast<bf4v*>(g + 8) = bf4v{lv[0], lv[1], lv[2], lv[3]};
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (gm + v < M) {
+              g[v] = hv[v];
+              g[v + 8] = lv[v];
+            }
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (gm + v >= M) continue;
+          gdptr p = gstore_ptr(Cb) + (int64_t)(gm + v) * it.ldc + gn;
+          *p = it.accumulate ? (*p + tot[a][b][v]) : tot[a][b][v];
+        }
+      }
+    }
+}
+
+// Pair-group packing (PackTask), one thread per (row, group)
+__global__ __launch_bounds__(256) void pack_pairs_kernel(const PackTask* __restrict__ tasks,
+                                                         const double* __restrict__ base) {
+  const PackTask t = tasks[blockIdx.x];
+  const int G = (t.K + 7) >> 3;
+  const int64_t n = (int64_t)t.rows * G;
+  const double* src = base ? base + (intptr_t)t.src : t.src;
+  for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.y * blockDim.x) {
+    const int r = (int)(e / G), g = (int)(e % G);
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = 8 * g + q;
+      v[q] = k < t.K ? src[(int64_t)r * t.srs + (int64_t)k * t.sks] : 0.0;
+    }
+    bf8v h, l;
+    split8(v, h, l);
+    double* d = t.dst + 4 * e;
+    *reinterpret_cast<bf8v*>(d) = h;
+    *reinterpret_cast<bf8v*>(d + 2) = l;
   }
 }
 
@@ -762,14 +999,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
       bh[b] = *reinterpret_cast<const bf8v*>(&bh_[rr][kc]);
       bl[b] = *reinterpret_cast<const bf8v*>(&bl_[rr][kc]);
     }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-      }
+    bf16x3_products(ah, al, bh, bl, acc);
     wave_lds_sync();
     c = c2; k0 = k2; cb = nc;
   }
@@ -2684,8 +2914,9 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   part_items.clear();
   int64_t psize = 0;
   if (!group_items.empty() && group_items.back() != (int)items.size()) end_group();
-  // split-K: chunk the contribution lists of heavy items
-  if (split_k > 0 && group_items.empty()) {
+  // split-K: chunk the contribution lists of heavy items (pre-split batches:
+  // also every item with more than kPreTerms terms, whatever split_k)
+  if ((split_k > 0 || presplit) && group_items.empty()) {
     std::vector<GemmItem> extra;
     struct Pending { size_t item; int first_extra, n; };
     std::vector<Pending> pend;
@@ -2694,12 +2925,15 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
       GemmItem& it = items[i];
       int64_t ktot = 0;
       for (int c = it.c0; c < it.c1; ++c) ktot += contribs[c].K;
-      if (ktot <= split_k || it.c1 - it.c0 < 2) continue;
-      // chunk boundaries by accumulated K
+      const bool heavy = split_k > 0 && ktot > split_k;
+      const bool many = presplit && it.c1 - it.c0 > kPreTerms;
+      if (!(heavy || many) || it.c1 - it.c0 < 2) continue;
+      // chunk boundaries by accumulated K (and term count)
       std::vector<int> cuts{it.c0};
       int64_t acc = 0;
       for (int c = it.c0; c < it.c1; ++c) {
-        if (acc >= split_k && c > cuts.back()) { cuts.push_back(c); acc = 0; }
+        const bool full = (split_k > 0 && acc >= split_k) || (presplit && c - cuts.back() >= kPreTerms);
+        if (full && c > cuts.back()) { cuts.push_back(c); acc = 0; }
         acc += contribs[c].K;
       }
       cuts.push_back(it.c1);
@@ -2863,7 +3097,11 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   if (tiles.empty()) return;
   require(precision >= kFP64 && precision <= kBF16x3, "unknown precision", kErrArgument);
   const dim3 block(kThreads);
-  if (wave_tiles) {
+  if (presplit) {
+    require(precision == kBF16x3 && !wave_tiles, "pre-split batches are BF16x3 on 64x64 tiles", kErrInternal);
+    hipLaunchKernelGGL(gemm_bf16x3_pre_kernel, dim3((unsigned)tiles.size()), block, 0, stream, dev_tiles,
+                       dev_items, dev_contribs, in, out);
+  } else if (wave_tiles) {
     const int ng = (int)(tiles.size() / 4);
     const dim3 grid(launch_grid(ng));
     if ((int)grid.x < ng)
@@ -3005,6 +3243,14 @@ void launch_copies(const std::vector<CopyTask>& tasks, DevBuf<CopyTask>& scratch
   DMRG_HIP(hipGetLastError());
   // the descriptor buffer may be reused by the next call: wait for the kernel
   DMRG_HIP(hipStreamSynchronize(stream));
+}
+
+void launch_pack(const PackTask* dev_tasks, int ntasks, int64_t max_groups, const double* base,
+                 hipStream_t stream) {
+  if (ntasks <= 0) return;
+  const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (max_groups + 255) / 256));
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3((unsigned)ntasks, gy), dim3(256), 0, stream, dev_tasks, base);
+  DMRG_HIP(hipGetLastError());
 }
 
 void dot(const double* x, const double* y, int64_t n, double* out, DevBuf<double>& partial,

@@ -130,6 +130,37 @@ def test_heff_mixed_within_tolerance(prec, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("split_k", ["0", "40"])
+def test_heff_presplit_matches_unpacked_bf16x3(monkeypatch, split_k):
+    # the pre-split BF16x3 stages (operands packed once into bf16 pair
+    # groups, T^T written as pairs) against the on-the-fly split: the same
+    # products, each term scaled in FP64 instead of before the split
+    dmo, site = state("4x3_d4_left")
+    monkeypatch.setenv("DMRGCC_SPLIT_K", split_k)
+    monkeypatch.setenv("DMRGCC_TILE", "64")
+    pre, lat = make_plan(dmo, site)
+    monkeypatch.setenv("DMRGCC_PRESPLIT", "0")
+    old, _ = make_plan(dmo, site)
+    n = lat.ket.dim()
+    x = np.random.default_rng(11).standard_normal(n)
+    y0 = np.random.default_rng(12).standard_normal(n)
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x, y0.copy())
+    got = {}
+    for key, plan in (("pre", pre), ("old", old)):
+        yd = torch.tensor(y0, device="cuda")
+        plan.apply(torch.tensor(x, device="cuda"), yd, dm.BF16X3)
+        got[key] = yd.cpu().numpy()
+        err = np.abs(got[key] - ref).max() / np.abs(ref).max()
+        assert err < 2e-4, (key, err)
+    diff = np.abs(got["pre"] - got["old"]).max() / np.abs(ref).max()
+    assert diff < 2e-5, diff
+    # the FP64 and FP32 forms of the pre-split plan keep the unpacked T
+    for prec, tol in ((dm.FP64, 1e-12), (dm.FP32, 1e-5)):
+        yd = torch.tensor(y0, device="cuda")
+        pre.apply(torch.tensor(x, device="cuda"), yd, prec)
+        assert np.abs(yd.cpu().numpy() - ref).max() < tol * np.abs(ref).max()
+
+
 def test_heff_is_symmetric_and_equals_dense():
     dmo, site = state("3x2_d3")
     plan, lat = make_plan(dmo, site)
