@@ -222,7 +222,14 @@ int dmrgcc_zheff_create(int d, int lb, int rb, const double* W, int nl, int nr, 
                         const int64_t* R_off, const double* R_data, void* stream,
                         dmrgcc_heff** out);
 /* y += H_eff x  (the reference's accumulate-into-state semantics).
- * x, y: device vectors of psi_dim doubles. */
+ * x, y: device vectors of psi_dim doubles.  BF16x3 on a real plan tiled
+ * 64x64 runs the pre-split form: the plan's L and R blocks are packed once
+ * into bf16 hi/lo pairs, x is packed per apply, and every term is scaled in
+ * FP64 after its product.  A batch (dmrgcc_heff_batch_*) and the 32x32
+ * tiling split the FP64 operands on the fly, with the scale folded into the
+ * split; the two forms agree to the BF16x3 accuracy, not bit for bit.  The
+ * environment variable DMRGCC_PRESPLIT=0, read when a plan is built, keeps
+ * that plan on the on-the-fly form. */
 int dmrgcc_heff_apply(dmrgcc_heff* h, const double* x, double* y, int precision);
 /* y = alpha * H_eff x + beta * y on `stream` (NULL: the plan's own stream),
  * returning without waiting: only kernel launches (no copies, memsets or

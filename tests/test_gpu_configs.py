@@ -236,7 +236,10 @@ def test_spinful_m64_trajectory_matches_oracle():
 # --------------------------------------------------------------------------
 # 64-chain scan
 # --------------------------------------------------------------------------
-def test_chain_scan_batch_matches_plans_and_oracle():
+def test_chain_scan_batch_matches_plans_and_oracle(monkeypatch):
+    # the batch runs the on-the-fly split BF16x3 form: the plans are built
+    # without the pre-split stages to compare bit for bit
+    monkeypatch.setenv("DMRGCC_PRESPLIT", "0")
     draws = workloads.chain_draws(64)
     runs, plans, feasible, infeasible = [], [], [], []
     t0 = time.time()
