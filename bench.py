@@ -358,10 +358,11 @@ def main():
 
     def s2_kernel(name):
         wave = tiling(name)[0] == 32
-        base = {"fp64": "gemm_%s<double>", "bf16x3": "gemm_bf16x3%s"}[name]
         if name == "fp64":
-            return base % ("wave_kernel" if wave else "mfma_kernel")
-        return base % ("_wave_kernel" if wave else "_kernel")
+            return "gemm_%s<double>" % ("wave_kernel" if wave else "mfma_kernel")
+        if infos[name].get("presplit"):
+            return "gemm_bf16x3_pre_kernel"
+        return "gemm_bf16x3%s" % ("_wave_kernel" if wave else "_kernel")
 
     def s2_grid(name):
         # 32x32 wave tiles run four tiles per workgroup

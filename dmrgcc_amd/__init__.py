@@ -721,7 +721,7 @@ class HeffPlan:
         self._set_words()
 
     def _set_words(self):
-        v = np.zeros(10)
+        v = np.zeros(11)
         _check(lib().dmrgcc_heff_info(self._h, _dp(v)))
         self.words = int(v[0]) * (2 if self.complex else 1)
 
@@ -761,11 +761,12 @@ class HeffPlan:
         return self
 
     def info(self):
-        v = np.zeros(10)
+        v = np.zeros(11)
         _check(lib().dmrgcc_heff_info(self._h, _dp(v)))
         return {"psi_dim": int(v[0]), "flops_ref": v[1], "flops_exec": v[2], "bytes_min": v[3],
                 "tasks": int(v[4]), "tiles": int(v[5]), "flops_stage1": v[6],
-                "flops_stage2": v[7], "tiles_stage1": int(v[8]), "tiles_stage2": int(v[9])}
+                "flops_stage2": v[7], "tiles_stage1": int(v[8]), "tiles_stage2": int(v[9]),
+                "presplit": bool(v[10])}
 
     def time(self, precision=FP64, reps=20):
         """Mean milliseconds per stage-1 launch, stage-2 launch (with its

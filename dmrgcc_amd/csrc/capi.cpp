@@ -652,6 +652,7 @@ int dmrgcc_heff_info(const dmrgcc_heff* h, double* info) {
     info[7] = h->plan.stage2().flops;
     info[8] = (double)h->plan.stage1().tiles.size();
     info[9] = (double)h->plan.stage2().tiles.size();
+    info[10] = h->plan.use_pre(kBF16x3) ? 1.0 : 0.0;
   });
 }
 

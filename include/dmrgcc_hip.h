@@ -243,7 +243,9 @@ int dmrgcc_heff_apply_async(dmrgcc_heff* h, const double* x, double* y, double a
 /* info[0] = psi_dim, [1] = F_ref (reference-order flops), [2] = F_exec,
  * [3] = minimum HBM bytes per matvec, [4] = stage-1 tasks, [5] = tiles,
  * [6] = stage-1 flops, [7] = stage-2 flops, [8] = stage-1 tiles,
- * [9] = stage-2 tiles. */
+ * [9] = stage-2 tiles, [10] = 1 when a BF16x3 matvec runs on the pre-split
+ * bf16 pair operands (gemm_bf16x3_pre_kernel), else 0.  info holds 11
+ * doubles. */
 int dmrgcc_heff_info(const dmrgcc_heff* h, double* info);
 int dmrgcc_heff_destroy(dmrgcc_heff* h);
 
