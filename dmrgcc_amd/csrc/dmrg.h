@@ -166,9 +166,10 @@ class HeffPlan {
   // stages on the pre-split kernel.  FP64 and FP32 keep the FP64 T.
   void set_presplit(bool on) { want_pre_ = on; }
   bool presplit() const { return pre_; }
+  // whether an apply in `precision` runs on the pre-split operands
+  bool use_pre(int precision) const { return pre_ && precision == kBF16x3 && !s1_.wave_tiles; }
 
  private:
-  bool use_pre(int precision) const { return pre_ && precision == kBF16x3 && !s1_.wave_tiles; }
   GemmBatch s1_, s2_;
   int want_tile_ = -1, want_split_ = -1;
   // pre-split BF16x3 stages: their batches, items and terms (for retile), and
