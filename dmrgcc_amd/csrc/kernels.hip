@@ -1009,21 +1009,41 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_wave_kernel(
 }
 
 // split-K reduction: C (+)= sum of the partial buffers, in chunk order
+// (s = ((0 + p_1) + p_2) + ..., then C += s).  Thread t of block x owns
+// elements 4 (256 x + t) .. + 3 of task y's M x N output, read and written
+// row-major; the loads of four partials for all four elements are issued
+// together, so a thread waits for memory once per four partials (the rows-
+// by-workgroup form waited once per element row).
+constexpr int kSplitEl = 4;
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitTask* __restrict__ tasks,
                                                            double* __restrict__ out_base) {
   const SplitTask t = tasks[blockIdx.y];
+  const int64_t mn = (int64_t)t.M * t.N;
+  const int64_t e0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kSplitEl;
+  if (e0 >= mn) return;
   gdptr C = gstore_ptr((t.flags & kRelC) ? out_base + (intptr_t)t.C : t.C);
   gcdptr part = gload_ptr(t.part);
-  const int64_t mn = (int64_t)t.M * t.N;
   const int es = (t.flags & kStrided2C) ? 2 : 1;
-  // rows over the workgroups, columns over the threads
-  for (int i = blockIdx.x; i < t.M; i += gridDim.x) {
-    for (int j = threadIdx.x; j < t.N; j += blockDim.x) {
-      const int64_t e = (int64_t)i * t.N + j;
-      double s = 0.0;
-      for (int c = 0; c < t.nparts; ++c) s += part[c * mn + e];
-      C[(int64_t)i * t.ldc + (int64_t)j * es] += s;
-    }
+  double sum[kSplitEl];
+#pragma unroll
+  for (int q = 0; q < kSplitEl; ++q) sum[q] = 0.0;
+  for (int c0 = 0; c0 < t.nparts; c0 += 4) {
+    double v[4][kSplitEl];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+      for (int q = 0; q < kSplitEl; ++q)  // clamped addresses, always in bounds
+        v[cc][q] = part[(int64_t)min(c0 + cc, t.nparts - 1) * mn + min(e0 + q, mn - 1)];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+      if (c0 + cc < t.nparts)
+#pragma unroll
+        for (int q = 0; q < kSplitEl; ++q) sum[q] += v[cc][q];
+  }
+#pragma unroll
+  for (int q = 0; q < kSplitEl; ++q) {
+    const int64_t e = e0 + q;
+    if (e < mn) C[(e / t.N) * t.ldc + (e % t.N) * es] += sum[q];
   }
 }
 
@@ -3118,9 +3138,9 @@ void GemmBatch::launch(hipStream_t stream, int precision, const double* in, doub
   }
   DMRG_HIP(hipGetLastError());
   if (reduce && !splits.empty()) {
-    int mx = 0;
-    for (const SplitTask& t : splits) mx = std::max(mx, t.M);
-    const unsigned gx = (unsigned)std::max(1, std::min(mx, 64));
+    int64_t mx = 0;
+    for (const SplitTask& t : splits) mx = std::max(mx, (int64_t)t.M * t.N);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, (mx + 256 * kSplitEl - 1) / (256 * kSplitEl));
     hipLaunchKernelGGL(split_reduce_kernel, dim3(gx, (unsigned)splits.size()), dim3(256), 0, stream,
                        dev_splits, out);
     DMRG_HIP(hipGetLastError());

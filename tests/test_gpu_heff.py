@@ -80,6 +80,42 @@ def test_heff_split_k(monkeypatch, split_k, prec, tile):
     assert err < (1e-12 if prec == dm.FP64 else 2e-4), err
 
 
+@pytest.mark.parametrize("tile", ["32", "64"])
+@pytest.mark.parametrize("prec", [dm.FP64, dm.FP32, dm.BF16X3])
+def test_heff_split_k_repeats_bit_identical(monkeypatch, prec, tile):
+    # heavy output blocks cut into split-K chunks: back-to-back matvecs, and
+    # graph replays with eager matvecs between them, give the same bits
+    dmo, site = state("4x3_d4_left")
+    monkeypatch.setenv("DMRGCC_SPLIT_K", "40")
+    monkeypatch.setenv("DMRGCC_TILE", tile)
+    plan, lat = make_plan(dmo, site)
+    n = lat.ket.dim()
+    x = torch.tensor(np.random.default_rng(9).standard_normal(n), device="cuda")
+    outs = []
+    for _ in range(4):
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        plan.apply(x, y, prec)
+        outs.append(y.cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    yg = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        plan.apply_async(x, yg, 1.0, 0.0, prec, stream=s)
+    for rep in range(3):
+        ye = torch.zeros(n, dtype=torch.float64, device="cuda")
+        plan.apply(x, ye, prec)   # eager launches between the replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(yg.cpu().numpy(), outs[0]), rep
+        assert np.array_equal(ye.cpu().numpy(), outs[0]), rep
+    ref = heff_apply(lat, dmo.W[site], dmo.Lenv[site], dmo.Renv[site], x.cpu().numpy(), np.zeros(n))
+    tol = {dm.FP64: 1e-12, dm.FP32: 1e-5, dm.BF16X3: 2e-4}[prec]
+    assert np.abs(outs[0] - ref).max() < tol * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("prec", [dm.FP64, dm.BF16X3])
 def test_heff_tune_keeps_results(prec):
     # the autotuner re-tiles the plan with every candidate and keeps the
