@@ -273,6 +273,12 @@ double* Context::lanczos_host() {
   return lz_host_;
 }
 
+double* Context::lanczos_host_dev() {
+  // the device's address of the same pinned buffer, for kernels that write it
+  if (!lz_host_dev_) DMRG_HIP(hipHostGetDevicePointer((void**)&lz_host_dev_, lanczos_host(), 0));
+  return lz_host_dev_;
+}
+
 std::vector<hipEvent_t> Context::events(int n) {
   while ((int)events_.size() < n) {
     hipEvent_t e;
@@ -679,10 +685,8 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
         xmax = std::max<int64_t>(xmax, (int64_t)b.rows * groups(b.cols));
       }
       if (!lrpack.empty()) {
-        DevBuf<PackTask> dl;
-        upload(dl, lrpack, stream);
-        launch_pack(dl.data(), (int)lrpack.size(), lrmax, nullptr, stream);
-        DMRG_HIP(hipStreamSynchronize(stream));  // dl is released here
+        upload(d_lrpack_, lrpack, stream);
+        launch_pack(d_lrpack_.data(), (int)lrpack.size(), lrmax, nullptr, stream);
       }
       upload(d_xpack_, xpack, stream);
       nxpack_ = (int)xpack.size();
@@ -1729,29 +1733,42 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
   double* coef_dev = sc + kSlot * num_iter;
   double* rq_dev = coef_dev + num_iter;
   double* host = cx.lanczos_host();
+  double* host_dev = cx.lanczos_host_dev();
   DevBuf<double> psi((size_t)num_iter * nd), h((size_t)nd);
   double* P = psi.data();
   DMRG_HIP(hipMemcpyAsync(P, ket, nd * sizeof(double), hipMemcpyDeviceToDevice, cx.stream));
   fill(h.data(), nd, 0.0, cx.stream);
   std::vector<double> alpha, beta;
   double old_energy = 10000.0, energy = 0.0;
-  // partial-sum buffers of the fused vector steps (two alternate in the
-  // Gram-Schmidt chain)
+  // partial-sum buffers of the fused vector steps: part_a holds the alpha
+  // partials, part_b the beta^2 partials, and the Gram-Schmidt chain
+  // alternates between part_c and part_a
   cx.partial.ensure(3 * 2 * kDotPartials);
   double* part_a = cx.partial.data();
   double* part_b = part_a + 2 * kDotPartials;
+  double* part_c = part_b + 2 * kDotPartials;
   std::vector<hipEvent_t> done = cx.events(num_iter);
   // psi_{i+1} = h / beta_i, h = -beta_i psi_i, then GramSchmidtMethod
   // (real_tensor_lanczos.cc:140-151, :188-201; complex_tensor_lanczos.cc:
   // 178-192): for j = 0..i, c = <psi_j|psi_{i+1}> on the updated vector,
   // psi_{i+1} -= c psi_j; then psi_{i+1} /= |psi_{i+1}|.  One launch per
   // projection: each forms c from the previous launch's partial sums and the
-  // partials of the next overlap.  beta_i is read on the device.
-  auto next_vector = [&](int i) {
+  // partials of the next overlap.  beta_i is formed on the device.  With
+  // `publish`, the first launch also forms alpha_i and beta_i^2 from
+  // iteration i's partials and writes them to slot i and to pinned host
+  // memory, and done[i] is recorded right after it: iteration i needs no
+  // launch of its own for its coefficients, and no copy.
+  auto next_vector = [&](int i, bool publish) {
     double* pn = P + (int64_t)(i + 1) * nd;
-    double* pin = part_a;
-    double* pout = part_b;
-    lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, sc + kSlot * i + 2);
+    double* pin = part_c;
+    double* pout = part_a;
+    if (publish) {
+      lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, nullptr, part_a, part_b,
+              sc + kSlot * i, host_dev + kSlot * i);
+      DMRG_HIP(hipEventRecord(done[i], cx.stream));
+    } else {
+      lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, sc + kSlot * i + 2);
+    }
     for (int j = 0; j <= i; ++j) {
       const double* xnext = j < i ? P + (int64_t)(j + 1) * nd : nullptr;
       lz_project(pn, P + (int64_t)j * nd, xnext, n, cz, pin, pout, cx.stream);
@@ -1759,32 +1776,33 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     }
     lz_normalize(pn, n, cz, pin, cx.stream);
   };
-  // iteration i on the stream: psi_i (when i > 0), h += H psi_i,
-  // alpha = <psi_i|h>, h -= alpha psi_i, beta^2 = <h|h>, then the three
-  // scalars to pinned host memory behind event done[i]
+  // the coefficients of iteration i by a launch of their own: (alpha,
+  // beta^2) to slot i and to pinned host memory, then done[i]
+  auto publish_alone = [&](int i) {
+    lz_finals(part_a, part_b, n, cz, sc + kSlot * i, cx.stream, host_dev + kSlot * i);
+    DMRG_HIP(hipEventRecord(done[i], cx.stream));
+  };
+  // iteration i on the stream: psi_i (when i > 0, publishing iteration i-1's
+  // coefficients unless `blocking`), h += H psi_i, alpha = <psi_i|h>,
+  // h -= alpha psi_i, beta^2 = <h|h> (as partials)
+  bool blocking = false;
   auto launch_iteration = [&](int i) {
-    if (i > 0) next_vector(i - 1);
+    if (i > 0) next_vector(i - 1, !blocking);
     double* pi = P + (int64_t)i * nd;
     plan.apply(pi, h.data(), precision, cx.stream);
     lz_dot_partial(pi, h.data(), n, cz, part_a, cx.stream);   // alpha = <psi|H|psi>
     lz_alpha(h.data(), pi, n, cz, part_a, part_b, cx.stream);  // h -= alpha psi; <h|h>
-    lz_finals(part_a, part_b, n, cz, sc + kSlot * i, cx.stream);
-    DMRG_HIP(hipMemcpyAsync(host + kSlot * i, sc + kSlot * i, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                            cx.stream));
-    DMRG_HIP(hipEventRecord(done[i], cx.stream));
+    if (blocking) publish_alone(i);
   };
-  // The host decides the reference's stop for iteration i while the device
-  // already runs iteration i + 1 (launched before the host waits).  When i
-  // stops, that iteration is discarded: it writes only psi_{i+1}, h and its
-  // own slot, which the Ritz vector and the refinement do not read (the
-  // refinement clears h first).  DMRGCC_LANCZOS_BLOCKING=1 launches each
-  // iteration only after the previous stop test (same results, no overlap).
-  const bool blocking = std::getenv("DMRGCC_LANCZOS_BLOCKING") != nullptr;
+  blocking = std::getenv("DMRGCC_LANCZOS_BLOCKING") != nullptr;
   launch_iteration(0);
   for (int i = 0; i < num_iter; ++i) {
     const int k = i + 1;
     const double t0 = now();
-    if (!blocking && k < num_iter) launch_iteration(k);
+    if (!blocking) {
+      if (k < num_iter) launch_iteration(k);  // publishes iteration i first
+      else publish_alone(i);
+    }
     DMRG_HIP(hipEventSynchronize(done[i]));
     st.matvec_seconds += now() - t0;
     st.flops += plan.fref();

@@ -106,6 +106,7 @@ struct Context {
   // iteration alpha and beta^2, then the Ritz coefficients), separate from
   // the staging above so that it never waits for another copy
   double* lanczos_host();
+  double* lanczos_host_dev();  // the device's address of lanczos_host()
   // at least n events (timing disabled), reused between calls
   std::vector<hipEvent_t> events(int n);
 
@@ -113,6 +114,7 @@ struct Context {
   double* pinned_ = nullptr;
   int64_t pinned_n_ = 0;
   double* lz_host_ = nullptr;
+  double* lz_host_dev_ = nullptr;
   std::vector<hipEvent_t> events_;
 };
 
@@ -180,6 +182,7 @@ class HeffPlan {
   std::vector<GemmContrib> pterms1_, pterms2_;
   DevBuf<double> Xs_, Ts_, LRs_;
   DevBuf<PackTask> d_xpack_;
+  DevBuf<PackTask> d_lrpack_;  // the one-off L/R packing's descriptors (kept: no wait at build)
   int nxpack_ = 0;
   int64_t xpack_groups_ = 0;
   // items and terms of both stages before tiling, kept for retile()

@@ -267,13 +267,20 @@ void lz_dot_partial(const double* x, const double* y, int64_t n, bool cplx, doub
 // alpha = final(part_in); h -= alpha psi; part_out <- partials of <h|h>
 void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* part_in,
               double* part_out, hipStream_t stream);
-// out[0..1] = final(part_a) (alpha; real: out[1] = 0), out[2] = Re final(part_b) (beta^2)
+// out[0..1] = final(part_a) (alpha; real: out[1] = 0), out[2] = Re final(part_b) (beta^2);
+// out_host (device-visible pinned memory, optional) receives the same three
 void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
-               hipStream_t stream);
+               hipStream_t stream, double* out_host = nullptr);
 // nxt = h / beta; h = -beta psi; part_out <- partials of <x0|nxt>.  With
-// beta2 (device), beta = sqrt(*beta2) is formed on the device instead.
+// beta2 (device), beta = sqrt(*beta2) is formed on the device instead.  With
+// part_a / part_b (the alpha and beta^2 partials of the step that produced
+// h), the kernel forms beta itself and also does lz_finals' work: slot[0..2]
+// and slot_host[0..2] (device-visible pinned memory) receive (alpha, beta^2);
+// part_out must then be a third buffer.
 void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream, const double* beta2 = nullptr);
+             double beta, double* part_out, hipStream_t stream, const double* beta2 = nullptr,
+             const double* part_a = nullptr, const double* part_b = nullptr, double* slot = nullptr,
+             double* slot_host = nullptr);
 // c = final(part_in); y -= c xj; part_out <- partials of <xnext|y> (xnext null: <y|y>)
 void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
                 const double* part_in, double* part_out, hipStream_t stream);

@@ -1375,7 +1375,7 @@ __global__ __launch_bounds__(256) void lz_alpha_kernel(double* __restrict__ h,
 template <bool kC>
 __global__ __launch_bounds__(256) void lz_finals_kernel(const double* __restrict__ part_a,
                                                         const double* __restrict__ part_b, int np,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out, double* __restrict__ out_host) {
   double ar, ai, br, bi;
   lz_final<kC>(part_a, np, ar, ai);
   lz_final<false>(part_b, np, br, bi);
@@ -1383,6 +1383,11 @@ __global__ __launch_bounds__(256) void lz_finals_kernel(const double* __restrict
     out[0] = ar;
     out[1] = ai;
     out[2] = br;
+    if (out_host) {  // pinned host memory, read after the stream's event
+      out_host[0] = ar;
+      out_host[1] = ai;
+      out_host[2] = br;
+    }
   }
 }
 
@@ -1391,10 +1396,35 @@ __global__ __launch_bounds__(256) void lz_next_kernel(double* __restrict__ nxt, 
                                                       const double* __restrict__ psi,
                                                       const double* __restrict__ x0, int64_t n,
                                                       double beta, const double* __restrict__ beta2,
-                                                      double* __restrict__ part_out) {
-  // beta from the device (sqrt of beta^2, correctly rounded like the
-  // host's), when the host has not read it yet
-  if (beta2) beta = sqrt(beta2[0]);
+                                                      double* __restrict__ part_out,
+                                                      const double* __restrict__ part_a,
+                                                      const double* __restrict__ part_b, int np,
+                                                      double* __restrict__ slot,
+                                                      double* __restrict__ slot_host) {
+  if (part_b) {
+    // the previous step's coefficients from its partials, as lz_finals
+    // forms them: beta^2 in every block, alpha in block 0, which also
+    // publishes (alpha, beta^2) to the device slot and to pinned host memory
+    double br, bi;
+    lz_final<false>(part_b, np, br, bi);
+    beta = sqrt(br);
+    if (blockIdx.x == 0) {
+      double ar, ai;
+      lz_final<kC>(part_a, np, ar, ai);
+      if (threadIdx.x == 0) {
+        slot[0] = ar;
+        slot[1] = ai;
+        slot[2] = br;
+        slot_host[0] = ar;
+        slot_host[1] = ai;
+        slot_host[2] = br;
+      }
+    }
+  } else if (beta2) {
+    // beta from the device (sqrt of beta^2, correctly rounded like the
+    // host's), when the host has not read it yet
+    beta = sqrt(beta2[0]);
+  }
   double sr = 0.0, si = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -3337,22 +3367,27 @@ void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* 
 }
 
 void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
-               hipStream_t stream) {
+               hipStream_t stream, double* out_host) {
   const int g = dot_grid(n);
-  if (cplx) hipLaunchKernelGGL(lz_finals_kernel<true>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out);
-  else hipLaunchKernelGGL(lz_finals_kernel<false>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out);
+  if (cplx)
+    hipLaunchKernelGGL(lz_finals_kernel<true>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out, out_host);
+  else
+    hipLaunchKernelGGL(lz_finals_kernel<false>, dim3(1), dim3(256), 0, stream, part_a, part_b, g, out, out_host);
   DMRG_HIP(hipGetLastError());
 }
 
 void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream, const double* beta2) {
+             double beta, double* part_out, hipStream_t stream, const double* beta2, const double* part_a,
+             const double* part_b, double* slot, double* slot_host) {
   const int g = dot_grid(n);
+  require(!part_b || (part_a && slot && slot_host && part_out != part_a && part_out != part_b),
+          "lz_next: publishing needs both partial buffers, the slots and a separate output", kErrInternal);
   if (cplx)
     hipLaunchKernelGGL(lz_next_kernel<true>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
-                       part_out);
+                       part_out, part_a, part_b, g, slot, slot_host);
   else
     hipLaunchKernelGGL(lz_next_kernel<false>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
-                       part_out);
+                       part_out, part_a, part_b, g, slot, slot_host);
   DMRG_HIP(hipGetLastError());
 }
 
