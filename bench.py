@@ -141,8 +141,32 @@ def site_lanczos(run, plan, site, niter, dm):
     # wall time of a whole solve (start vector in HBM to Ritz vector, the
     # mixed forms' FP64 refinement matvec included) per Lanczos iteration
     per_it = {k: ms[k] / it[k] for k in ms}
+    # marginal cost of one iteration: the same solve capped at it and at
+    # it - 1 iterations (a solve that ends at its cap launches nothing
+    # beyond it), timed the same way; the difference holds iteration it's
+    # matvec, its coefficients and its it - 1 Gram-Schmidt projections, and
+    # none of the per-solve costs (start copy, Ritz vector, refinement, host
+    # return).  A solve that stops by its own test, as in a sweep, has also
+    # launched the next iteration, which it discards.
+    def capped(prec, cap):
+        best = None
+        for _ in range(4):
+            v = x0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.lanczos(v, cap, prec, refine=True)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best * 1e3
+    marg = {}
+    for name, prec in (("fp64", dm.FP64), ("fp32", dm.FP32), ("bf16x3", dm.BF16X3)):
+        if it[name] >= 2:
+            marg[name] = capped(prec, it[name]) - capped(prec, it[name] - 1)
     return {"niter": niter, "energy": e, "iterations": it, "solve_ms": ms, "ms_per_iteration": per_it,
             "matvec_ms": mv, "iteration_over_matvec": {k: per_it[k] / mv[k] for k in ms},
+            "marginal_iteration_ms": marg,
+            "marginal_iteration_over_matvec": {k: marg[k] / mv[k] for k in marg},
             "dE_rel": {k: abs(e[k] - ref) / abs(ref) for k in ("fp32", "bf16x3")}}
 
 
