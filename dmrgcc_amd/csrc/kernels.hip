@@ -1545,14 +1545,20 @@ __global__ __launch_bounds__(64) void svd_fro2_kernel(const SvdProblem* __restri
 //  1. forms the kJc x kJc Gram matrix of its columns with FP64 MFMA: the waves
 //     take interleaved 16-row slabs and keep partial 16 x 16 blocks, summed in
 //     a fixed order in LDS (deterministic);
-//  2. runs one sweep of two-sided Jacobi on it in LDS (wave 0; the rotation
-//     angles are those of the one-sided method, computed from the Gram
-//     entries), accumulating the rotation Q;
-//  3. applies Q to its columns of G and V with MFMA (slabs over the waves).
+//  2. runs one sweep of two-sided Jacobi on it in registers (wave 0,
+//     inner_sweep16; the rotation angles are those of the one-sided method,
+//     computed from the Gram entries), accumulating the rotation Q;
+//  3. applies Q to its columns of G, and the previous round's rotation to its
+//     columns of V, with MFMA (slabs over the waves).
 // Pairs whose columns are already orthogonal to tol (or numerically zero) are
 // not rotated; a visit that rotates nothing leaves its columns untouched.
 // The round time is one visit's critical path, so the visit width and the
-// number of waves per visit are compile-time tunables.
+// number of waves per visit are compile-time tunables.  DMRG_JACOBI_JB=16
+// gives 32-column visits (three local rounds of two 16-column sweeps each,
+// every column pair of the two blocks rotated once per visit): half the
+// rounds per sweep at the same sweep count, but on the 8x4 m = 1000 sweep a
+// round took 41 us against 16 us (half the visits, twice the bytes each), so
+// 16-column visits stay the default.
 #ifndef DMRG_JACOBI_JB
 #define DMRG_JACOBI_JB 8
 #endif
@@ -1565,6 +1571,7 @@ constexpr int kJw = DMRG_JACOBI_WAVES;
 constexpr int kJn = kJc / 16;  // 16 x 16 MFMA blocks per visit edge
 static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
 static_assert(kJw >= 2 && kJw <= 16, "2 to 16 waves per visit (wave 0 sweeps while the others update V)");
+static_assert(kJc == 16 || kJw >= 8, "32-column visits update their 32 x 32 Gram with eight waves");
 
 // Slabs of 16 rows handled together by one wave (their loads in flight at once)
 #ifndef DMRG_JACOBI_SLABS
@@ -1620,6 +1627,90 @@ __device__ inline void apply_rotation(double* __restrict__ X, int64_t ld, int ro
       }
     }
   }
+}
+
+// One sweep of two-sided Jacobi in registers on the 16 x 16 Gram
+// G[sel[r]][sel[c]] (r, c < 16) of an LDS array with row length LD, by one
+// wave; its rotation goes to Q[sel[r]][sel[c]].  Returns whether any pair
+// rotated (the same value in every lane).  Pairs (a, b) rotate when
+// G_ab^2 > tol2 G_aa G_bb and both norms exceed zero2.
+template <int LD>
+__device__ inline int inner_sweep16(const double (*G)[LD], double (*Q)[LD], const int* sel, int lane,
+                                    double tol2, double zero2) {
+  // The inner sweep in registers, XOR order: step d rotates the pairs
+  // (x, x ^ d), d = 1 .. 15 (every pair once).  Lane (c, q) = c + 16 q
+  // holds G[c ^ (4 q + s)][c] and Q[c ^ (4 q + s)][c] in slot s < 4 and the
+  // diagonal D = G[c][c]; every lane forms its column's rotation from the
+  // same three values as its partner (bit-identical angles).  Per step:
+  // column update G J, Q J (the partner column sits at lane ^ (d | (d >> 2)
+  // << 4), slot s ^ (d & 3)), then the row update through the symmetry of
+  // J^T G J: G'[r][c] = G'[c][r] = (G J)[c][r] cs + (G J)[c ^ d][r] s_c.
+  const int c = lane & 15, q = lane >> 4;
+  double g[4], qv[4];
+#pragma unroll
+  for (int s2 = 0; s2 < 4; ++s2) {
+    g[s2] = G[sel[c ^ (4 * q + s2)]][sel[c]];
+    qv[s2] = (4 * q + s2) == 0 ? 1.0 : 0.0;
+  }
+  double D = G[sel[c]][sel[c]];
+  int any = 0;
+#pragma unroll
+  for (int d = 1; d < 16; ++d) {
+    const int hb = d >= 8 ? 8 : d >= 4 ? 4 : d >= 2 ? 2 : 1;  // top bit of d
+    const bool lo = (c & hb) == 0;
+    const int a = lo ? c : (c ^ d);
+    const double Dp = bperm_d(D, lane ^ d);
+    // G[a ^ d][a] sits in column a at offset d: quarter d >> 2, slot d & 3
+    const double gm = bperm_d(g[d & 3], a + 16 * (d >> 2));
+    const double al = lo ? D : Dp, be = lo ? Dp : D;
+    double cs = 1.0, sn = 0.0;
+    const bool doit = al > zero2 && be > zero2 && gm * gm > tol2 * al * be;
+    if (doit) {
+      // tan(theta) = sgn(dd) gg / (|dd| + hypot(dd, gg)), dd = be - al,
+      // gg = 2 gm (the one-sided angle); rsq/rcp with one Newton step each
+      const double dd = be - al, gg = 2.0 * gm;
+      const double w = dd * dd + gg * gg;
+      double ri = __builtin_amdgcn_rsq(w);
+      ri = ri * (1.5 - 0.5 * w * ri * ri);
+      const double den = fabs(dd) + w * ri;
+      double rc = __builtin_amdgcn_rcp(den);
+      rc = rc * (2.0 - den * rc);
+      const double t = (dd >= 0.0 ? gg : -gg) * rc;
+      const double u = 1.0 + t * t;
+      double ci = __builtin_amdgcn_rsq(u);
+      ci = ci * (1.5 - 0.5 * u * ci * ci);
+      cs = ci;
+      sn = ci * t;
+      any = 1;
+    }
+    // a step that rotates no pair leaves G and Q as they are: skip its
+    // exchanges (wave-uniform test)
+    if (!__any(doit)) continue;
+    // new column a = cs a - sn b, new column b = sn a + cs b
+    const double sg = lo ? -sn : sn;
+    const int mask = d | ((d >> 2) << 4);
+    double h[4], pq[4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      const double pg = bperm_d(g[s2 ^ (d & 3)], lane ^ mask);
+      pq[s2] = bperm_d(qv[s2 ^ (d & 3)], lane ^ mask);  // all read before any update
+      h[s2] = cs * g[s2] + sg * pg;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) qv[s2] = cs * qv[s2] + sg * pq[s2];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      const int r = c ^ (4 * q + s2);
+      const double hc = bperm_d(h[s2], r + 16 * q);                      // (G J)[c][r]
+      const double hd = bperm_d(h[s2 ^ (d & 3)], r + 16 * (q ^ (d >> 2)));  // (G J)[c ^ d][r]
+      g[s2] = cs * hc + sg * hd;
+    }
+    D = lo ? cs * cs * al - 2.0 * cs * sn * gm + sn * sn * be
+           : sn * sn * al + 2.0 * cs * sn * gm + cs * cs * be;
+  }
+#pragma unroll
+  for (int s2 = 0; s2 < 4; ++s2) Q[sel[c ^ (4 * q + s2)]][sel[c]] = qv[s2];
+  return __any(any);
 }
 
 // column c of visit slot `slot` (< kJc) of block pair pr in round `round`
@@ -1739,153 +1830,97 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     G[r][c] = sum;
     Q[r][c] = r == c ? 1.0 : 0.0;
   }
+  if (kJc == 32 && tid == 0) flag = 0;  // (32-column visits: set by the sweeping waves)
   __syncthreads();
   if constexpr (kJc == 16) {
   if (wave == 0) {
-    // The inner sweep in registers, XOR order: step d rotates the pairs
-    // (x, x ^ d), d = 1 .. 15 (every pair once).  Lane (c, q) = c + 16 q
-    // holds G[c ^ (4 q + s)][c] and Q[c ^ (4 q + s)][c] in slot s < 4 and the
-    // diagonal D = G[c][c]; every lane forms its column's rotation from the
-    // same three values as its partner (bit-identical angles).  Per step:
-    // column update G J, Q J (the partner column sits at lane ^ (d | (d >> 2)
-    // << 4), slot s ^ (d & 3)), then the row update through the symmetry of
-    // J^T G J: G'[r][c] = G'[c][r] = (G J)[c][r] cs + (G J)[c ^ d][r] s_c.
     const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
-    const double tol2 = tol * tol;
     const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
-    const int c = lane & 15, q = lane >> 4;
-    double g[4], qv[4];
-#pragma unroll
-    for (int s2 = 0; s2 < 4; ++s2) {
-      g[s2] = G[c ^ (4 * q + s2)][c];
-      qv[s2] = (4 * q + s2) == 0 ? 1.0 : 0.0;
-    }
-    double D = G[c][c];
-    int any = 0;
-#pragma unroll
-    for (int d = 1; d < 16; ++d) {
-      const int hb = d >= 8 ? 8 : d >= 4 ? 4 : d >= 2 ? 2 : 1;  // top bit of d
-      const bool lo = (c & hb) == 0;
-      const int a = lo ? c : (c ^ d);
-      const double Dp = bperm_d(D, lane ^ d);
-      // G[a ^ d][a] sits in column a at offset d: quarter d >> 2, slot d & 3
-      const double gm = bperm_d(g[d & 3], a + 16 * (d >> 2));
-      const double al = lo ? D : Dp, be = lo ? Dp : D;
-      double cs = 1.0, sn = 0.0;
-      const bool doit = al > zero2 && be > zero2 && gm * gm > tol2 * al * be;
-      if (doit) {
-        // tan(theta) = sgn(dd) gg / (|dd| + hypot(dd, gg)), dd = be - al,
-        // gg = 2 gm (the one-sided angle); rsq/rcp with one Newton step each
-        const double dd = be - al, gg = 2.0 * gm;
-        const double w = dd * dd + gg * gg;
-        double ri = __builtin_amdgcn_rsq(w);
-        ri = ri * (1.5 - 0.5 * w * ri * ri);
-        const double den = fabs(dd) + w * ri;
-        double rc = __builtin_amdgcn_rcp(den);
-        rc = rc * (2.0 - den * rc);
-        const double t = (dd >= 0.0 ? gg : -gg) * rc;
-        const double u = 1.0 + t * t;
-        double ci = __builtin_amdgcn_rsq(u);
-        ci = ci * (1.5 - 0.5 * u * ci * ci);
-        cs = ci;
-        sn = ci * t;
-        any = 1;
-      }
-      // new column a = cs a - sn b, new column b = sn a + cs b
-      const double sg = lo ? -sn : sn;
-      const int mask = d | ((d >> 2) << 4);
-      double h[4], pq[4];
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2) {
-        const double pg = bperm_d(g[s2 ^ (d & 3)], lane ^ mask);
-        pq[s2] = bperm_d(qv[s2 ^ (d & 3)], lane ^ mask);  // all read before any update
-        h[s2] = cs * g[s2] + sg * pg;
-      }
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2) qv[s2] = cs * qv[s2] + sg * pq[s2];
-#pragma unroll
-      for (int s2 = 0; s2 < 4; ++s2) {
-        const int r = c ^ (4 * q + s2);
-        const double hc = bperm_d(h[s2], r + 16 * q);                      // (G J)[c][r]
-        const double hd = bperm_d(h[s2 ^ (d & 3)], r + 16 * (q ^ (d >> 2)));  // (G J)[c ^ d][r]
-        g[s2] = cs * hc + sg * hd;
-      }
-      D = lo ? cs * cs * al - 2.0 * cs * sn * gm + sn * sn * be
-             : sn * sn * al + 2.0 * cs * sn * gm + cs * cs * be;
-    }
-#pragma unroll
-    for (int s2 = 0; s2 < 4; ++s2) Q[c ^ (4 * q + s2)][c] = qv[s2];
-    any = __any(any);
+    const int ident[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+    const int any = inner_sweep16<kJc + 1>(G, Q, ident, lane, tol * tol, zero2);
     if (lane == 0) flag = any;
   } else if (prev) {
     apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
   }
   } else {
-  // 32-column visits: one sweep of two-sided Jacobi in LDS on wave 0
-  if (wave == 0) {
-    const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
-    const double tol2 = tol * tol;
-    const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
-    constexpr int kLpp = 128 / kJc;       // lanes per pair
-    constexpr int kRpl = kJc / kLpp;      // rows (and columns) per lane
-    const int pp = lane / kLpp, i0 = (lane % kLpp) * kRpl;
-    int any = 0;
-    for (int r = 0; r < kJc - 1; ++r) {
-      int a, b;
-      if (pp == 0) { a = r; b = kJc - 1; }
-      else { a = (r + pp) % (kJc - 1); b = (r + kJc - 1 - pp) % (kJc - 1); }
-      if (a > b) { const int t = a; a = b; b = t; }
-      const double al = G[a][a], be = G[b][b], gm = G[a][b];
-      double cs = 1.0, sn = 0.0;
-      const bool doit = al > zero2 && be > zero2 && gm * gm > tol2 * al * be;
-      if (doit) {
-        // tan(theta) = sgn(d) g / (|d| + hypot(d, g)) with d = be - al,
-        // g = 2 gm (the one-sided angle); hardware rsq/rcp with one Newton
-        // step each.  c = (1 + t^2)^-1/2 and s = c t keep c^2 + s^2 = 1 to a
-        // few ulp whatever the accuracy of t.
-        const double d = be - al, g = 2.0 * gm;
-        const double w = d * d + g * g;
-        double ri = __builtin_amdgcn_rsq(w);
-        ri = ri * (1.5 - 0.5 * w * ri * ri);
-        const double den = fabs(d) + w * ri;
-        double rc = __builtin_amdgcn_rcp(den);
-        rc = rc * (2.0 - den * rc);
-        const double t = (d >= 0.0 ? g : -g) * rc;
-        const double u = 1.0 + t * t;
-        double ci = __builtin_amdgcn_rsq(u);
-        ci = ci * (1.5 - 0.5 * u * ci * ci);
-        cs = ci;
-        sn = ci * t;
-        any = 1;
-      }
-      wave_lds_sync();
-      if (doit) {  // G <- G J, Q <- Q J  (columns a, b)
+  // 32-column visits: three local rounds over the four 8-column sub-blocks
+  // s0 = slots 0-7, s1 = 8-15 (block bi), s2 = 16-23, s3 = 24-31 (block bj):
+  // (s0 s1)(s2 s3), (s0 s2)(s1 s3), (s0 s3)(s1 s2).  In each, waves 0 and 1
+  // sweep one 16-column pair each (inner_sweep16 on its part of the 32 x 32
+  // Gram), their rotations form Qr, and the workgroup updates G <- Qr^T G Qr
+  // and Q <- Q Qr on MFMA.  Waves 2 .. kJw-1 meanwhile apply the previous
+  // round's rotation to V, a third of its rows per local round.  Every
+  // column pair of the two blocks is rotated once per visit, so a sweep needs
+  // half the rounds of 16-column visits.
+  const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
+  const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
+  double (*Qr)[kJc + 1] = Gp[1];  // (partial Grams no longer needed)
+  double (*Tm)[kJc + 1] = Gp[2];
+  double (*Qn)[kJc + 1] = Gp[3];
+  const int vchunk = (N + 2) / 3;
+  int any = 0;
+  for (int lr = 0; lr < 3; ++lr) {
+    if (wave < 2) {
+      // sub-block pairs of this local round: (0 1)(2 3), (0 2)(1 3), (0 3)(1 2)
+      const int sa = wave == 0 ? 0 : (lr == 0 ? 2 : 1);
+      const int sb = wave == 0 ? lr + 1 : (lr == 0 ? 3 : (lr == 1 ? 3 : 2));
+      int sel[16];
 #pragma unroll
-        for (int ii = i0; ii < i0 + kRpl; ++ii) {
-          const double ga = G[ii][a], gb = G[ii][b];
-          G[ii][a] = cs * ga - sn * gb;
-          G[ii][b] = sn * ga + cs * gb;
-          const double qa = Q[ii][a], qb = Q[ii][b];
-          Q[ii][a] = cs * qa - sn * qb;
-          Q[ii][b] = sn * qa + cs * qb;
-        }
+      for (int k = 0; k < 8; ++k) {
+        sel[k] = 8 * sa + k;
+        sel[8 + k] = 8 * sb + k;
       }
-      wave_lds_sync();
-      if (doit) {  // G <- J^T G  (rows a, b)
-#pragma unroll
-        for (int ii = i0; ii < i0 + kRpl; ++ii) {
-          const double ga = G[a][ii], gb = G[b][ii];
-          G[a][ii] = cs * ga - sn * gb;
-          G[b][ii] = sn * ga + cs * gb;
-        }
+      any |= inner_sweep16<kJc + 1>(G, Qr, sel, lane, tol * tol, zero2);
+      // zero this pair's rows in the other pair's columns (the two pairs
+      // partition the slots, so Qr is written whole)
+      const int oa = wave == 1 ? 0 : (lr == 0 ? 2 : 1);
+      const int ob = wave == 1 ? lr + 1 : (lr == 0 ? 3 : (lr == 1 ? 3 : 2));
+      for (int e = lane; e < 16 * 16; e += 64) {
+        const int r = e >> 4, c = e & 15;
+        Qr[sel[r]][c < 8 ? 8 * oa + c : 8 * ob + c - 8] = 0.0;
       }
-      wave_lds_sync();
+    } else if (prev) {
+      const int r0 = lr * vchunk, r1 = min(N, r0 + vchunk);
+      if (r1 > r0) apply_rotation(p.V + r0, N, r1 - r0, colp, Qp, lane, wave - 2, kJw - 2);
     }
-    any = __any(any);
-    if (lane == 0) flag = any;
-  } else if (prev) {
-    apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
+    __syncthreads();
+    // Tm = G Qr (waves 0-3), Qn = Q Qr (waves 4-7): one 16 x 16 block each
+    {
+      const int i = lane & 15, kk = lane >> 4;
+      const int w4 = wave & 3, bi = w4 >> 1, bj = w4 & 1;
+      const double (*L)[kJc + 1] = wave < 4 ? G : Q;
+      double (*O)[kJc + 1] = wave < 4 ? Tm : Qn;
+      if (wave < 8) {
+        d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < kJc / 4; ++t)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L[16 * bi + i][4 * t + kk], Qr[4 * t + kk][16 * bj + i],
+                                                     acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) O[16 * bi + kk + 4 * v][16 * bj + i] = acc[v];
+      }
+    }
+    __syncthreads();
+    // G = Qr^T Tm (waves 0-3); Q = Qn (waves 4-7)
+    {
+      const int i = lane & 15, kk = lane >> 4;
+      const int w4 = wave & 3, bi = w4 >> 1, bj = w4 & 1;
+      if (wave < 4) {
+        d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < kJc / 4; ++t)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Qr[4 * t + kk][16 * bi + i], Tm[4 * t + kk][16 * bj + i],
+                                                     acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) G[16 * bi + kk + 4 * v][16 * bj + i] = acc[v];
+      } else if (wave < 8) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Q[16 * bi + kk + 4 * v][16 * bj + i] = Qn[16 * bi + kk + 4 * v][16 * bj + i];
+      }
+    }
+    __syncthreads();
   }
+  if (wave < 2 && lane == 0 && any) flag = 1;
   }
   __syncthreads();
   int* qf = qflag + (round & 1) * nvisits + blockIdx.x;
