@@ -2161,6 +2161,8 @@ __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
         s = ci * t;
         any = 1;
       }
+      // a step that rotates no pair leaves G and Q as they are (wave-uniform)
+      if (!__any(doit)) continue;
       // J = [[c, s e], [-s conj(e)]...]: x_a' = c x_a - s conj(e) x_b,
       // x_b' = s e x_a + c x_b, with e = (ph_r, ph_i)
       const double mr = s * ph_r, mi = s * ph_i;  // s e
