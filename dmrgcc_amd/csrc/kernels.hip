@@ -2011,7 +2011,7 @@ __global__ __launch_bounds__(256) void zjacobi_init_kernel(const SvdProblem* __r
 // rows [0, rows) of the visit's complex columns: X[:, col] <- X[:, col] * Q
 __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int rows, const int* col,
                                        const double (*Qr)[kZc + 1], const double (*Qi)[kZc + 1],
-                                       int lane, int wave) {
+                                       int lane, int wave, int nwaves) {
   const int i = lane & 15, kk = lane >> 4;
   int cs[4];
 #pragma unroll
@@ -2024,7 +2024,7 @@ __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int r
     qi[t] = Qi[4 * t + kk][i];
     nqi[t] = -qi[t];
   }
-  for (int base = wave * 16; base < rows; base += 16 * kZw) {
+  for (int base = wave * 16; base < rows; base += 16 * nwaves) {
     double ar[4], ai[4];
     const int r = base + i;
 #pragma unroll
@@ -2056,32 +2056,63 @@ __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int r
   }
 }
 
+// One round, as jacobi_gram_round_kernel: the rotation of a visit is applied
+// to G at once and to V one round later (launch `round` applies the previous
+// round's rotation of the same block-pair slot to V on waves 1 .. kZw-1
+// while wave 0 runs the inner sweep; a launch with round == the sweep's
+// round count only flushes the last rotations).  Rotations and flags go
+// through qbuf (Qr then Qi, 2 kZc^2 doubles per slot) and qflag,
+// double-buffered by round parity.
 __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
-    const int* __restrict__ vpair, int round, int* __restrict__ rotated) {
+    const int* __restrict__ vpair, int round, int* __restrict__ rotated,
+    double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
   __shared__ double Pr[kZw][kZc][kZc + 1];
   __shared__ double Pi[kZw][kZc][kZc + 1];
   __shared__ double Qr[kZc][kZc + 1];
   __shared__ double Qi[kZc][kZc + 1];
-  __shared__ int col[kZc];
+  __shared__ double Qpr[kZc][kZc + 1];  // the previous round's rotation
+  __shared__ double Qpi[kZc][kZc + 1];
+  __shared__ int col[kZc], colp[kZc];
   __shared__ int flag;
+  static_assert(2 * kZc * kZc == 64 * kZw, "one previous-rotation entry per thread");
   double (*Gr)[kZc + 1] = Pr[0];
   double (*Gi)[kZc + 1] = Pi[0];
   constexpr int kB = kZc / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the previous round's flag and rotation of this slot (always in bounds)
+  const int pflag = round >= 1 ? qflag[((round - 1) & 1) * nvisits + blockIdx.x] : 0;
+  const double qpre =
+      round >= 1 ? qbuf[((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (2 * kZc * kZc) + tid] : 0.0;
   // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
   const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kB - 1) / kB, nb2 = nb + (nb & 1);
-  if (round >= nb2 - 1) return;
-  int bi, bj;
-  if (pr == 0) { bi = round; bj = nb2 - 1; }
-  else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+  const bool cur = round < nb2 - 1;
+  const bool prev = round >= 1 && round - 1 < nb2 - 1 && pflag != 0;
+  if (!cur && !prev) return;
+  auto visit_col = [&](int rd, int slot) {
+    int bi, bj;
+    if (pr == 0) { bi = rd; bj = nb2 - 1; }
+    else { bi = (rd + pr) % (nb2 - 1); bj = (rd + nb2 - 1 - pr) % (nb2 - 1); }
+    const int blk = slot < kB ? bi : bj;
+    const int c = blk * kB + (slot & (kB - 1));
+    return (blk < nb && c < N) ? c : -1;
+  };
   if (tid < kZc) {
-    const int blk = tid < kB ? bi : bj;
-    const int c = blk * kB + (tid & (kB - 1));
-    col[tid] = (blk < nb && c < N) ? c : -1;
+    col[tid] = cur ? visit_col(round, tid) : -1;
+    colp[tid] = prev ? visit_col(round - 1, tid) : -1;
+  }
+  if (prev) {
+    const int e = tid & (kZc * kZc - 1);
+    if (tid < kZc * kZc) Qpr[e / kZc][e % kZc] = qpre;
+    else Qpi[e / kZc][e % kZc] = qpre;
+  }
+  __syncthreads();
+  if (!cur) {
+    zapply_rotation(p.V, N, N, colp, Qpr, Qpi, lane, wave, kZw);
+    return;
   }
   __syncthreads();
   {
@@ -2195,12 +2226,25 @@ __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
     }
     any = __any(any);
     if (lane == 0) flag = any;
+  } else if (prev) {
+    zapply_rotation(p.V, N, N, colp, Qpr, Qpi, lane, wave - 1, kZw - 1);
   }
   __syncthreads();
-  if (!flag) return;
-  zapply_rotation(p.G, M, M, col, Qr, Qi, lane, wave);
-  zapply_rotation(p.V, N, N, col, Qr, Qi, lane, wave);
-  if (tid == 0) rotated[pi] = 1;
+  int* qf = qflag + (round & 1) * nvisits + blockIdx.x;
+  if (!flag) {
+    if (tid == 0) *qf = 0;
+    return;
+  }
+  zapply_rotation(p.G, M, M, col, Qr, Qi, lane, wave, kZw);
+  double* qd = qbuf + ((int64_t)(round & 1) * nvisits + blockIdx.x) * (2 * kZc * kZc);
+  {
+    const int e = tid & (kZc * kZc - 1);
+    qd[tid] = tid < kZc * kZc ? Qr[e / kZc][e % kZc] : Qi[e / kZc][e % kZc];
+  }
+  if (tid == 0) {
+    *qf = 1;
+    rotated[pi] = 1;
+  }
 }
 
 // sigma and the phase convention, one workgroup per problem, wave per column
@@ -3810,12 +3854,15 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
   int* rotated = sweeps_out.data();
   int* vprob = rotated + np;
   int* qflag = vprob + 2 * max_visits;
-  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * kJc * kJc);
-  std::vector<int> hrot(np);
+  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * 2 * kZc * kZc);
+  std::vector<int> hrot(np), last(np, 0);
+  int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
+    ++sweeps_done;
     std::vector<int> h(np, 0), pairs;
     int rounds = 0;
     for (int i : active) {
+      last[i] = sweep + 1;
       const int nb = (probs[i].N + kB - 1) / kB, nb2 = nb + (nb & 1);
       for (int pr = 0; pr < nb2 / 2; ++pr) {
         h.push_back(i);
@@ -3827,9 +3874,10 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
     int* vpair = vprob + visits;
     h.insert(h.end(), pairs.begin(), pairs.end());
     upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
-    for (int r = 0; r < rounds; ++r)
+    // rounds 0 .. rounds-1, then the flush of the last rotations into V
+    for (int r = 0; r <= rounds; ++r)
       hipLaunchKernelGGL(zjacobi_round_kernel, dim3(visits), dim3(64 * kZw), 0, stream, scratch.data(),
-                         vprob, vpair, r, rotated);
+                         vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
     DMRG_HIP(hipStreamSynchronize(stream));
@@ -3843,6 +3891,14 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
   hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np, (fin_n + kFinCols - 1) / kFinCols), dim3(256), 0, stream,
                      scratch.data());
   DMRG_HIP(hipGetLastError());
+  if (std::getenv("DMRG_SVD_STATS")) {
+    int maxN = 0;
+    for (const SvdProblem& q : probs) maxN = std::max(maxN, q.N);
+    std::printf("ZSVDSTAT np=%d maxN=%d sweeps=%d |", np, maxN, sweeps_done);
+    for (int i = 0; i < np; ++i)
+      if ((int64_t)probs[i].M * probs[i].N > 20000) std::printf(" %dx%d:%d", probs[i].M, probs[i].N, last[i]);
+    std::printf("\n");
+  }
 }
 
 // ---------------------------------------------------------------------------
