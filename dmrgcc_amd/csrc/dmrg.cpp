@@ -575,6 +575,13 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
   }
   const int cs = cplx_ ? 2 : 1;
   T_.resize((size_t)std::max<int64_t>(cs * tsize, 1));
+  // stage-1 items that read one R block share a placement key (their tiles on
+  // one XCD, the block fetched into one L2); DMRGCC_XCD_KEYS=0 turns it off
+  std::vector<int> rbase(R.size() + 1, 0);
+  for (size_t i = 0; i < R.size(); ++i) rbase[i + 1] = rbase[i] + R[i].n();
+  const char* kev = std::getenv("DMRGCC_XCD_KEYS");
+  const bool rkeys = !(kev && kev[0] == '0');
+  auto rkey = [&](const Task& t) { return rkeys ? rbase[t.o2] + t.ri : -1; };
   // stage 1: T = K * R  (K relative to the input vector)
   for (const Task& t : tasks) {
     if (t.outs.empty()) continue;
@@ -587,7 +594,7 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
                  {ZTerm{krel, kb.cols, Rb.ptr(t.ri), rbk.cols, kb.cols, kRelA, false, false, 1.0, 0.0}});
       continue;
     }
-    s1_.begin_item(T_.data() + t.toff, rbk.cols, kb.rows, rbk.cols, 0);
+    s1_.begin_item(T_.data() + t.toff, rbk.cols, kb.rows, rbk.cols, 0, 0, rkey(t));
     s1_.add(krel, kb.cols, Rb.ptr(t.ri), rbk.cols, kb.cols, kRelA, 1.0);
   }
   // stage 2: group terms by output block, keeping the reference's term order
@@ -698,7 +705,7 @@ void HeffPlan::build_blocks(const BlockTensor& ket, const std::vector<int>& bphy
         const Block& kb = ket.blk[t.kp];
         const Block& rbk = R[t.o2].blk[t.ri];
         if (kb.cols <= 0) continue;  // (no K: the FP64 T of such a task is zero as well)
-        p1_.begin_item(Ts_.data() + toff_s[ti], groups(kb.rows), kb.rows, rbk.cols, 0, kPairC);
+        p1_.begin_item(Ts_.data() + toff_s[ti], groups(kb.rows), kb.rows, rbk.cols, 0, kPairC, rkey(t));
         p1_.add(Xs_.data() + xoff[t.kp], groups(kb.cols), LRs_.data() + roff[t.o2][t.ri], groups(rbk.rows),
                 kb.cols, 0, 1.0);
       }

@@ -119,6 +119,9 @@ struct GemmBatch {
   GemmTile* dev_tiles = nullptr;
   SplitTask* dev_splits = nullptr;
   std::vector<int> part_items;  // split-K chunk items awaiting bind_parts()
+  // Optional placement key per item (begin_item): items with one key have
+  // their tiles on one XCD, so operands they share are fetched into one L2.
+  std::vector<int> item_key;
   int split_k = default_split_k();
   // 32: 32x32 wave tiles (four independent tiles per workgroup); otherwise
   // 64x64 workgroup tiles (environment DMRGCC_TILE)
@@ -139,10 +142,11 @@ struct GemmBatch {
 
   void clear() {
     items.clear(); contribs.clear(); tiles.clear(); splits.clear();
-    group_items.clear(); group_tiles.clear(); part_items.clear(); flops = 0;
+    group_items.clear(); group_tiles.clear(); part_items.clear(); item_key.clear(); flops = 0;
   }
-  // Start an output item; subsequent add() calls attach terms to it.
-  void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0);
+  // Start an output item; subsequent add() calls attach terms to it.  key >= 0:
+  // the item's placement key (items sharing a key go to one XCD).
+  void begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags = 0, int key = -1);
   void add(const double* A, int64_t lda, const double* B, int64_t ldb, int K,
            int flags, double scale);
   int end_group() { group_items.push_back((int)items.size()); return (int)group_items.size() - 1; }

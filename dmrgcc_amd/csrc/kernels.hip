@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "kernels.h"
 
@@ -2919,7 +2920,11 @@ __global__ __launch_bounds__(256) void sort_scatter_rows_kernel(const SortJob* _
 // ===========================================================================
 // host wrappers
 // ===========================================================================
-void GemmBatch::begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags) {
+void GemmBatch::begin_item(double* C, int64_t ldc, int M, int N, int accumulate, int flags, int key) {
+  if (key >= 0 || !item_key.empty()) {
+    item_key.resize(items.size(), -1);
+    item_key.push_back(key);
+  }
   GemmItem it{};
   it.flags = flags;
   it.C = C;
@@ -3044,6 +3049,9 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   group_tiles.clear();
   part_items.clear();
   int64_t psize = 0;
+  // placement keys of the items (split-K chunks take their item's)
+  std::vector<int> key(items.size(), -1);
+  for (size_t i = 0; i < item_key.size() && i < items.size(); ++i) key[i] = item_key[i];
   if (!group_items.empty() && group_items.back() != (int)items.size()) end_group();
   // split-K: chunk the contribution lists of heavy items (pre-split batches:
   // also every item with more than kPreTerms terms, whatever split_k)
@@ -3093,6 +3101,9 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
         part_items.push_back((int)items.size());
         items.push_back(e);
       }
+      key.resize(items.size(), -1);
+      for (const Pending& pd : pend)
+        for (int e = 0; e < pd.n; ++e) key[base + pd.first_extra + e] = key[pd.item];
       for (const Pending& pd : pend) {
         const GemmItem& it = items[pd.item];
         splits.push_back(SplitTask{it.C, it.ldc, it.M, it.N, pd.n, it.flags,
@@ -3118,6 +3129,43 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   // H_eff launches measured here the 64x64 form is the faster (BF16x3:
   // 0.105 vs 0.158 ms per C2 matvec), so it is the default.
   wave_tiles = group_items.empty() && tile_m == kTile && tile_n == kTile && tile_mode == 32;
+  // With placement keys: the items of one key form a group whose tiles all go
+  // to one XCD queue; groups (and unkeyed items) to the least-loaded queue,
+  // costliest first.
+  const bool keyed = tile_m == tile_n && std::any_of(key.begin(), key.end(), [](int k) { return k >= 0; });
+  auto place_keyed = [&](const std::vector<int>& order, int ts, std::vector<std::vector<GemmTile>>& q) {
+    std::unordered_map<int, int> kg;
+    std::vector<std::vector<int>> groups;
+    std::vector<double> gcost;
+    for (int i : order) {
+      const GemmItem& it = items[i];
+      if (it.M <= 0 || it.N <= 0) continue;
+      if (it.c0 == it.c1 && it.accumulate) continue;
+      int g = (int)groups.size();
+      if (key[i] >= 0) {
+        auto f = kg.find(key[i]);
+        if (f != kg.end()) g = f->second;
+        else kg[key[i]] = g;
+      }
+      if (g == (int)groups.size()) {
+        groups.emplace_back();
+        gcost.push_back(0.0);
+      }
+      groups[g].push_back(i);
+      gcost[g] += (cost[i] + 32.0) * (double)(((it.M + ts - 1) / ts) * ((it.N + ts - 1) / ts));
+    }
+    std::vector<int> gorder(groups.size());
+    std::iota(gorder.begin(), gorder.end(), 0);
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) { return gcost[a] > gcost[b]; });
+    std::vector<double> load(kXcds, 0.0);
+    for (int g : gorder) {
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int i : groups[g])
+        for (int m0 = 0; m0 < items[i].M; m0 += ts)
+          for (int n0 = 0; n0 < items[i].N; n0 += ts) q[x].push_back(GemmTile{i, m0, n0, 0});
+      load[x] += gcost[g];
+    }
+  };
   std::vector<int> ends = group_items;
   if (ends.empty()) ends.push_back((int)items.size());
   int first = 0;
@@ -3129,7 +3177,8 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     std::vector<std::vector<GemmTile>> q(kXcds);
     std::vector<double> load(kXcds, 0.0);
-    for (int i : order) {
+    if (keyed) place_keyed(order, kWT, q);
+    for (int i : keyed ? std::vector<int>{} : order) {
       const GemmItem& it = items[i];
       if (it.M <= 0 || it.N <= 0) continue;
       if (it.c0 == it.c1 && it.accumulate) continue;
@@ -3160,7 +3209,8 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     std::vector<std::vector<GemmTile>> q(kXcds);
     std::vector<double> load(kXcds, 0.0);
-    for (int i : order) {
+    if (keyed) place_keyed(order, tile_m, q);
+    for (int i : keyed ? std::vector<int>{} : order) {
       const GemmItem& it = items[i];
       if (it.M <= 0 || it.N <= 0) continue;
       if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
