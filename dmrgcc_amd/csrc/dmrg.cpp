@@ -138,6 +138,43 @@ void fill_random(const std::vector<RandomDraw>& ds, double* rnd, int cs, GlibcRa
   rng.skip(total);
 }
 
+// The random matrices of `ds` made on the device into R (device memory): the
+// draws fill_random makes on the host, from the stream's current window,
+// and `rng` then advanced past them as fill_random leaves it.  Lane c of
+// rng_stream starts chunk c of the draws from the window that the
+// coefficients of x^(c kRngChunk) mod P give (GlibcRand::chunk_powers),
+// made once per context and kept on the device.
+void random_matrices_device(const std::vector<RandomDraw>& ds, int cs, double* R, Context& cx) {
+  int64_t total = 0, most = 0;
+  std::vector<RngDraw> dd;
+  dd.reserve(ds.size());
+  for (const RandomDraw& d : ds) {
+    const int64_t n = (int64_t)d.rows * d.cols;
+    dd.push_back(RngDraw{total, d.roff, d.rows, d.cols, d.w, d.wi, d.diag ? 1 : 0, 0});
+    total += (int64_t)cs * n;
+    most = std::max(most, n);
+  }
+  if (total == 0) return;
+  const int64_t nchunks = (total + kRngChunk - 1) / kRngChunk;
+  const int64_t have = (int64_t)cx.rng_coef_host.size() / 31;
+  if (nchunks > have) {
+    const int64_t want = std::max(nchunks, 2 * have);
+    cx.rng_coef_host.resize((size_t)want * 31);
+    GlibcRand::chunk_powers(kRngChunk, have, want - have,
+                            have ? cx.rng_coef_host.data() + 31 * (have - 1) : nullptr,
+                            cx.rng_coef_host.data() + 31 * have);
+    upload(cx.rng_coef, cx.rng_coef_host, cx.stream);
+  }
+  std::vector<uint32_t> w(61);
+  cx.rng.window(w.data());
+  upload(cx.rng_w0, w, cx.stream);
+  cx.rng_raw.ensure((size_t)(nchunks * kRngChunk));
+  rng_stream(cx.rng_w0.data(), cx.rng_coef.data(), total, cx.rng_raw.data(), cx.stream);
+  upload(cx.rng_draws, dd, cx.stream);
+  rng_layout(cx.rng_draws.data(), (int)dd.size(), most, cx.rng_raw.data(), R, cs, cx.stream);
+  cx.rng.skip(total);
+}
+
 // One piece of a block assembled by the expansion / match fills.
 struct Piece {
   const double* src;  // nullptr: zeros
@@ -2078,9 +2115,6 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
     cx_.rng.skip(pending);
     return;
   }
-  // drawn straight into pinned host memory, so the upload is one DMA copy
-  double* rnd = cx_.pinned(rnd_size);
-  fill_random(draws, rnd, cs, cx_.rng);
   // ---- dims of the expansion sectors; zero blocks for empty ones ----
   if (left) {
     for (int r = 0; r < E.nr(); ++r) {
@@ -2139,9 +2173,8 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   fill(E.ket.data.data(), E.ket.doubles(), 0.0, cx_.stream);
   // ---- device: first = L*K (or K*R), then noise * first * random ----
   DevBuf<double> F((size_t)std::max<int64_t>(cs * fsize, 1)), Rd((size_t)std::max<int64_t>(rnd_size, 1));
-  if (rnd_size > 0)
-    DMRG_HIP(hipMemcpyAsync(Rd.data(), rnd, rnd_size * sizeof(double), hipMemcpyHostToDevice,
-                            cx_.stream));
+  // the random matrices, drawn on the device (the draws of fill_random)
+  random_matrices_device(draws, cs, Rd.data(), cx_);
   GemmBatch s1, s2;
   std::vector<char> fused(firsts.size(), 0);
   for (const Term& t : terms) fused[t.f] = 1;

@@ -109,6 +109,13 @@ struct Context {
   double* lanczos_host_dev();  // the device's address of lanczos_host()
   // at least n events (timing disabled), reused between calls
   std::vector<hipEvent_t> events(int n);
+  // Device rand() stream state (random_matrices_device, dmrg.cpp): the chunk
+  // coefficients made so far (host copy and device table), the window, the
+  // raw draws and the matrix descriptors
+  std::vector<uint32_t> rng_coef_host;
+  DevBuf<uint32_t> rng_coef, rng_w0;
+  DevBuf<uint8_t> rng_raw;
+  DevBuf<RngDraw> rng_draws;
 
  private:
   double* pinned_ = nullptr;

@@ -307,6 +307,26 @@ struct ColGatherTask {
   double* dst;
   int64_t dsr, dsc;
 };
+// Device rand() stream (the glibc TYPE_3 generator of GlibcRand): draws
+// 0 .. total-1 after the state whose generation window is w0 (61 words,
+// GlibcRand::window), as bytes (draw % 100).  Lane c makes draws
+// [c kRngChunk, (c + 1) kRngChunk) from the window coef[c] (31 words,
+// GlibcRand::chunk_powers with step kRngChunk) gives, so `out` holds
+// ceil(total / kRngChunk) kRngChunk bytes and coef that many chunks.
+constexpr int kRngChunk = 992;  // 8 runs of 124 = 4 x 31 draws
+void rng_stream(const uint32_t* w0, const uint32_t* coef, int64_t total, uint8_t* out, hipStream_t stream);
+// One random matrix of the expansion: RandomMatrix(rows, cols) from draws
+// first .. (row-major at R[roff], filled column by column; complex: two
+// draws per entry, the imaginary part first), then its diagonal set to
+// 0.1 (w + i wi) when diag.
+struct RngDraw {
+  int64_t first, roff;
+  int rows, cols;
+  double w, wi;
+  int diag, pad;
+};
+void rng_layout(const RngDraw* draws, int ndraws, int64_t max_entries, const uint8_t* raw, double* R, int cs,
+                hipStream_t stream);
 void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGatherTask>& scratch,
                        hipStream_t stream);
 

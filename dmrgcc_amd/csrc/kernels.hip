@@ -3552,6 +3552,87 @@ void lanczos_step(double* psi_next, double* h, const double* psi, int64_t n, dou
   DMRG_HIP(hipGetLastError());
 }
 
+namespace {
+// Lane c: its chunk's window st[t] = sum_j coef[c][j] w[j + t] (the 31
+// latest raw values, oldest first), then the recurrence r = r[-31] + r[-3]
+// unrolled over 124 steps (st[u % 31] += st[(u + 28) % 31]), four draws per
+// 32-bit store.
+__global__ __launch_bounds__(256) void rng_stream_kernel(const uint32_t* __restrict__ w0,
+                                                          const uint32_t* __restrict__ coef, int64_t nchunks,
+                                                          uint32_t* __restrict__ out) {
+  __shared__ uint32_t w[61];
+  if (threadIdx.x < 61) w[threadIdx.x] = w0[threadIdx.x];
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  uint32_t st[31];
+  {
+    uint32_t cf[31];
+#pragma unroll
+    for (int j = 0; j < 31; ++j) cf[j] = coef[31 * c + j];
+#pragma unroll
+    for (int t = 0; t < 31; ++t) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int j = 0; j < 31; ++j) v += cf[j] * w[j + t];
+      st[t] = v;
+    }
+  }
+  uint32_t* o = out + c * (kRngChunk / 4);
+  for (int run = 0; run < kRngChunk / 124; ++run) {
+#pragma unroll
+    for (int q = 0; q < 31; ++q) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int u = (4 * q + b) % 31;
+        st[u] += st[(u + 28) % 31];
+        word |= ((st[u] >> 1) % 100u) << (8 * b);
+      }
+      o[31 * run + q] = word;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void rng_layout_kernel(const RngDraw* __restrict__ draws,
+                                                          const uint8_t* __restrict__ raw,
+                                                          double* __restrict__ R, int cs) {
+  const RngDraw d = draws[blockIdx.y];
+  const int64_t n = (int64_t)d.rows * d.cols;
+  double* out = R + d.roff;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / d.cols), j = (int)(e % d.cols);
+    const int64_t k = (int64_t)j * d.rows + i;  // its draw: column-major fill
+    const bool dg = d.diag && i == j;
+    if (cs == 1) {
+      out[e] = dg ? 0.1 * d.w : 0.01 * (double)raw[d.first + k];
+    } else {
+      out[2 * e] = dg ? 0.1 * d.w : 0.01 * (double)raw[d.first + 2 * k + 1];
+      out[2 * e + 1] = dg ? 0.1 * d.wi : 0.01 * (double)raw[d.first + 2 * k];
+    }
+  }
+}
+}  // namespace
+
+void rng_stream(const uint32_t* w0, const uint32_t* coef, int64_t total, uint8_t* out, hipStream_t stream) {
+  if (total <= 0) return;
+  const int64_t nchunks = (total + kRngChunk - 1) / kRngChunk;
+  hipLaunchKernelGGL(rng_stream_kernel, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, stream, w0, coef,
+                     nchunks, reinterpret_cast<uint32_t*>(out));
+  DMRG_HIP(hipGetLastError());
+}
+
+void rng_layout(const RngDraw* draws, int ndraws, int64_t max_entries, const uint8_t* raw, double* R, int cs,
+                hipStream_t stream) {
+  if (ndraws <= 0 || max_entries <= 0) return;
+  const unsigned gx = (unsigned)std::min<int64_t>(64, (max_entries + 255) / 256);
+  for (int d0 = 0; d0 < ndraws; d0 += 65535) {
+    const unsigned gy = (unsigned)std::min(65535, ndraws - d0);
+    hipLaunchKernelGGL(rng_layout_kernel, dim3(gx, gy), dim3(256), 0, stream, draws + d0, raw, R, cs);
+    DMRG_HIP(hipGetLastError());
+  }
+}
+
 void launch_col_gather(const std::vector<ColGatherTask>& tasks, DevBuf<ColGatherTask>& scratch,
                        hipStream_t stream) {
   std::vector<ColGatherTask> live;

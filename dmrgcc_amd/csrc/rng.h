@@ -85,6 +85,32 @@ class GlibcRand {
   }
   uint64_t count() const { return count_; }
 
+  // The generation window: w[0..30] the 31 latest raw values, oldest first,
+  // and w[31..60] the next 30 raw values (the values skip() combines).
+  void window(uint32_t* w) const {
+    for (int j = 0; j < 31; ++j) w[j] = st_[(f_ + j) % 31];
+    for (int j = 31; j < 61; ++j) w[j] = w[j - 3] + w[j - 31];
+  }
+  // Coefficients of x^(k step) mod P(x) for k = k0 .. k0 + count - 1, 31
+  // each, into out; prev holds those of x^((k0 - 1) step) when k0 > 0.
+  // With c = x^n mod P, sum_j c_j w[j + t] (t < 31) is the window after n
+  // draws, so chunk k of a stream cut into chunks of `step` draws starts
+  // from the window that these coefficients give.
+  static void chunk_powers(int64_t step, int64_t k0, int64_t count, const uint32_t* prev, uint32_t* out) {
+    uint32_t base[31], cur[31];
+    pow_mod(step, base);
+    if (k0 == 0) {
+      for (int i = 0; i < 31; ++i) cur[i] = i == 0 ? 1u : 0u;
+    } else {
+      for (int i = 0; i < 31; ++i) cur[i] = prev[i];
+      mul_mod(cur, base);
+    }
+    for (int64_t k = 0; k < count; ++k) {
+      for (int i = 0; i < 31; ++i) out[31 * k + i] = cur[i];
+      mul_mod(cur, base);
+    }
+  }
+
  private:
   // a = a * b mod P(x) on degree-30 coefficient vectors (mod 2^32)
   static void mul_mod(uint32_t* a, const uint32_t* b) {
