@@ -942,7 +942,9 @@ def sector_svd(mats, stream=None, precondition=0):
     of device tensors with S descending.  precondition: 0 plain Jacobi; 1 QR
     first and Jacobi on R^T (long-side factor exactly orthonormal, short-side
     one only for singular values well above eps*|A|); 2 two QRs (short-side
-    factor exactly orthonormal)."""
+    factor exactly orthonormal).  Complex matrices ignore precondition: sectors
+    with min(M, N) >= 48 are QR-preconditioned (two QRs, columns presorted by
+    norm) unless the environment sets DMRGCC_ZSVD_QR=0."""
     import torch
     n = len(mats)
     cz = n > 0 and mats[0].is_complex()

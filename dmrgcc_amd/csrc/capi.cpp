@@ -973,7 +973,7 @@ int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N
         cps.push_back(c);
       }
       probs.push_back(SvdProblem{work.data() + p.g, work.data() + p.v, work.data() + p.s,
-                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0, 0});
+                                 work.data() + p.s + p.k, p.m, p.k, p.trans ? 1 : 0, 2});
     }
     launch_copies(cps, cx.copy_scratch, cx.stream);
     zjacobi_svd(probs, cx.svd_scratch, cx.svd_sweeps, cx.stream);

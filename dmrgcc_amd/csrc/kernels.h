@@ -371,7 +371,9 @@ void jacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratc
 // dimension M entries); one-sided block Jacobi with complex rotations, G <-
 // G V, V unitary, sigma_j = |G e_j|.  Phase convention: the largest-modulus
 // entry of every left vector is made real positive, the right vector taking
-// the same phase.  precond is ignored (no QR preconditioning).
+// the same phase.  precond != 0 with N >= 48: QR-preconditioned (A P = Q_1
+// R_1 with the columns presorted by norm, R_1^H = Q_2 R_2, Jacobi on R_2^H);
+// the left factor is then formed from Q_1 and V = P Q_2 J.
 void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
                  DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps = 80);
 

@@ -2838,6 +2838,386 @@ __global__ __launch_bounds__(256) void qr_finish_kernel(const QrJob* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Complex QR preconditioning.  A, Y, P, X and C hold interleaved complex
+// entries; leading dimensions count complex entries.  Reflectors follow
+// LAPACK's zlarfg: H = I - tau v v^H with H^H x = (beta, 0, ..., 0), beta
+// real, v = (1, ...).  Panel k has Q_k = H_1 ... H_bw = I - Y T Y^H (T upper
+// triangular, zlarft's forward columnwise recursion) and P_k = Y_k T_k, so the
+// factorisation applies Q_k^H = I - Y P^H to the trailing columns and the
+// back-transformation applies Q_k = I - P Y^H: both are X <- X - L (R^H X),
+// as in the real form.  Panels are at most kZqrB columns wide.
+// ---------------------------------------------------------------------------
+constexpr int kZqrB = 16;
+
+// Panel k of every problem, one workgroup each.  kLds: the panel is staged in
+// LDS (2 (m b + b b + b) doubles of dynamic LDS); otherwise it is worked on in
+// place in global memory and LDS holds T and tau (2 (b b + b) doubles).
+template <bool kLds>
+__global__ __launch_bounds__(512) void zqr_panel_kernel(const QrJob* __restrict__ probs, int k, int b) {
+  extern __shared__ double smem[];
+  __shared__ double red[8];
+  __shared__ double colss[2];
+  const QrJob p = probs[blockIdx.x];
+  const int M = p.M, N = p.N;
+  const int c0 = k * b;
+  if (c0 >= N) return;
+  const int bw = min(b, N - c0);
+  const int m = M - c0;  // >= bw because M >= N
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x >> 6);
+  double* Ag = p.A + 2 * (c0 + (int64_t)c0 * M);  // panel origin, ld M
+  double* pan;
+  int64_t ld;
+  double* T;
+  if (kLds) {
+    pan = smem;
+    ld = m;
+    T = smem + 2 * (int64_t)m * bw;
+    const int64_t n2 = 2 * (int64_t)m;
+    for (int64_t e = tid; e < n2 * bw; e += blockDim.x) {
+      const int j = (int)(e / n2);
+      pan[e] = Ag[e - j * n2 + 2 * (int64_t)j * M];
+    }
+  } else {
+    pan = Ag;
+    ld = M;
+    T = smem;
+  }
+  double* tau = T + 2 * b * b;
+  __syncthreads();
+  {
+    double ss = 0.0;  // |x[1:]|^2 of column 0
+    for (int i = 2 + tid; i < 2 * m; i += blockDim.x) ss += pan[i] * pan[i];
+    ss = wave_sum(ss);
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += red[w];
+      colss[0] = t;
+    }
+    __syncthreads();
+  }
+  // Column j: every thread forms the reflector from alpha = x[0] and
+  // |x[1:]|^2 (the same values for every thread); 32 threads per trailing
+  // column apply H^H = I - conj(tau) v v^H (shuffle-reduced dot), the group of
+  // column j + 1 also forming that column's |x[1:]|^2 for the next step, and x
+  // is scaled into v after the barrier.  No reflector (tau = 0) when x[1:] = 0
+  // and alpha is real.
+  for (int j = 0; j < bw; ++j) {
+    double* x = pan + 2 * ((int64_t)j * ld + j);  // x[0 .. len-1], len = m - j
+    const int len = m - j;
+    const double t = colss[j & 1];
+    const double ar = x[0], ai = x[1];
+    const bool refl = t > 0.0 || ai != 0.0;
+    double tr = 0.0, ti = 0.0, beta = ar, sr = 0.0, si = 0.0;  // tau, beta, scal = 1 / (alpha - beta)
+    if (refl) {
+      beta = -copysign(sqrt(ar * ar + ai * ai + t), ar);
+      tr = (beta - ar) / beta;
+      ti = -ai / beta;
+      const double dr = ar - beta, den = dr * dr + ai * ai;
+      sr = dr / den;
+      si = -ai / den;
+    }
+    if (tid == 0) {
+      tau[2 * j] = tr;
+      tau[2 * j + 1] = ti;
+    }
+    const int c = j + 1 + (tid >> 5), sub = tid & 31;
+    if (c < bw) {
+      double* y = pan + 2 * ((int64_t)c * ld + j);
+      const bool next = c == j + 1;
+      double acc = 0.0;
+      if (refl) {
+        double dr = 0.0, di = 0.0;  // sum over i >= 1 of conj(x_i) y_i
+#pragma unroll 4
+        for (int i = 1 + sub; i < len; i += 32) {
+          const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+          dr += xr * yr + xi * yi;
+          di += xr * yi - xi * yr;
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) {
+          dr += __shfl_xor(dr, o, 64);
+          di += __shfl_xor(di, o, 64);
+        }
+        // w = v^H y = y_0 + conj(scal) dot; d = conj(tau) w; y -= d v
+        const double wr = y[0] + (sr * dr + si * di), wi = y[1] + (sr * di - si * dr);
+        const double Dr = tr * wr + ti * wi, Di = tr * wi - ti * wr;
+        const double Sr = Dr * sr - Di * si, Si = Dr * si + Di * sr;  // d scal
+#pragma unroll 4
+        for (int i = 1 + sub; i < len; i += 32) {
+          const double xr = x[2 * i], xi = x[2 * i + 1];
+          const double yr = y[2 * i] - (Sr * xr - Si * xi), yi = y[2 * i + 1] - (Sr * xi + Si * xr);
+          y[2 * i] = yr;
+          y[2 * i + 1] = yi;
+          if (next && i >= 2) acc += yr * yr + yi * yi;
+        }
+        if (sub == 0) {
+          y[0] -= Dr;
+          y[1] -= Di;
+        }
+      } else if (next) {
+        for (int i = 2 + sub; i < len; i += 32) acc += y[2 * i] * y[2 * i] + y[2 * i + 1] * y[2 * i + 1];
+      }
+      if (next) {
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (sub == 0) colss[(j + 1) & 1] = acc;
+      }
+    }
+    __syncthreads();
+    if (refl)
+      for (int i = 1 + tid; i < len; i += blockDim.x) {
+        const double xr = x[2 * i], xi = x[2 * i + 1];
+        x[2 * i] = xr * sr - xi * si;
+        x[2 * i + 1] = xr * si + xi * sr;
+      }
+    if (tid == 0) {
+      x[0] = beta;
+      x[1] = 0.0;
+    }
+  }
+  __syncthreads();
+  // S[l][j] = y_l^H y_j (l < j) into the strict lower triangle of T (T[j][l]),
+  // four threads per pair; Y[r][c] = 0 (r < c), 1 (r = c), v_c[r] (r > c)
+  {
+    const int npair = bw * (bw - 1) / 2;
+    const int pr = tid >> 2, sub = tid & 3;
+    if (pr < npair) {  // uniform over the pair's four lanes
+      int j = 1;
+      while ((j + 1) * j / 2 <= pr) ++j;
+      const int l = pr - j * (j - 1) / 2;
+      const double* yl = pan + 2 * (int64_t)l * ld;
+      const double* yj = pan + 2 * (int64_t)j * ld;
+      double s_r = 0.0, s_i = 0.0;
+      for (int r = j + 1 + sub; r < m; r += 4) {
+        const double a = yl[2 * r], bq = yl[2 * r + 1], cq = yj[2 * r], dq = yj[2 * r + 1];
+        s_r += a * cq + bq * dq;
+        s_i += a * dq - bq * cq;
+      }
+      s_r += __shfl_xor(s_r, 2, 64);
+      s_i += __shfl_xor(s_i, 2, 64);
+      s_r += __shfl_xor(s_r, 1, 64);
+      s_i += __shfl_xor(s_i, 1, 64);
+      if (sub == 0) {  // row j: conj(v_l[j]) * 1
+        T[2 * (j * b + l)] = s_r + yl[2 * j];
+        T[2 * (j * b + l) + 1] = s_i - yl[2 * j + 1];
+      }
+    }
+  }
+  __syncthreads();
+  // T upper triangular: T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
+  // (S[l][j] at T[j][l]); row r of column j by 16 threads
+  for (int j = 0; j < bw; ++j) {
+    const int r = tid >> 4, sub = tid & 15;
+    if (r < j) {
+      double s_r = 0.0, s_i = 0.0;
+      for (int l = r + sub; l < j; l += 16) {
+        const double a = T[2 * (r * b + l)], bq = T[2 * (r * b + l) + 1];
+        const double cq = T[2 * (j * b + l)], dq = T[2 * (j * b + l) + 1];
+        s_r += a * cq - bq * dq;
+        s_i += a * dq + bq * cq;
+      }
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        s_r += __shfl_xor(s_r, o, 64);
+        s_i += __shfl_xor(s_i, o, 64);
+      }
+      if (sub == 0) {
+        const double tjr = tau[2 * j], tji = tau[2 * j + 1];
+        T[2 * (r * b + j)] = -(tjr * s_r - tji * s_i);
+        T[2 * (r * b + j) + 1] = -(tjr * s_i + tji * s_r);
+      }
+    } else if (r == j && sub == 0) {
+      T[2 * (j * b + j)] = tau[2 * j];
+      T[2 * (j * b + j) + 1] = tau[2 * j + 1];
+    }
+    __syncthreads();
+  }
+  // Y_k (unit lower trapezoidal) and P_k = Y_k T_k, m x bw each (ld m)
+  double* Yk = p.Y + 2 * (int64_t)k * b * M;
+  double* Pk = p.P + 2 * (int64_t)k * b * M;
+  for (int64_t e = tid; e < (int64_t)m * bw; e += blockDim.x) {
+    const int j = (int)(e / m), i = (int)(e % m);
+    const double* yi = pan + 2 * ((int64_t)j * ld + i);
+    Yk[2 * e] = i < j ? 0.0 : (i == j ? 1.0 : yi[0]);
+    Yk[2 * e + 1] = i <= j ? 0.0 : yi[1];
+    double s_r = 0.0, s_i = 0.0;
+    const int lmax = min(j, i);
+    for (int l = 0; l <= lmax; ++l) {
+      const double* yl = pan + 2 * ((int64_t)l * ld + i);
+      const double a = l == i ? 1.0 : yl[0], bq = l == i ? 0.0 : yl[1];
+      const double cq = T[2 * (l * b + j)], dq = T[2 * (l * b + j) + 1];
+      s_r += a * cq - bq * dq;
+      s_i += a * dq + bq * cq;
+    }
+    Pk[2 * e] = s_r;
+    Pk[2 * e + 1] = s_i;
+  }
+  if (kLds) {  // R block of the panel back to A (upper triangle of its top bw rows)
+    for (int64_t e = tid; e < (int64_t)bw * bw; e += blockDim.x) {
+      const int j = (int)(e / bw), i = (int)(e % bw);
+      if (i <= j) {
+        Ag[2 * (i + (int64_t)j * M)] = pan[2 * ((int64_t)j * ld + i)];
+        Ag[2 * (i + (int64_t)j * M) + 1] = pan[2 * ((int64_t)j * ld + i) + 1];
+      }
+    }
+  }
+}
+
+// Complex block reflector of panel k, X <- X - L (R^H X) with (L, R) = (Y_k,
+// P_k) while factorising and (P_k, Y_k) in the back-transformation.  As
+// qr_update_kernel: workgroup (column block, job) takes kQrUpCols columns of
+// X; Z = R^H X_blk on FP64 MFMA (four real chains: Re Z = Rr^T Xr + Ri^T Xi,
+// Im Z = Rr^T Xi - Ri^T Xr) with the rows split over the waves and the partial
+// Z summed in a fixed order in LDS, then X_blk -= L Z in 16-row slabs.
+template <bool kBack>
+__global__ __launch_bounds__(512) void zqr_update_kernel(const QrJob* __restrict__ probs, int k, int b) {
+  constexpr int kW = 8;
+  __shared__ double Zp[kW][2][16][kQrUpCols + 1];
+  __shared__ double Z[2][16][kQrUpCols + 1];
+  const QrJob p = probs[blockIdx.y];
+  const int M = p.M, N = p.N;
+  const int c0 = k * b;
+  if (c0 >= N) return;
+  const int bw = min(b, N - c0), m = M - c0;
+  const int x0 = kBack ? 0 : c0 + bw;  // first column of X
+  const int ncol = N - x0;
+  const int j0 = blockIdx.x * kQrUpCols;
+  if (j0 >= ncol) return;
+  double* X = (kBack ? p.C : p.A) + 2 * (c0 + (int64_t)(x0 + j0) * M);  // ld M, m rows
+  const double* Yk = p.Y + 2 * (int64_t)k * b * M;                       // ld m
+  const double* Pk = p.P + 2 * (int64_t)k * b * M;
+  const double* L = kBack ? Pk : Yk;
+  const double* R = kBack ? Yk : Pk;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  const bool colok = j0 + i < ncol;
+  const double* xc = X + 2 * (int64_t)(colok ? i : 0) * M;
+  const bool rok = i < bw;
+  const double* rc = R + 2 * (int64_t)(rok ? i : 0) * m;
+  {
+    d4v zr = {0.0, 0.0, 0.0, 0.0}, zi = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 16 * wave; r0 < m; r0 += 16 * kW) {
+      double xr[4], xi[4], rr[4], ri[4], nri[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = r0 + 4 * kk + t;
+        const int rq = min(r, m - 1);
+        const double a = xc[2 * rq], bq = xc[2 * rq + 1], cq = rc[2 * rq], dq = rc[2 * rq + 1];
+        const bool okx = colok && r < m, okr = rok && r < m;
+        xr[t] = okx ? a : 0.0;
+        xi[t] = okx ? bq : 0.0;
+        rr[t] = okr ? cq : 0.0;
+        ri[t] = okr ? dq : 0.0;
+        nri[t] = -ri[t];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        zr = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[t], xr[t], zr, 0, 0, 0);
+        zr = __builtin_amdgcn_mfma_f64_16x16x4f64(ri[t], xi[t], zr, 0, 0, 0);
+        zi = __builtin_amdgcn_mfma_f64_16x16x4f64(rr[t], xi[t], zi, 0, 0, 0);
+        zi = __builtin_amdgcn_mfma_f64_16x16x4f64(nri[t], xr[t], zi, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Zp[wave][0][kk + 4 * v][i] = zr[v];
+      Zp[wave][1][kk + 4 * v][i] = zi[v];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < 2 * 16 * kQrUpCols; e += blockDim.x) {
+    const int h = e / (16 * kQrUpCols), r = (e / kQrUpCols) % 16, c = e % kQrUpCols;
+    double s = Zp[0][h][r][c];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) s += Zp[w][h][r][c];
+    Z[h][r][c] = s;
+  }
+  __syncthreads();
+  // Re X -= Lr Zr - Li Zi, Im X -= Lr Zi + Li Zr (rows of Z >= bw are zero)
+  double zbr[4], zbi[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    zbr[t] = Z[0][4 * t + kk][i];
+    zbi[t] = Z[1][4 * t + kk][i];
+  }
+  for (int r0 = 16 * wave; r0 < m; r0 += 16 * kW) {
+    const int ra = min(r0 + i, m - 1);
+    double nlr[4], li[4], nli[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int l = 4 * t + kk;
+      const double* lp = L + 2 * ((int64_t)min(l, bw - 1) * m + ra);  // clamped, in bounds
+      const double a = lp[0], bq = lp[1];
+      const bool ok = l < bw && r0 + i < m;
+      nlr[t] = ok ? -a : 0.0;
+      li[t] = ok ? bq : 0.0;
+      nli[t] = -li[t];
+    }
+    d4v ar, ai;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + kk + 4 * v;
+      const int rq = min(r, m - 1);
+      const double a = xc[2 * rq], bq = xc[2 * rq + 1];
+      const bool ok = colok && r < m;
+      ar[v] = ok ? a : 0.0;
+      ai[v] = ok ? bq : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      ar = __builtin_amdgcn_mfma_f64_16x16x4f64(nlr[t], zbr[t], ar, 0, 0, 0);
+      ar = __builtin_amdgcn_mfma_f64_16x16x4f64(li[t], zbi[t], ar, 0, 0, 0);
+      ai = __builtin_amdgcn_mfma_f64_16x16x4f64(nlr[t], zbi[t], ai, 0, 0, 0);
+      ai = __builtin_amdgcn_mfma_f64_16x16x4f64(nli[t], zbr[t], ai, 0, 0, 0);
+    }
+    if (colok) {
+      double* xw = X + 2 * (int64_t)i * M;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + kk + 4 * v;
+        if (r < m) {
+          xw[2 * r] = ar[v];
+          xw[2 * r + 1] = ai[v];
+        }
+      }
+    }
+  }
+}
+
+// X = R^H (N x N complex) from the upper triangle of the factorised A
+__global__ __launch_bounds__(256) void zqr_rt_kernel(const QrJob* __restrict__ probs) {
+  const QrJob p = probs[blockIdx.y];
+  const int N = p.N, M = p.M;
+  const int64_t nn = (int64_t)N * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / N), j = (int)(e % N);  // X[j + i N] = conj(R[i][j])
+    const double* a = p.A + 2 * (i + (int64_t)j * M);
+    const bool up = j >= i;
+    p.X[2 * e] = up ? a[0] : 0.0;
+    p.X[2 * e + 1] = up ? -a[1] : 0.0;
+  }
+}
+
+// C = [S; 0] (M x N complex) ahead of the back-transformation C <- Q C
+__global__ __launch_bounds__(256) void zqr_seed_kernel(const QrJob* __restrict__ probs) {
+  const QrJob p = probs[blockIdx.y];
+  if (!p.S) return;
+  const int N = p.N, M = p.M;
+  const int64_t mn = (int64_t)M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / M), i = (int)(e % M);
+    const bool in = i < N;
+    const double* s = p.S + 2 * ((int64_t)(in ? i : 0) + (int64_t)j * N);
+    p.C[2 * e] = in ? s[0] : 0.0;
+    p.C[2 * e + 1] = in ? s[1] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Column presort of the QR-preconditioned problems.  The first QR factors
 // A P, P ordering the columns by decreasing norm (ties: lower index first);
 // the Jacobi on the resulting triangle then needs fewer sweeps.  A P = U S W^T
@@ -2852,6 +3232,7 @@ struct SortJob {
   int* perm;        // column j of A P is column perm[j] of A
   double* C;        // N x N (ld N) whose rows are scattered back through P
   int M, N;
+  int cs;           // doubles per entry: 1 real, 2 interleaved complex
 };
 constexpr int kSortMaxN = 8192;  // keys staged in LDS
 
@@ -2860,10 +3241,11 @@ __global__ __launch_bounds__(256) void sort_colnorm_kernel(const SortJob* __rest
   const SortJob p = jobs[blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int jend = min(p.N, (int)(blockIdx.y + 1) * kFinCols);
+  const int mc = p.cs * p.M;
   for (int j = blockIdx.y * kFinCols + wave; j < jend; j += 4) {
-    const double* a = p.A + (int64_t)j * p.M;
+    const double* a = p.A + (int64_t)j * mc;
     double s = 0.0;
-    for (int i = lane; i < p.M; i += 64) s += a[i] * a[i];
+    for (int i = lane; i < mc; i += 64) s += a[i] * a[i];
     s = wave_sum(s);
     if (lane == 0) p.key[j] = s;
   }
@@ -2891,24 +3273,27 @@ __global__ __launch_bounds__(1024) void sort_rank_kernel(const SortJob* __restri
 __global__ __launch_bounds__(256) void sort_gather_kernel(const SortJob* __restrict__ jobs) {
   const SortJob p = jobs[blockIdx.x];
   const int c0 = blockIdx.y * kFinCols, c1 = min(p.N, c0 + kFinCols);
-  const int64_t n = (int64_t)max(c1 - c0, 0) * p.M;
+  const int mc = p.cs * p.M;
+  const int64_t n = (int64_t)max(c1 - c0, 0) * mc;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const int j = c0 + (int)(e / p.M), i = (int)(e % p.M);
-    p.dst[e + (int64_t)c0 * p.M] = p.A[i + (int64_t)p.perm[j] * p.M];
+    const int j = c0 + (int)(e / mc), i = (int)(e % mc);
+    p.dst[e + (int64_t)c0 * mc] = p.A[i + (int64_t)p.perm[j] * mc];
   }
 }
 
 // dst[perm[i], j] = C[i, j]  (mode 0), then C = dst  (mode 1)
 __global__ __launch_bounds__(256) void sort_scatter_rows_kernel(const SortJob* __restrict__ jobs, int mode) {
   const SortJob p = jobs[blockIdx.x];
-  const int N = p.N;
+  const int N = p.N, cs = p.cs;
   const int c0 = blockIdx.y * kFinCols, c1 = min(N, c0 + kFinCols);
-  const int64_t n = (int64_t)max(c1 - c0, 0) * N;
+  const int64_t n = (int64_t)max(c1 - c0, 0) * N * cs;
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const int64_t at = e + (int64_t)c0 * N;
+    const int64_t at = e + (int64_t)c0 * N * cs;  // double index
     if (mode == 0) {
-      const int j = (int)(at / N), i = (int)(at % N);
-      p.dst[p.perm[i] + (int64_t)j * N] = p.C[at];
+      const int64_t en = at / cs;  // entry index
+      const int part = (int)(at - en * cs);
+      const int j = (int)(en / N), i = (int)(en % N);
+      p.dst[(p.perm[i] + (int64_t)j * N) * cs + part] = p.C[at];
     } else {
       p.C[at] = p.dst[at];
     }
@@ -3669,35 +4054,45 @@ namespace {
 constexpr int kQrMinN = 48;  // smaller problems are not worth the QR launches
 
 // One batched blocked-Householder QR over a set of jobs: per panel k the
-// panel kernel, then the trailing update Q_k^T A (qr_update_kernel); the
-// back-transformation C <- Q_1 ... Q_npan C applies the panels in reverse.
+// panel kernel, then the trailing update Q_k^H A (qr_update_kernel or its
+// complex twin); the back-transformation C <- Q_1 ... Q_npan C applies the
+// panels in reverse.
 struct QrPhase {
   std::vector<QrJob> jobs;
   DevBuf<QrJob> d_jobs;
   DevBuf<double> work;
-  int b = 32, npan = 0, maxN = 0, maxM = 0, reg_u = 0;
+  int b = 32, npan = 0, maxN = 0, maxM = 0, reg_u = 0, cs = 1;
   bool lds = true;
   size_t lds_bytes = 0;
 
   // jobs: every field but Y and P set by the caller (X included); the plan
-  // adds the Y and P buffers.
-  void plan(std::vector<QrJob> in, hipStream_t stream) {
+  // adds the Y and P buffers.  cs = 2: interleaved complex jobs.
+  void plan(std::vector<QrJob> in, hipStream_t stream, int cs_ = 1) {
     jobs = std::move(in);
+    cs = cs_;
     maxM = maxN = 0;
     for (const QrJob& j : jobs) { maxM = std::max(maxM, j.M); maxN = std::max(maxN, j.N); }
-    // widest panel whose LDS copy fits (m*b + b*b + b doubles plus the static bytes)
+    // widest panel whose LDS copy fits (cs (m*b + b*b + b) doubles plus the static bytes)
     const size_t budget = 160 * 1024 - 256;
     lds = false;
-    for (int bb : {32, 16, 8}) {
-      const size_t need = ((size_t)maxM * bb + (size_t)bb * bb + bb) * sizeof(double);
-      if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
-    }
-    if (!lds) { b = 32; lds_bytes = ((size_t)b * b + b) * sizeof(double); }
-    // register column loop (b = 32, panel columns padded to 64 kU rows)
     reg_u = 0;
-    if (lds && b == 32 && maxM <= 64 * 9) {
-      reg_u = maxM <= 64 * 4 ? 4 : 9;
-      lds_bytes = ((size_t)64 * reg_u * b + (size_t)b * b + b) * sizeof(double);
+    if (cs == 2) {
+      for (int bb : {kZqrB, 8}) {
+        const size_t need = 2 * ((size_t)maxM * bb + (size_t)bb * bb + bb) * sizeof(double);
+        if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
+      }
+      if (!lds) { b = kZqrB; lds_bytes = 2 * ((size_t)b * b + b) * sizeof(double); }
+    } else {
+      for (int bb : {32, 16, 8}) {
+        const size_t need = ((size_t)maxM * bb + (size_t)bb * bb + bb) * sizeof(double);
+        if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
+      }
+      if (!lds) { b = 32; lds_bytes = ((size_t)b * b + b) * sizeof(double); }
+      // register column loop (b = 32, panel columns padded to 64 kU rows)
+      if (lds && b == 32 && maxM <= 64 * 9) {
+        reg_u = maxM <= 64 * 4 ? 4 : 9;
+        lds_bytes = ((size_t)64 * reg_u * b + (size_t)b * b + b) * sizeof(double);
+      }
     }
     npan = (maxN + b - 1) / b;
     int64_t total = 0;
@@ -3705,21 +4100,38 @@ struct QrPhase {
     for (const QrJob& j : jobs) {
       const int np = (j.N + b - 1) / b;
       off.push_back(total);
-      total += 2 * (int64_t)np * b * j.M;
+      total += 2 * (int64_t)cs * np * b * j.M;
     }
     work.ensure((size_t)std::max<int64_t>(total, 1));
     for (size_t t = 0; t < jobs.size(); ++t) {
       QrJob& r = jobs[t];
       const int np = (r.N + b - 1) / b;
       r.Y = work.data() + off[t];
-      r.P = r.Y + (int64_t)np * b * r.M;
+      r.P = r.Y + (int64_t)cs * np * b * r.M;
     }
     upload(d_jobs, jobs, stream);
   }
 
-  // A = Q R in place, X = R^T
+  // A = Q R in place, X = R^H
   void factor(hipStream_t stream) const {
     const unsigned nj = (unsigned)jobs.size();
+    if (cs == 2) {
+      for (int k = 0; k < npan; ++k) {
+        if (lds)
+          hipLaunchKernelGGL(zqr_panel_kernel<true>, dim3(nj), dim3(512), lds_bytes, stream, d_jobs.data(), k, b);
+        else
+          hipLaunchKernelGGL(zqr_panel_kernel<false>, dim3(nj), dim3(512), lds_bytes, stream, d_jobs.data(), k, b);
+        DMRG_HIP(hipGetLastError());
+        const unsigned gx = (unsigned)((maxN - std::min(maxN, k * b) + kQrUpCols - 1) / kQrUpCols);
+        if (gx > 0)
+          hipLaunchKernelGGL(zqr_update_kernel<false>, dim3(gx, nj), dim3(512), 0, stream, d_jobs.data(), k, b);
+        DMRG_HIP(hipGetLastError());
+      }
+      const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxN * maxN + 255) / 256);
+      hipLaunchKernelGGL(zqr_rt_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
+      DMRG_HIP(hipGetLastError());
+      return;
+    }
     for (int k = 0; k < npan; ++k) {
       if (reg_u == 4)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 4>), dim3(nj), dim3(512), lds_bytes, stream,
@@ -3744,12 +4156,21 @@ struct QrPhase {
   }
 
   // C = Q [S; 0] (or Q C when S is null), then the finish of flagged jobs
+  // (real jobs only)
   void back(hipStream_t stream) const {
     const unsigned nj = (unsigned)jobs.size();
     const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)maxM * maxN + 255) / 256);
+    const unsigned gu = (unsigned)((maxN + kQrUpCols - 1) / kQrUpCols);
+    if (cs == 2) {
+      hipLaunchKernelGGL(zqr_seed_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
+      DMRG_HIP(hipGetLastError());
+      for (int k = npan - 1; k >= 0; --k)
+        hipLaunchKernelGGL(zqr_update_kernel<true>, dim3(gu, nj), dim3(512), 0, stream, d_jobs.data(), k, b);
+      DMRG_HIP(hipGetLastError());
+      return;
+    }
     hipLaunchKernelGGL(qr_seed_kernel, dim3(gx, nj), dim3(256), 0, stream, d_jobs.data());
     DMRG_HIP(hipGetLastError());
-    const unsigned gu = (unsigned)((maxN + kQrUpCols - 1) / kQrUpCols);
     for (int k = npan - 1; k >= 0; --k)
       hipLaunchKernelGGL(qr_update_kernel<true>, dim3(gu, nj), dim3(512), 0, stream, d_jobs.data(), k, b);
     DMRG_HIP(hipGetLastError());
@@ -3816,7 +4237,7 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
       // the first QR factors A P (a presorted copy) when the keys fit in LDS
       const double* A0 = p.G;
       if (p.N <= kSortMaxN) {
-        SortJob sj{p.G, sb, sb + (int64_t)p.M * p.N, sp, mode[i] == 1 ? X3 : p.V, p.M, p.N};
+        SortJob sj{p.G, sb, sb + (int64_t)p.M * p.N, sp, mode[i] == 1 ? X3 : p.V, p.M, p.N, 1};
         sb += (int64_t)p.M * p.N + p.N;
         sp += p.N;
         sorts.push_back(sj);
@@ -3955,12 +4376,86 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
   }
 }
 
-void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scratch,
+void zjacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scratch,
                  DevBuf<int>& sweeps_out, hipStream_t stream, int max_sweeps) {
-  if (probs.empty()) return;
-  const int np = (int)probs.size();
-  for (const SvdProblem& p : probs)
+  if (probs_in.empty()) return;
+  const int np = (int)probs_in.size();
+  for (const SvdProblem& p : probs_in)
     require(p.M >= p.N && p.N >= 1, "zjacobi_svd: needs M >= N >= 1", kErrInternal);
+  // QR preconditioning (problems with precond set and N >= kQrMinN): with
+  // A P = Q_1 R_1 (P: the columns in decreasing norm order) and R_1^H = Q_2 R_2
+  // the Jacobi runs on X = R_2^H (X J = W), and A = (Q_1 [W; 0]) (P Q_2 J)^H:
+  // the left factor and sigma come from G = Q_1 [W; 0], V = P Q_2 J is
+  // unitary to rounding.  DMRGCC_ZSVD_QR=0 turns it off.
+  const char* qenv = std::getenv("DMRGCC_ZSVD_QR");
+  const bool qr_on = !qenv || std::atoi(qenv) != 0;
+  std::vector<SvdProblem> probs = probs_in;
+  std::vector<int> pq;
+  for (int i = 0; i < np; ++i)
+    if (qr_on && probs_in[i].precond != 0 && probs_in[i].N >= kQrMinN) pq.push_back(i);
+  QrPhase ph1, ph2;
+  DevBuf<double> xbuf, sbuf;
+  DevBuf<int> sperm;
+  DevBuf<SortJob> d_sort;
+  std::vector<SortJob> sorts;
+  int sort_n = 0;
+  if (!pq.empty()) {
+    int64_t total = 0, stotal = 0, ptotal = 0;
+    for (int i : pq) {
+      const SvdProblem& p = probs_in[i];
+      total += 4 * (int64_t)p.N * p.N;  // X_1, X_2 (complex)
+      if (p.N <= kSortMaxN) {
+        stotal += 2 * (int64_t)p.M * p.N + p.N;
+        ptotal += p.N;
+      }
+    }
+    xbuf.ensure((size_t)total);
+    sbuf.ensure((size_t)std::max<int64_t>(stotal, 1));
+    sperm.ensure((size_t)std::max<int64_t>(ptotal, 1));
+    std::vector<QrJob> j1, j2;
+    double* x = xbuf.data();
+    double* sb = sbuf.data();
+    int* sp = sperm.data();
+    for (int i : pq) {
+      const SvdProblem& p = probs_in[i];
+      const int64_t nn = (int64_t)p.N * p.N;
+      double* X1 = x;
+      double* X2 = x + 2 * nn;
+      x += 4 * nn;
+      const double* A0 = p.G;
+      if (p.N <= kSortMaxN) {  // the first QR factors a presorted copy
+        SortJob sj{p.G, sb, sb + 2 * (int64_t)p.M * p.N, sp, p.V, p.M, p.N, 2};
+        sb += 2 * (int64_t)p.M * p.N + p.N;
+        sp += p.N;
+        sorts.push_back(sj);
+        sort_n = std::max(sort_n, p.N);
+        A0 = sj.dst;
+      }
+      QrJob a{};
+      a.A = const_cast<double*>(A0); a.X = X1; a.C = p.G; a.S = X2;  // G = Q_1 [W; 0]
+      a.M = p.M; a.N = p.N;
+      j1.push_back(a);
+      QrJob c{};
+      c.A = X1; c.X = X2; c.C = p.V;  // V = Q_2 J
+      c.M = p.N; c.N = p.N;
+      j2.push_back(c);
+      probs[i].G = X2;
+      probs[i].M = p.N;
+    }
+    ph1.plan(j1, stream, 2);
+    ph2.plan(j2, stream, 2);
+    if (!sorts.empty()) {
+      upload(d_sort, sorts, stream);
+      const unsigned ns = (unsigned)sorts.size(), gy = (unsigned)((sort_n + kFinCols - 1) / kFinCols);
+      hipLaunchKernelGGL(sort_colnorm_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data());
+      hipLaunchKernelGGL(sort_rank_kernel, dim3(ns), dim3(1024), sort_n * sizeof(double), stream,
+                         d_sort.data());
+      hipLaunchKernelGGL(sort_gather_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data());
+      DMRG_HIP(hipGetLastError());
+    }
+    ph1.factor(stream);
+    ph2.factor(stream);
+  }
   upload(scratch, probs, stream);
   int init_n = 1;
   for (const SvdProblem& q : probs) init_n = std::max(init_n, q.N);
@@ -4017,17 +4512,31 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs, DevBuf<SvdProblem>& scrat
       if (hrot[i]) still.push_back(i);
     active.swap(still);
   }
+  if (!pq.empty()) {
+    ph2.back(stream);  // V = Q_2 J
+    if (!sorts.empty()) {  // rows of V back through P
+      const unsigned ns = (unsigned)sorts.size(), gy = (unsigned)((sort_n + kFinCols - 1) / kFinCols);
+      hipLaunchKernelGGL(sort_scatter_rows_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data(), 0);
+      hipLaunchKernelGGL(sort_scatter_rows_kernel, dim3(ns, gy), dim3(256), 0, stream, d_sort.data(), 1);
+      DMRG_HIP(hipGetLastError());
+    }
+    ph1.back(stream);  // G = Q_1 [W; 0]
+    upload(scratch, probs_in, stream);
+  }
   int fin_n = 1;
-  for (const SvdProblem& q : probs) fin_n = std::max(fin_n, q.N);
+  for (const SvdProblem& q : probs_in) fin_n = std::max(fin_n, q.N);
   hipLaunchKernelGGL(zjacobi_finish_kernel, dim3(np, (fin_n + kFinCols - 1) / kFinCols), dim3(256), 0, stream,
                      scratch.data());
   DMRG_HIP(hipGetLastError());
+  // the phases' buffers are released when this function returns
+  if (!pq.empty()) DMRG_HIP(hipStreamSynchronize(stream));
   if (std::getenv("DMRG_SVD_STATS")) {
     int maxN = 0;
-    for (const SvdProblem& q : probs) maxN = std::max(maxN, q.N);
-    std::printf("ZSVDSTAT np=%d maxN=%d sweeps=%d |", np, maxN, sweeps_done);
+    for (const SvdProblem& q : probs_in) maxN = std::max(maxN, q.N);
+    std::printf("ZSVDSTAT np=%d qr=%d maxN=%d sweeps=%d |", np, (int)pq.size(), maxN, sweeps_done);
     for (int i = 0; i < np; ++i)
-      if ((int64_t)probs[i].M * probs[i].N > 20000) std::printf(" %dx%d:%d", probs[i].M, probs[i].N, last[i]);
+      if ((int64_t)probs_in[i].M * probs_in[i].N > 20000)
+        std::printf(" %dx%d:%d", probs_in[i].M, probs_in[i].N, last[i]);
     std::printf("\n");
   }
 }

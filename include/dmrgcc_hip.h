@@ -359,9 +359,10 @@ int dmrgcc_sector_svd(int n, const double* const* A, const int* M, const int* N,
 
 /* Complex (ComplexMatrix::SVDMatrix, complex_matrix.cc:601-637; zgesdd in
  * the reference): A_i, U_i, Vt_i interleaved complex row-major (A = U S Vt,
- * Vt = V^H), S_i real descending.  Complex one-sided block Jacobi, no QR
- * preconditioning; the largest-modulus entry of every column of U is real
- * positive. */
+ * Vt = V^H), S_i real descending.  Complex one-sided block Jacobi; sectors
+ * with min(M, N) >= 48 are QR-preconditioned as in the sweep (environment
+ * DMRGCC_ZSVD_QR=0 turns that off); the largest-modulus entry of every column
+ * of U is real positive. */
 int dmrgcc_zsector_svd(int n, const double* const* A, const int* M, const int* N,
                        double* const* U, double* const* S, double* const* Vt, void* stream);
 

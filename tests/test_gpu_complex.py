@@ -200,7 +200,10 @@ def test_zlanczos_matches_oracle(name):
 
 
 @pytest.mark.parametrize("shapes", [[(1, 1), (5, 5), (40, 12), (12, 40)], [(120, 90), (90, 120)],
-                                    [(300, 64), (33, 257)]])
+                                    [(300, 64), (33, 257)],
+                                    # QR panels of 16 columns in LDS (many panels, square), of 8
+                                    # columns in LDS (M > 622), in global memory (M > 1267)
+                                    [(600, 500), (470, 470)], [(1300, 64)], [(1400, 100), (100, 1500)]])
 def test_zsector_svd(shapes):
     # complex sector SVD (ComplexMatrix::SVDMatrix) against numpy on graded
     # spectra down to 1e-12, with the phase convention of the oracle
@@ -230,3 +233,39 @@ def test_zsector_svd(shapes):
         ur = ur * fix_signs(ur)[None, :]
         big = sg > 1e-4
         assert np.abs(ur[:, big] - u[:, big]).max() < 1e-8
+
+
+@pytest.mark.parametrize("qr", ["1", "0"])
+def test_zsector_svd_degenerate(qr, monkeypatch):
+    # rank-deficient sectors with exact zero columns, a real-valued complex
+    # matrix (reflectors with real alpha and no tail) and repeated singular
+    # values, with and without the QR preconditioning
+    monkeypatch.setenv("DMRGCC_ZSVD_QR", qr)
+    rng = np.random.default_rng(5)
+    a1 = rng.standard_normal((200, 40)) + 1j * rng.standard_normal((200, 40))
+    a1 = a1 @ (rng.standard_normal((40, 90)) + 1j * rng.standard_normal((40, 90)))  # rank 40
+    a1[:, 7] = 0
+    a1[:, 50:54] = 0
+    a2 = rng.standard_normal((150, 80)).astype(np.complex128)
+    a2[:, 0] = 0
+    q, _ = np.linalg.qr(rng.standard_normal((120, 60)) + 1j * rng.standard_normal((120, 60)))
+    p, _ = np.linalg.qr(rng.standard_normal((60, 60)) + 1j * rng.standard_normal((60, 60)))
+    s3 = np.repeat([2.0, 1.0, 0.5], 20)
+    a3 = (q * s3) @ p.conj().T
+    mats = [a1, a2, a3]
+    U, S, Vt = dm.sector_svd([cdev(a) for a in mats])
+    for a, u, sg, vt in zip(mats, U, S, Vt):
+        u, sg, vt = u.cpu().numpy(), sg.cpu().numpy(), vt.cpu().numpy()
+        M, N = a.shape
+        k = min(M, N)
+        ref = np.linalg.svd(a, compute_uv=False)
+        scale = ref[0]
+        assert np.abs(sg - ref).max() < 1e-13 * max(M, N) * scale
+        assert np.abs(vt @ vt.conj().T - np.eye(k)).max() < 1e-12
+        # left vectors of (numerically) zero singular values are not defined
+        # (an exactly zero one gives a NaN column): the live part only
+        live = sg > 1e-10 * scale
+        uu = u[:, live]
+        assert np.isfinite(uu).all()
+        assert np.abs((uu * sg[live]) @ vt[live] - a).max() < 1e-13 * max(M, N) * scale
+        assert np.abs(uu.conj().T @ uu - np.eye(live.sum())).max() < 1e-11
