@@ -1571,7 +1571,7 @@ constexpr int kJc = 2 * kJb;
 constexpr int kJw = DMRG_JACOBI_WAVES;
 constexpr int kJn = kJc / 16;  // 16 x 16 MFMA blocks per visit edge
 static_assert(kJc == 16 || kJc == 32, "visits of 16 or 32 columns");
-static_assert(kJw >= 2 && kJw <= 16, "2 to 16 waves per visit (wave 0 sweeps while the others update V)");
+static_assert(kJw >= 2 && kJw <= 16, "2 to 16 waves per visit");
 static_assert(kJc == 16 || kJw >= 8, "32-column visits update their 32 x 32 Gram with eight waves");
 
 // Slabs of 16 rows handled together by one wave (their loads in flight at once)
@@ -1725,61 +1725,71 @@ __device__ inline int visit_column(int round, int pr, int nb, int nb2, int N, in
   return (blk < nb && c < N) ? c : -1;
 }
 
-// One round.  The rotation of a visit is applied to G at once and to V one
-// round later: launch `round` also applies the previous round's rotation of
-// the same (problem, block pair) to V, by waves 1 .. kJw-1 while wave 0 runs
-// the inner sweep (V is read by nothing else during the sweep, and every V
-// column still receives its rotations in round order).  The rotation and a
-// flag go through qbuf / qflag, double-buffered by round parity; a launch
-// with round == the sweep's round count only flushes the last rotations.
+// The V half of a round launch (workgroups nvisits .. 2 nvisits - 1): visit
+// slot v applies the previous round's rotation of its (problem, block pair)
+// to V with the whole workgroup, on CUs of its own, while the G half of the
+// same launch runs the current round (V is read by nothing else during the
+// sweep, and every V column still receives its rotations in round order).
+template <int kC, int kW, typename QT>
+__device__ inline bool round_v_part(const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
+                                    const int* __restrict__ vpair, int round, const double* __restrict__ qbuf,
+                                    const int* __restrict__ qflag, int nvisits, int kb, int cs, QT apply) {
+  if ((int)blockIdx.x < nvisits) return false;
+  const int v = (int)blockIdx.x - nvisits;
+  if (round < 1 || !qflag[((round - 1) & 1) * nvisits + v]) return true;
+  __shared__ double Qp[2][kC][kC + 1];  // the previous round's rotation (complex: re, im)
+  __shared__ int colp[kC];
+  const int pi = vprob[v], pr = vpair[v];
+  const SvdProblem p = probs[pi];
+  const int N = p.N;
+  const int nb = (N + kb - 1) / kb, nb2 = nb + (nb & 1);
+  if (round - 1 >= nb2 - 1) return true;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double* qs = qbuf + ((int64_t)((round - 1) & 1) * nvisits + v) * (cs * kC * kC);
+  for (int e = tid; e < cs * kC * kC; e += 64 * kW) {
+    const int h = e / (kC * kC), r = (e / kC) % kC, c = e % kC;
+    Qp[h][r][c] = qs[e];
+  }
+  if (tid < kC) {
+    int bi, bj;
+    if (pr == 0) { bi = round - 1; bj = nb2 - 1; }
+    else { bi = (round - 1 + pr) % (nb2 - 1); bj = (round - 1 + nb2 - 1 - pr) % (nb2 - 1); }
+    const int blk = tid < kb ? bi : bj;
+    const int c = blk * kb + (tid % kb);
+    colp[tid] = (blk < nb && c < N) ? c : -1;
+  }
+  __syncthreads();
+  apply(p.V, N, colp, Qp, lane, wave);
+  return true;
+}
+
+// One round.  Launched with 2 nvisits workgroups: the first nvisits run the
+// round's visits (Gram, inner sweep, rotation of G), the others apply the
+// previous round's rotations to V (round_v_part).  The rotation and a flag
+// go through qbuf / qflag, double-buffered by round parity; a launch with
+// round == the sweep's round count only flushes the last rotations.
 __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
     const int* __restrict__ vpair, int round, int* __restrict__ rotated,
     double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
+  if (round_v_part<kJc, kJw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kJb, 1,
+                             [](double* V, int N, const int* colp, const double (*Qp)[kJc][kJc + 1], int lane,
+                                int wave) { apply_rotation(V, N, N, colp, Qp[0], lane, wave, kJw); }))
+    return;
   __shared__ double Gp[kJw][kJc][kJc + 1];  // partial Grams, one per wave
   __shared__ double Q[kJc][kJc + 1];
-  __shared__ double Qp[kJc][kJc + 1];       // the previous round's rotation
-  __shared__ int col[kJc], colp[kJc];
+  __shared__ int col[kJc];
   __shared__ int flag;
   double (*G)[kJc + 1] = Gp[0];  // the summed Gram overwrites wave 0's partial
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // The previous round's flag and rotation of this visit slot depend only on
-  // the launch index, so their loads are issued with the descriptor's (the
-  // rotation is read whether or not it is used: the slot is always in bounds)
-  const int pflag = round >= 1 ? qflag[((round - 1) & 1) * nvisits + blockIdx.x] : 0;
-  double qpre[(kJc * kJc + 64 * kJw - 1) / (64 * kJw)];
-  if (round >= 1) {
-    const double* qs = qbuf + ((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (kJc * kJc);
-#pragma unroll
-    for (int u = 0; u * 64 * kJw < kJc * kJc; ++u) {
-      const int e = tid + u * 64 * kJw;
-      qpre[u] = e < kJc * kJc ? qs[e] : 0.0;
-    }
-  }
   // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
   const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
-  const bool cur = round < nb2 - 1;
-  const bool prev = round >= 1 && round - 1 < nb2 - 1 && pflag != 0;
-  if (!cur && !prev) return;
-  if (tid < kJc) {
-    col[tid] = cur ? visit_column(round, pr, nb, nb2, N, tid) : -1;
-    colp[tid] = prev ? visit_column(round - 1, pr, nb, nb2, N, tid) : -1;
-  }
-  if (prev) {
-#pragma unroll
-    for (int u = 0; u * 64 * kJw < kJc * kJc; ++u) {
-      const int e = tid + u * 64 * kJw;
-      if (e < kJc * kJc) Qp[e / kJc][e % kJc] = qpre[u];
-    }
-  }
+  if (round >= nb2 - 1) return;
+  if (tid < kJc) col[tid] = visit_column(round, pr, nb, nb2, N, tid);
   __syncthreads();
-  if (!cur) {
-    apply_rotation(p.V, N, N, colp, Qp, lane, wave, kJw);
-    return;
-  }
   // partial Gram of this wave's slabs: lane (i, kk) feeds X[row][col_{16h+i}]
   {
     const int i = lane & 15, kk = lane >> 4;
@@ -1840,8 +1850,6 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     const int ident[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
     const int any = inner_sweep16<kJc + 1>(G, Q, ident, lane, tol * tol, zero2);
     if (lane == 0) flag = any;
-  } else if (prev) {
-    apply_rotation(p.V, N, N, colp, Qp, lane, wave - 1, kJw - 1);
   }
   } else {
   // 32-column visits: three local rounds over the four 8-column sub-blocks
@@ -1849,16 +1857,13 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
   // (s0 s1)(s2 s3), (s0 s2)(s1 s3), (s0 s3)(s1 s2).  In each, waves 0 and 1
   // sweep one 16-column pair each (inner_sweep16 on its part of the 32 x 32
   // Gram), their rotations form Qr, and the workgroup updates G <- Qr^T G Qr
-  // and Q <- Q Qr on MFMA.  Waves 2 .. kJw-1 meanwhile apply the previous
-  // round's rotation to V, a third of its rows per local round.  Every
-  // column pair of the two blocks is rotated once per visit, so a sweep needs
+  // and Q <- Q Qr on MFMA.  Every column pair of the two blocks is rotated once per visit, so a sweep needs
   // half the rounds of 16-column visits.
   const double tol = 2.220446049250313e-16 * (double)M;  // above the dot-product round-off
   const double zero2 = 4.930380657631324e-32 * (*p.fro2);  // (eps |A|_F)^2
   double (*Qr)[kJc + 1] = Gp[1];  // (partial Grams no longer needed)
   double (*Tm)[kJc + 1] = Gp[2];
   double (*Qn)[kJc + 1] = Gp[3];
-  const int vchunk = (N + 2) / 3;
   int any = 0;
   for (int lr = 0; lr < 3; ++lr) {
     if (wave < 2) {
@@ -1880,9 +1885,6 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
         const int r = e >> 4, c = e & 15;
         Qr[sel[r]][c < 8 ? 8 * oa + c : 8 * ob + c - 8] = 0.0;
       }
-    } else if (prev) {
-      const int r0 = lr * vchunk, r1 = min(N, r0 + vchunk);
-      if (r1 > r0) apply_rotation(p.V + r0, N, r1 - r0, colp, Qp, lane, wave - 2, kJw - 2);
     }
     __syncthreads();
     // Tm = G Qr (waves 0-3), Qn = Q Qr (waves 4-7): one 16 x 16 block each
@@ -2057,63 +2059,43 @@ __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int r
   }
 }
 
-// One round, as jacobi_gram_round_kernel: the rotation of a visit is applied
-// to G at once and to V one round later (launch `round` applies the previous
-// round's rotation of the same block-pair slot to V on waves 1 .. kZw-1
-// while wave 0 runs the inner sweep; a launch with round == the sweep's
-// round count only flushes the last rotations).  Rotations and flags go
-// through qbuf (Qr then Qi, 2 kZc^2 doubles per slot) and qflag,
-// double-buffered by round parity.
+// One round, as jacobi_gram_round_kernel: launched with 2 nvisits
+// workgroups, the first nvisits running the round's visits and the others
+// applying the previous round's rotations to V (round_v_part).  Rotations and
+// flags go through qbuf (Qr then Qi, 2 kZc^2 doubles per slot) and qflag,
+// double-buffered by round parity; a launch with round == the sweep's round
+// count only flushes the last rotations.
 __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
     const int* __restrict__ vpair, int round, int* __restrict__ rotated,
     double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
+  constexpr int kB = kZc / 2;
+  if (round_v_part<kZc, kZw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kB, 2,
+                             [](double* V, int N, const int* colp, const double (*Qp)[kZc][kZc + 1], int lane,
+                                int wave) { zapply_rotation(V, N, N, colp, Qp[0], Qp[1], lane, wave, kZw); }))
+    return;
   __shared__ double Pr[kZw][kZc][kZc + 1];
   __shared__ double Pi[kZw][kZc][kZc + 1];
   __shared__ double Qr[kZc][kZc + 1];
   __shared__ double Qi[kZc][kZc + 1];
-  __shared__ double Qpr[kZc][kZc + 1];  // the previous round's rotation
-  __shared__ double Qpi[kZc][kZc + 1];
-  __shared__ int col[kZc], colp[kZc];
+  __shared__ int col[kZc];
   __shared__ int flag;
-  static_assert(2 * kZc * kZc == 64 * kZw, "one previous-rotation entry per thread");
   double (*Gr)[kZc + 1] = Pr[0];
   double (*Gi)[kZc + 1] = Pi[0];
-  constexpr int kB = kZc / 2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // the previous round's flag and rotation of this slot (always in bounds)
-  const int pflag = round >= 1 ? qflag[((round - 1) & 1) * nvisits + blockIdx.x] : 0;
-  const double qpre =
-      round >= 1 ? qbuf[((int64_t)((round - 1) & 1) * nvisits + blockIdx.x) * (2 * kZc * kZc) + tid] : 0.0;
   // visit blockIdx.x is column-block pair pr of problem pi (host-built map)
   const int pi = vprob[blockIdx.x], pr = vpair[blockIdx.x];
   const SvdProblem p = probs[pi];
   const int M = p.M, N = p.N;
   const int nb = (N + kB - 1) / kB, nb2 = nb + (nb & 1);
-  const bool cur = round < nb2 - 1;
-  const bool prev = round >= 1 && round - 1 < nb2 - 1 && pflag != 0;
-  if (!cur && !prev) return;
-  auto visit_col = [&](int rd, int slot) {
-    int bi, bj;
-    if (pr == 0) { bi = rd; bj = nb2 - 1; }
-    else { bi = (rd + pr) % (nb2 - 1); bj = (rd + nb2 - 1 - pr) % (nb2 - 1); }
-    const int blk = slot < kB ? bi : bj;
-    const int c = blk * kB + (slot & (kB - 1));
-    return (blk < nb && c < N) ? c : -1;
-  };
+  if (round >= nb2 - 1) return;
   if (tid < kZc) {
-    col[tid] = cur ? visit_col(round, tid) : -1;
-    colp[tid] = prev ? visit_col(round - 1, tid) : -1;
-  }
-  if (prev) {
-    const int e = tid & (kZc * kZc - 1);
-    if (tid < kZc * kZc) Qpr[e / kZc][e % kZc] = qpre;
-    else Qpi[e / kZc][e % kZc] = qpre;
-  }
-  __syncthreads();
-  if (!cur) {
-    zapply_rotation(p.V, N, N, colp, Qpr, Qpi, lane, wave, kZw);
-    return;
+    int bi, bj;
+    if (pr == 0) { bi = round; bj = nb2 - 1; }
+    else { bi = (round + pr) % (nb2 - 1); bj = (round + nb2 - 1 - pr) % (nb2 - 1); }
+    const int blk = tid < kB ? bi : bj;
+    const int c = blk * kB + (tid & (kB - 1));
+    col[tid] = (blk < nb && c < N) ? c : -1;
   }
   __syncthreads();
   {
@@ -2227,8 +2209,6 @@ __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
     }
     any = __any(any);
     if (lane == 0) flag = any;
-  } else if (prev) {
-    zapply_rotation(p.V, N, N, colp, Qpr, Qpi, lane, wave - 1, kZw - 1);
   }
   __syncthreads();
   int* qf = qflag + (round & 1) * nvisits + blockIdx.x;
@@ -2319,15 +2299,17 @@ struct QrJob {
 // staged in LDS (dynamic, m*b + b*b + b doubles); otherwise it is worked on in
 // place in global memory and LDS holds only T and tau (b*b + b doubles).
 // kU > 0 (b = 32, m <= 64 kU, LDS sized for 64 kU rows): the column loop keeps the panel in
-// registers instead, wave w holding columns w, w + 8, w + 16, w + 24 and lane
+// registers instead, wave w of kWv holding columns w, w + kWv, ... and lane
 // l rows l, l + 64, ...; a column is written to LDS once, by its wave, when
 // its last update is done, and each reflector value read from there serves
-// the four columns of a lane.
-template <bool kLds, int kU = 0>
-__global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__ probs, int k,
+// the 32 / kWv columns of a lane.  A column step is bound by the dependent
+// FP64 chain of one wave (reductions, square root, divisions), not by the
+// barrier: kU = 9 panels take 71 us with 16 waves against 75 us with 8.
+template <bool kLds, int kU = 0, int kWv = 8>
+__global__ __launch_bounds__(64 * kWv) void qr_panel_kernel(const QrJob* __restrict__ probs, int k,
                                                        int b) {
   extern __shared__ double smem[];
-  __shared__ double red[8];
+  __shared__ double red[kWv];
   const QrJob p = probs[blockIdx.x];
   const int M = p.M, N = p.N;
   const int c0 = k * b;
@@ -2376,14 +2358,15 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   // one barrier.
   __shared__ double colss[2];
   if constexpr (kU > 0) {
-    // wave w holds columns w, w + 8, w + 16, w + 24 of the panel, lane l rows
-    // l + 64 u (u < kU): a reflector value read from LDS serves four columns.
+    // wave w holds columns w, w + kWv, ... of the panel, lane l rows l + 64 u
+    // (u < kU): a reflector value read from LDS serves kQ columns.
     // The LDS columns have 64 kU rows (ld), zero past m, so only register 0
     // (rows 0 .. 63, the ones that can be <= j < 32) needs a row test.
-    const int nwv = (int)(blockDim.x >> 6);  // 8
-    double y[4][kU];
+    constexpr int nwv = kWv;
+    constexpr int kQ = 32 / kWv;  // columns per wave
+    double y[kQ][kU];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       const int c = wave + nwv * q;
       const double* src = Ag + (int64_t)min(c, bw - 1) * M;
 #pragma unroll
@@ -2414,7 +2397,7 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
         double* xp = pan + (int64_t)(j - 1) * ld;
         const double sc = tj_prev != 0.0 ? scal_prev : 1.0;
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq)
+        for (int qq = 0; qq < kQ; ++qq)
           if (qq == q) {
             if (lane >= j - 1) xp[lane] = lane == j - 1 ? beta_prev : y[qq][0] * sc;
 #pragma unroll
@@ -2429,18 +2412,18 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       xv[0] = lane > j ? xv[0] : 0.0;
       const double t = colss[j & 1];
       const double alpha = x[j];
-      // x^T y over rows > j for the wave's four columns, reduced together
+      // x^T y over rows > j for the wave's kQ columns, reduced together
       // (a column that is done or absent gives an unused value)
-      double dot[4];
+      double dot[kQ];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kQ; ++q) {
         double dp[2] = {0.0, 0.0};
 #pragma unroll
         for (int u = 0; u < kU; ++u) dp[u & 1] += xv[u] * y[q][u];
         dot[q] = row16_sum(dp[0] + dp[1]);
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < kQ; ++q)
         dot[q] = (readlane_d(dot[q], 0) + readlane_d(dot[q], 16)) +
                  (readlane_d(dot[q], 32) + readlane_d(dot[q], 48));
       double tj = 0.0, beta = alpha, scal = 0.0;
@@ -2451,7 +2434,7 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       }
       if (tid == 0) tau[j] = tj;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < kQ; ++q) {
         const int c = wave + nwv * q;
         if (c <= j || c >= bw) continue;  // uniform over the wave
         // d = tau (y_j + scal x^T y); y -= d v, v = (1, scal x[j+1:]); row
@@ -2486,7 +2469,7 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
       double* xp = pan + (int64_t)(bw - 1) * ld;
       const double sc = tj_prev != 0.0 ? scal_prev : 1.0;
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq)
+      for (int qq = 0; qq < kQ; ++qq)
         if (qq == q) {
           if (lane >= bw - 1) xp[lane] = lane == bw - 1 ? beta_prev : y[qq][0] * sc;
 #pragma unroll
@@ -2605,23 +2588,30 @@ __global__ __launch_bounds__(512) void qr_panel_kernel(const QrJob* __restrict__
   }
   __syncthreads();
   // T upper triangular (dlarft): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
-  // (S[l][j] = y_l^T y_j sits at T[j][l]); row r of column j by 16 threads,
-  // shuffle-reduced within their 16 lanes
-  for (int j = 0; j < bw; ++j) {
+  // (S[l][j] = y_l^T y_j sits at T[j][l]).  Row r of T depends only on its own
+  // earlier entries, T[r][j] = -tau_j sum_{r <= l < j} T[r][l] S[l][j], so the
+  // 16 lanes of row r build it left to right with no barrier: lane sub keeps
+  // T[r][r + sub] and T[r][r + sub + 16] in registers and the partial sums are
+  // reduced within the 16 lanes by DPP.
+  {
     const int r = tid >> 4, sub = tid & 15;
-    if (r < j) {
-      double s = 0.0;
-      for (int l = r + sub; l < j; l += 16) s += T[r * b + l] * T[j * b + l];
-      s += __shfl_xor(s, 8, 64);
-      s += __shfl_xor(s, 4, 64);
-      s += __shfl_xor(s, 2, 64);
-      s += __shfl_xor(s, 1, 64);
-      if (sub == 0) T[r * b + j] = -tau[j] * s;
-    } else if (r == j && sub == 0) {
-      T[j * b + j] = tau[j];
+    double t0 = 0.0, t1 = 0.0;
+    if (r < bw) {
+      for (int j = r; j < bw; ++j) {
+        double sum = 0.0;
+        const int l0 = r + sub, l1 = r + sub + 16;
+        if (l0 < j) sum += t0 * T[j * b + l0];
+        if (l1 < j) sum += t1 * T[j * b + l1];
+        sum = row16_sum(sum);  // the 16 lanes of row r form one DPP row
+        const double v = j == r ? tau[j] : -tau[j] * sum;
+        if (j - r == sub) t0 = v;
+        if (j - r == sub + 16) t1 = v;
+      }
+      if (r + sub < bw) T[r * b + r + sub] = t0;
+      if (r + sub + 16 < bw) T[r * b + r + sub + 16] = t1;
     }
-    __syncthreads();
   }
+  __syncthreads();
   // Y_k (unit lower trapezoidal) and P_k = Y_k T_k
   double* Yk = p.Y + (int64_t)k * b * M;
   double* Pk = p.P + (int64_t)k * b * M;
@@ -3007,33 +2997,37 @@ __global__ __launch_bounds__(512) void zqr_panel_kernel(const QrJob* __restrict_
   }
   __syncthreads();
   // T upper triangular: T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] S[0:j, j]
-  // (S[l][j] at T[j][l]); row r of column j by 16 threads
-  for (int j = 0; j < bw; ++j) {
-    const int r = tid >> 4, sub = tid & 15;
-    if (r < j) {
-      double s_r = 0.0, s_i = 0.0;
-      for (int l = r + sub; l < j; l += 16) {
-        const double a = T[2 * (r * b + l)], bq = T[2 * (r * b + l) + 1];
-        const double cq = T[2 * (j * b + l)], dq = T[2 * (j * b + l) + 1];
-        s_r += a * cq - bq * dq;
-        s_i += a * dq + bq * cq;
-      }
-#pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-        s_r += __shfl_xor(s_r, o, 64);
-        s_i += __shfl_xor(s_i, o, 64);
-      }
-      if (sub == 0) {
+  // (S[l][j] at T[j][l]); as in the real panel, the 16 lanes of row r build
+  // it left to right with no barrier, lane sub keeping T[r][r + sub]
+  {
+    const int r = tid >> 4, sub = tid & 15;  // bw <= 16: one entry per lane
+    double tr0 = 0.0, ti0 = 0.0;
+    if (r < bw) {
+      for (int j = r; j < bw; ++j) {
+        double s_r = 0.0, s_i = 0.0;
+        const int l = r + sub;
+        if (l < j) {
+          const double cq = T[2 * (j * b + l)], dq = T[2 * (j * b + l) + 1];
+          s_r += tr0 * cq - ti0 * dq;
+          s_i += tr0 * dq + ti0 * cq;
+        }
+        s_r = row16_sum(s_r);
+        s_i = row16_sum(s_i);
         const double tjr = tau[2 * j], tji = tau[2 * j + 1];
-        T[2 * (r * b + j)] = -(tjr * s_r - tji * s_i);
-        T[2 * (r * b + j) + 1] = -(tjr * s_i + tji * s_r);
+        const double vr = j == r ? tjr : -(tjr * s_r - tji * s_i);
+        const double vi = j == r ? tji : -(tjr * s_i + tji * s_r);
+        if (j - r == sub) {
+          tr0 = vr;
+          ti0 = vi;
+        }
       }
-    } else if (r == j && sub == 0) {
-      T[2 * (j * b + j)] = tau[2 * j];
-      T[2 * (j * b + j) + 1] = tau[2 * j + 1];
+      if (r + sub < bw) {
+        T[2 * (r * b + r + sub)] = tr0;
+        T[2 * (r * b + r + sub) + 1] = ti0;
+      }
     }
-    __syncthreads();
   }
+  __syncthreads();
   // Y_k (unit lower trapezoidal) and P_k = Y_k T_k, m x bw each (ld m)
   double* Yk = p.Y + 2 * (int64_t)k * b * M;
   double* Pk = p.P + 2 * (int64_t)k * b * M;
@@ -4137,7 +4131,7 @@ struct QrPhase {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 4>), dim3(nj), dim3(512), lds_bytes, stream,
                            d_jobs.data(), k, b);
       else if (reg_u == 9)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 9>), dim3(nj), dim3(512), lds_bytes, stream,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 9, 16>), dim3(nj), dim3(1024), lds_bytes, stream,
                            d_jobs.data(), k, b);
       else if (lds)
         hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(nj), dim3(512), lds_bytes, stream,
@@ -4327,7 +4321,7 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
     upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
     // rounds 0 .. rounds-1, then the flush of the last rotations into V
     for (int r = 0; r <= rounds; ++r)
-      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(visits), dim3(64 * kJw), 0, stream,
+      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(2 * visits), dim3(64 * kJw), 0, stream,
                          scratch.data(), vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -4502,7 +4496,7 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& sc
     upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
     // rounds 0 .. rounds-1, then the flush of the last rotations into V
     for (int r = 0; r <= rounds; ++r)
-      hipLaunchKernelGGL(zjacobi_round_kernel, dim3(visits), dim3(64 * kZw), 0, stream, scratch.data(),
+      hipLaunchKernelGGL(zjacobi_round_kernel, dim3(2 * visits), dim3(64 * kZw), 0, stream, scratch.data(),
                          vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
     DMRG_HIP(hipGetLastError());
     DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
