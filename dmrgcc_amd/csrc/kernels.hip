@@ -1725,16 +1725,40 @@ __device__ inline int visit_column(int round, int pr, int nb, int nb2, int N, in
   return (blk < nb && c < N) ? c : -1;
 }
 
+// Sweep state of the round launches, per problem: rotated (set by the visits
+// that rotate), active (problems still sweeping; the visits of the others
+// exit at once), and host, the device address of pinned host memory that
+// receives rotated at the end of each sweep.
+struct SweepCtl {
+  int* rotated;
+  int* active;
+  int* host;
+  int np;
+};
+
 // The V half of a round launch (workgroups nvisits .. 2 nvisits - 1): visit
 // slot v applies the previous round's rotation of its (problem, block pair)
 // to V with the whole workgroup, on CUs of its own, while the G half of the
 // same launch runs the current round (V is read by nothing else during the
 // sweep, and every V column still receives its rotations in round order).
+// The flush launch of a sweep has one more workgroup, which ends the sweep:
+// rotated goes to the host and becomes the next sweep's active, and is
+// cleared (the visits that wrote it ran in earlier launches).
 template <int kC, int kW, typename QT>
 __device__ inline bool round_v_part(const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
                                     const int* __restrict__ vpair, int round, const double* __restrict__ qbuf,
-                                    const int* __restrict__ qflag, int nvisits, int kb, int cs, QT apply) {
+                                    const int* __restrict__ qflag, int nvisits, int kb, int cs, SweepCtl ctl,
+                                    QT apply) {
   if ((int)blockIdx.x < nvisits) return false;
+  if ((int)blockIdx.x >= 2 * nvisits) {
+    for (int i = threadIdx.x; i < ctl.np; i += blockDim.x) {
+      const int r = ctl.rotated[i];
+      ctl.host[i] = r;
+      ctl.active[i] = r;
+      ctl.rotated[i] = 0;
+    }
+    return true;
+  }
   const int v = (int)blockIdx.x - nvisits;
   if (round < 1 || !qflag[((round - 1) & 1) * nvisits + v]) return true;
   __shared__ double Qp[2][kC][kC + 1];  // the previous round's rotation (complex: re, im)
@@ -1767,12 +1791,14 @@ __device__ inline bool round_v_part(const SvdProblem* __restrict__ probs, const 
 // round's visits (Gram, inner sweep, rotation of G), the others apply the
 // previous round's rotations to V (round_v_part).  The rotation and a flag
 // go through qbuf / qflag, double-buffered by round parity; a launch with
-// round == the sweep's round count only flushes the last rotations.
+// round == the sweep's round count only flushes the last rotations and ends
+// the sweep.
 __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
-    const int* __restrict__ vpair, int round, int* __restrict__ rotated,
+    const int* __restrict__ vpair, int round, SweepCtl ctl,
     double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
-  if (round_v_part<kJc, kJw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kJb, 1,
+  int* __restrict__ rotated = ctl.rotated;
+  if (round_v_part<kJc, kJw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kJb, 1, ctl,
                              [](double* V, int N, const int* colp, const double (*Qp)[kJc][kJc + 1], int lane,
                                 int wave) { apply_rotation(V, N, N, colp, Qp[0], lane, wave, kJw); }))
     return;
@@ -1788,6 +1814,10 @@ __global__ __launch_bounds__(64 * kJw) void jacobi_gram_round_kernel(
   const int M = p.M, N = p.N;
   const int nb = (N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
   if (round >= nb2 - 1) return;
+  if (!ctl.active[pi]) {  // converged in an earlier sweep: nothing for V either
+    if (tid == 0) qflag[(round & 1) * nvisits + blockIdx.x] = 0;
+    return;
+  }
   if (tid < kJc) col[tid] = visit_column(round, pr, nb, nb2, N, tid);
   __syncthreads();
   // partial Gram of this wave's slabs: lane (i, kk) feeds X[row][col_{16h+i}]
@@ -2064,13 +2094,14 @@ __device__ inline void zapply_rotation(double* __restrict__ X, int64_t ld, int r
 // applying the previous round's rotations to V (round_v_part).  Rotations and
 // flags go through qbuf (Qr then Qi, 2 kZc^2 doubles per slot) and qflag,
 // double-buffered by round parity; a launch with round == the sweep's round
-// count only flushes the last rotations.
+// count only flushes the last rotations and ends the sweep.
 __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
     const SvdProblem* __restrict__ probs, const int* __restrict__ vprob,
-    const int* __restrict__ vpair, int round, int* __restrict__ rotated,
+    const int* __restrict__ vpair, int round, SweepCtl ctl,
     double* __restrict__ qbuf, int* __restrict__ qflag, int nvisits) {
   constexpr int kB = kZc / 2;
-  if (round_v_part<kZc, kZw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kB, 2,
+  int* __restrict__ rotated = ctl.rotated;
+  if (round_v_part<kZc, kZw>(probs, vprob, vpair, round, qbuf, qflag, nvisits, kB, 2, ctl,
                              [](double* V, int N, const int* colp, const double (*Qp)[kZc][kZc + 1], int lane,
                                 int wave) { zapply_rotation(V, N, N, colp, Qp[0], Qp[1], lane, wave, kZw); }))
     return;
@@ -2089,6 +2120,10 @@ __global__ __launch_bounds__(64 * kZw) void zjacobi_round_kernel(
   const int M = p.M, N = p.N;
   const int nb = (N + kB - 1) / kB, nb2 = nb + (nb & 1);
   if (round >= nb2 - 1) return;
+  if (!ctl.active[pi]) {  // converged in an earlier sweep: nothing for V either
+    if (tid == 0) qflag[(round & 1) * nvisits + blockIdx.x] = 0;
+    return;
+  }
   if (tid < kZc) {
     int bi, bj;
     if (pr == 0) { bi = round; bj = nb2 - 1; }
@@ -4047,6 +4082,86 @@ void fill(double* x, int64_t n, double v, hipStream_t stream) {
 namespace {
 constexpr int kQrMinN = 48;  // smaller problems are not worth the QR launches
 
+// Pinned host memory for the per-sweep flags, one buffer per host thread (a
+// driver call runs on one thread and waits for each of its sweeps).
+int* sweep_host_flags(int np, int** dev) {
+  struct Buf {
+    int* host = nullptr;
+    int* dev = nullptr;
+    int cap = 0;
+  };
+  thread_local Buf b;
+  if (np > b.cap) {
+    if (b.host) DMRG_HIP(hipHostFree(b.host));
+    b.cap = std::max(np, 256);
+    DMRG_HIP(hipHostMalloc((void**)&b.host, (size_t)b.cap * sizeof(int), hipHostMallocDefault));
+    DMRG_HIP(hipHostGetDevicePointer((void**)&b.dev, b.host, 0));
+  }
+  *dev = b.dev;
+  return b.host;
+}
+
+// The sweeps of a batched Jacobi.  One visit map serves the whole call (every
+// problem with N > 1, kb-column blocks, a visit per block pair); a sweep is
+// rounds 0 .. R-1 (R: the most rounds of a still active problem) and a flush
+// launch, whose extra workgroup leaves the per-problem rotation flags in
+// pinned host memory and makes them the next sweep's active set, so the host
+// only waits for the sweep and reads them.  launch(grid, round, ctl, vprob,
+// vpair, qflag, visits) issues one round launch.  Returns the sweeps run;
+// last[i] = the last sweep problem i took part in.
+template <typename Launch>
+int run_sweeps(const std::vector<SvdProblem>& probs, int kb, DevBuf<int>& ctl_buf, hipStream_t stream,
+               int max_sweeps, std::vector<int>& last, int& visits_out, Launch launch) {
+  const int np = (int)probs.size();
+  std::vector<int> active, nrounds(np, 0), map;
+  std::vector<int> pairs;
+  for (int i = 0; i < np; ++i)
+    if (probs[i].N > 1) {
+      active.push_back(i);
+      const int nb = (probs[i].N + kb - 1) / kb, nb2 = nb + (nb & 1);
+      nrounds[i] = nb2 - 1;
+      for (int pr = 0; pr < nb2 / 2; ++pr) {
+        map.push_back(i);
+        pairs.push_back(pr);
+      }
+    }
+  const int visits = (int)pairs.size();
+  visits_out = visits;
+  last.assign(np, 0);
+  if (active.empty()) return 0;
+  // device ints: active[np] | rotated[np] | vprob[visits] | vpair[visits] |
+  // qflag[2 visits]
+  ctl_buf.ensure(2 * (size_t)np + 4 * (size_t)visits);
+  std::vector<int> h(2 * (size_t)np, 0);
+  for (int i : active) h[i] = 1;
+  h.insert(h.end(), map.begin(), map.end());
+  h.insert(h.end(), pairs.begin(), pairs.end());
+  upload_bytes(ctl_buf.data(), h.data(), h.size() * sizeof(int), stream);
+  SweepCtl ctl{ctl_buf.data() + np, ctl_buf.data(), nullptr, np};
+  int* hflags = sweep_host_flags(np, &ctl.host);
+  const int* vprob = ctl_buf.data() + 2 * np;
+  const int* vpair = vprob + visits;
+  int* qflag = ctl_buf.data() + 2 * np + 2 * visits;
+  int sweeps = 0;
+  for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
+    ++sweeps;
+    int rounds = 0;
+    for (int i : active) {
+      rounds = std::max(rounds, nrounds[i]);
+      last[i] = sweep + 1;
+    }
+    for (int r = 0; r < rounds; ++r) launch(dim3(2 * visits), r, ctl, vprob, vpair, qflag, visits);
+    launch(dim3(2 * visits + 1), rounds, ctl, vprob, vpair, qflag, visits);  // flush, sweep end
+    DMRG_HIP(hipGetLastError());
+    DMRG_HIP(hipStreamSynchronize(stream));
+    std::vector<int> still;
+    for (int i : active)
+      if (hflags[i]) still.push_back(i);
+    active.swap(still);
+  }
+  return sweeps;
+}
+
 // One batched blocked-Householder QR over a set of jobs: per panel k the
 // panel kernel, then the trailing update Q_k^H A (qr_update_kernel or its
 // complex twin); the back-transformation C <- Q_1 ... Q_npan C applies the
@@ -4285,54 +4400,25 @@ void jacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& scr
                      fro_part.data());
   hipLaunchKernelGGL(svd_fro2_kernel, dim3(np), dim3(64), 0, stream, scratch.data(), fro_part.data(), init_gy);
   DMRG_HIP(hipGetLastError());
-  std::vector<int> active;
-  int max_visits = 0;
-  for (int i = 0; i < np; ++i)
-    if (probs[i].N > 1) {
-      active.push_back(i);
-      const int nb = (probs[i].N + kJb - 1) / kJb;
-      max_visits += (nb + (nb & 1)) / 2;
-    }
-  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits] |
-  // qflag[2 max_visits]; the rotations of two rounds in qbuf
-  sweeps_out.ensure((size_t)np + 4 * (size_t)max_visits);
-  int* rotated = sweeps_out.data();
-  int* vprob = rotated + np;
-  int* qflag = vprob + 2 * max_visits;
-  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * kJc * kJc);
-  std::vector<int> hrot(np), last(np, 0);
-  int sweeps_done = 0;
-  for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
-    ++sweeps_done;
-    // visit q of a round = (problem, column-block pair) from this map
-    std::vector<int> h(np, 0), pairs;
-    int rounds = 0;
-    for (int i : active) {
-      const int nb = (probs[i].N + kJb - 1) / kJb, nb2 = nb + (nb & 1);
-      for (int pr = 0; pr < nb2 / 2; ++pr) {
-        h.push_back(i);
-        pairs.push_back(pr);
+  // the rotations of two rounds in qbuf
+  DevBuf<double> qbuf;
+  std::vector<int> last;
+  int visits = 0;
+  {
+    int nv = 0;
+    for (const SvdProblem& q : probs)
+      if (q.N > 1) {
+        const int nb = (q.N + kJb - 1) / kJb;
+        nv += (nb + (nb & 1)) / 2;
       }
-      rounds = std::max(rounds, nb2 - 1);
-    }
-    const int visits = (int)pairs.size();
-    int* vpair = vprob + visits;
-    h.insert(h.end(), pairs.begin(), pairs.end());
-    upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
-    // rounds 0 .. rounds-1, then the flush of the last rotations into V
-    for (int r = 0; r <= rounds; ++r)
-      hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(2 * visits), dim3(64 * kJw), 0, stream,
-                         scratch.data(), vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
-    DMRG_HIP(hipGetLastError());
-    DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
-    DMRG_HIP(hipStreamSynchronize(stream));
-    std::vector<int> still;
-    for (int i : active) {
-      last[i] = sweep + 1;
-      if (hrot[i]) still.push_back(i);
-    }
-    active.swap(still);
+    qbuf.ensure((size_t)std::max(2 * nv, 1) * kJc * kJc);
   }
+  const int sweeps_done = run_sweeps(
+      probs, kJb, sweeps_out, stream, max_sweeps, last, visits,
+      [&](dim3 grid, int r, SweepCtl ctl, const int* vprob, const int* vpair, int* qflag, int nv) {
+        hipLaunchKernelGGL(jacobi_gram_round_kernel, grid, dim3(64 * kJw), 0, stream, scratch.data(), vprob,
+                           vpair, r, ctl, qbuf.data(), qflag, nv);
+      });
   if (nq1 > 0) ph3.back(stream);        // V = Q_3 J             (precond 1)
   if (!pq.empty()) {
     ph2.back(stream);                   // X3 = Q_2 W or V = Q_2 J
@@ -4460,52 +4546,25 @@ void zjacobi_svd(const std::vector<SvdProblem>& probs_in, DevBuf<SvdProblem>& sc
   hipLaunchKernelGGL(svd_fro2_kernel, dim3(np), dim3(64), 0, stream, scratch.data(), fro_part.data(), init_gy);
   DMRG_HIP(hipGetLastError());
   constexpr int kB = kZc / 2;
-  std::vector<int> active;
-  int max_visits = 0;
-  for (int i = 0; i < np; ++i)
-    if (probs[i].N > 1) {
-      active.push_back(i);
-      const int nb = (probs[i].N + kB - 1) / kB;
-      max_visits += (nb + (nb & 1)) / 2;
-    }
-  // device ints: rotated[np] | vprob[max_visits] | vpair[max_visits] |
-  // qflag[2 max_visits]; the rotations of two rounds in qbuf
-  sweeps_out.ensure((size_t)np + 4 * (size_t)max_visits);
-  int* rotated = sweeps_out.data();
-  int* vprob = rotated + np;
-  int* qflag = vprob + 2 * max_visits;
-  DevBuf<double> qbuf((size_t)std::max(2 * max_visits, 1) * 2 * kZc * kZc);
-  std::vector<int> hrot(np), last(np, 0);
-  int sweeps_done = 0;
-  for (int sweep = 0; sweep < max_sweeps && !active.empty(); ++sweep) {
-    ++sweeps_done;
-    std::vector<int> h(np, 0), pairs;
-    int rounds = 0;
-    for (int i : active) {
-      last[i] = sweep + 1;
-      const int nb = (probs[i].N + kB - 1) / kB, nb2 = nb + (nb & 1);
-      for (int pr = 0; pr < nb2 / 2; ++pr) {
-        h.push_back(i);
-        pairs.push_back(pr);
+  // the rotations of two rounds in qbuf (Qr then Qi per slot)
+  DevBuf<double> qbuf;
+  std::vector<int> last;
+  int visits = 0;
+  {
+    int nv = 0;
+    for (const SvdProblem& q : probs)
+      if (q.N > 1) {
+        const int nb = (q.N + kB - 1) / kB;
+        nv += (nb + (nb & 1)) / 2;
       }
-      rounds = std::max(rounds, nb2 - 1);
-    }
-    const int visits = (int)pairs.size();
-    int* vpair = vprob + visits;
-    h.insert(h.end(), pairs.begin(), pairs.end());
-    upload_bytes(rotated, h.data(), h.size() * sizeof(int), stream);
-    // rounds 0 .. rounds-1, then the flush of the last rotations into V
-    for (int r = 0; r <= rounds; ++r)
-      hipLaunchKernelGGL(zjacobi_round_kernel, dim3(2 * visits), dim3(64 * kZw), 0, stream, scratch.data(),
-                         vprob, vpair, r, rotated, qbuf.data(), qflag, visits);
-    DMRG_HIP(hipGetLastError());
-    DMRG_HIP(hipMemcpyAsync(hrot.data(), rotated, np * sizeof(int), hipMemcpyDeviceToHost, stream));
-    DMRG_HIP(hipStreamSynchronize(stream));
-    std::vector<int> still;
-    for (int i : active)
-      if (hrot[i]) still.push_back(i);
-    active.swap(still);
+    qbuf.ensure((size_t)std::max(2 * nv, 1) * 2 * kZc * kZc);
   }
+  const int sweeps_done = run_sweeps(
+      probs, kB, sweeps_out, stream, max_sweeps, last, visits,
+      [&](dim3 grid, int r, SweepCtl ctl, const int* vprob, const int* vpair, int* qflag, int nv) {
+        hipLaunchKernelGGL(zjacobi_round_kernel, grid, dim3(64 * kZw), 0, stream, scratch.data(), vprob, vpair,
+                           r, ctl, qbuf.data(), qflag, nv);
+      });
   if (!pq.empty()) {
     ph2.back(stream);  // V = Q_2 J
     if (!sorts.empty()) {  // rows of V back through P
