@@ -150,7 +150,7 @@ def site_lanczos(run, plan, site, niter, dm):
     # launched the next iteration, which it discards.
     def capped(prec, cap):
         best = None
-        for _ in range(4):
+        for _ in range(12):
             v = x0.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
