@@ -1659,9 +1659,12 @@ void RealTensorRundmrg::merge(int leigh, int site) {
   double* sc = cx_.scal.data();
   dot(nk.data.data(), nk.data.data(), nk.doubles(), sc, cx_.partial, cx_.stream);
   scale_inv_sqrt(nk.data.data(), nk.doubles(), sc, cx_.stream, 1);
-  cx_.sync();
+  // no wait here: the replaced ket and canonical tensor stay allocated until
+  // the next sync, for the queued GEMMs that read them
+  cx_.retire(std::move(ket.data));
   ket = std::move(nk);
   nb->has_can = false;
+  cx_.retire(std::move(nb->can.data));
   nb->can = BlockTensor{};
   nb->match_dim.clear();
 }
@@ -1683,27 +1686,38 @@ void RealTensorRundmrg::Initialize() {
 // --------------------------------------------------------------------------
 // Lanczos (RealTensorLanczos::LanczosMethod, real_tensor_lanczos.cc:14-169)
 // --------------------------------------------------------------------------
-double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
-  HeffPlan plan;
+std::unique_ptr<HeffPlan> RealTensorRundmrg::make_plan(int site) {
+  auto plan = std::make_unique<HeffPlan>();
   const double t0 = now();
   const int64_t n = lat_[site].ket.size;
   const bool tune = p_.autotune && n >= p_.tune_min_dim && n >= 4 * tuned_psi_;
-  if (!tune) plan.set_tiling(tune_tile_, tune_split_);
-  plan.set_presplit(p_.precision == kBF16x3);  // the packed operands serve BF16x3 only
-  plan.build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
+  if (!tune) plan->set_tiling(tune_tile_, tune_split_);
+  plan->set_presplit(p_.precision == kBF16x3);  // the packed operands serve BF16x3 only
+  plan->build(lat_[site], ham_.sites()[site], L_[site], R_[site], cx_.stream);
   if (tune) {
     // the plan ends up tiled with the fastest candidate; the trial matvecs
     // run on scratch vectors, the state is untouched
-    plan.tune(p_.precision, 3, cx_.stream);
-    tune_tile_ = plan.tile_mode() == 32 ? 32 : 64;
-    tune_split_ = plan.split_k();
+    plan->tune(p_.precision, 3, cx_.stream);
+    tune_tile_ = plan->tile_mode() == 32 ? 32 : 64;
+    tune_split_ = plan->split_k();
     tuned_psi_ = n;
     ++stats_.tunes;
     stats_.tile_mode = tune_tile_;
     stats_.split_k = tune_split_;
   }
   ph_.plan += now() - t0;
-  return lanczos_solve(plan, lat_[site].ket.data.data(), num_iter, p_.precision, p_.refine, cx_,
+  return plan;
+}
+
+double RealTensorRundmrg::LanczosMethod(int num_iter, int site) {
+  // a plan built ahead for this site by the previous step (same lattice,
+  // operator and environments: nothing changes them in between)
+  std::unique_ptr<HeffPlan> plan;
+  if (next_plan_ && next_plan_site_ == site) plan = std::move(next_plan_);
+  else plan = make_plan(site);
+  next_plan_.reset();
+  next_plan_site_ = -1;
+  return lanczos_solve(*plan, lat_[site].ket.data.data(), num_iter, p_.precision, p_.refine, cx_,
                        stats_);
 }
 
@@ -2451,6 +2465,11 @@ void RealTensorRundmrg::Step(int h, int j) {
   merge(l2r ? 0 : 1, nb);
   ph_.merge += now() - tp;
   if (p_.verbose >= 2) trace(h, nb, energy, "merge");
+  // the next step works on site nb (in this half sweep, or first in the next
+  // one): its plan is built on the host now, while the GPU runs the
+  // environment update and merge queued above
+  next_plan_ = make_plan(nb);
+  next_plan_site_ = nb;
   if ((l2r && j == L_sites_ - 2) || (!l2r && j == 1)) {
     res_.energies.push_back(half_e_);
     res_.trunc_err.push_back(half_terr_);

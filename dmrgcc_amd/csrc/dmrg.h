@@ -98,7 +98,17 @@ struct Context {
   ~Context();
   Context(const Context&) = delete;
   Context& operator=(const Context&) = delete;
-  void sync() { DMRG_HIP(hipStreamSynchronize(stream)); }
+  // waits for the stream, then drops the buffers retired since the last sync
+  void sync() {
+    DMRG_HIP(hipStreamSynchronize(stream));
+    retired.clear();
+  }
+  // keeps a device buffer alive until the next sync(): for buffers that work
+  // already queued on the stream may still read when their owner lets go
+  void retire(DevBuf<double>&& b) {
+    if (b.data()) retired.push_back(std::move(b));
+  }
+  std::vector<DevBuf<double>> retired;
   // pinned host staging of at least n doubles, reused between calls (grown
   // when needed; growing discards the contents)
   double* pinned(int64_t n);
@@ -407,6 +417,9 @@ class RealTensorRundmrg {
   void DefineTensorSpace(int first_block, int first_dim);
   void Initialize();
   double LanczosMethod(int num_iter, int site);
+  // the H_eff plan of `site` as LanczosMethod builds it (tiling choice and
+  // autotune included)
+  std::unique_ptr<HeffPlan> make_plan(int site);
   Lattice& lattice(int i) { return lat_[i]; }
   Env& left_env(int i) { return L_[i]; }
   Env& right_env(int i) { return R_[i]; }
@@ -456,6 +469,10 @@ class RealTensorRundmrg {
   PhaseTimes ph_;
   int64_t tuned_psi_ = 0;                  // psi_dim of the last autotuned plan
   int tune_tile_ = -1, tune_split_ = -1;   // its choice, used by later plans
+  // the next step's plan, built at the end of a step while the GPU still runs
+  // that step's environment update and merge
+  std::unique_ptr<HeffPlan> next_plan_;
+  int next_plan_site_ = -1;
 };
 
 }  // namespace dmrg
