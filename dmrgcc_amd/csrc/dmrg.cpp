@@ -957,7 +957,11 @@ void finish_env(Env& out, std::vector<std::vector<std::vector<EnvTerm>>>& terms,
   if (!tm) {
     s1.launch(cx.stream, kFP64);
     s2.launch(cx.stream, kFP64);
-    cx.sync();
+    // no wait: F and the descriptors stay allocated until the next sync, so
+    // the host goes on while the GPU computes the environment
+    cx.retire(std::move(F));
+    cx.retire(s1);
+    cx.retire(s2);
     return;
   }
   hipEvent_t e[3];
@@ -1665,6 +1669,7 @@ void RealTensorRundmrg::merge(int leigh, int site) {
   ket = std::move(nk);
   nb->has_can = false;
   cx_.retire(std::move(nb->can.data));
+  cx_.retire(gb);
   nb->can = BlockTensor{};
   nb->match_dim.clear();
 }
@@ -2238,7 +2243,11 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   s2.finalize(cx_.stream);
   s1.launch(cx_.stream, kFP64);
   s2.launch(cx_.stream, kFP64);
-  cx_.sync();
+  // no wait: the buffers stay allocated until the next sync (the SVD's)
+  cx_.retire(std::move(F));
+  cx_.retire(std::move(Rd));
+  cx_.retire(s1);
+  cx_.retire(s2);
 }
 
 namespace {

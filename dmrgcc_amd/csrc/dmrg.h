@@ -104,11 +104,17 @@ struct Context {
     retired.clear();
   }
   // keeps a device buffer alive until the next sync(): for buffers that work
-  // already queued on the stream may still read when their owner lets go
-  void retire(DevBuf<double>&& b) {
-    if (b.data()) retired.push_back(std::move(b));
+  // already queued on the stream may still use when their owner lets go
+  template <typename T>
+  void retire(DevBuf<T>&& b) {
+    if (b.data()) retired.push_back(std::make_shared<DevBuf<T>>(std::move(b)));
   }
-  std::vector<DevBuf<double>> retired;
+  void retire(GemmBatch& g) {
+    retire(std::move(g.d_desc));
+    retire(std::move(g.d_splits));
+    retire(std::move(g.part));
+  }
+  std::vector<std::shared_ptr<void>> retired;
   // pinned host staging of at least n doubles, reused between calls (grown
   // when needed; growing discards the contents)
   double* pinned(int64_t n);

@@ -107,13 +107,11 @@ struct GemmBatch {
   std::vector<GemmContrib> contribs;
   std::vector<GemmTile> tiles;
   std::vector<SplitTask> splits;
-  DevBuf<GemmItem> d_items;
-  DevBuf<GemmContrib> d_contribs;
-  DevBuf<GemmTile> d_tiles;
-  DevBuf<SplitTask> d_splits;
+  DevBuf<double> d_desc;       // finalize: items, contribs, tiles, splits in one buffer
+  DevBuf<SplitTask> d_splits;  // merge: the merged batch's split tasks
   DevBuf<double> part;
-  // where launch() reads the descriptors: the buffers above (finalize) or a
-  // caller's arena (place)
+  // where launch() reads the descriptors: d_desc (finalize) or a caller's
+  // arena (place)
   GemmItem* dev_items = nullptr;
   GemmContrib* dev_contribs = nullptr;
   GemmTile* dev_tiles = nullptr;

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <unordered_map>
@@ -3403,14 +3404,24 @@ void GemmBatch::finalize(hipStream_t stream, int tile_m, int tile_n) {
     part.ensure((size_t)psize);
     bind_parts(part.data());
   }
-  upload(d_items, items, stream);
-  upload(d_contribs, contribs, stream);
-  upload(d_tiles, tiles, stream);
-  if (!splits.empty()) upload(d_splits, splits, stream);
-  dev_items = d_items.data();
-  dev_contribs = d_contribs.data();
-  dev_tiles = d_tiles.data();
-  dev_splits = d_splits.data();
+  // the four descriptor arrays in one device buffer and one upload (each
+  // array 256-byte aligned)
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t bi = al(items.size() * sizeof(GemmItem)), bc = al(contribs.size() * sizeof(GemmContrib));
+  const size_t bt = al(tiles.size() * sizeof(GemmTile)), bs = al(splits.size() * sizeof(SplitTask));
+  const size_t total = bi + bc + bt + bs;
+  std::vector<char> img(std::max<size_t>(total, 1));
+  if (!items.empty()) std::memcpy(img.data(), items.data(), items.size() * sizeof(GemmItem));
+  if (!contribs.empty()) std::memcpy(img.data() + bi, contribs.data(), contribs.size() * sizeof(GemmContrib));
+  if (!tiles.empty()) std::memcpy(img.data() + bi + bc, tiles.data(), tiles.size() * sizeof(GemmTile));
+  if (!splits.empty()) std::memcpy(img.data() + bi + bc + bt, splits.data(), splits.size() * sizeof(SplitTask));
+  d_desc.ensure(std::max<size_t>(total, 8) / sizeof(double));
+  if (total) upload_bytes(d_desc.data(), img.data(), total, stream);
+  char* base = reinterpret_cast<char*>(d_desc.data());
+  dev_items = reinterpret_cast<GemmItem*>(base);
+  dev_contribs = reinterpret_cast<GemmContrib*>(base + bi);
+  dev_tiles = reinterpret_cast<GemmTile*>(base + bi + bc);
+  dev_splits = reinterpret_cast<SplitTask*>(base + bi + bc + bt);
 }
 
 void GemmBatch::bind_parts(double* base) {
