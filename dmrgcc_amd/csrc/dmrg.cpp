@@ -7,6 +7,7 @@
 #include "dmrg.h"
 
 #include <algorithm>
+#include <future>
 #include <unordered_map>
 #include <chrono>
 #include <cmath>
@@ -1591,6 +1592,7 @@ double canonicalize(Lattice& lat, int leigh, int max_dim, double canpre, Context
   }
   launch_copies(cps, cx_.copy_scratch, cx_.stream);
   launch_col_gather(gts, cx_.gather_scratch, cx_.stream);
+  cx_.retire(std::move(ket.data));  // read by the copies just queued
   ket = std::move(nk);
   lat.can = std::move(can);
   lat.has_can = true;
@@ -1968,7 +1970,7 @@ std::vector<std::vector<int>> RealTensorRundmrg::expan_lattice(int leigh, int si
 }
 
 void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
-                                          const std::vector<std::vector<int>>& mapping) {
+                                          const std::vector<std::vector<int>>& mapping, ExpanWork* defer) {
   // leigh 0: LeftExpanTensorContraction (real_tensor_contraction.cc:169-459)
   // leigh 1: RightExpanTensorContraction (real_tensor_contraction.cc:477-765)
   const Lattice& lat = lat_[site];
@@ -2241,6 +2243,14 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   }
   s1.finalize(cx_.stream);
   s2.finalize(cx_.stream);
+  if (defer) {
+    defer->s1 = std::move(s1);
+    defer->s2 = std::move(s2);
+    defer->F = std::move(F);
+    defer->Rd = std::move(Rd);
+    defer->pending = true;
+    return;
+  }
   s1.launch(cx_.stream, kFP64);
   s2.launch(cx_.stream, kFP64);
   // no wait: the buffers stay allocated until the next sync (the SVD's)
@@ -2248,6 +2258,17 @@ void RealTensorRundmrg::expan_contraction(int leigh, int site, Lattice& E,
   cx_.retire(std::move(Rd));
   cx_.retire(s1);
   cx_.retire(s2);
+}
+
+void RealTensorRundmrg::launch_expansion(ExpanWork& w) {
+  if (!w.pending) return;
+  w.s1.launch(cx_.stream, kFP64);
+  w.s2.launch(cx_.stream, kFP64);
+  cx_.retire(std::move(w.F));
+  cx_.retire(std::move(w.Rd));
+  cx_.retire(w.s1);
+  cx_.retire(w.s2);
+  w.pending = false;
 }
 
 namespace {
@@ -2446,17 +2467,28 @@ void RealTensorRundmrg::Step(int h, int j) {
     half_t0_ = now();
   }
   double tp = now();
+  // The expansion's host work (sector bookkeeping, term lists, the random
+  // draws, the GEMM descriptors) reads only the block structure of the site,
+  // its environment and the rand() state, none of which the Lanczos solve
+  // touches: it runs on a worker thread while the Lanczos iterations run, and
+  // only the two GEMMs that read the new ket are launched afterwards.  The
+  // rand() draws keep their order (the solve draws none).
+  ExpanWork xw;
+  auto prep = std::async(std::launch::async, [&] {
+    xw.mapping = l2r ? expan_lattice(1, j, ham_.bond_tables()[j + 1], xw.E)
+                     : expan_lattice(0, j, ham_.bond_tables()[j], xw.E);
+    expan_contraction(l2r ? 0 : 1, j, xw.E, xw.mapping, &xw);
+  });
   const double energy = LanczosMethod(p_.niter[h], j);
   half_e_ = energy;
   ph_.lanczos += now() - tp; tp = now();
   if (p_.verbose >= 2) trace(h, j, energy, "lanczos");
+  prep.get();
   // ResetTensorNetwork keeps the block list but empties the matrices; the
   // reset environment is not read again before it is recomputed.
   for (auto& b : (l2r ? R_[j] : L_[j])) b.define(std::vector<int>(), std::vector<int>());
-  Lattice E;
-  auto mapping = l2r ? expan_lattice(1, j, ham_.bond_tables()[j + 1], E)
-                     : expan_lattice(0, j, ham_.bond_tables()[j], E);
-  expan_contraction(l2r ? 0 : 1, j, E, mapping);
+  Lattice& E = xw.E;
+  launch_expansion(xw);
   ph_.expan += now() - tp; tp = now();
   if (p_.verbose >= 2) trace(h, j, energy, "expan_contraction");
   expand(l2r ? 0 : 1, j, E);

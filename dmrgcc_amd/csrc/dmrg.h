@@ -447,7 +447,21 @@ class RealTensorRundmrg {
   void merge(int leigh, int site);
   std::vector<std::vector<int>> expan_lattice(int leigh, int site, const QNTable& op_table,
                                               Lattice& E);
-  void expan_contraction(int leigh, int site, Lattice& E, const std::vector<std::vector<int>>& mapping);
+  // The expansion's GEMMs and buffers when their launch is deferred (the
+  // host work done ahead, while the Lanczos iterations run)
+  struct ExpanWork {
+    Lattice E;
+    std::vector<std::vector<int>> mapping;
+    GemmBatch s1, s2;
+    DevBuf<double> F, Rd;
+    bool pending = false;
+  };
+  // defer: build everything and queue what does not read the ket (the zero
+  // fill of E, the random matrices, the descriptors) but leave the two
+  // GEMMs to launch_expansion
+  void expan_contraction(int leigh, int site, Lattice& E, const std::vector<std::vector<int>>& mapping,
+                         ExpanWork* defer = nullptr);
+  void launch_expansion(ExpanWork& w);
   void expand(int leigh, int site, Lattice& E);
   void match(int leigh, int site);
   void trace(int h, int j, double e, const char* what);
