@@ -2334,14 +2334,14 @@ struct QrJob {
 // Factorise panel k of every problem (one workgroup each).  kLds: the panel is
 // staged in LDS (dynamic, m*b + b*b + b doubles); otherwise it is worked on in
 // place in global memory and LDS holds only T and tau (b*b + b doubles).
-// kU > 0 (b = 32, m <= 64 kU, LDS sized for 64 kU rows): the column loop keeps the panel in
+// kU > 0 (b = kB, m <= 64 kU, LDS sized for 64 kU rows): the column loop keeps the panel in
 // registers instead, wave w of kWv holding columns w, w + kWv, ... and lane
 // l rows l, l + 64, ...; a column is written to LDS once, by its wave, when
 // its last update is done, and each reflector value read from there serves
-// the 32 / kWv columns of a lane.  A column step is bound by the dependent
+// the kB / kWv columns of a lane.  A column step is bound by the dependent
 // FP64 chain of one wave (reductions, square root, divisions), not by the
 // barrier: kU = 9 panels take 71 us with 16 waves against 75 us with 8.
-template <bool kLds, int kU = 0, int kWv = 8>
+template <bool kLds, int kU = 0, int kWv = 8, int kB = 32>
 __global__ __launch_bounds__(64 * kWv) void qr_panel_kernel(const QrJob* __restrict__ probs, int k,
                                                        int b) {
   extern __shared__ double smem[];
@@ -2397,9 +2397,10 @@ __global__ __launch_bounds__(64 * kWv) void qr_panel_kernel(const QrJob* __restr
     // wave w holds columns w, w + kWv, ... of the panel, lane l rows l + 64 u
     // (u < kU): a reflector value read from LDS serves kQ columns.
     // The LDS columns have 64 kU rows (ld), zero past m, so only register 0
-    // (rows 0 .. 63, the ones that can be <= j < 32) needs a row test.
+    // (rows 0 .. 63, the ones that can be <= j < kB) needs a row test.
     constexpr int nwv = kWv;
-    constexpr int kQ = 32 / kWv;  // columns per wave
+    static_assert(kB % kWv == 0 && kB <= 64, "whole columns per wave, panel rows j < 64 in register 0");
+    constexpr int kQ = kB / kWv;  // columns per wave
     double y[kQ][kU];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -2474,7 +2475,7 @@ __global__ __launch_bounds__(64 * kWv) void qr_panel_kernel(const QrJob* __restr
         const int c = wave + nwv * q;
         if (c <= j || c >= bw) continue;  // uniform over the wave
         // d = tau (y_j + scal x^T y); y -= d v, v = (1, scal x[j+1:]); row
-        // j < 32 sits in lane j, register 0
+        // j < kB sits in lane j, register 0
         if (tj != 0.0) {
           const double yj = readlane_d(y[q][0], j);
           const double d = (yj + scal * dot[q]) * tj;
@@ -4208,11 +4209,13 @@ struct QrPhase {
         if (need <= budget) { b = bb; lds = true; lds_bytes = need; break; }
       }
       if (!lds) { b = 32; lds_bytes = ((size_t)b * b + b) * sizeof(double); }
-      // register column loop (b = 32, panel columns padded to 64 kU rows)
-      if (lds && b == 32 && maxM <= 64 * 9) {
-        reg_u = maxM <= 64 * 4 ? 4 : 9;
-        lds_bytes = ((size_t)64 * reg_u * b + (size_t)b * b + b) * sizeof(double);
-      }
+      // register column loop (panel columns padded to 64 kU rows): 32-column
+      // panels up to 576 rows, 16-column panels up to 1024 rows (16 waves, a
+      // column each), 8-column panels up to 2048 rows (8 waves)
+      if (lds && b == 32 && maxM <= 64 * 9) reg_u = maxM <= 64 * 4 ? 4 : 9;
+      else if (lds && b == 16 && maxM <= 64 * 16) reg_u = 16;
+      else if (lds && b == 8 && maxM <= 64 * 32) reg_u = 32;
+      if (reg_u) lds_bytes = ((size_t)64 * reg_u * b + (size_t)b * b + b) * sizeof(double);
     }
     npan = (maxN + b - 1) / b;
     int64_t total = 0;
@@ -4259,6 +4262,12 @@ struct QrPhase {
       else if (reg_u == 9)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 9, 16>), dim3(nj), dim3(1024), lds_bytes, stream,
                            d_jobs.data(), k, b);
+      else if (reg_u == 16)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 16, 16, 16>), dim3(nj), dim3(1024), lds_bytes,
+                           stream, d_jobs.data(), k, b);
+      else if (reg_u == 32)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(qr_panel_kernel<true, 32, 8, 8>), dim3(nj), dim3(512), lds_bytes,
+                           stream, d_jobs.data(), k, b);
       else if (lds)
         hipLaunchKernelGGL(qr_panel_kernel<true>, dim3(nj), dim3(512), lds_bytes, stream,
                            d_jobs.data(), k, b);

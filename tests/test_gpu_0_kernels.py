@@ -75,11 +75,14 @@ def test_sector_svd(shape):
 
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("shapes", [[(120, 90)], [(200, 150), (150, 200), (64, 64), (90, 20)],
-                                    [(600, 560)], [(700, 650), (650, 700)], [(2100, 100)],
+                                    [(600, 560)], [(700, 650), (650, 700)], [(1024, 300)],
+                                    [(1100, 90)], [(1800, 150)], [(2048, 64)], [(2100, 100)],
                                     [(3000, 60)]])
 def test_sector_svd_preconditioned(shapes, mode):
-    # QR-preconditioned path: LDS panels of 32 / 16 / 8 columns and the global
-    # panel (M too large for LDS); mixed with problems below the QR size
+    # QR-preconditioned path: register panels of 32 columns (up to 576 rows),
+    # 16 columns (up to 1024 rows) and 8 columns (up to 2048 rows), LDS panels
+    # of 16 / 8 columns beyond those, and the global panel (M too large for
+    # LDS); mixed with problems below the QR size
     rng = np.random.default_rng(11)
     mats, spec = [], []
     for (M, N) in shapes:
