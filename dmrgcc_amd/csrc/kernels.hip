@@ -3485,6 +3485,7 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
     std::vector<GemmItem> extra;
     struct Pending { size_t item; int first_extra, n; };
     std::vector<Pending> pend;
+    std::vector<int> cuts;
     const size_t nitems = items.size();
     for (size_t i = 0; i < nitems; ++i) {
       GemmItem& it = items[i];
@@ -3494,7 +3495,7 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
       const bool many = presplit && it.c1 - it.c0 > kPreTerms;
       if (!(heavy || many) || it.c1 - it.c0 < 2) continue;
       // chunk boundaries by accumulated K (and term count)
-      std::vector<int> cuts{it.c0};
+      cuts.assign(1, it.c0);
       int64_t acc = 0;
       for (int c = it.c0; c < it.c1; ++c) {
         const bool full = (split_k > 0 && acc >= split_k) || (presplit && c - cuts.back() >= kPreTerms);
@@ -3523,6 +3524,8 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
       // chunk outputs hold offsets into the partial buffer until
       // bind_parts() gives it an address
       const size_t base = items.size();
+      items.reserve(base + extra.size());
+      part_items.reserve(part_items.size() + extra.size());
       for (GemmItem& e : extra) {
         part_items.push_back((int)items.size());
         items.push_back(e);
@@ -3559,39 +3562,85 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   // to one XCD queue; groups (and unkeyed items) to the least-loaded queue,
   // costliest first.
   const bool keyed = tile_m == tile_n && std::any_of(key.begin(), key.end(), [](int k) { return k >= 0; });
-  auto place_keyed = [&](const std::vector<int>& order, int ts, std::vector<std::vector<GemmTile>>& q) {
-    std::unordered_map<int, int> kg;
-    std::vector<std::vector<int>> groups;
-    std::vector<double> gcost;
-    for (int i : order) {
-      const GemmItem& it = items[i];
-      if (it.M <= 0 || it.N <= 0) continue;
-      if (it.c0 == it.c1 && it.accumulate) continue;
-      int g = (int)groups.size();
-      if (key[i] >= 0) {
-        auto f = kg.find(key[i]);
-        if (f != kg.end()) g = f->second;
-        else kg[key[i]] = g;
-      }
-      if (g == (int)groups.size()) {
-        groups.emplace_back();
-        gcost.push_back(0.0);
-      }
-      groups[g].push_back(i);
-      gcost[g] += (cost[i] + 32.0) * (double)(((it.M + ts - 1) / ts) * ((it.N + ts - 1) / ts));
-    }
-    std::vector<int> gorder(groups.size());
-    std::iota(gorder.begin(), gorder.end(), 0);
-    std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) { return gcost[a] > gcost[b]; });
+  // costliest first, ties in item order (the order a stable sort gives)
+  auto by_cost = [&](int a, int b) { return cost[a] > cost[b] || (cost[a] == cost[b] && a < b); };
+  auto live = [&](const GemmItem& it) { return it.M > 0 && it.N > 0 && !(it.c0 == it.c1 && it.accumulate); };
+  auto ntiles = [](const GemmItem& it, int ts) { return ((it.M + ts - 1) / ts) * ((it.N + ts - 1) / ts); };
+  // The items of one launch group in queue order: qx[n] is the XCD queue of
+  // the n-th item of `seq`; the tiles are then laid out queue by queue from
+  // counts, without growing vectors.
+  std::vector<int> seq, qx, qoff(kXcds + 1), qfill(kXcds);
+  std::vector<GemmTile> qt;
+  auto assign = [&](const std::vector<int>& order, int ts) {
+    seq.clear();
+    qx.clear();
     std::vector<double> load(kXcds, 0.0);
-    for (int g : gorder) {
-      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      for (int i : groups[g])
-        for (int m0 = 0; m0 < items[i].M; m0 += ts)
-          for (int n0 = 0; n0 < items[i].N; n0 += ts) q[x].push_back(GemmTile{i, m0, n0, 0});
-      load[x] += gcost[g];
+    auto least = [&] { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
+    if (keyed) {
+      // items of one key form a group whose tiles all go to one queue; groups
+      // (and unkeyed items) to the least-loaded queue, costliest first
+      std::unordered_map<int, int> kg;
+      std::vector<int> gfirst, gnext, glast;
+      std::vector<double> gcost;
+      std::vector<int> nextof(items.size(), -1);
+      for (int i : order) {
+        const GemmItem& it = items[i];
+        if (!live(it)) continue;
+        int g = (int)gfirst.size();
+        if (key[i] >= 0) {
+          auto f = kg.find(key[i]);
+          if (f != kg.end()) g = f->second;
+          else kg[key[i]] = g;
+        }
+        if (g == (int)gfirst.size()) {
+          gfirst.push_back(i);
+          glast.push_back(i);
+          gcost.push_back(0.0);
+        } else {
+          nextof[glast[g]] = i;
+          glast[g] = i;
+        }
+        gcost[g] += (cost[i] + 32.0) * (double)ntiles(it, ts);
+      }
+      std::vector<int> gorder(gfirst.size());
+      std::iota(gorder.begin(), gorder.end(), 0);
+      std::sort(gorder.begin(), gorder.end(),
+                [&](int x, int y) { return gcost[x] > gcost[y] || (gcost[x] == gcost[y] && x < y); });
+      for (int g : gorder) {
+        const int x = least();
+        for (int i = gfirst[g]; i >= 0; i = nextof[i]) {
+          seq.push_back(i);
+          qx.push_back(x);
+        }
+        load[x] += gcost[g];
+      }
+    } else {
+      for (int i : order) {
+        const GemmItem& it = items[i];
+        if (!live(it)) continue;
+        const int x = least();
+        seq.push_back(i);
+        qx.push_back(x);
+        load[x] += (cost[i] + 32.0) * ntiles(it, ts);
+      }
+    }
+    // tiles per queue, then each queue's tiles in item order
+    std::fill(qoff.begin(), qoff.end(), 0);
+    for (size_t n = 0; n < seq.size(); ++n) qoff[qx[n] + 1] += ntiles(items[seq[n]], ts);
+    for (int x = 0; x < kXcds; ++x) qoff[x + 1] += qoff[x];
+    qt.resize((size_t)qoff[kXcds]);
+    for (int x = 0; x < kXcds; ++x) qfill[x] = qoff[x];
+    for (size_t n = 0; n < seq.size(); ++n) {
+      const GemmItem& it = items[seq[n]];
+      GemmTile* d = qt.data() + qfill[qx[n]];
+      int c = 0;
+      for (int m0 = 0; m0 < it.M; m0 += ts)
+        for (int n0 = 0; n0 < it.N; n0 += ts) d[c++] = GemmTile{seq[n], m0, n0, 0};
+      qfill[qx[n]] += c;
     }
   };
+  auto qsize = [&](int x) { return (size_t)(qoff[x + 1] - qoff[x]); };
+  auto qat = [&](int x, size_t k) { return qt[(size_t)qoff[x] + k]; };
   std::vector<int> ends = group_items;
   if (ends.empty()) ends.push_back((int)items.size());
   int first = 0;
@@ -3600,29 +3649,15 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
     // workgroup); workgroup w runs on XCD w % kXcds
     std::vector<int> order(items.size());
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<std::vector<GemmTile>> q(kXcds);
-    std::vector<double> load(kXcds, 0.0);
-    if (keyed) place_keyed(order, kWT, q);
-    for (int i : keyed ? std::vector<int>{} : order) {
-      const GemmItem& it = items[i];
-      if (it.M <= 0 || it.N <= 0) continue;
-      if (it.c0 == it.c1 && it.accumulate) continue;
-      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      int nt = 0;
-      for (int m0 = 0; m0 < it.M; m0 += kWT)
-        for (int n0 = 0; n0 < it.N; n0 += kWT, ++nt) q[x].push_back(GemmTile{i, m0, n0, 0});
-      load[x] += (cost[i] + 32.0) * nt;
-    }
+    std::sort(order.begin(), order.end(), by_cost);
+    assign(order, kWT);
     size_t depth = 0;
-    for (auto& v : q) {
-      while (v.size() % 4) v.push_back(GemmTile{-1, 0, 0, 0});
-      depth = std::max(depth, v.size() / 4);
-    }
+    for (int x = 0; x < kXcds; ++x) depth = std::max(depth, (qsize(x) + 3) / 4);
+    tiles.resize(depth * kXcds * 4);
+    size_t o = 0;
     for (size_t k = 0; k < depth; ++k)
       for (int x = 0; x < kXcds; ++x)
-        for (int w = 0; w < 4; ++w)
-          tiles.push_back(4 * k + w < q[x].size() ? q[x][4 * k + w] : GemmTile{-1, 0, 0, 0});
+        for (int w = 0; w < 4; ++w) tiles[o++] = 4 * k + w < qsize(x) ? qat(x, 4 * k + w) : GemmTile{-1, 0, 0, 0};
     while (tiles.size() >= 4 && tiles[tiles.size() - 1].item < 0 && tiles[tiles.size() - 2].item < 0 &&
            tiles[tiles.size() - 3].item < 0 && tiles[tiles.size() - 4].item < 0)
       tiles.resize(tiles.size() - 4);
@@ -3631,26 +3666,17 @@ int64_t GemmBatch::layout(int tile_m, int tile_n) {
   }
   for (int end : ends) {
     std::vector<int> order;
+    order.reserve((size_t)std::max(end - first, 0));
     for (int i = first; i < end; ++i) order.push_back(i);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<std::vector<GemmTile>> q(kXcds);
-    std::vector<double> load(kXcds, 0.0);
-    if (keyed) place_keyed(order, tile_m, q);
-    for (int i : keyed ? std::vector<int>{} : order) {
-      const GemmItem& it = items[i];
-      if (it.M <= 0 || it.N <= 0) continue;
-      if (it.c0 == it.c1 && it.accumulate) continue;  // nothing to add
-      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      int nt = 0;
-      for (int m0 = 0; m0 < it.M; m0 += tile_m)
-        for (int n0 = 0; n0 < it.N; n0 += tile_n, ++nt) q[x].push_back(GemmTile{i, m0, n0, 0});
-      load[x] += (cost[i] + 32.0) * nt;
-    }
+    std::sort(order.begin(), order.end(), by_cost);
+    assign(order, tile_m);
     size_t depth = 0;
-    for (const auto& v : q) depth = std::max(depth, v.size());
+    for (int x = 0; x < kXcds; ++x) depth = std::max(depth, qsize(x));
+    const size_t t0 = tiles.size();
+    tiles.resize(t0 + depth * kXcds);
+    size_t o = t0;
     for (size_t k = 0; k < depth; ++k)
-      for (int x = 0; x < kXcds; ++x)
-        tiles.push_back(k < q[x].size() ? q[x][k] : GemmTile{-1, 0, 0, 0});
+      for (int x = 0; x < kXcds; ++x) tiles[o++] = k < qsize(x) ? qat(x, k) : GemmTile{-1, 0, 0, 0};
     // trailing empty tiles of the last row are not launched
     while (!tiles.empty() && tiles.back().item < 0 && (int)tiles.size() > (group_tiles.empty() ? 0 : group_tiles.back()))
       tiles.pop_back();
