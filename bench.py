@@ -142,15 +142,15 @@ def site_lanczos(run, plan, site, niter, dm):
     # mixed forms' FP64 refinement matvec included) per Lanczos iteration
     per_it = {k: ms[k] / it[k] for k in ms}
     # marginal cost of one iteration: the same solve capped at it and at
-    # it - 1 iterations (a solve that ends at its cap launches nothing
-    # beyond it), timed the same way; the difference holds iteration it's
-    # matvec, its coefficients and its it - 1 Gram-Schmidt projections, and
-    # none of the per-solve costs (start copy, Ritz vector, refinement, host
+    # it - 1 iterations, best of 40 solves each (a solve that ends at its cap
+    # launches nothing beyond it); the difference holds iteration it's
+    # matvec, its coefficients and the Gram-Schmidt of its vector against the
+    # it - 1 before it, and none of the per-solve costs (start copy, Ritz vector, refinement, host
     # return).  A solve that stops by its own test, as in a sweep, has also
     # launched the next iteration, which it discards.
     def capped(prec, cap):
         best = None
-        for _ in range(12):
+        for _ in range(40):
             v = x0.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()

@@ -1806,46 +1806,55 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
   std::vector<double> alpha, beta;
   double old_energy = 10000.0, energy = 0.0;
   // partial-sum buffers of the fused vector steps: part_a holds the alpha
-  // partials, part_b the beta^2 partials, and the Gram-Schmidt chain
-  // alternates between part_c and part_a
-  cx.partial.ensure(3 * 2 * kDotPartials);
+  // partials, part_b the beta^2 partials, part_m the overlaps of h with the
+  // basis for the next vector's Gram-Schmidt
+  cx.partial.ensure(2 * 2 * kDotPartials + (size_t)(2 * num_iter) * kDotPartials);
   double* part_a = cx.partial.data();
   double* part_b = part_a + 2 * kDotPartials;
-  double* part_c = part_b + 2 * kDotPartials;
-  std::vector<hipEvent_t> done = cx.events(num_iter);
+  double* part_m = part_b + 2 * kDotPartials;
+  // Slot i's ready flag (host[kSlot i + 3]) is set by the launch that
+  // publishes iteration i's coefficients, after them; the host polls it
+  // instead of recording an event in the stream (an event between two
+  // launches idles the GPU for several microseconds).
+  for (int i = 0; i < num_iter; ++i) host[kSlot * i + 3] = 0.0;
   // psi_{i+1} = h / beta_i, h = -beta_i psi_i, then GramSchmidtMethod
   // (real_tensor_lanczos.cc:140-151, :188-201; complex_tensor_lanczos.cc:
-  // 178-192): for j = 0..i, c = <psi_j|psi_{i+1}> on the updated vector,
-  // psi_{i+1} -= c psi_j; then psi_{i+1} /= |psi_{i+1}|.  One launch per
-  // projection: each forms c from the previous launch's partial sums and the
-  // partials of the next overlap.  beta_i is formed on the device.  With
-  // `publish`, the first launch also forms alpha_i and beta_i^2 from
-  // iteration i's partials and writes them to slot i and to pinned host
-  // memory, and done[i] is recorded right after it: iteration i needs no
-  // launch of its own for its coefficients, and no copy.
+  // 178-192), which projects out psi_0 .. psi_i and normalises: one launch,
+  // lz_next_all, from the overlaps <psi_j|h> that iteration i's
+  // lz_alpha_multi formed (c_j = <psi_j|h> / beta_i, r^2 = 1 - sum |c_j|^2).
+  // For the orthonormal basis this is the reference's vector: its sequential
+  // coefficients differ by terms of second order in the overlaps, which the
+  // recurrence leaves at the rounding level.  beta_i is formed on the device.
+  // With `publish`, the launch also forms alpha_i and beta_i^2 from iteration
+  // i's partials and writes them to slot i and to pinned host memory with the
+  // ready flag: iteration i needs no launch of its own for its coefficients,
+  // and no copy.
   auto next_vector = [&](int i, bool publish) {
     double* pn = P + (int64_t)(i + 1) * nd;
-    double* pin = part_c;
-    double* pout = part_a;
-    if (publish) {
-      lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, nullptr, part_a, part_b,
-              sc + kSlot * i, host_dev + kSlot * i);
-      DMRG_HIP(hipEventRecord(done[i], cx.stream));
-    } else {
-      lz_next(pn, h.data(), P + (int64_t)i * nd, P, n, cz, 0.0, pin, cx.stream, sc + kSlot * i + 2);
-    }
-    for (int j = 0; j <= i; ++j) {
-      const double* xnext = j < i ? P + (int64_t)(j + 1) * nd : nullptr;
-      lz_project(pn, P + (int64_t)j * nd, xnext, n, cz, pin, pout, cx.stream);
-      std::swap(pin, pout);
-    }
-    lz_normalize(pn, n, cz, pin, cx.stream);
+    lz_next_all(pn, h.data(), P + (int64_t)i * nd, P, nd, i + 1, n, cz, part_m, cx.stream, nullptr, part_a, part_b,
+                publish ? sc + kSlot * i : nullptr, publish ? host_dev + kSlot * i : nullptr);
   };
   // the coefficients of iteration i by a launch of their own: (alpha,
-  // beta^2) to slot i and to pinned host memory, then done[i]
+  // beta^2) to slot i and to pinned host memory, with the ready flag
   auto publish_alone = [&](int i) {
     lz_finals(part_a, part_b, n, cz, sc + kSlot * i, cx.stream, host_dev + kSlot * i);
-    DMRG_HIP(hipEventRecord(done[i], cx.stream));
+  };
+  // wait for slot i's flag; a stream that fails or drains without setting it
+  // ends the wait with an error
+  auto wait_published = [&](int i) {
+    const volatile double* flag = host + kSlot * i + 3;
+    for (uint64_t spin = 1; *flag != 1.0; ++spin) {
+      if ((spin & 0xfff) == 0) {
+        const hipError_t q = hipStreamQuery(cx.stream);
+        if (q == hipSuccess) {
+          require(*flag == 1.0, "lanczos: the stream drained without publishing the coefficients",
+                  kErrInternal);
+          break;
+        }
+        if (q != hipErrorNotReady) DMRG_HIP(q);
+      }
+      __builtin_ia32_pause();
+    }
   };
   // iteration i on the stream: psi_i (when i > 0, publishing iteration i-1's
   // coefficients unless `blocking`), h += H psi_i, alpha = <psi_i|h>,
@@ -1856,7 +1865,8 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
     double* pi = P + (int64_t)i * nd;
     plan.apply(pi, h.data(), precision, cx.stream);
     lz_dot_partial(pi, h.data(), n, cz, part_a, cx.stream);   // alpha = <psi|H|psi>
-    lz_alpha(h.data(), pi, n, cz, part_a, part_b, cx.stream);  // h -= alpha psi; <h|h>
+    // h -= alpha psi; partials of <h|h> and of <psi_j|h>, j <= i
+    lz_alpha_multi(h.data(), pi, P, nd, i + 1, n, cz, part_a, part_b, part_m, cx.stream);
     if (blocking) publish_alone(i);
   };
   blocking = std::getenv("DMRGCC_LANCZOS_BLOCKING") != nullptr;
@@ -1868,7 +1878,7 @@ double lanczos_solve(HeffPlan& plan, double* ket, int num_iter, int precision, i
       if (k < num_iter) launch_iteration(k);  // publishes iteration i first
       else publish_alone(i);
     }
-    DMRG_HIP(hipEventSynchronize(done[i]));
+    wait_published(i);
     st.matvec_seconds += now() - t0;
     st.flops += plan.fref();
     ++st.matvecs;

@@ -266,28 +266,27 @@ int dot_grid(int64_t n);
 // part <- partials of <x|y>
 void lz_dot_partial(const double* x, const double* y, int64_t n, bool cplx, double* part,
                     hipStream_t stream);
-// alpha = final(part_in); h -= alpha psi; part_out <- partials of <h|h>
-void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* part_in,
-              double* part_out, hipStream_t stream);
 // out[0..1] = final(part_a) (alpha; real: out[1] = 0), out[2] = Re final(part_b) (beta^2);
-// out_host (device-visible pinned memory, optional) receives the same three
+// out_host (device-visible pinned memory, optional) receives the same three,
+// then out_host[3] = 1 (the ready flag the host polls)
 void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
                hipStream_t stream, double* out_host = nullptr);
-// nxt = h / beta; h = -beta psi; part_out <- partials of <x0|nxt>.  With
-// beta2 (device), beta = sqrt(*beta2) is formed on the device instead.  With
-// part_a / part_b (the alpha and beta^2 partials of the step that produced
-// h), the kernel forms beta itself and also does lz_finals' work: slot[0..2]
-// and slot_host[0..2] (device-visible pinned memory) receive (alpha, beta^2);
-// part_out must then be a third buffer.
-void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream, const double* beta2 = nullptr,
-             const double* part_a = nullptr, const double* part_b = nullptr, double* slot = nullptr,
-             double* slot_host = nullptr);
-// c = final(part_in); y -= c xj; part_out <- partials of <xnext|y> (xnext null: <y|y>)
-void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
-                const double* part_in, double* part_out, hipStream_t stream);
-// y /= sqrt(Re final(part_in))
-void lz_normalize(double* y, int64_t n, bool cplx, const double* part_in, hipStream_t stream);
+// The Lanczos steps after the alpha dot (basis x_j = x + j ld, j < nj, each
+// of n entries; partial buffers of dot_grid(n) partials per value; part_m
+// holds (cplx ? 2 : 1) nj groups).  lz_alpha_multi: alpha = final(part_a),
+// h -= alpha psi, part_b <- partials of |h|^2, part_m <- partials of every
+// <x_j|h>.  lz_next_all: beta from part_b (or sqrt(*beta2) on the device when
+// part_b is null), c_j = <x_j|h> / beta, nxt = (h / beta - sum_j c_j x_j) / r
+// with r^2 = 1 - sum_j |c_j|^2, h = -beta psi.  With slot, block 0 also forms
+// alpha from part_a and writes (alpha, beta^2) to slot[0..2] and slot_host[0..2]
+// (device-visible pinned memory), then slot_host[3] = 1 (the ready flag the
+// host polls).
+void lz_alpha_multi(double* h, const double* psi, const double* x, int64_t ld, int nj, int64_t n, bool cplx,
+                    const double* part_a, double* part_b, double* part_m, hipStream_t stream);
+void lz_next_all(double* nxt, double* h, const double* psi, const double* x, int64_t ld, int nj, int64_t n,
+                 bool cplx, const double* part_m, hipStream_t stream, const double* beta2 = nullptr,
+                 const double* part_a = nullptr, const double* part_b = nullptr, double* slot = nullptr,
+                 double* slot_host = nullptr);
 
 // Column gather for the SVD scatter: for i < rows, jj < ncol:
 //   dst[i*dsr + jj*dsc] = scale[jj] * src[(r0 + i) + perm[jj]*ld]

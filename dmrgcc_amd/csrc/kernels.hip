@@ -1348,33 +1348,6 @@ __global__ __launch_bounds__(256) void lz_dot_partial_kernel(const double* __res
 }
 
 template <bool kC>
-__global__ __launch_bounds__(256) void lz_alpha_kernel(double* __restrict__ h,
-                                                       const double* __restrict__ psi, int64_t n,
-                                                       const double* __restrict__ part_in, int np,
-                                                       double* __restrict__ part_out) {
-  double cr, ci;
-  lz_final<kC>(part_in, np, cr, ci);
-  const double ar = -1.0 * cr, ai = -1.0 * ci;
-  double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (kC) {
-      const double xr = psi[2 * i], xi = psi[2 * i + 1];
-      const double yr = h[2 * i] + (ar * xr - ai * xi);
-      const double yi = h[2 * i + 1] + (ar * xi + ai * xr);
-      h[2 * i] = yr;
-      h[2 * i + 1] = yi;
-      s += yr * yr + yi * yi;
-    } else {
-      const double y = h[i] + ar * psi[i];
-      h[i] = y;
-      s += y * y;
-    }
-  }
-  lz_block_store<false>(s, 0.0, part_out);
-}
-
-template <bool kC>
 __global__ __launch_bounds__(256) void lz_finals_kernel(const double* __restrict__ part_a,
                                                         const double* __restrict__ part_b, int np,
                                                         double* __restrict__ out, double* __restrict__ out_host) {
@@ -1385,34 +1358,151 @@ __global__ __launch_bounds__(256) void lz_finals_kernel(const double* __restrict
     out[0] = ar;
     out[1] = ai;
     out[2] = br;
-    if (out_host) {  // pinned host memory, read after the stream's event
+    if (out_host) {  // pinned host memory: the values, then the ready flag
       out_host[0] = ar;
       out_host[1] = ai;
       out_host[2] = br;
+      __threadfence_system();
+      out_host[3] = 1.0;
+      __threadfence_system();
     }
   }
 }
 
+// ---------------------------------------------------------------------------
+// The rest of a Lanczos iteration after the matvec and the alpha dot, in two
+// launches.  The reference (real_tensor_lanczos.cc:80-151, GramSchmidtMethod
+// :188-201) subtracts alpha psi_i, forms beta = |h|, y = h / beta, projects y
+// against psi_0 .. psi_i one after the other and normalises.  Here
+// lz_alpha_multi subtracts alpha psi_i and forms the partials of |h|^2 and of
+// every <psi_j|h>; lz_next_all forms beta, c_j = <psi_j|h> / beta and writes
+// (h / beta - sum_j c_j psi_j) / r with r^2 = 1 - sum_j |c_j|^2 (|h / beta|
+// = 1 by the definition of beta).  For an orthonormal basis this is the
+// reference's vector: its sequential coefficients differ from c_j by terms of
+// second order in the overlaps, which the three-term recurrence leaves at the
+// rounding level, and |y - sum c_j psi_j|^2 = |y|^2 - sum |c_j|^2.  Partial
+// sums: grid g = dot_grid(n), element e to thread e mod (256 g), a fixed block
+// combine; every block that needs a coefficient sums the partials in one
+// fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kMultiChunk = 4;  // overlaps accumulated per pass over the elements
+
+// block sums of NV per-thread values; value v < nvalid goes to part[(base +
+// v) g + blockIdx]
+template <int NV>
+__device__ inline void lz_block_store_multi(const double (&val)[NV], int nvalid, int base,
+                                            double* __restrict__ part, int g) {
+  __shared__ double red[2 * kMultiChunk][4];
+  static_assert(NV <= 2 * kMultiChunk, "values per block store");
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double t = wave_sum(val[v]);
+    if ((threadIdx.x & 63) == 0) red[v][w] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nvalid) {
+    const int v = threadIdx.x;
+    part[(int64_t)(base + v) * g + blockIdx.x] = (red[v][0] + red[v][1]) + (red[v][2] + red[v][3]);
+  }
+  __syncthreads();
+}
+
+// alpha = final(part_a); h -= alpha psi; part_b <- partials of |h|^2;
+// part_m <- partials of <x_j|h> for j < nj (x_j = x + j ld; complex: value
+// pairs, groups 2 j and 2 j + 1)
 template <bool kC>
-__global__ __launch_bounds__(256) void lz_next_kernel(double* __restrict__ nxt, double* __restrict__ h,
-                                                      const double* __restrict__ psi,
-                                                      const double* __restrict__ x0, int64_t n,
-                                                      double beta, const double* __restrict__ beta2,
-                                                      double* __restrict__ part_out,
-                                                      const double* __restrict__ part_a,
-                                                      const double* __restrict__ part_b, int np,
-                                                      double* __restrict__ slot,
-                                                      double* __restrict__ slot_host) {
+__global__ __launch_bounds__(256) void lz_alpha_multi_kernel(double* __restrict__ h,
+                                                             const double* __restrict__ psi,
+                                                             const double* __restrict__ x, int64_t ld, int nj,
+                                                             int64_t n, const double* __restrict__ part_a,
+                                                             double* __restrict__ part_b,
+                                                             double* __restrict__ part_m) {
+  const int g = gridDim.x;
+  double cr, ci;
+  lz_final<kC>(part_a, g, cr, ci);
+  const double ar = -1.0 * cr, ai = -1.0 * ci;
+  const int64_t stride = (int64_t)g * blockDim.x;
+  const int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  for (int64_t e = e0; e < n; e += stride) {
+    if (kC) {
+      const double xr = psi[2 * e], xi = psi[2 * e + 1];
+      const double yr = h[2 * e] + (ar * xr - ai * xi);
+      const double yi = h[2 * e + 1] + (ar * xi + ai * xr);
+      h[2 * e] = yr;
+      h[2 * e + 1] = yi;
+      nrm += yr * yr + yi * yi;
+    } else {
+      const double y = h[e] + ar * psi[e];
+      h[e] = y;
+      nrm += y * y;
+    }
+  }
+  {
+    const double v[1] = {nrm};
+    lz_block_store_multi(v, 1, 0, part_b, g);
+  }
+  // overlaps, kMultiChunk vectors per pass (this thread's own elements of h)
+  for (int j0 = 0; j0 < nj; j0 += kMultiChunk) {
+    double sr[kMultiChunk], si[kMultiChunk];
+#pragma unroll
+    for (int u = 0; u < kMultiChunk; ++u) sr[u] = si[u] = 0.0;
+    for (int64_t e = e0; e < n; e += stride) {
+      if (kC) {
+        const double yr = h[2 * e], yi = h[2 * e + 1];
+#pragma unroll
+        for (int u = 0; u < kMultiChunk; ++u)
+          if (j0 + u < nj) {
+            const double* xj = x + (int64_t)(j0 + u) * ld;
+            const double xr = xj[2 * e], xi = xj[2 * e + 1];
+            sr[u] += xr * yr + xi * yi;
+            si[u] += xr * yi - xi * yr;
+          }
+      } else {
+        const double y = h[e];
+#pragma unroll
+        for (int u = 0; u < kMultiChunk; ++u)
+          if (j0 + u < nj) sr[u] += x[(int64_t)(j0 + u) * ld + e] * y;
+      }
+    }
+    const int nu = min(kMultiChunk, nj - j0);
+    if (kC) {
+      double v[2 * kMultiChunk];
+#pragma unroll
+      for (int u = 0; u < kMultiChunk; ++u) {
+        v[2 * u] = sr[u];
+        v[2 * u + 1] = si[u];
+      }
+      lz_block_store_multi(v, 2 * nu, 2 * j0, part_m, g);
+    } else {
+      lz_block_store_multi(sr, nu, j0, part_m, g);
+    }
+  }
+}
+
+// beta: formed from part_b (block 0 then also forms alpha from part_a and
+// publishes (alpha, beta^2) to slot and slot_host, then the ready flag
+// slot_host[3]), or sqrt of the device value beta2.  c_j = final(part_m group)
+// / beta; nxt = (h / beta - sum_j c_j x_j) / r, r^2 = 1 - sum_j |c_j|^2 (j
+// ascending; a non-positive r^2, a breakdown the solve's beta test stops at,
+// divides by 1); h = -beta psi.
+template <bool kC>
+__global__ __launch_bounds__(256) void lz_next_all_kernel(
+    double* __restrict__ nxt, double* __restrict__ h, const double* __restrict__ psi,
+    const double* __restrict__ x, int64_t ld, int nj, int64_t n, const double* __restrict__ part_m,
+    const double* __restrict__ part_a, const double* __restrict__ part_b, const double* __restrict__ beta2,
+    double* __restrict__ slot, double* __restrict__ slot_host) {
+  extern __shared__ double coef[];  // (kC ? 2 : 1) nj values
+  const int g = gridDim.x;
+  double beta;
   if (part_b) {
-    // the previous step's coefficients from its partials, as lz_finals
-    // forms them: beta^2 in every block, alpha in block 0, which also
-    // publishes (alpha, beta^2) to the device slot and to pinned host memory
     double br, bi;
-    lz_final<false>(part_b, np, br, bi);
+    lz_final<false>(part_b, g, br, bi);
     beta = sqrt(br);
-    if (blockIdx.x == 0) {
+    if (slot && blockIdx.x == 0) {
       double ar, ai;
-      lz_final<kC>(part_a, np, ar, ai);
+      lz_final<kC>(part_a, g, ar, ai);
       if (threadIdx.x == 0) {
         slot[0] = ar;
         slot[1] = ai;
@@ -1420,78 +1510,48 @@ __global__ __launch_bounds__(256) void lz_next_kernel(double* __restrict__ nxt, 
         slot_host[0] = ar;
         slot_host[1] = ai;
         slot_host[2] = br;
+        // the ready flag after the values, both pushed out to host memory
+        __threadfence_system();
+        slot_host[3] = 1.0;
+        __threadfence_system();
       }
     }
-  } else if (beta2) {
-    // beta from the device (sqrt of beta^2, correctly rounded like the
-    // host's), when the host has not read it yet
+  } else {
     beta = sqrt(beta2[0]);
   }
-  double sr = 0.0, si = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (kC) {
-      const double yr = h[2 * i] / beta, yi = h[2 * i + 1] / beta;
-      nxt[2 * i] = yr;
-      nxt[2 * i + 1] = yi;
-      h[2 * i] = -beta * psi[2 * i];
-      h[2 * i + 1] = -beta * psi[2 * i + 1];
-      const double xr = x0[2 * i], xi = x0[2 * i + 1];
-      sr += xr * yr + xi * yi;
-      si += xr * yi - xi * yr;
-    } else {
-      const double y = h[i] / beta;
-      nxt[i] = y;
-      h[i] = -beta * psi[i];
-      sr += x0[i] * y;
-    }
+  const int nvals = (kC ? 2 : 1) * nj;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int v = w; v < nvals; v += 4) {
+    double s = 0.0;
+    for (int i = lane; i < g; i += 64) s += part_m[(int64_t)v * g + i];
+    s = wave_sum(s);
+    if (lane == 0) coef[v] = s / beta;
   }
-  lz_block_store<kC>(sr, si, part_out);
-}
-
-template <bool kC>
-__global__ __launch_bounds__(256) void lz_project_kernel(double* __restrict__ y,
-                                                         const double* __restrict__ xj,
-                                                         const double* __restrict__ xnext, int64_t n,
-                                                         const double* __restrict__ part_in, int np,
-                                                         double* __restrict__ part_out) {
-  double cr, ci;
-  lz_final<kC>(part_in, np, cr, ci);
-  const double ar = -1.0 * cr, ai = -1.0 * ci;
-  double sr = 0.0, si = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  __syncthreads();
+  double r2 = 1.0;
+  for (int v = 0; v < nvals; ++v) r2 -= coef[v] * coef[v];
+  const double r = r2 > 0.0 ? sqrt(r2) : 1.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
     if (kC) {
-      const double xr = xj[2 * i], xi = xj[2 * i + 1];
-      const double yr = y[2 * i] + (ar * xr - ai * xi);
-      const double yi = y[2 * i + 1] + (ar * xi + ai * xr);
-      y[2 * i] = yr;
-      y[2 * i + 1] = yi;
-      if (xnext) {
-        const double zr = xnext[2 * i], zi = xnext[2 * i + 1];
-        sr += zr * yr + zi * yi;
-        si += zr * yi - zi * yr;
-      } else {
-        sr += yr * yr + yi * yi;
+      double vr = h[2 * e] / beta, vi = h[2 * e + 1] / beta;
+      for (int j = 0; j < nj; ++j) {
+        const double* xj = x + (int64_t)j * ld;
+        const double xr = xj[2 * e], xi = xj[2 * e + 1], cr = coef[2 * j], ci = coef[2 * j + 1];
+        vr -= cr * xr - ci * xi;
+        vi -= cr * xi + ci * xr;
       }
+      nxt[2 * e] = vr / r;
+      nxt[2 * e + 1] = vi / r;
+      h[2 * e] = -beta * psi[2 * e];
+      h[2 * e + 1] = -beta * psi[2 * e + 1];
     } else {
-      const double v = y[i] + ar * xj[i];
-      y[i] = v;
-      sr += (xnext ? xnext[i] : v) * v;
+      double v = h[e] / beta;
+      for (int j = 0; j < nj; ++j) v -= coef[j] * x[(int64_t)j * ld + e];
+      nxt[e] = v / r;
+      h[e] = -beta * psi[e];
     }
   }
-  lz_block_store<kC>(sr, si, part_out);
-}
-
-template <bool kC>
-__global__ __launch_bounds__(256) void lz_normalize_kernel(double* __restrict__ y, int64_t nd,
-                                                           const double* __restrict__ part_in, int np) {
-  double cr, ci;
-  lz_final<kC>(part_in, np, cr, ci);
-  const double r = sqrt(cr);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd;
-       i += (int64_t)gridDim.x * blockDim.x)
-    y[i] /= r;
 }
 
 inline int grid_for(int64_t n) {
@@ -3939,16 +3999,6 @@ void lz_dot_partial(const double* x, const double* y, int64_t n, bool cplx, doub
   DMRG_HIP(hipGetLastError());
 }
 
-void lz_alpha(double* h, const double* psi, int64_t n, bool cplx, const double* part_in,
-              double* part_out, hipStream_t stream) {
-  const int g = dot_grid(n);
-  if (cplx)
-    hipLaunchKernelGGL(lz_alpha_kernel<true>, dim3(g), dim3(256), 0, stream, h, psi, n, part_in, g, part_out);
-  else
-    hipLaunchKernelGGL(lz_alpha_kernel<false>, dim3(g), dim3(256), 0, stream, h, psi, n, part_in, g, part_out);
-  DMRG_HIP(hipGetLastError());
-}
-
 void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx, double* out,
                hipStream_t stream, double* out_host) {
   const int g = dot_grid(n);
@@ -3959,41 +4009,36 @@ void lz_finals(const double* part_a, const double* part_b, int64_t n, bool cplx,
   DMRG_HIP(hipGetLastError());
 }
 
-void lz_next(double* nxt, double* h, const double* psi, const double* x0, int64_t n, bool cplx,
-             double beta, double* part_out, hipStream_t stream, const double* beta2, const double* part_a,
-             const double* part_b, double* slot, double* slot_host) {
+void lz_alpha_multi(double* h, const double* psi, const double* x, int64_t ld, int nj, int64_t n, bool cplx,
+                    const double* part_a, double* part_b, double* part_m, hipStream_t stream) {
+  if (n <= 0) return;
   const int g = dot_grid(n);
-  require(!part_b || (part_a && slot && slot_host && part_out != part_a && part_out != part_b),
-          "lz_next: publishing needs both partial buffers, the slots and a separate output", kErrInternal);
   if (cplx)
-    hipLaunchKernelGGL(lz_next_kernel<true>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
-                       part_out, part_a, part_b, g, slot, slot_host);
+    hipLaunchKernelGGL(lz_alpha_multi_kernel<true>, dim3(g), dim3(256), 0, stream, h, psi, x, ld, nj, n, part_a,
+                       part_b, part_m);
   else
-    hipLaunchKernelGGL(lz_next_kernel<false>, dim3(g), dim3(256), 0, stream, nxt, h, psi, x0, n, beta, beta2,
-                       part_out, part_a, part_b, g, slot, slot_host);
+    hipLaunchKernelGGL(lz_alpha_multi_kernel<false>, dim3(g), dim3(256), 0, stream, h, psi, x, ld, nj, n, part_a,
+                       part_b, part_m);
   DMRG_HIP(hipGetLastError());
 }
 
-void lz_project(double* y, const double* xj, const double* xnext, int64_t n, bool cplx,
-                const double* part_in, double* part_out, hipStream_t stream) {
+void lz_next_all(double* nxt, double* h, const double* psi, const double* x, int64_t ld, int nj, int64_t n,
+                 bool cplx, const double* part_m, hipStream_t stream, const double* beta2, const double* part_a,
+                 const double* part_b, double* slot, double* slot_host) {
+  if (n <= 0) return;
+  require(nj >= 1, "lz_next_all: needs at least one basis vector", kErrInternal);
+  require((part_b && (!slot || (part_a && slot_host))) || (!part_b && beta2),
+          "lz_next_all: beta needs the beta^2 partials or the device value; publishing needs alpha's partials "
+          "and both slots",
+          kErrInternal);
   const int g = dot_grid(n);
+  const size_t lds = (size_t)(cplx ? 2 : 1) * nj * sizeof(double);
   if (cplx)
-    hipLaunchKernelGGL(lz_project_kernel<true>, dim3(g), dim3(256), 0, stream, y, xj, xnext, n, part_in, g,
-                       part_out);
+    hipLaunchKernelGGL(lz_next_all_kernel<true>, dim3(g), dim3(256), lds, stream, nxt, h, psi, x, ld, nj, n, part_m,
+                       part_a, part_b, beta2, slot, slot_host);
   else
-    hipLaunchKernelGGL(lz_project_kernel<false>, dim3(g), dim3(256), 0, stream, y, xj, xnext, n, part_in, g,
-                       part_out);
-  DMRG_HIP(hipGetLastError());
-}
-
-void lz_normalize(double* y, int64_t n, bool cplx, const double* part_in, hipStream_t stream) {
-  const int64_t nd = cplx ? 2 * n : n;
-  if (nd <= 0) return;
-  const int g = dot_grid(n);
-  if (cplx)
-    hipLaunchKernelGGL(lz_normalize_kernel<true>, dim3(grid_for(nd)), dim3(256), 0, stream, y, nd, part_in, g);
-  else
-    hipLaunchKernelGGL(lz_normalize_kernel<false>, dim3(grid_for(nd)), dim3(256), 0, stream, y, nd, part_in, g);
+    hipLaunchKernelGGL(lz_next_all_kernel<false>, dim3(g), dim3(256), lds, stream, nxt, h, psi, x, ld, nj, n,
+                       part_m, part_a, part_b, beta2, slot, slot_host);
   DMRG_HIP(hipGetLastError());
 }
 
